@@ -1,0 +1,326 @@
+/*
+ * rp_api.h -- C ABI of the MI355X (gfx950) wavefront spectral path tracer.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, integer status codes,
+ * no C++ exceptions across the boundary.  Every synchronous call on one
+ * handle must be externally serialised (one host thread per handle at a time).
+ *
+ * Reference interfaces replaced (rlefko/rypath):
+ *   rp_scene_desc          RenderScene                      src/render.hpp:248-306
+ *   rp_scene_create        build_metal_renderer             src/metal_render.hpp:73-93
+ *   rp_sample              MetalRenderScene::sample         src/metal_render.hpp:56-57
+ *   rp_request             RenderRequest                    src/metal_render.hpp:14-32
+ *   rp_path_record         PacketSample (+ diagnostics)     src/render.hpp:307-316
+ *   rp_render_frame        FrameSession::submit/Frame::wait src/frame_session.hpp:145-200
+ *   rp_status              GeometryStatus                   src/metal_geometry.hpp:22
+ *   rp_path_status         render_wire status codes         src/render_wire.h:20-26
+ * Entry points with no counterpart in the reference (new patterns):
+ *   rp_trace_closest / rp_trace_any  (traversal probes, parity + roofline)
+ *   rp_batch_*                       (uneven multi-scene persistent work pool)
+ *   rp_autotune                      (queue capacity / workgroup / sort-key width)
+ *   rp_backward                      (path-replay adjoint)
+ */
+#ifndef RP_API_H
+#define RP_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RP_API_VERSION 1u
+#define RP_NONE 0xffffffffu
+#define RP_SPECTRUM_SAMPLES 471u /* 360..830 nm at 1 nm, nearest-nm lookup */
+
+/* ---- status codes (GeometryStatus, src/metal_geometry.hpp:22) ---------- */
+typedef enum rp_status {
+    RP_SUCCESS = 0,
+    RP_NOT_BUILT = 1,   /* library built without a device backend */
+    RP_NO_DEVICE = 2,   /* no HIP device visible */
+    RP_FAILURE = 3,     /* invalid input or device error; see rp_last_error */
+    RP_INVALID = 4      /* invalid argument (null pointer, bad size, bad enum) */
+} rp_status;
+
+/* ---- per-path status (render_wire::Status, src/render_wire.h:20-26) ---- */
+typedef enum rp_path_status {
+    RP_PATH_SUCCESS = 0,
+    RP_PATH_EMERGENCY_DEPTH = 1,
+    RP_PATH_INVALID_ARITHMETIC = 2,
+    RP_PATH_INVALID_HIT = 3,
+    RP_PATH_BOUNDARY_EXHAUSTION = 4
+} rp_path_status;
+
+/* ---- spectra ------------------------------------------------------------ */
+typedef enum rp_spectrum_kind {
+    RP_SPECTRUM_CONSTANT = 0, /* c[0] */
+    RP_SPECTRUM_SIGMOID = 1,  /* PBRT RGB sigmoid s(x), x = fma(fma(c0,l,c1),l,c2), l in nm */
+    RP_SPECTRUM_TABLE = 2,    /* 471 samples at `table` (offset in floats into spectra pool) */
+    RP_SPECTRUM_CAUCHY = 3,   /* c0 + c1 / (l_um * l_um) */
+    RP_SPECTRUM_LINEAR = 4    /* linear from c0 at 360 nm to c1 at 830 nm */
+} rp_spectrum_kind;
+
+typedef struct rp_spectrum {
+    uint32_t kind;
+    float c[3];
+    float scale;     /* multiplies the value (sigmoid: unbounded scale; others: 1) */
+    uint32_t table;  /* float offset into rp_scene_desc.spectra (TABLE kind) */
+} rp_spectrum;
+
+/* ---- materials (SpectralMaterial, src/render.hpp:108-129) -------------- */
+typedef enum rp_material_kind {
+    RP_MATERIAL_DIFFUSE = 0,
+    RP_MATERIAL_CONDUCTOR = 1,
+    RP_MATERIAL_DIELECTRIC = 2,
+    RP_MATERIAL_INTERFACE = 3 /* null interface: medium boundary, no scattering */
+} rp_material_kind;
+
+typedef struct rp_material {
+    uint32_t kind;
+    uint32_t remap_roughness; /* nonzero: alpha = sqrt(roughness) (PBRT) */
+    float roughness_u, roughness_v;
+    rp_spectrum reflectance; /* diffuse */
+    rp_spectrum eta;         /* conductor / dielectric */
+    rp_spectrum k;           /* conductor */
+} rp_material;
+
+/* ---- objects: material + area emission + medium interface per object --- */
+typedef struct rp_object {
+    uint32_t material;
+    uint32_t emission;       /* index into emissions or RP_NONE */
+    uint32_t medium_inside;  /* RP_NONE = vacuum */
+    uint32_t medium_outside;
+} rp_object;
+
+typedef struct rp_emission {
+    rp_spectrum radiance; /* emitted radiance L(lambda) */
+    uint32_t two_sided;   /* zero: emits on the geometric-normal side only */
+    uint32_t pad[3];
+} rp_emission;
+
+typedef struct rp_spot_light {
+    float position[3];
+    float direction[3];   /* unit cone axis */
+    float cos_falloff_start, cos_falloff_end;
+    rp_spectrum intensity;
+} rp_spot_light;
+
+typedef struct rp_medium {
+    rp_spectrum sigma_a, sigma_s; /* 1/unit */
+    float g;                      /* Henyey-Greenstein asymmetry, clamped to [-0.99, 0.99] */
+    uint32_t pad[3];
+} rp_medium;
+
+/* RasterCamera (src/render.hpp:153-158): d = corner + x*dx + y*dy, x,y in [0,1) */
+typedef struct rp_camera {
+    float origin[3];
+    float corner[3];
+    float dx[3];
+    float dy[3];
+    uint32_t width, height;
+    uint32_t medium; /* RP_NONE = vacuum */
+    uint32_t pad;
+} rp_camera;
+
+typedef struct rp_scene_desc {
+    uint32_t api_version;
+    uint32_t vertex_count, triangle_count, object_count;
+    const float *positions;      /* 3 * vertex_count */
+    const uint32_t *indices;     /* 3 * triangle_count */
+    const uint32_t *tri_object;  /* triangle_count */
+    const rp_object *objects;
+    uint32_t material_count, emission_count, medium_count, spot_count;
+    const rp_material *materials;
+    const rp_emission *emissions;
+    const rp_medium *media;
+    const rp_spot_light *spots;
+    const float *spectra;        /* pool of 471-sample tables */
+    uint32_t spectra_count;      /* floats in the pool */
+    uint32_t has_environment;
+    rp_spectrum environment;     /* constant-direction environment radiance */
+    rp_camera camera;
+    uint32_t max_depth;          /* PBRT maxdepth: scattering events */
+    uint32_t hero_wavelengths;   /* 0 = uniform (1/470), 1 = visible (PBRT fit) */
+} rp_scene_desc;
+
+typedef struct rp_build_options {
+    uint32_t leaf_size;       /* max triangles per BVH leaf (1..8), 0 = default 4 */
+    uint32_t sah_bins;        /* 0 = default 32 */
+    uint32_t threads;         /* host build threads, 0 = hardware concurrency */
+    uint32_t device;          /* HIP device ordinal */
+} rp_build_options;
+
+typedef struct rp_build_info {
+    double build_seconds, upload_seconds;
+    uint64_t node_count, leaf_count, device_bytes;
+    uint32_t max_depth, max_stack; /* BVH depth and worst-case traversal stack */
+    uint32_t emitter_count, light_count;
+} rp_build_info;
+
+/* ---- sampler ------------------------------------------------------------ */
+typedef enum rp_randomization {
+    RP_RANDOMIZE_NONE = 0, RP_RANDOMIZE_PERMUTE = 1, RP_RANDOMIZE_FASTOWEN = 2, RP_RANDOMIZE_OWEN = 3
+} rp_randomization;
+
+/* RenderRequest (src/metal_render.hpp:14-32) */
+typedef struct rp_request {
+    uint64_t seed;            /* sampler seed (PBRT signed 32-bit seed in the low word) */
+    uint32_t randomization;   /* rp_randomization */
+    uint32_t pixel_begin, pixel_count;
+    uint32_t sample_count;
+    uint64_t sample_begin;
+    uint32_t emergency_limit; /* 0 = default 1024 */
+    uint32_t boundary_limit;  /* 0 = default 4096 */
+} rp_request;
+
+/* PacketSample (src/render.hpp:307-316) plus parity diagnostics. 104 bytes. */
+typedef struct rp_path_record {
+    uint32_t pixel, status;
+    uint64_t sample;
+    uint64_t sobol_index;
+    float wavelengths[4];
+    float wavelength_pdfs[4];
+    float radiance[4];
+    float xyz[3];
+    uint32_t primary_triangle; /* scene triangle id of the camera ray's hit, RP_NONE = miss */
+    uint32_t vertices;          /* path vertices including the camera */
+    uint32_t camera_words[2];   /* Sobol words of dimensions 0 and 1 */
+    uint32_t pad;
+} rp_path_record;
+
+/* ---- frames ------------------------------------------------------------- */
+typedef enum rp_mode {
+    RP_MODE_MEGAKERNEL = 0,      /* one thread = one complete path (plain) */
+    RP_MODE_WAVEFRONT = 1,       /* queues, unsorted */
+    RP_MODE_WAVEFRONT_SORTED = 2 /* queues, shade queue sorted by material key */
+} rp_mode;
+
+typedef struct rp_frame_config {
+    uint32_t mode;             /* rp_mode */
+    uint32_t spp;
+    uint64_t sample_begin;
+    uint64_t seed;
+    uint32_t randomization;
+    uint32_t capacity;         /* wavefront paths in flight, 0 = auto */
+    uint32_t group_size;       /* threads per workgroup, 0 = auto */
+    uint32_t sort_bits;        /* material sort-key width, 0 = auto */
+    uint32_t flags;            /* RP_FRAME_COUNT_TRAFFIC: replay-count BVH node/triangle fetches */
+    uint32_t pad;
+} rp_frame_config;
+
+#define RP_FRAME_COUNT_TRAFFIC 1u
+
+typedef struct rp_frame_stats {
+    double seconds;                 /* device time of the frame (events) */
+    uint64_t paths;                 /* W*H*spp */
+    uint64_t extension_rays, shadow_rays;
+    uint64_t failed_paths;          /* status != success */
+    double trace_seconds;           /* wavefront: device time in closest-hit kernels */
+    uint64_t extend_launches;       /* wavefront: closest-hit kernel launches */
+    uint64_t node_fetches, triangle_tests; /* with RP_FRAME_COUNT_TRAFFIC: closest-hit traversal totals */
+} rp_frame_stats;
+
+/* ---- traversal probe ray/hit records ------------------------------------ */
+typedef struct rp_ray {
+    float origin[3]; float tmax;
+    float direction[3]; uint32_t pad;
+} rp_ray;
+
+typedef struct rp_hit {
+    uint32_t triangle; /* scene triangle id, RP_NONE = miss */
+    float t, b1, b2;   /* barycentrics of vertices 1 and 2 */
+} rp_hit;
+
+typedef struct rp_scene rp_scene;
+typedef struct rp_batch rp_batch;
+
+/* Library version and device availability. */
+uint32_t rp_api_version(void);
+int rp_device_count(void);
+/* Last error of the calling thread's most recent failing call (NUL-terminated). */
+size_t rp_last_error(char *buffer, size_t capacity);
+
+/* Build: deep-copies all scene data, builds the BVH on the host, uploads. */
+int rp_scene_create(const rp_scene_desc *desc, const rp_build_options *options, rp_scene **out);
+int rp_scene_info(const rp_scene *scene, rp_build_info *info);
+void rp_scene_destroy(rp_scene *scene);
+
+/* Per-path records for request->pixel_count * request->sample_count paths,
+ * lane = pixel_offset + pixel_count * sample_offset (src/render.metal:240-242).
+ * `out` is host memory with room for that many records. */
+int rp_sample(rp_scene *scene, const rp_request *request, rp_path_record *out, uint64_t capacity);
+
+/* One frame of W*H*spp paths.  Film: per-pixel mean XYZ (float32, W*H*3) of the
+ * samples [sample_begin, sample_begin+spp), reduced in increasing sample order
+ * with Neumaier-compensated double sums.  xyz_host may be NULL. */
+int rp_render_frame(rp_scene *scene, const rp_frame_config *config, float *xyz_host,
+                    rp_frame_stats *stats);
+/* Same, with the film written to caller-owned device memory (W*H*3 floats). */
+int rp_render_frame_device(rp_scene *scene, const rp_frame_config *config, float *xyz_device,
+                           rp_frame_stats *stats);
+
+/* Closest-hit / any-hit traversal of caller rays (host buffers). any-hit writes
+ * 1 (occluded) or 0 per ray into `occluded`. */
+int rp_trace_closest(rp_scene *scene, const rp_ray *rays, uint64_t count, rp_hit *hits,
+                     double *seconds);
+int rp_trace_any(rp_scene *scene, const rp_ray *rays, uint64_t count, uint8_t *occluded,
+                 double *seconds);
+
+/* ---- batched work pool (uneven scenes, one persistent launch) ----------- */
+typedef struct rp_batch_item {
+    rp_scene *scene;
+    uint32_t spp;
+    uint32_t pad;
+    uint64_t seed;
+} rp_batch_item;
+
+typedef struct rp_batch_config {
+    uint32_t capacity;    /* paths in flight, 0 = autotuned/default */
+    uint32_t group_size;  /* 0 = autotuned/default */
+    uint32_t sort_bits;   /* 0 = autotuned/default */
+    uint32_t use_graph;   /* nonzero: replay the bounce loop as a captured HIP graph */
+    uint32_t randomization;
+    uint32_t pad;
+} rp_batch_config;
+
+typedef struct rp_batch_stats {
+    double seconds;
+    uint64_t paths;
+    uint64_t extension_rays, shadow_rays;
+    uint64_t failed_paths;
+    uint32_t graph_launches, pad;
+} rp_batch_stats;
+
+int rp_batch_create(const rp_batch_item *items, uint32_t count, rp_batch **out);
+/* Renders every item; xyz_host[i] (may be NULL) receives item i's film. */
+int rp_batch_render(rp_batch *batch, const rp_batch_config *config, float **xyz_host,
+                    rp_batch_stats *stats);
+void rp_batch_destroy(rp_batch *batch);
+
+/* Autotuner: times candidate (capacity, group_size, sort_bits) triples on the
+ * batch and returns the fastest in `best`. */
+int rp_autotune(rp_batch *batch, rp_batch_config *best, double *best_seconds);
+
+/* ---- differentiable rendering ------------------------------------------- */
+typedef enum rp_param_kind {
+    RP_PARAM_REFLECTANCE_BIN = 0, /* material `target`, bin `bin` of `bins` over 360..830 nm */
+    RP_PARAM_ROUGHNESS = 1,       /* material `target` (u and v roughness) */
+    RP_PARAM_SIGMA_S_BIN = 2      /* medium `target`, bin `bin` of `bins` */
+} rp_param_kind;
+
+typedef struct rp_param {
+    uint32_t kind, target, bin, bins;
+} rp_param;
+
+/* Replace parameter values (same layout as rp_backward's gradient). */
+int rp_set_params(rp_scene *scene, const rp_param *params, uint32_t count, const float *values);
+/* Path-replay adjoint: dL/dparams for the frame defined by `config`, given
+ * dL/dXYZ per pixel (W*H*3 floats, host).  grad receives `count` floats. */
+int rp_backward(rp_scene *scene, const rp_frame_config *config, const rp_param *params,
+                uint32_t count, const float *dl_dxyz, float *grad, rp_frame_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RP_API_H */

@@ -1,0 +1,1427 @@
+// Oracle transport: scalar, one path at a time, restating the reference's
+// hero-wavelength path tracer (shaders/packet_pt_event_body.inc,
+// shaders/hero_pt_helpers.h, src/hero_spectrum_shared.h) with PBRT v4 BSDFs
+// (shaders/conductor.h, shaders/dielectric.h), homogeneous media
+// (shaders/medium.h) and the PBRT watertight triangle test.
+// TEST INFRASTRUCTURE ONLY (see oracle_internal.h).
+#include "oracle_internal.h"
+
+#include <atomic>
+#include <numeric>
+#include <thread>
+
+namespace orc {
+
+// =========================================================== scene + BVH
+struct Light {
+    uint32_t kind; // 0 triangle, 1 spot, 2 environment
+    uint32_t index;
+    float pmf, cdf;
+    float inv_area;
+};
+
+struct Scene {
+    rp_scene_desc d{};
+    std::vector<float> positions;
+    std::vector<uint32_t> indices, tri_object;
+    std::vector<rp_object> objects;
+    std::vector<rp_material> materials;
+    std::vector<rp_emission> emissions;
+    std::vector<rp_medium> media;
+    std::vector<rp_spot_light> spots;
+    std::vector<float> spectra;
+    // binary BVH
+    struct Node {
+        float lo[3], hi[3];
+        uint32_t left_or_first, count; // count == 0: interior (children left, left+1)
+    };
+    std::vector<Node> nodes;
+    std::vector<uint32_t> order;
+    std::vector<Light> lights;
+    std::vector<uint32_t> tri_light; // triangle -> light index or RP_NONE
+    float scene_radius = 0;
+    uint32_t m = 0;
+    V3 vert(uint32_t i) const { return {positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]}; }
+    void tri(uint32_t t, V3 &a, V3 &b, V3 &c) const {
+        a = vert(indices[3 * t]);
+        b = vert(indices[3 * t + 1]);
+        c = vert(indices[3 * t + 2]);
+    }
+};
+
+static void build_bvh(Scene &s) {
+    uint32_t n = s.d.triangle_count;
+    s.order.resize(n);
+    std::iota(s.order.begin(), s.order.end(), 0u);
+    std::vector<V3> cen(n), lo(n), hi(n);
+    for (uint32_t t = 0; t < n; ++t) {
+        V3 a, b, c;
+        s.tri(t, a, b, c);
+        lo[t] = {std::min({a.x, b.x, c.x}), std::min({a.y, b.y, c.y}), std::min({a.z, b.z, c.z})};
+        hi[t] = {std::max({a.x, b.x, c.x}), std::max({a.y, b.y, c.y}), std::max({a.z, b.z, c.z})};
+        cen[t] = {0.5f * (lo[t].x + hi[t].x), 0.5f * (lo[t].y + hi[t].y), 0.5f * (lo[t].z + hi[t].z)};
+    }
+    s.nodes.clear();
+    s.nodes.reserve(2 * n / 2 + 2);
+    struct Item { uint32_t node, begin, end; };
+    std::vector<Item> stack;
+    s.nodes.push_back({});
+    stack.push_back({0, 0, n});
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        Scene::Node nd{};
+        float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+        float cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (uint32_t i = it.begin; i < it.end; ++i) {
+            uint32_t t = s.order[i];
+            for (int a = 0; a < 3; ++a) {
+                l[a] = std::min(l[a], get(lo[t], a));
+                h[a] = std::max(h[a], get(hi[t], a));
+                cl[a] = std::min(cl[a], get(cen[t], a));
+                ch[a] = std::max(ch[a], get(cen[t], a));
+            }
+        }
+        // Pad boxes so that the triangle test alone decides hits.
+        for (int a = 0; a < 3; ++a) {
+            float pad = std::max(std::fabs(l[a]), std::fabs(h[a])) * 0x1p-18f + 1e-30f;
+            nd.lo[a] = l[a] - pad;
+            nd.hi[a] = h[a] + pad;
+        }
+        uint32_t count = it.end - it.begin;
+        int axis = 0;
+        float ext = ch[0] - cl[0];
+        for (int a = 1; a < 3; ++a)
+            if (ch[a] - cl[a] > ext) { ext = ch[a] - cl[a]; axis = a; }
+        if (count <= 4 || !(ext > 0)) {
+            nd.left_or_first = it.begin;
+            nd.count = count;
+            s.nodes[it.node] = nd;
+            continue;
+        }
+        uint32_t mid = it.begin + count / 2;
+        std::nth_element(s.order.begin() + it.begin, s.order.begin() + mid, s.order.begin() + it.end,
+                         [&](uint32_t x, uint32_t y) {
+                             float cx = get(cen[x], axis), cy = get(cen[y], axis);
+                             return cx < cy || (cx == cy && x < y);
+                         });
+        uint32_t left = uint32_t(s.nodes.size());
+        s.nodes.push_back({});
+        s.nodes.push_back({});
+        nd.left_or_first = left;
+        nd.count = 0;
+        s.nodes[it.node] = nd;
+        stack.push_back({left, it.begin, mid});
+        stack.push_back({left + 1, mid, it.end});
+    }
+}
+
+// PBRT v4 watertight ray/triangle test (Woop, Benthin, Wald 2013) with the
+// conservative t > 0 error bound.
+static bool intersect_triangle(V3 o, V3 d, float tmax, V3 p0, V3 p1, V3 p2, float &t_out, float &b0o,
+                               float &b1o, float &b2o) {
+    V3 p0t = p0 - o, p1t = p1 - o, p2t = p2 - o;
+    V3 ad = vabs(d);
+    int kz = ad.x > ad.y ? (ad.x > ad.z ? 0 : 2) : (ad.y > ad.z ? 1 : 2);
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    V3 dp = {get(d, kx), get(d, ky), get(d, kz)};
+    p0t = {get(p0t, kx), get(p0t, ky), get(p0t, kz)};
+    p1t = {get(p1t, kx), get(p1t, ky), get(p1t, kz)};
+    p2t = {get(p2t, kx), get(p2t, ky), get(p2t, kz)};
+    float sx = -dp.x / dp.z, sy = -dp.y / dp.z, sz = 1.0f / dp.z;
+    p0t.x = p0t.x + sx * p0t.z;
+    p0t.y = p0t.y + sy * p0t.z;
+    p1t.x = p1t.x + sx * p1t.z;
+    p1t.y = p1t.y + sy * p1t.z;
+    p2t.x = p2t.x + sx * p2t.z;
+    p2t.y = p2t.y + sy * p2t.z;
+    float e0 = dop(p1t.x, p2t.y, p1t.y, p2t.x);
+    float e1 = dop(p2t.x, p0t.y, p2t.y, p0t.x);
+    float e2 = dop(p0t.x, p1t.y, p0t.y, p1t.x);
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        e0 = float(double(p2t.y) * double(p1t.x) - double(p2t.x) * double(p1t.y));
+        e1 = float(double(p0t.y) * double(p2t.x) - double(p0t.x) * double(p2t.y));
+        e2 = float(double(p1t.y) * double(p0t.x) - double(p1t.x) * double(p0t.y));
+    }
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return false;
+    float det = e0 + e1 + e2;
+    if (det == 0.0f) return false;
+    p0t.z = p0t.z * sz;
+    p1t.z = p1t.z * sz;
+    p2t.z = p2t.z * sz;
+    float ts = e0 * p0t.z + e1 * p1t.z + e2 * p2t.z;
+    if (det < 0 && (ts >= 0 || ts < tmax * det)) return false;
+    if (det > 0 && (ts <= 0 || ts > tmax * det)) return false;
+    float inv = 1.0f / det;
+    float b0 = e0 * inv, b1 = e1 * inv, b2 = e2 * inv;
+    float t = ts * inv;
+    float max_zt = std::max(std::max(std::fabs(p0t.z), std::fabs(p1t.z)), std::fabs(p2t.z));
+    float delta_z = gamma_n(3) * max_zt;
+    float max_xt = std::max(std::max(std::fabs(p0t.x), std::fabs(p1t.x)), std::fabs(p2t.x));
+    float max_yt = std::max(std::max(std::fabs(p0t.y), std::fabs(p1t.y)), std::fabs(p2t.y));
+    float delta_x = gamma_n(5) * (max_xt + max_zt);
+    float delta_y = gamma_n(5) * (max_yt + max_zt);
+    float delta_e = 2.0f * (gamma_n(2) * max_xt * max_yt + delta_y * max_xt + delta_x * max_yt);
+    float max_e = std::max(std::max(std::fabs(e0), std::fabs(e1)), std::fabs(e2));
+    float delta_t = 3.0f * (gamma_n(3) * max_e * max_zt + delta_e * max_zt + delta_z * max_e) * std::fabs(inv);
+    if (t <= delta_t) return false;
+    t_out = t;
+    b0o = b0;
+    b1o = b1;
+    b2o = b2;
+    return true;
+}
+
+struct Hit {
+    uint32_t tri = RP_NONE;
+    float t = INFINITY, b0 = 0, b1 = 0, b2 = 0;
+};
+
+static bool box_hit(const Scene::Node &n, V3 o, V3 inv, float tmax) {
+    float t0 = 0.0f, t1 = tmax;
+    for (int a = 0; a < 3; ++a) {
+        float oa = get(o, a), ia = get(inv, a);
+        float tn = (n.lo[a] - oa) * ia, tf = (n.hi[a] - oa) * ia;
+        if (tn > tf) std::swap(tn, tf);
+        tf *= 1.0f + 2.0f * gamma_n(3);
+        if (tn == tn) t0 = tn > t0 ? tn : t0;
+        if (tf == tf) t1 = tf < t1 ? tf : t1;
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
+// Closest hit: smallest t, ties to the smallest triangle id (order independent).
+static Hit closest(const Scene &s, V3 o, V3 d, float tmax) {
+    Hit h;
+    if (s.nodes.empty()) return h;
+    V3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    uint32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const auto &n = s.nodes[stack[--sp]];
+        if (!box_hit(n, o, inv, std::min(tmax, h.t))) continue;
+        if (n.count) {
+            for (uint32_t i = 0; i < n.count; ++i) {
+                uint32_t tri = s.order[n.left_or_first + i];
+                V3 a, b, c;
+                s.tri(tri, a, b, c);
+                float t, b0, b1, b2;
+                if (intersect_triangle(o, d, tmax, a, b, c, t, b0, b1, b2) &&
+                    (t < h.t || (t == h.t && tri < h.tri))) {
+                    h = {tri, t, b0, b1, b2};
+                }
+            }
+        } else {
+            stack[sp++] = n.left_or_first;
+            stack[sp++] = n.left_or_first + 1;
+        }
+    }
+    return h;
+}
+
+// ====================================================== surface geometry
+struct SurfacePoint {
+    V3 p, n, err;
+};
+static SurfacePoint surface_point(const Scene &s, const Hit &h) {
+    V3 a, b, c;
+    s.tri(h.tri, a, b, c);
+    SurfacePoint sp;
+    sp.p = h.b0 * a + h.b1 * b + h.b2 * c;
+    V3 abs_sum = vabs(h.b0 * a) + vabs(h.b1 * b) + vabs(h.b2 * c);
+    sp.err = gamma_n(7) * abs_sum;
+    sp.n = normalize(cross(a - c, b - c));
+    return sp;
+}
+// PBRT v4 OffsetRayOrigin.
+static V3 offset_origin(V3 p, V3 err, V3 n, V3 w) {
+    float dd = dot(vabs(n), err);
+    V3 off = dd * n;
+    if (dot(w, n) < 0) off = -off;
+    V3 po = p + off;
+    float *c = &po.x;
+    const float *oc = &off.x;
+    for (int i = 0; i < 3; ++i) {
+        if (oc[i] > 0) c[i] = next_up(c[i]);
+        else if (oc[i] < 0) c[i] = next_down(c[i]);
+    }
+    return po;
+}
+
+struct Frame {
+    V3 x, y, z;
+    V3 local(V3 v) const { return {dot(v, x), dot(v, y), dot(v, z)}; }
+    V3 world(V3 v) const { return x * v.x + y * v.y + z * v.z; }
+};
+// Duff et al. 2017 branchless orthonormal basis (PBRT v4 CoordinateSystem).
+static Frame frame_from_z(V3 n) {
+    float sign = std::copysign(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    Frame f;
+    f.x = {1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    f.y = {b, sign + n.y * n.y * a, -n.y};
+    f.z = n;
+    return f;
+}
+
+// ================================================================ BSDFs
+struct Lane {
+    uint32_t kind = RP_MATERIAL_DIFFUSE;
+    float r = 0, eta = 1, k = 0, ax = 0, ay = 0;
+    bool smooth = false;
+};
+
+static Lane prepare_lane(const Scene &s, const rp_material &m, float lambda) {
+    Lane l;
+    l.kind = m.kind;
+    const float *pool = s.spectra.data();
+    if (m.kind == RP_MATERIAL_DIFFUSE) {
+        l.r = spectrum_eval(m.reflectance, lambda, pool);
+        return l;
+    }
+    l.eta = spectrum_eval(m.eta, lambda, pool);
+    if (m.kind == RP_MATERIAL_CONDUCTOR) l.k = spectrum_eval(m.k, lambda, pool);
+    float u = m.roughness_u, v = m.roughness_v;
+    l.ax = m.remap_roughness ? std::sqrt(u) : u;
+    l.ay = m.remap_roughness ? std::sqrt(v) : v;
+    if (m.kind == RP_MATERIAL_DIELECTRIC && l.eta == 0.0f) l.eta = 1.0f;
+    l.smooth = std::max(l.ax, l.ay) < 1e-3f || (m.kind == RP_MATERIAL_DIELECTRIC && l.eta == 1.0f);
+    if (!l.smooth) {
+        l.ax = std::max(l.ax, 1e-4f);
+        l.ay = std::max(l.ay, 1e-4f);
+    }
+    return l;
+}
+
+struct Cx {
+    float re, im;
+};
+static Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+static Cx cdiv(Cx a, Cx b) {
+    float scale = 1.0f / (b.re * b.re + b.im * b.im);
+    return {scale * (a.re * b.re + a.im * b.im), scale * (a.im * b.re - a.re * b.im)};
+}
+static float cnorm(Cx a) { return a.re * a.re + a.im * a.im; }
+static Cx csqrt(Cx z) {
+    float n = std::sqrt(cnorm(z));
+    if (n == 0.0f) return {0.0f, 0.0f};
+    float t1 = std::sqrt(0.5f * (n + std::fabs(z.re)));
+    float t2 = 0.5f * z.im / t1;
+    if (z.re >= 0.0f) return {t1, t2};
+    return {std::fabs(t2), std::copysign(t1, z.im)};
+}
+// PBRT v4 FrComplex.
+static float fresnel_complex(float cos_i, float eta, float k) {
+    cos_i = std::min(std::max(cos_i, 0.0f), 1.0f);
+    Cx e = {eta, k};
+    float sin2_i = 1.0f - cos_i * cos_i;
+    Cx sin2_t = cdiv({sin2_i, 0.0f}, cmul(e, e));
+    Cx cos_t = csqrt({1.0f - sin2_t.re, -sin2_t.im});
+    Cx ec = {e.re * cos_i, e.im * cos_i};
+    Cx r_parl = cdiv({ec.re - cos_t.re, ec.im - cos_t.im}, {ec.re + cos_t.re, ec.im + cos_t.im});
+    Cx ect = cmul(e, cos_t);
+    Cx r_perp = cdiv({cos_i - ect.re, -ect.im}, {cos_i + ect.re, ect.im});
+    return (cnorm(r_parl) + cnorm(r_perp)) * 0.5f;
+}
+// PBRT v4 FrDielectric.
+static float fresnel_dielectric(float cos_i, float eta) {
+    cos_i = std::min(std::max(cos_i, -1.0f), 1.0f);
+    if (cos_i < 0.0f) {
+        eta = 1.0f / eta;
+        cos_i = -cos_i;
+    }
+    float sin2_i = 1.0f - cos_i * cos_i;
+    float sin2_t = sin2_i / (eta * eta);
+    if (sin2_t >= 1.0f) return 1.0f;
+    float cos_t = std::sqrt(std::max(0.0f, 1.0f - sin2_t));
+    float r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    return (r_parl * r_parl + r_perp * r_perp) * 0.5f;
+}
+// Trowbridge-Reitz D and Smith Lambda (shaders/conductor.h:72-90 forms).
+static float ggx_d(const Lane &l, V3 m) {
+    if (m.z * m.z * m.z * m.z < 1e-16f) return 0.0f;
+    V3 v = {m.x / l.ax, m.y / l.ay, m.z};
+    float t = dot(v, v);
+    return 1.0f / (kPi * l.ax * l.ay * t * t);
+}
+static float ggx_lambda(const Lane &l, V3 w) {
+    V3 v = {1.0f, l.ax * w.x / w.z, l.ay * w.y / w.z};
+    return 0.5f * (length(v) - 1.0f);
+}
+// Heitz 2018 visible-normal sampling (PBRT v4 Sample_wm).
+static V3 ggx_sample_wm(const Lane &l, V3 w, float u0, float u1) {
+    V3 wh = normalize(V3{l.ax * w.x, l.ay * w.y, w.z});
+    if (wh.z < 0) wh = -wh;
+    V3 t1 = wh.z < 0.99999f ? normalize(V3{-wh.y, wh.x, 0.0f}) : V3{1.0f, 0.0f, 0.0f};
+    V3 t2 = cross(wh, t1);
+    float r = std::sqrt(u0);
+    float sn, cs;
+    sincos_turns(u1, sn, cs);
+    float px = r * cs, py = r * sn;
+    float h = std::sqrt(1.0f - px * px);
+    float a = (1.0f + wh.z) * 0.5f;
+    py = (1.0f - a) * h + a * py;
+    float pz = std::sqrt(std::max(0.0f, 1.0f - (px * px + py * py)));
+    V3 nh = px * t1 + py * t2 + pz * wh;
+    return normalize(V3{l.ax * nh.x, l.ay * nh.y, std::max(1e-6f, nh.z)});
+}
+
+struct Eval {
+    float f = 0, pdf = 0;
+};
+
+static Eval eval_lane(const Lane &l, V3 wo, V3 wi) {
+    Eval e;
+    if (l.kind == RP_MATERIAL_DIFFUSE) {
+        if (!(wo.z * wi.z > 0)) return e;
+        e.f = l.r * kInvPi;
+        e.pdf = std::fabs(wi.z) * kInvPi;
+        return e;
+    }
+    if (l.smooth || wo.z == 0.0f || wi.z == 0.0f) return e;
+    if (l.kind == RP_MATERIAL_CONDUCTOR) {
+        if (!(wo.z * wi.z > 0)) return e;
+        V3 h = wo + wi;
+        if (dot(h, h) == 0.0f) return e;
+        V3 m = normalize(h);
+        float F = fresnel_complex(std::fabs(dot(wo, m)), l.eta, l.k);
+        float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+        e.f = D * F / (4.0f * std::fabs(wo.z) * std::fabs(wi.z) * (1.0f + lo + li));
+        e.pdf = D / (4.0f * std::fabs(wo.z) * (1.0f + lo));
+        return e;
+    }
+    // rough dielectric
+    bool reflect = wo.z * wi.z > 0;
+    float etap = 1.0f;
+    if (!reflect) etap = wo.z > 0 ? l.eta : 1.0f / l.eta;
+    V3 h = wi * etap + wo;
+    if (dot(h, h) == 0.0f) return e;
+    V3 m = normalize(h);
+    if (m.z < 0) m = -m;
+    float dmi = dot(m, wi), dmo = dot(m, wo);
+    if (dmi * wi.z < 0 || dmo * wo.z < 0) return e;
+    float F = fresnel_dielectric(dmo, l.eta);
+    float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+    float G = 1.0f / (1.0f + lo + li);
+    float pdf_wm = D / (std::fabs(wo.z) * (1.0f + lo)) * std::fabs(dmo);
+    float R = F, T = 1.0f - F;
+    if (reflect) {
+        e.f = D * G * F / std::fabs(4.0f * wi.z * wo.z);
+        e.pdf = pdf_wm / (4.0f * std::fabs(dmo)) * (R / (R + T));
+    } else {
+        float s = dmi + dmo / etap;
+        float denom = s * s * wi.z * wo.z;
+        float ft = D * T * G * std::fabs(dmi * dmo / denom);
+        e.f = ft / (etap * etap);
+        e.pdf = pdf_wm * (std::fabs(dmi) / (s * s)) * (T / (R + T));
+    }
+    return e;
+}
+
+struct Sampled {
+    V3 wi;
+    float pdf = 0;
+    float eta = 1; // relative eta of a transmission event (1 otherwise)
+    bool delta = false, transmission = false, ok = false;
+};
+static bool refract(V3 wi, V3 n, float eta, float &etap, V3 &wt) {
+    float cos_i = dot(n, wi);
+    if (cos_i < 0) {
+        eta = 1.0f / eta;
+        cos_i = -cos_i;
+        n = -n;
+    }
+    float sin2_i = std::max(0.0f, 1.0f - cos_i * cos_i);
+    float sin2_t = sin2_i / (eta * eta);
+    if (sin2_t >= 1.0f) return false;
+    float cos_t = std::sqrt(std::max(0.0f, 1.0f - sin2_t));
+    wt = -wi / eta + (cos_i / eta - cos_t) * n;
+    etap = eta;
+    return true;
+}
+
+// Concentric-disk cosine hemisphere (PBRT v4 SampleCosineHemisphere).
+static V3 cosine_hemisphere(float u0, float u1) {
+    float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (ox != 0.0f || oy != 0.0f) {
+        float r, turns;
+        if (std::fabs(ox) > std::fabs(oy)) {
+            r = ox;
+            turns = 0.125f * (oy / ox);
+        } else {
+            r = oy;
+            turns = 0.25f - 0.125f * (ox / oy);
+        }
+        float sn, cs;
+        sincos_turns(turns, sn, cs);
+        dx = r * cs;
+        dy = r * sn;
+    }
+    float z = std::sqrt(std::max(0.0f, 1.0f - dx * dx - dy * dy));
+    return {dx, dy, z};
+}
+
+static Sampled sample_lane(const Lane &l, V3 wo, float uc, float u0, float u1) {
+    Sampled s;
+    if (wo.z == 0.0f) return s;
+    if (l.kind == RP_MATERIAL_DIFFUSE) {
+        V3 wi = cosine_hemisphere(u0, u1);
+        if (wo.z < 0) wi.z = -wi.z;
+        s.wi = wi;
+        s.pdf = std::fabs(wi.z) * kInvPi;
+        s.ok = s.pdf > 0;
+        return s;
+    }
+    if (l.kind == RP_MATERIAL_CONDUCTOR) {
+        if (l.smooth) {
+            s.wi = {-wo.x, -wo.y, wo.z};
+            s.pdf = 1.0f;
+            s.delta = true;
+            s.ok = true;
+            return s;
+        }
+        V3 m = ggx_sample_wm(l, wo, u0, u1);
+        V3 wi = 2.0f * dot(wo, m) * m - wo;
+        if (!(wo.z * wi.z > 0)) return s;
+        float lo = ggx_lambda(l, wo);
+        float D = ggx_d(l, m);
+        if (D == 0.0f) return s;
+        s.wi = wi;
+        s.pdf = D / (4.0f * std::fabs(wo.z) * (1.0f + lo));
+        s.ok = s.pdf > 0;
+        return s;
+    }
+    // dielectric
+    if (l.smooth) {
+        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
+        if (uc < R / (R + T)) {
+            s.wi = {-wo.x, -wo.y, wo.z};
+            s.pdf = R / (R + T);
+        } else {
+            float etap;
+            V3 wt;
+            if (!refract(wo, {0.0f, 0.0f, 1.0f}, l.eta, etap, wt)) return s;
+            s.wi = wt;
+            s.pdf = T / (R + T);
+            s.eta = etap;
+            s.transmission = true;
+        }
+        s.delta = true;
+        s.ok = s.pdf > 0 && s.wi.z != 0.0f;
+        return s;
+    }
+    V3 m = ggx_sample_wm(l, wo, u0, u1);
+    float R = fresnel_dielectric(dot(wo, m), l.eta), T = 1.0f - R;
+    float lo = ggx_lambda(l, wo), D = ggx_d(l, m);
+    float pdf_wm = D / (std::fabs(wo.z) * (1.0f + lo)) * std::fabs(dot(wo, m));
+    if (uc < R / (R + T)) {
+        V3 wi = 2.0f * dot(wo, m) * m - wo;
+        if (!(wo.z * wi.z > 0)) return s;
+        s.wi = wi;
+        s.pdf = pdf_wm / (4.0f * std::fabs(dot(wo, m))) * (R / (R + T));
+    } else {
+        float etap;
+        V3 wi;
+        if (!refract(wo, m, l.eta, etap, wi)) return s;
+        if (wo.z * wi.z > 0 || wi.z == 0.0f) return s;
+        float sd = dot(wi, m) + dot(wo, m) / etap;
+        float dwm_dwi = std::fabs(dot(wi, m)) / (sd * sd);
+        s.wi = wi;
+        s.pdf = pdf_wm * dwm_dwi * (T / (R + T));
+        s.eta = etap;
+        s.transmission = true;
+    }
+    s.ok = s.pdf > 0 && D > 0;
+    return s;
+}
+
+// Companion-lane mass and density of a Dirac event (shaders/hero_scattering.h:98-150).
+static void dirac_lane(const Lane &l, V3 wo, V3 wi, float uc, float u0, float u1, float &mass, float &pdf) {
+    mass = pdf = 0.0f;
+    if (wo.z == 0.0f || wi.z == 0.0f) return;
+    bool transmission = wo.z * wi.z < 0;
+    if (l.kind == RP_MATERIAL_CONDUCTOR && l.smooth) {
+        if (transmission || !(wi.x == -wo.x && wi.y == -wo.y && wi.z == wo.z)) return;
+        pdf = 1.0f;
+        mass = fresnel_complex(std::fabs(wo.z), l.eta, l.k);
+        return;
+    }
+    if (l.kind == RP_MATERIAL_DIELECTRIC && l.smooth) {
+        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
+        Sampled c = sample_lane(l, wo, transmission ? R / (R + T) : 0.0f, u0, u1);
+        (void)uc;
+        if (!c.ok || !(c.wi.x == wi.x && c.wi.y == wi.y && c.wi.z == wi.z)) return;
+        pdf = c.pdf;
+        float scale = transmission ? 1.0f / (c.eta * c.eta) : 1.0f;
+        mass = (transmission ? T : R) * scale;
+    }
+}
+
+// ================================================================ lights
+static float avg_spectrum(const Scene &s, const rp_spectrum &sp) {
+    double acc = 0;
+    for (int i = 0; i < 471; ++i) acc += double(spectrum_eval(sp, float(360 + i), s.spectra.data()));
+    return float(acc / 471.0);
+}
+
+static void build_lights(Scene &s) {
+    std::vector<double> power;
+    s.lights.clear();
+    s.tri_light.assign(s.d.triangle_count, RP_NONE);
+    for (uint32_t t = 0; t < s.d.triangle_count; ++t) {
+        const auto &obj = s.objects[s.tri_object[t]];
+        if (obj.emission == RP_NONE) continue;
+        const auto &em = s.emissions[obj.emission];
+        V3 a, b, c;
+        s.tri(t, a, b, c);
+        double ex = double(b.x) - a.x, ey = double(b.y) - a.y, ez = double(b.z) - a.z;
+        double fx = double(c.x) - a.x, fy = double(c.y) - a.y, fz = double(c.z) - a.z;
+        double cx = ey * fz - ez * fy, cy = ez * fx - ex * fz, cz = ex * fy - ey * fx;
+        double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        if (!(area > 0)) continue;
+        double p = double(kPi) * area * double(avg_spectrum(s, em.radiance)) * (em.two_sided ? 2.0 : 1.0);
+        s.tri_light[t] = uint32_t(s.lights.size());
+        s.lights.push_back({0u, t, 0.0f, 0.0f, float(1.0 / area)});
+        power.push_back(p);
+    }
+    for (uint32_t i = 0; i < s.spots.size(); ++i) {
+        const auto &sp = s.spots[i];
+        double p = double(avg_spectrum(s, sp.intensity)) * 2.0 * double(kPi) *
+                   (1.0 - 0.5 * (double(sp.cos_falloff_start) + double(sp.cos_falloff_end)));
+        s.lights.push_back({1u, i, 0.0f, 0.0f, 0.0f});
+        power.push_back(p);
+    }
+    if (s.d.has_environment) {
+        double r = s.scene_radius;
+        double p = double(kPi) * r * r * 4.0 * double(kPi) * double(avg_spectrum(s, s.d.environment));
+        s.lights.push_back({2u, 0u, 0.0f, 0.0f, 0.0f});
+        power.push_back(p);
+    }
+    double total = 0;
+    for (double p : power) total += p;
+    double cum = 0;
+    for (size_t i = 0; i < s.lights.size(); ++i) {
+        cum += power[i];
+        s.lights[i].pmf = total > 0 ? float(power[i] / total) : 0.0f;
+        s.lights[i].cdf = total > 0 ? float(cum / total) : 0.0f;
+    }
+    if (!s.lights.empty() && total > 0) s.lights.back().cdf = 1.0f;
+}
+
+static void compute_radius(Scene &s) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t v = 0; v < s.d.vertex_count; ++v)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], double(s.positions[3 * v + a]));
+            hi[a] = std::max(hi[a], double(s.positions[3 * v + a]));
+        }
+    double r2 = 0;
+    for (int a = 0; a < 3; ++a) {
+        double h = 0.5 * (hi[a] - lo[a]);
+        r2 += h * h;
+    }
+    s.scene_radius = s.d.vertex_count ? float(std::sqrt(r2)) : 0.0f;
+}
+
+// ============================================================ transport
+struct Ctx {
+    const Scene *s;
+    uint64_t seed;
+    uint32_t mode;
+    uint32_t emergency;
+    uint32_t boundary;
+};
+
+static std::array<uint32_t, 4> draws(const Ctx &c, uint64_t index, uint32_t block) {
+    uint32_t seed_bits = uint32_t(c.seed);
+    uint32_t d = 4 * block;
+    return {sobol_word(index, d, c.mode, seed_bits), sobol_word(index, d + 1, c.mode, seed_bits),
+            sobol_word(index, d + 2, c.mode, seed_bits), sobol_word(index, d + 3, c.mode, seed_bits)};
+}
+// Philox medium stream, shaders/render.metal:128-136 (medium_local_block / medium_stream_v1).
+static std::array<uint32_t, 4> medium_draws(const Ctx &c, uint32_t pixel, uint64_t sample, uint32_t &draw) {
+    return philox({pixel, uint32_t(sample), uint32_t(sample >> 32), 0x20000000u | draw++},
+                  {uint32_t(c.seed), uint32_t(c.seed >> 32) ^ 0x6d656431u});
+}
+
+static float mixture(const float d[4], const float p[4]) {
+    return (d[0] * p[0] + d[1] * p[1]) + (d[2] * p[2] + d[3] * p[3]);
+}
+static float dsum(const float d[4]) { return (d[0] + d[1]) + (d[2] + d[3]); }
+static void normalize_density(float d[4]) {
+    float m = std::max(std::max(d[0], d[1]), std::max(d[2], d[3]));
+    if (!(m > 0) || !std::isfinite(m)) return;
+    int e;
+    (void)std::frexp(m, &e);
+    for (int k = 0; k < 4; ++k) d[k] = std::ldexp(d[k], -e);
+}
+// Cumulative balance update (src/hero_spectrum_shared.h:116-139).
+static bool advance(float density[4], float beta[4], const float num[4], const float pdf[4]) {
+    float nd[4];
+    for (int k = 0; k < 4; ++k) nd[k] = density[k] * pdf[k];
+    float before = dsum(density), after = dsum(nd);
+    if (!(before > 0 && after > 0) || !std::isfinite(after)) return false;
+    float ratio = before / after;
+    for (int k = 0; k < 4; ++k) {
+        beta[k] = (beta[k] * num[k]) * ratio;
+        if (!std::isfinite(beta[k])) return false;
+        density[k] = nd[k];
+    }
+    normalize_density(density);
+    return true;
+}
+static bool positive_normal(float v) {
+    uint32_t w = std::bit_cast<uint32_t>(v);
+    return w >= 0x00800000u && w < 0x7f800000u;
+}
+// Throughput roulette (shaders/hero_pt_helpers.h:56-85, src/packet_roulette_shared.h:61-108).
+static bool hero_continue(float beta[4], const float eta_scale[4], bool enabled, uint32_t depth, float u) {
+    if (!enabled || depth <= 1) return true;
+    float maximum = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float v = beta[k] * eta_scale[k];
+        if (!std::isfinite(v) || v < 0 || (beta[k] > 0 && !positive_normal(v))) return true;
+        maximum = std::max(maximum, v);
+    }
+    if (!positive_normal(maximum) || maximum >= 1.0f) return true;
+    float q = 1.0f - maximum;
+    float survival = 1.0f - q;
+    if (!(q > 0 && q < 1) || !positive_normal(survival)) return true;
+    float surv[4];
+    for (int k = 0; k < 4; ++k) {
+        surv[k] = beta[k];
+        if (std::bit_cast<uint32_t>(beta[k]) == 0) continue;
+        surv[k] = beta[k] / survival;
+        if (!positive_normal(surv[k])) return true;
+    }
+    if (u < q) return false;
+    for (int k = 0; k < 4; ++k) beta[k] = surv[k];
+    return true;
+}
+
+struct MediumCoeff {
+    float st[4], ss[4];
+    float g;
+};
+static MediumCoeff medium_coeff(const Scene &s, uint32_t m, const Wavelengths &wl) {
+    MediumCoeff c{};
+    const auto &md = s.media[m];
+    for (int k = 0; k < 4; ++k) {
+        float a = spectrum_eval(md.sigma_a, wl.lambda[k], s.spectra.data());
+        float sc = spectrum_eval(md.sigma_s, wl.lambda[k], s.spectra.data());
+        c.ss[k] = sc;
+        c.st[k] = a + sc;
+    }
+    c.g = std::min(std::max(md.g, -0.99f), 0.99f);
+    return c;
+}
+// Henyey-Greenstein, PBRT outward convention: cos = dot(wo, wi).
+static float hg(float cos_t, float g) {
+    float denom = 1.0f + g * g + 2.0f * g * cos_t;
+    return kInv4Pi * (1.0f - g * g) / (denom * std::sqrt(denom));
+}
+static V3 hg_sample(V3 wo, float g, float u0, float u1, float &pdf) {
+    float cos_t;
+    if (std::fabs(g) < 1e-3f) cos_t = 1.0f - 2.0f * u0;
+    else {
+        float sq = (1.0f - g * g) / (1.0f + g - 2.0f * g * u0);
+        cos_t = -1.0f / (2.0f * g) * (1.0f + g * g - sq * sq);
+    }
+    cos_t = std::min(std::max(cos_t, -1.0f), 1.0f);
+    float sin_t = std::sqrt(std::max(0.0f, 1.0f - cos_t * cos_t));
+    float sn, cs;
+    sincos_turns(u1, sn, cs);
+    Frame f = frame_from_z(wo);
+    V3 wi = f.world({sin_t * cs, sin_t * sn, cos_t});
+    pdf = hg(cos_t, g);
+    return wi;
+}
+
+static uint32_t outgoing_medium(const rp_object &o, uint32_t current, V3 n, V3 w) {
+    if (o.medium_inside == o.medium_outside) return current;
+    return dot(n, w) > 0 ? o.medium_outside : o.medium_inside;
+}
+
+// Visibility with transmittance through null interfaces and homogeneous media.
+// Returns false when blocked by a scattering surface.
+static bool transmittance(const Ctx &c, V3 o, V3 d, float tmax, uint32_t medium, const Wavelengths &wl,
+                          float tr[4], uint32_t &crossings) {
+    const Scene &s = *c.s;
+    for (int k = 0; k < 4; ++k) tr[k] = 1.0f;
+    for (;;) {
+        Hit h = closest(s, o, d, tmax);
+        float seg = h.tri == RP_NONE ? tmax : h.t;
+        if (medium != RP_NONE) {
+            MediumCoeff mc = medium_coeff(s, medium, wl);
+            float len = length(d);
+            for (int k = 0; k < 4; ++k) tr[k] = tr[k] * exp_f(-mc.st[k] * (seg * len));
+        }
+        if (h.tri == RP_NONE) return true;
+        const auto &obj = s.objects[s.tri_object[h.tri]];
+        if (s.materials[obj.material].kind != RP_MATERIAL_INTERFACE) return false;
+        if (++crossings > c.boundary) return false;
+        SurfacePoint sp = surface_point(s, h);
+        medium = outgoing_medium(obj, medium, sp.n, d);
+        V3 no = offset_origin(sp.p, sp.err, sp.n, d);
+        // Continue toward the same target point.
+        V3 target = o + tmax * d;
+        if (std::isinf(tmax)) {
+            o = no;
+        } else {
+            d = target - no;
+            o = no;
+            tmax = 1.0f;
+        }
+    }
+}
+
+struct PathOut {
+    uint32_t status = RP_PATH_SUCCESS;
+    float L[4] = {0, 0, 0, 0};
+    Wavelengths wl{};
+    uint64_t index = 0;
+    uint32_t primary = RP_NONE;
+    uint32_t vertices = 0;
+    uint32_t w0 = 0, w1 = 0;
+    uint64_t ext_rays = 0, shadow_rays = 0;
+};
+
+struct LightSample {
+    V3 p, n;
+    V3 err;
+    float le[4] = {0, 0, 0, 0};
+    float pdf = 0; // solid angle, including selection pmf
+    bool delta = false, infinite = false, ok = false;
+    V3 wi;
+    float dist = 0;
+};
+
+static uint32_t select_light(const Scene &s, float u) {
+    uint32_t lo = 0, hi = uint32_t(s.lights.size());
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (u < s.lights[mid].cdf) hi = mid;
+        else lo = mid + 1;
+    }
+    return std::min(lo, uint32_t(s.lights.size()) - 1);
+}
+
+static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, float ul, float u0, float u1) {
+    LightSample ls;
+    if (s.lights.empty()) return ls;
+    uint32_t li = select_light(s, ul);
+    const Light &L = s.lights[li];
+    if (!(L.pmf > 0)) return ls;
+    if (L.kind == 0) {
+        V3 a, b, c;
+        s.tri(L.index, a, b, c);
+        float b0, b1;
+        if (u0 < u1) {
+            b0 = u0 * 0.5f;
+            b1 = u1 - b0;
+        } else {
+            b1 = u1 * 0.5f;
+            b0 = u0 - b1;
+        }
+        float b2 = 1.0f - b0 - b1;
+        ls.p = b0 * a + b1 * b + b2 * c;
+        ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
+        ls.n = normalize(cross(a - c, b - c));
+        V3 dv = ls.p - p;
+        float d2 = dot(dv, dv);
+        if (!(d2 > 0)) return ls;
+        ls.dist = std::sqrt(d2);
+        ls.wi = dv / ls.dist;
+        float cos_l = dot(ls.n, -ls.wi);
+        const auto &em = s.emissions[s.objects[s.tri_object[L.index]].emission];
+        if (!em.two_sided && !(cos_l > 0)) return ls;
+        if (cos_l == 0.0f) return ls;
+        ls.pdf = ((L.pmf * L.inv_area) * d2) / std::fabs(cos_l);
+        for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(em.radiance, wl.lambda[k], s.spectra.data());
+        ls.ok = ls.pdf > 0 && std::isfinite(ls.pdf);
+        return ls;
+    }
+    if (L.kind == 1) {
+        const auto &sp = s.spots[L.index];
+        V3 pos = {sp.position[0], sp.position[1], sp.position[2]};
+        V3 dv = pos - p;
+        float d2 = dot(dv, dv);
+        if (!(d2 > 0)) return ls;
+        ls.dist = std::sqrt(d2);
+        ls.wi = dv / ls.dist;
+        V3 axis = {sp.direction[0], sp.direction[1], sp.direction[2]};
+        float cos_t = dot(-ls.wi, axis);
+        float t = (cos_t - sp.cos_falloff_end) / (sp.cos_falloff_start - sp.cos_falloff_end);
+        t = std::min(std::max(t, 0.0f), 1.0f);
+        float fall = t * t * (3.0f - 2.0f * t);
+        for (int k = 0; k < 4; ++k)
+            ls.le[k] = spectrum_eval(sp.intensity, wl.lambda[k], s.spectra.data()) * fall / d2;
+        ls.p = pos;
+        ls.n = {0, 0, 0};
+        ls.err = {0, 0, 0};
+        ls.pdf = L.pmf;
+        ls.delta = true;
+        ls.ok = true;
+        return ls;
+    }
+    // environment: uniform sphere
+    float z = 1.0f - 2.0f * u0;
+    float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+    float sn, cs;
+    sincos_turns(u1, sn, cs);
+    ls.wi = {r * cs, r * sn, z};
+    ls.infinite = true;
+    ls.pdf = L.pmf * kInv4Pi;
+    for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(s.d.environment, wl.lambda[k], s.spectra.data());
+    ls.ok = true;
+    return ls;
+}
+
+static float env_pdf(const Scene &s) {
+    if (!s.d.has_environment || s.lights.empty()) return 0.0f;
+    return s.lights.back().pmf * kInv4Pi;
+}
+
+// One complete hero path: src/render.metal:238-312 (begin_sample, primary_ray)
+// followed by the event loop of shaders/packet_pt_event_body.inc.
+static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
+    const Scene &s = *c.s;
+    PathOut out;
+    const uint32_t W = s.d.camera.width, H = s.d.camera.height;
+    const uint32_t px = pixel % W, py = pixel / W;
+    uint32_t seed_bits = uint32_t(c.seed);
+    out.index = sobol_index(s.m, px, py, sample);
+    uint32_t w0 = sobol_word(out.index, 0, RP_RANDOMIZE_NONE, 0);
+    uint32_t w1 = sobol_word(out.index, 1, RP_RANDOMIZE_NONE, 0);
+    uint32_t w2 = sobol_word(out.index, 2, c.mode, seed_bits);
+    out.w0 = w0;
+    out.w1 = w1;
+    Wavelengths wl = sample_wavelengths(sobol_uniform(w2), s.d.hero_wavelengths);
+    out.wl = wl;
+    float ux = sobol_camera_uniform(w0, s.m, px), uy = sobol_camera_uniform(w1, s.m, py);
+    float fx = (float(px) + ux) / float(W), fy = (float(py) + uy) / float(H);
+    const rp_camera &cam = s.d.camera;
+    V3 d = {cam.corner[0] + fx * cam.dx[0] + fy * cam.dy[0], cam.corner[1] + fx * cam.dx[1] + fy * cam.dy[1],
+            cam.corner[2] + fx * cam.dx[2] + fy * cam.dy[2]};
+    float scale = std::max(std::fabs(d.x), std::max(std::fabs(d.y), std::fabs(d.z)));
+    d = normalize(d / scale);
+    V3 o = {cam.origin[0], cam.origin[1], cam.origin[2]};
+    float tmax = INFINITY;
+
+    float beta[4] = {1, 1, 1, 1}, density[4] = {1, 1, 1, 1};
+    float prev_density[4] = {1, 1, 1, 1}, prev_pdf[4] = {0, 0, 0, 0};
+    float eta_scale[4] = {1, 1, 1, 1};
+    bool rr_enabled = true, prev_delta = false;
+    V3 prev_p = o;
+    uint32_t count = 1, crossings = 0, medium = cam.medium, mdraw = 0;
+    const uint32_t capacity = s.d.max_depth + 2;
+    auto add = [&](const float v[4]) {
+        for (int k = 0; k < 4; ++k) out.L[k] = out.L[k] + v[k];
+    };
+
+    for (;;) {
+        if (count >= capacity) break;
+        if (!(beta[0] > 0 || beta[1] > 0 || beta[2] > 0 || beta[3] > 0)) break;
+        Hit h = closest(s, o, d, tmax);
+        ++out.ext_rays;
+        if (count == 1 && crossings == 0) out.primary = h.tri;
+        float seg = h.tri == RP_NONE ? INFINITY : h.t;
+
+        // ---- homogeneous medium along the segment (hero free flight)
+        if (medium != RP_NONE) {
+            MediumCoeff mc = medium_coeff(s, medium, wl);
+            auto mw = medium_draws(c, pixel, sample, mdraw);
+            float uf = philox_u01(mw[0]);
+            float tf = mc.st[0] > 0 ? -log_f(1.0f - uf) / mc.st[0] : INFINITY;
+            if (tf < seg) {
+                // scattering event in the medium at distance tf
+                float num[4], pdf[4];
+                for (int k = 0; k < 4; ++k) {
+                    float tr = exp_f(-mc.st[k] * tf);
+                    num[k] = mc.ss[k] * tr;
+                    pdf[k] = mc.st[k] * tr;
+                }
+                if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                for (int k = 0; k < 4; ++k) prev_pdf[k] = prev_pdf[k] * pdf[k];
+                V3 mp = o + tf * d;
+                ++count;
+                crossings = 0;
+                if (count == capacity) break;
+                if (count - 2 >= c.emergency) { out.status = RP_PATH_EMERGENCY_DEPTH; break; }
+                auto dr = draws(c, out.index, 4 + count - 2);
+                float direct_density[4], direct_beta[4];
+                std::memcpy(direct_density, density, sizeof density);
+                bool direct_allowed = hero_continue(beta, eta_scale, rr_enabled, count - 1, sobol_uniform(dr[3]));
+                std::memcpy(direct_beta, beta, sizeof beta);
+                V3 wo = -d;
+                if (!direct_allowed) break;
+                // phase sampling (hero g is wavelength independent)
+                float ppdf;
+                V3 wi = hg_sample(wo, mc.g, sobol_uniform(dr[1]), sobol_uniform(dr[2]), ppdf);
+                std::memcpy(prev_density, density, sizeof density);
+                float pnum[4] = {ppdf, ppdf, ppdf, ppdf}, ppdfs[4] = {ppdf, ppdf, ppdf, ppdf};
+                if (!advance(density, beta, pnum, ppdfs)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                for (int k = 0; k < 4; ++k) prev_pdf[k] = ppdf;
+                prev_delta = false;
+                prev_p = mp;
+                // NEE from the medium vertex
+                auto lw = draws(c, out.index, 133 + count - 2);
+                LightSample ls = sample_light(s, mp, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]), sobol_uniform(lw[2]));
+                if (ls.ok) {
+                    float ph = hg(dot(wo, ls.wi), mc.g);
+                    float val[4];
+                    bool any = false;
+                    for (int k = 0; k < 4; ++k) {
+                        val[k] = ((direct_beta[k] * ph) * (ls.le[k] / ls.pdf));
+                        any = any || val[k] > 0;
+                    }
+                    if (any) {
+                        float tr[4];
+                        uint32_t cr = 0;
+                        V3 to;
+                        float tm;
+                        V3 sd;
+                        if (ls.infinite) { sd = ls.wi; tm = INFINITY; }
+                        else {
+                            to = ls.delta ? ls.p : offset_origin(ls.p, ls.err, ls.n, mp - ls.p);
+                            sd = to - mp;
+                            tm = 1.0f - 1e-4f;
+                        }
+                        ++out.shadow_rays;
+                        if (transmittance(c, mp, sd, tm, medium, wl, tr, cr)) {
+                            for (int k = 0; k < 4; ++k) val[k] = val[k] * tr[k];
+                            if (!ls.delta) {
+                                float lp[4] = {ls.pdf, ls.pdf, ls.pdf, ls.pdf};
+                                float pp[4] = {ph, ph, ph, ph};
+                                float a = mixture(direct_density, lp), b = mixture(direct_density, pp);
+                                float w = a / (a + b);
+                                for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
+                            }
+                            add(val);
+                        }
+                    }
+                }
+                o = mp;
+                d = wi;
+                tmax = INFINITY;
+                continue;
+            }
+            // passes to the surface (or escapes): transmission probability per lane
+            if (std::isfinite(seg)) {
+                float num[4], pdf[4];
+                for (int k = 0; k < 4; ++k) {
+                    float tr = exp_f(-mc.st[k] * seg);
+                    num[k] = tr;
+                    pdf[k] = tr;
+                }
+                if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                for (int k = 0; k < 4; ++k) prev_pdf[k] = prev_pdf[k] * pdf[k];
+            } else {
+                break; // infinite medium, nothing reached
+            }
+        }
+
+        if (h.tri == RP_NONE) {
+            if (s.d.has_environment) {
+                float val[4];
+                for (int k = 0; k < 4; ++k)
+                    val[k] = beta[k] * spectrum_eval(s.d.environment, wl.lambda[k], s.spectra.data());
+                if (!(count == 1 || prev_delta)) {
+                    float lp = env_pdf(s);
+                    float lpv[4] = {lp, lp, lp, lp};
+                    float a = mixture(prev_density, prev_pdf), b = mixture(prev_density, lpv);
+                    float w = a / (a + b);
+                    for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
+                }
+                add(val);
+            }
+            break;
+        }
+        SurfacePoint sp = surface_point(s, h);
+        const rp_object &obj = s.objects[s.tri_object[h.tri]];
+        const rp_material &mat = s.materials[obj.material];
+        if (mat.kind == RP_MATERIAL_INTERFACE) {
+            if (crossings++ >= c.boundary) { out.status = RP_PATH_BOUNDARY_EXHAUSTION; break; }
+            medium = outgoing_medium(obj, medium, sp.n, d);
+            o = offset_origin(sp.p, sp.err, sp.n, d);
+            tmax = INFINITY;
+            continue;
+        }
+        V3 wo = -d;
+        ++count;
+        crossings = 0;
+        // arrival emission (hero_endpoint)
+        if (obj.emission != RP_NONE && s.tri_light[h.tri] != RP_NONE) {
+            const auto &em = s.emissions[obj.emission];
+            float cos_l = dot(sp.n, wo);
+            if (em.two_sided || cos_l > 0) {
+                float val[4];
+                for (int k = 0; k < 4; ++k) val[k] = beta[k] * spectrum_eval(em.radiance, wl.lambda[k], s.spectra.data());
+                if (!(count == 2 || prev_delta)) {
+                    const Light &L = s.lights[s.tri_light[h.tri]];
+                    V3 dv = sp.p - prev_p;
+                    float d2 = dot(dv, dv);
+                    float lp = ((L.pmf * L.inv_area) * d2) / std::fabs(cos_l);
+                    float lpv[4] = {lp, lp, lp, lp};
+                    float a = mixture(prev_density, prev_pdf), b = mixture(prev_density, lpv);
+                    float w = a / (a + b);
+                    for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
+                }
+                add(val);
+            }
+        }
+        if (count == capacity) break;
+        if (count - 2 >= c.emergency) { out.status = RP_PATH_EMERGENCY_DEPTH; break; }
+        Frame fr = frame_from_z(sp.n);
+        V3 wol = fr.local(wo);
+        Lane lanes[4];
+        for (int k = 0; k < 4; ++k) lanes[k] = prepare_lane(s, mat, wl.lambda[k]);
+        bool connectible = !lanes[0].smooth || lanes[0].kind == RP_MATERIAL_DIFFUSE;
+        auto dr = draws(c, out.index, 4 + count - 2);
+        float direct_density[4], direct_beta[4];
+        std::memcpy(direct_density, density, sizeof density);
+        bool direct_allowed = hero_continue(beta, eta_scale, rr_enabled, count - 1, sobol_uniform(dr[3]));
+        std::memcpy(direct_beta, beta, sizeof beta);
+        bool cont = direct_allowed;
+        if (cont) {
+            float uc = sobol_uniform(dr[0]), u0 = sobol_uniform(dr[1]), u1 = sobol_uniform(dr[2]);
+            Sampled hs = sample_lane(lanes[0], wol, uc, u0, u1);
+            if (!hs.ok) cont = false;
+            else {
+                float num[4], pdf[4], etas[4];
+                if (hs.delta) {
+                    for (int k = 0; k < 4; ++k) {
+                        if (k == 0) {
+                            float m, p;
+                            dirac_lane(lanes[0], wol, hs.wi, uc, u0, u1, m, p);
+                            num[0] = m;
+                            pdf[0] = hs.pdf;
+                        } else dirac_lane(lanes[k], wol, hs.wi, uc, u0, u1, num[k], pdf[k]);
+                        etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
+                    }
+                } else {
+                    for (int k = 0; k < 4; ++k) {
+                        Eval e = eval_lane(lanes[k], wol, hs.wi);
+                        num[k] = e.f * std::fabs(hs.wi.z);
+                        pdf[k] = e.pdf;
+                        etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
+                    }
+                    pdf[0] = hs.pdf;
+                }
+                std::memcpy(prev_density, direct_density, sizeof density);
+                if (!advance(density, beta, num, pdf)) {
+                    out.status = RP_PATH_INVALID_ARITHMETIC;
+                    cont = false;
+                } else {
+                    std::memcpy(prev_pdf, pdf, sizeof pdf);
+                    if (hs.transmission && rr_enabled)
+                        for (int k = 0; k < 4; ++k) eta_scale[k] = eta_scale[k] * (etas[k] * etas[k]);
+                    prev_delta = hs.delta;
+                    prev_p = sp.p;
+                    V3 wi = fr.world(hs.wi);
+                    wi = normalize(wi);
+                    medium = outgoing_medium(obj, medium, sp.n, wi);
+                    o = offset_origin(sp.p, sp.err, sp.n, wi);
+                    d = wi;
+                    tmax = INFINITY;
+                }
+            }
+        }
+        // direct lighting (hero_direct) with the pre-sampling state
+        if (out.status == RP_PATH_SUCCESS && direct_allowed && connectible) {
+            auto lw = draws(c, out.index, 133 + count - 2);
+            LightSample ls = sample_light(s, sp.p, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]), sobol_uniform(lw[2]));
+            if (ls.ok) {
+                V3 wil = fr.local(ls.wi);
+                float cosv = std::fabs(dot(sp.n, ls.wi));
+                float val[4], bp[4];
+                bool any = false;
+                for (int k = 0; k < 4; ++k) {
+                    Eval e = eval_lane(lanes[k], wol, wil);
+                    bp[k] = e.pdf;
+                    val[k] = ((direct_beta[k] * e.f) * (ls.le[k] / ls.pdf)) * cosv;
+                    any = any || val[k] > 0;
+                }
+                if (any) {
+                    V3 from = offset_origin(sp.p, sp.err, sp.n, ls.wi);
+                    V3 sd;
+                    float tm;
+                    if (ls.infinite) { sd = ls.wi; tm = INFINITY; }
+                    else {
+                        V3 to = ls.delta ? ls.p : offset_origin(ls.p, ls.err, ls.n, from - ls.p);
+                        sd = to - from;
+                        tm = 1.0f - 1e-4f;
+                    }
+                    float tr[4];
+                    uint32_t cr = 0;
+                    uint32_t nm = outgoing_medium(obj, medium, sp.n, ls.wi);
+                    ++out.shadow_rays;
+                    if (transmittance(c, from, sd, tm, nm, wl, tr, cr)) {
+                        for (int k = 0; k < 4; ++k) val[k] = val[k] * tr[k];
+                        if (!ls.delta) {
+                            float lp[4] = {ls.pdf, ls.pdf, ls.pdf, ls.pdf};
+                            float a = mixture(direct_density, lp), b = mixture(direct_density, bp);
+                            float w = a / (a + b);
+                            for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
+                        }
+                        add(val);
+                    }
+                }
+            }
+        }
+        if (!cont) break;
+    }
+    out.vertices = count;
+    return out;
+}
+
+// Hero sensor: sum_k cie(lambda_k) * (L_k / pdf_k / 4)  (shaders/frame_sensor.h:32-53).
+static void sensor_xyz(const PathOut &p, float xyz[3]) {
+    xyz[0] = xyz[1] = xyz[2] = 0.0f;
+    const auto &cie = cie_table();
+    for (int k = 0; k < 4; ++k) {
+        if (!(p.wl.pdf[k] > 0)) continue;
+        int i = int(std::floor(p.wl.lambda[k] + 0.5f)) - 360;
+        i = std::min(std::max(i, 0), 470);
+        float w = (p.L[k] / p.wl.pdf[k]) / 4.0f;
+        for (int ch = 0; ch < 3; ++ch) xyz[ch] = xyz[ch] + cie.xyz[i][ch] * w;
+    }
+}
+
+} // namespace orc
+
+// ================================================================ C API
+using namespace orc;
+
+extern "C" {
+
+int or_init(const char *sobol_path, const char *cie_path) {
+    return (load_sobol_tables(sobol_path) && load_cie_table(cie_path)) ? 0 : -1;
+}
+
+uint64_t or_sobol_index(uint32_t m, uint32_t x, uint32_t y, uint64_t sample) { return sobol_index(m, x, y, sample); }
+uint32_t or_sobol_word(uint64_t index, uint32_t dim, uint32_t mode, uint32_t seed) {
+    return sobol_word(index, dim, mode, seed);
+}
+uint32_t or_dimension_hash(uint32_t dim, uint32_t seed) { return sobol_dimension_hash(dim, seed); }
+float or_sobol_uniform(uint32_t w) { return sobol_uniform(w); }
+float or_camera_uniform(uint32_t w, uint32_t m, uint32_t p) { return sobol_camera_uniform(w, m, p); }
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    auto r = philox({ctr[0], ctr[1], ctr[2], ctr[3]}, {key[0], key[1]});
+    for (int i = 0; i < 4; ++i) out[i] = r[i];
+}
+float or_philox_u01(uint32_t w) { return philox_u01(w); }
+void or_sample_wavelengths(float u, uint32_t mode, float lambda[4], float pdf[4]) {
+    Wavelengths w = sample_wavelengths(u, mode);
+    for (int k = 0; k < 4; ++k) {
+        lambda[k] = w.lambda[k];
+        pdf[k] = w.pdf[k];
+    }
+}
+void or_cie_row(uint32_t i, float out[3]) {
+    for (int c = 0; c < 3; ++c) out[c] = cie_table().xyz[i][c];
+}
+float or_spectrum_eval(const rp_spectrum *s, float lambda, const float *pool) { return spectrum_eval(*s, lambda, pool); }
+void or_sincos_turns(float t, float *s, float *c) { sincos_turns(t, *s, *c); }
+float or_exp(float x) { return exp_f(x); }
+float or_log(float x) { return log_f(x); }
+
+// BSDF probes for analytic pins (kind: rp_material_kind; roughness already alpha if remap==0).
+void or_bsdf_eval(const rp_material *m, const float *spectra, float lambda, const float wo[3], const float wi[3],
+                  float *f, float *pdf) {
+    Scene s;
+    s.spectra.assign(spectra ? spectra : (const float *)nullptr, spectra ? spectra + 471 : nullptr);
+    Lane l = prepare_lane(s, *m, lambda);
+    Eval e = eval_lane(l, {wo[0], wo[1], wo[2]}, {wi[0], wi[1], wi[2]});
+    *f = e.f;
+    *pdf = e.pdf;
+}
+int or_bsdf_sample(const rp_material *m, const float *spectra, float lambda, const float wo[3], float uc, float u0,
+                   float u1, float wi[3], float *pdf, int *delta) {
+    Scene s;
+    s.spectra.assign(spectra ? spectra : (const float *)nullptr, spectra ? spectra + 471 : nullptr);
+    Lane l = prepare_lane(s, *m, lambda);
+    Sampled r = sample_lane(l, {wo[0], wo[1], wo[2]}, uc, u0, u1);
+    wi[0] = r.wi.x;
+    wi[1] = r.wi.y;
+    wi[2] = r.wi.z;
+    *pdf = r.pdf;
+    *delta = r.delta;
+    return r.ok ? 1 : 0;
+}
+float or_fresnel_complex(float c, float eta, float k) { return fresnel_complex(c, eta, k); }
+float or_fresnel_dielectric(float c, float eta) { return fresnel_dielectric(c, eta); }
+float or_hg(float cos_t, float g) { return hg(cos_t, g); }
+void or_hg_sample(const float wo[3], float g, float u0, float u1, float wi[3], float *pdf) {
+    V3 r = hg_sample({wo[0], wo[1], wo[2]}, g, u0, u1, *pdf);
+    wi[0] = r.x;
+    wi[1] = r.y;
+    wi[2] = r.z;
+}
+int or_intersect_triangle(const float o[3], const float d[3], float tmax, const float p[9], float out[4]) {
+    float t, b0, b1, b2;
+    bool hit = intersect_triangle({o[0], o[1], o[2]}, {d[0], d[1], d[2]}, tmax, {p[0], p[1], p[2]},
+                                  {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, t, b0, b1, b2);
+    out[0] = t;
+    out[1] = b0;
+    out[2] = b1;
+    out[3] = b2;
+    return hit ? 1 : 0;
+}
+
+void *or_scene_create(const rp_scene_desc *d) {
+    auto *s = new Scene();
+    s->d = *d;
+    s->positions.assign(d->positions, d->positions + 3 * size_t(d->vertex_count));
+    s->indices.assign(d->indices, d->indices + 3 * size_t(d->triangle_count));
+    s->tri_object.assign(d->tri_object, d->tri_object + d->triangle_count);
+    s->objects.assign(d->objects, d->objects + d->object_count);
+    s->materials.assign(d->materials, d->materials + d->material_count);
+    s->emissions.assign(d->emissions, d->emissions + d->emission_count);
+    s->media.assign(d->media, d->media + d->medium_count);
+    s->spots.assign(d->spots, d->spots + d->spot_count);
+    s->spectra.assign(d->spectra, d->spectra + d->spectra_count);
+    s->m = sobol_scale_exponent(d->camera.width, d->camera.height);
+    compute_radius(*s);
+    build_bvh(*s);
+    build_lights(*s);
+    return s;
+}
+void or_scene_destroy(void *s) { delete static_cast<Scene *>(s); }
+
+uint32_t or_light_count(void *sp) { return uint32_t(static_cast<Scene *>(sp)->lights.size()); }
+void or_light_table(void *sp, uint32_t *kind, uint32_t *index, float *pmf, float *cdf, float *inv_area) {
+    auto *s = static_cast<Scene *>(sp);
+    for (size_t i = 0; i < s->lights.size(); ++i) {
+        kind[i] = s->lights[i].kind;
+        index[i] = s->lights[i].index;
+        pmf[i] = s->lights[i].pmf;
+        cdf[i] = s->lights[i].cdf;
+        inv_area[i] = s->lights[i].inv_area;
+    }
+}
+
+// Closest hits for caller rays (rp_ray / rp_hit layouts).
+void or_trace_closest(void *sp, const rp_ray *rays, uint64_t n, rp_hit *hits, uint32_t threads) {
+    auto *s = static_cast<Scene *>(sp);
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint64_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            uint64_t b = next.fetch_add(1024);
+            if (b >= n) break;
+            uint64_t e = std::min(n, b + 1024);
+            for (uint64_t i = b; i < e; ++i) {
+                const rp_ray &r = rays[i];
+                Hit h = closest(*s, {r.origin[0], r.origin[1], r.origin[2]},
+                                {r.direction[0], r.direction[1], r.direction[2]}, r.tmax);
+                hits[i].triangle = h.tri;
+                hits[i].t = h.t;
+                hits[i].b1 = h.b1;
+                hits[i].b2 = h.b2;
+                if (h.tri == RP_NONE) hits[i].t = INFINITY, hits[i].b1 = hits[i].b2 = 0;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+}
+
+// Per-path records for request (lane = pixel_offset + pixel_count * sample_offset).
+int or_sample(void *sp, const rp_request *rq, rp_path_record *out, uint32_t threads) {
+    auto *s = static_cast<Scene *>(sp);
+    Ctx c{s, rq->seed, rq->randomization, rq->emergency_limit ? rq->emergency_limit : 1024u,
+          rq->boundary_limit ? rq->boundary_limit : 4096u};
+    uint64_t n = uint64_t(rq->pixel_count) * rq->sample_count;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint64_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            uint64_t b = next.fetch_add(256);
+            if (b >= n) break;
+            uint64_t e = std::min(n, b + 256);
+            for (uint64_t lane = b; lane < e; ++lane) {
+                uint32_t pixel = rq->pixel_begin + uint32_t(lane % rq->pixel_count);
+                uint64_t sample = rq->sample_begin + lane / rq->pixel_count;
+                PathOut p = trace_path(c, pixel, sample);
+                rp_path_record &r = out[lane];
+                std::memset(&r, 0, sizeof r);
+                r.pixel = pixel;
+                r.status = p.status;
+                r.sample = sample;
+                r.sobol_index = p.index;
+                for (int k = 0; k < 4; ++k) {
+                    r.wavelengths[k] = p.wl.lambda[k];
+                    r.wavelength_pdfs[k] = p.wl.pdf[k];
+                    r.radiance[k] = p.L[k];
+                }
+                if (p.status == RP_PATH_SUCCESS) sensor_xyz(p, r.xyz);
+                r.primary_triangle = p.primary;
+                r.vertices = p.vertices;
+                r.camera_words[0] = p.w0;
+                r.camera_words[1] = p.w1;
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    return 0;
+}
+
+// Full frame: per-pixel mean XYZ of samples [sample_begin, sample_begin+spp),
+// Neumaier-compensated double sums in increasing sample order (src/render.cpp:2326-2407).
+int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                    float *xyz, uint32_t threads, uint64_t *rays) {
+    auto *s = static_cast<Scene *>(sp);
+    Ctx c{s, seed, randomization, 1024u, 4096u};
+    uint32_t npx = s->d.camera.width * s->d.camera.height;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint32_t> next{0};
+    std::atomic<uint64_t> ray_count{0};
+    auto work = [&] {
+        uint64_t local_rays = 0;
+        for (;;) {
+            uint32_t b = next.fetch_add(64);
+            if (b >= npx) break;
+            uint32_t e = std::min(npx, b + 64);
+            for (uint32_t pixel = b; pixel < e; ++pixel) {
+                double sum[3] = {0, 0, 0}, corr[3] = {0, 0, 0};
+                for (uint32_t k = 0; k < spp; ++k) {
+                    PathOut p = trace_path(c, pixel, sample_begin + k);
+                    local_rays += p.ext_rays + p.shadow_rays;
+                    float v[3] = {0, 0, 0};
+                    if (p.status == RP_PATH_SUCCESS) sensor_xyz(p, v);
+                    for (int ch = 0; ch < 3; ++ch) {
+                        double x = double(v[ch]);
+                        double t = sum[ch] + x;
+                        if (std::fabs(sum[ch]) >= std::fabs(x)) corr[ch] += (sum[ch] - t) + x;
+                        else corr[ch] += (x - t) + sum[ch];
+                        sum[ch] = t;
+                    }
+                }
+                for (int ch = 0; ch < 3; ++ch) xyz[3 * size_t(pixel) + ch] = float((sum[ch] + corr[ch]) / double(spp));
+            }
+        }
+        ray_count += local_rays;
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    if (rays) *rays = ray_count.load();
+    return 0;
+}
+
+} // extern "C"

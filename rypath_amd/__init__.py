@@ -1,0 +1,308 @@
+"""ctypes bindings of the gfx950 path tracer's C ABI (include/rp_api.h).
+
+The HIP library rypath_amd/librypath_amd.so is required: importing this
+package raises if it is missing, and every rendering call runs on the GPU
+through the C ABI (there is no CPU fallback).
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "librypath_amd.so"
+
+if not _LIB_PATH.exists():
+    raise ImportError(f"{_LIB_PATH} is missing: build it with `python __graft_entry__.py` (hipcc, gfx950)")
+
+_lib = C.CDLL(str(_LIB_PATH))
+
+NONE = 0xFFFFFFFF
+SUCCESS, NOT_BUILT, NO_DEVICE, FAILURE, INVALID = range(5)
+SPECTRUM_CONSTANT, SPECTRUM_SIGMOID, SPECTRUM_TABLE, SPECTRUM_CAUCHY, SPECTRUM_LINEAR = range(5)
+MATERIAL_DIFFUSE, MATERIAL_CONDUCTOR, MATERIAL_DIELECTRIC, MATERIAL_INTERFACE = range(4)
+RANDOMIZE_NONE, RANDOMIZE_PERMUTE, RANDOMIZE_FASTOWEN, RANDOMIZE_OWEN = range(4)
+MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_WAVEFRONT_SORTED = range(3)
+SYNTH_CORNELL, SYNTH_TERRAIN, SYNTH_BATCH, SYNTH_VOLUME, SYNTH_INVERSE = range(5)
+PARAM_REFLECTANCE_BIN, PARAM_ROUGHNESS, PARAM_SIGMA_S_BIN = range(3)
+u32, u64, f32, f64 = C.c_uint32, C.c_uint64, C.c_float, C.c_double
+
+
+class Spectrum(C.Structure):
+    _fields_ = [("kind", u32), ("c", f32 * 3), ("scale", f32), ("table", u32)]
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", u32), ("remap_roughness", u32), ("roughness_u", f32), ("roughness_v", f32),
+                ("reflectance", Spectrum), ("eta", Spectrum), ("k", Spectrum)]
+
+
+class Object(C.Structure):
+    _fields_ = [("material", u32), ("emission", u32), ("medium_inside", u32), ("medium_outside", u32)]
+
+
+class Emission(C.Structure):
+    _fields_ = [("radiance", Spectrum), ("two_sided", u32), ("pad", u32 * 3)]
+
+
+class SpotLight(C.Structure):
+    _fields_ = [("position", f32 * 3), ("direction", f32 * 3), ("cos_falloff_start", f32),
+                ("cos_falloff_end", f32), ("intensity", Spectrum)]
+
+
+class Medium(C.Structure):
+    _fields_ = [("sigma_a", Spectrum), ("sigma_s", Spectrum), ("g", f32), ("pad", u32 * 3)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("origin", f32 * 3), ("corner", f32 * 3), ("dx", f32 * 3), ("dy", f32 * 3),
+                ("width", u32), ("height", u32), ("medium", u32), ("pad", u32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("api_version", u32), ("vertex_count", u32), ("triangle_count", u32), ("object_count", u32),
+                ("positions", C.POINTER(f32)), ("indices", C.POINTER(u32)), ("tri_object", C.POINTER(u32)),
+                ("objects", C.POINTER(Object)),
+                ("material_count", u32), ("emission_count", u32), ("medium_count", u32), ("spot_count", u32),
+                ("materials", C.POINTER(Material)), ("emissions", C.POINTER(Emission)),
+                ("media", C.POINTER(Medium)), ("spots", C.POINTER(SpotLight)), ("spectra", C.POINTER(f32)),
+                ("spectra_count", u32), ("has_environment", u32), ("environment", Spectrum),
+                ("camera", Camera), ("max_depth", u32), ("hero_wavelengths", u32)]
+
+
+class BuildOptions(C.Structure):
+    _fields_ = [("leaf_size", u32), ("sah_bins", u32), ("threads", u32), ("device", u32)]
+
+
+class BuildInfo(C.Structure):
+    _fields_ = [("build_seconds", f64), ("upload_seconds", f64), ("node_count", u64), ("leaf_count", u64),
+                ("device_bytes", u64), ("max_depth", u32), ("max_stack", u32), ("emitter_count", u32),
+                ("light_count", u32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("seed", u64), ("randomization", u32), ("pixel_begin", u32), ("pixel_count", u32),
+                ("sample_count", u32), ("sample_begin", u64), ("emergency_limit", u32), ("boundary_limit", u32)]
+
+
+class FrameConfig(C.Structure):
+    _fields_ = [("mode", u32), ("spp", u32), ("sample_begin", u64), ("seed", u64), ("randomization", u32),
+                ("capacity", u32), ("group_size", u32), ("sort_bits", u32), ("flags", u32), ("pad", u32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("seconds", f64), ("paths", u64), ("extension_rays", u64), ("shadow_rays", u64),
+                ("failed_paths", u64), ("trace_seconds", f64), ("extend_launches", u64), ("node_fetches", u64),
+                ("triangle_tests", u64)]
+
+
+class Ray(C.Structure):
+    _fields_ = [("origin", f32 * 3), ("tmax", f32), ("direction", f32 * 3), ("pad", u32)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("triangle", u32), ("t", f32), ("b1", f32), ("b2", f32)]
+
+
+class SyntheticOptions(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("max_depth", u32), ("grid", u32), ("spheres", u32),
+                ("sphere_subdivision", u32), ("target_triangles", u32), ("volume_subdivision", u32),
+                ("hg_g", f32), ("bins", u32), ("pad", u32)]
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("spp", u32), ("pad", u32), ("seed", u64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("capacity", u32), ("group_size", u32), ("sort_bits", u32), ("use_graph", u32),
+                ("randomization", u32), ("pad", u32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("seconds", f64), ("paths", u64), ("extension_rays", u64), ("shadow_rays", u64),
+                ("failed_paths", u64), ("graph_launches", u32), ("pad", u32)]
+
+
+class Param(C.Structure):
+    _fields_ = [("kind", u32), ("target", u32), ("bin", u32), ("bins", u32)]
+
+
+RECORD_DTYPE = np.dtype([
+    ("pixel", "<u4"), ("status", "<u4"), ("sample", "<u8"), ("sobol_index", "<u8"),
+    ("wavelengths", "<f4", 4), ("wavelength_pdfs", "<f4", 4), ("radiance", "<f4", 4), ("xyz", "<f4", 3),
+    ("primary_triangle", "<u4"), ("vertices", "<u4"), ("camera_words", "<u4", 2), ("pad", "<u4"),
+])
+assert RECORD_DTYPE.itemsize == 104
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("direction", "<f4", 3), ("pad", "<u4")])
+HIT_DTYPE = np.dtype([("triangle", "<u4"), ("t", "<f4"), ("b1", "<f4"), ("b2", "<f4")])
+
+
+def _sig(name, res, args):
+    fn = getattr(_lib, name)
+    fn.restype = res
+    fn.argtypes = args
+    return fn
+
+
+_sig("rp_api_version", u32, [])
+_sig("rp_device_count", C.c_int, [])
+_sig("rp_last_error", C.c_size_t, [C.c_char_p, C.c_size_t])
+_sig("rp_scene_create", C.c_int, [C.POINTER(SceneDesc), C.POINTER(BuildOptions), C.POINTER(C.c_void_p)])
+_sig("rp_scene_info", C.c_int, [C.c_void_p, C.POINTER(BuildInfo)])
+_sig("rp_scene_destroy", None, [C.c_void_p])
+_sig("rp_sample", C.c_int, [C.c_void_p, C.POINTER(Request), C.c_void_p, u64])
+_sig("rp_render_frame", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
+_sig("rp_render_frame_device", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
+_sig("rp_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
+_sig("rp_trace_any", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
+_sig("rp_batch_create", C.c_int, [C.POINTER(BatchItem), u32, C.POINTER(C.c_void_p)])
+_sig("rp_batch_render", C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(C.c_void_p), C.POINTER(BatchStats)])
+_sig("rp_batch_destroy", None, [C.c_void_p])
+_sig("rp_autotune", C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(f64)])
+_sig("rp_set_params", C.c_int, [C.c_void_p, C.POINTER(Param), u32, C.POINTER(f32)])
+_sig("rp_backward", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.POINTER(Param), u32, C.POINTER(f32),
+                              C.POINTER(f32), C.POINTER(FrameStats)])
+_sig("rp_synthetic_create", C.c_int, [u32, u64, C.POINTER(SyntheticOptions), C.POINTER(C.c_void_p)])
+_sig("rp_synthetic_desc", C.POINTER(SceneDesc), [C.c_void_p])
+_sig("rp_synthetic_destroy", None, [C.c_void_p])
+
+EXPORTED = [
+    "rp_api_version", "rp_device_count", "rp_last_error", "rp_scene_create", "rp_scene_info", "rp_scene_destroy",
+    "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_trace_closest", "rp_trace_any",
+    "rp_batch_create", "rp_batch_render", "rp_batch_destroy", "rp_autotune", "rp_set_params", "rp_backward",
+    "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
+]
+
+
+class RypathError(RuntimeError):
+    def __init__(self, where, code):
+        buf = C.create_string_buffer(4096)
+        _lib.rp_last_error(buf, len(buf))
+        super().__init__(f"{where} failed with status {code}: {buf.value.decode(errors='replace')}")
+        self.code = code
+
+
+def _check(where, code):
+    if code != SUCCESS:
+        raise RypathError(where, code)
+
+
+def library():
+    return _lib
+
+
+def device_count():
+    return _lib.rp_device_count()
+
+
+class Synthetic:
+    """Owned synthetic scene description (include/rp_synthetic.h)."""
+
+    def __init__(self, kind, seed=1, **options):
+        o = SyntheticOptions()
+        for k, v in options.items():
+            setattr(o, k, v)
+        h = C.c_void_p()
+        _check("rp_synthetic_create", _lib.rp_synthetic_create(kind, seed, C.byref(o), C.byref(h)))
+        self._h = h
+        self.desc = _lib.rp_synthetic_desc(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rp_synthetic_destroy(self._h)
+            self._h = None
+
+    @property
+    def d(self):
+        return self.desc.contents
+
+    def triangles(self):
+        d = self.d
+        pos = np.ctypeslib.as_array(d.positions, shape=(d.vertex_count, 3)).copy()
+        idx = np.ctypeslib.as_array(d.indices, shape=(d.triangle_count, 3)).copy()
+        return pos, idx
+
+
+def request(seed=1, pixel_begin=0, pixel_count=1, sample_begin=0, sample_count=1,
+            randomization=RANDOMIZE_FASTOWEN, emergency_limit=0, boundary_limit=0):
+    r = Request()
+    r.seed, r.randomization, r.pixel_begin, r.pixel_count = seed, randomization, pixel_begin, pixel_count
+    r.sample_begin, r.sample_count = sample_begin, sample_count
+    r.emergency_limit, r.boundary_limit = emergency_limit, boundary_limit
+    return r
+
+
+FRAME_COUNT_TRAFFIC = 1
+
+
+def frame_config(spp, mode=MODE_WAVEFRONT_SORTED, sample_begin=0, seed=1, randomization=RANDOMIZE_FASTOWEN,
+                 capacity=0, group_size=0, sort_bits=0, flags=0):
+    c = FrameConfig()
+    c.flags = flags
+    c.mode, c.spp, c.sample_begin, c.seed, c.randomization = mode, spp, sample_begin, seed, randomization
+    c.capacity, c.group_size, c.sort_bits = capacity, group_size, sort_bits
+    return c
+
+
+class Scene:
+    """Device scene: BVH built on the host, uploaded to HBM (rp_scene_create)."""
+
+    def __init__(self, desc, leaf_size=0, sah_bins=0, device=0):
+        o = BuildOptions()
+        o.leaf_size, o.sah_bins, o.device = leaf_size, sah_bins, device
+        h = C.c_void_p()
+        ptr = desc if isinstance(desc, C._Pointer) else C.pointer(desc)
+        _check("rp_scene_create", _lib.rp_scene_create(ptr, C.byref(o), C.byref(h)))
+        self._h = h
+        cam = ptr.contents.camera
+        self.width, self.height = cam.width, cam.height
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rp_scene_destroy(self._h)
+            self._h = None
+
+    def info(self):
+        i = BuildInfo()
+        _check("rp_scene_info", _lib.rp_scene_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in BuildInfo._fields_}
+
+    def sample(self, req):
+        n = req.pixel_count * req.sample_count
+        out = np.zeros(n, dtype=RECORD_DTYPE)
+        _check("rp_sample", _lib.rp_sample(self._h, C.byref(req), out.ctypes.data_as(C.c_void_p), n))
+        return out
+
+    def render_frame(self, cfg):
+        out = np.zeros((self.height, self.width, 3), dtype=np.float32)
+        st = FrameStats()
+        _check("rp_render_frame", _lib.rp_render_frame(self._h, C.byref(cfg), out.ctypes.data_as(C.c_void_p),
+                                                        C.byref(st)))
+        return out, {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
+    def render_frame_device(self, cfg, device_ptr):
+        st = FrameStats()
+        _check("rp_render_frame_device", _lib.rp_render_frame_device(self._h, C.byref(cfg), C.c_void_p(device_ptr),
+                                                                      C.byref(st)))
+        return {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        sec = f64()
+        _check("rp_trace_closest", _lib.rp_trace_closest(self._h, rays.ctypes.data_as(C.c_void_p), len(rays),
+                                                          hits.ctypes.data_as(C.c_void_p), C.byref(sec)))
+        return hits, sec.value
+
+    def trace_any(self, rays):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.zeros(len(rays), dtype=np.uint8)
+        sec = f64()
+        _check("rp_trace_any", _lib.rp_trace_any(self._h, rays.ctypes.data_as(C.c_void_p), len(rays),
+                                                  occ.ctypes.data_as(C.c_void_p), C.byref(sec)))
+        return occ, sec.value

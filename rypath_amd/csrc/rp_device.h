@@ -1,0 +1,1099 @@
+// Device-side transport for gfx950: math, sampling, BVH4 traversal, BSDFs,
+// lights, media and the hero-wavelength path event.  Everything here is
+// binary32 with no FMA contraction (the library is built with
+// -ffp-contract=off) and uses only the approximations defined in this file
+// for transcendentals, so per-path results are reproducible bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rp_api.h"
+
+#define RP_DEV __device__ __forceinline__
+#define RP_NOINLINE __device__ __attribute__((noinline))
+
+namespace rpd {
+
+// ============================================================== vectors
+struct F3 {
+    float x, y, z;
+};
+RP_DEV F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+RP_DEV F3 operator+(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+RP_DEV F3 operator-(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+RP_DEV F3 operator-(F3 a) { return {-a.x, -a.y, -a.z}; }
+RP_DEV F3 operator*(float s, F3 a) { return {s * a.x, s * a.y, s * a.z}; }
+RP_DEV F3 operator*(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+RP_DEV F3 operator/(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+RP_DEV float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RP_DEV F3 vabs(F3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+RP_DEV float comp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// std::min / std::max semantics (returns the first argument on ties / NaN).
+RP_DEV float omin(float a, float b) { return (b < a) ? b : a; }
+RP_DEV float omax(float a, float b) { return (a < b) ? b : a; }
+RP_DEV float dop(float a, float b, float c, float d) {
+    float cd = c * d;
+    float err = __builtin_fmaf(-c, d, cd);
+    float r = __builtin_fmaf(a, b, -cd);
+    return r + err;
+}
+RP_DEV F3 cross(F3 a, F3 b) { return {dop(a.y, b.z, a.z, b.y), dop(a.z, b.x, a.x, b.z), dop(a.x, b.y, a.y, b.x)}; }
+RP_DEV float length(F3 a) { return __builtin_sqrtf(dot(a, a)); }
+RP_DEV F3 normalize(F3 a) { return a / length(a); }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kInv4Pi = 0.07957747154594766788f;
+RP_DEV float gamma_n(int n) { return (float(n) * 0x1p-24f) / (1.0f - float(n) * 0x1p-24f); }
+
+// ======================================================= transcendentals
+// sin/cos of 2*pi*t: exact quadrant reduction in turns, Taylor on |a|<=pi/4.
+RP_DEV void sincos_turns(float t, float &s, float &c) {
+    float x = t * 4.0f;
+    float q = floorf(x + 0.5f);
+    float r = (x - q) * 0.25f;
+    float a = r * 6.28318530717958647692f;
+    float a2 = a * a;
+    float sp = a + a * a2 * (-1.0f / 6.0f + a2 * (1.0f / 120.0f + a2 * (-1.0f / 5040.0f + a2 * (1.0f / 362880.0f))));
+    float cp = 1.0f + a2 * (-0.5f + a2 * (1.0f / 24.0f + a2 * (-1.0f / 720.0f + a2 * (1.0f / 40320.0f))));
+    int qi = int(q) & 3;
+    if (qi == 0) { s = sp; c = cp; }
+    else if (qi == 1) { s = cp; c = -sp; }
+    else if (qi == 2) { s = -sp; c = -cp; }
+    else { s = -cp; c = sp; }
+}
+// exp: Cody-Waite two-part ln2 reduction, degree-7 Taylor.
+RP_DEV float exp_f(float x) {
+    if (x != x) return x;
+    if (!(x > -104.0f)) return 0.0f;
+    if (x > 88.72f) return __builtin_inff();
+    float k = floorf(x * 1.44269504088896341f + 0.5f);
+    float r = (x - k * 0.693145751953125f) - k * 1.42860682030941723212e-6f;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (1.0f / 6.0f + r * (1.0f / 24.0f + r * (1.0f / 120.0f + r * (1.0f / 720.0f + r * (1.0f / 5040.0f)))))));
+    return ldexpf(p, int(k));
+}
+// log: frexp to [sqrt(1/2), sqrt(2)), atanh series in (m-1)/(m+1).
+RP_DEV float log_f(float x) {
+    if (x != x || x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_inff();
+    if (x == __builtin_inff()) return x;
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0.70710678118654752f) { m = m * 2.0f; e -= 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float lm = s * (2.0f + s2 * (2.0f / 3.0f + s2 * (2.0f / 5.0f + s2 * (2.0f / 7.0f + s2 * (2.0f / 9.0f)))));
+    float ef = float(e);
+    return ef * 0.693145751953125f + (lm + ef * 1.42860682030941723212e-6f);
+}
+RP_DEV float next_up(float v) {
+    if (__builtin_isinf(v) && v > 0) return v;
+    if (v == -0.0f) v = 0.0f;
+    uint32_t u = __float_as_uint(v);
+    if (v >= 0) ++u; else --u;
+    return __uint_as_float(u);
+}
+RP_DEV float next_down(float v) {
+    if (__builtin_isinf(v) && v < 0) return v;
+    if (v == 0.0f) v = -0.0f;
+    uint32_t u = __float_as_uint(v);
+    if (v > 0) --u; else ++u;
+    return __uint_as_float(u);
+}
+
+// ======================================================== scene on device
+struct LightEntry {
+    uint32_t kind, index; // index: leaf-order triangle position / spot index
+    float pmf, cdf, inv_area;
+    uint32_t pad[3];
+};
+
+struct Scene {
+    const float4 *tris;     // 3 per triangle in BVH leaf order: (v0, scene id) (v1, object) (v2, light)
+    const uint4 *nodes;     // 4 per 64-byte node
+    uint32_t node_count, tri_count;
+    const rp_object *objects;
+    const rp_material *materials;
+    const rp_emission *emissions;
+    const rp_medium *media;
+    const rp_spot_light *spots;
+    const float *spectra;
+    const LightEntry *lights;
+    uint32_t light_count;
+    uint32_t has_environment;
+    const float4 *cie;          // 471 rows (x, y, z, 0)
+    const uint32_t *sobol_dir;  // transposed directions: [52][1024]
+    const uint64_t *sobol_intervals, *sobol_inverse;
+    rp_camera camera;
+    rp_spectrum environment;
+    uint32_t max_depth, hero_mode, m;
+    uint32_t has_interfaces, has_media;
+    float env_pdf;
+};
+
+// =============================================================== sampling
+RP_DEV uint64_t sobol_index(const Scene &S, uint32_t x, uint32_t y, uint64_t sample) {
+    uint32_t m = S.m;
+    if (m == 0) return sample;
+    uint32_t row = (m - 1) * 52;
+    uint64_t delta = 0;
+    uint64_t result = sample << (2 * m);
+    for (uint64_t s = sample; s; s &= s - 1) delta ^= S.sobol_intervals[row + __builtin_ctzll(s)];
+    uint64_t bits = ((uint64_t(x) << m) | y) ^ delta;
+    for (; bits; bits &= bits - 1) result ^= S.sobol_inverse[row + __builtin_ctzll(bits)];
+    return result;
+}
+RP_DEV uint32_t dimension_hash(uint32_t dimension, uint32_t seed_bits) {
+    const uint64_t mm = 0xc6a4a7935bd1e995ull;
+    uint64_t k = ((uint64_t(seed_bits) << 32) | dimension) * mm;
+    k ^= k >> 47;
+    k *= mm;
+    uint64_t h = (8 * mm) ^ k;
+    h *= mm;
+    h ^= h >> 47;
+    h *= mm;
+    h ^= h >> 47;
+    return uint32_t(h);
+}
+RP_DEV uint64_t mix64(uint64_t v) {
+    v ^= v >> 31;
+    v *= 0x7fb5d329728ea185ull;
+    v ^= v >> 27;
+    v *= 0x81dadef4bc2dd44dull;
+    return v ^ (v >> 33);
+}
+RP_DEV uint32_t scramble(uint32_t v, uint32_t dimension, uint32_t mode, uint32_t seed_bits) {
+    if (mode == RP_RANDOMIZE_NONE) return v;
+    uint32_t hash = dimension_hash(dimension, seed_bits);
+    if (mode == RP_RANDOMIZE_PERMUTE) return v ^ hash;
+    if (mode == RP_RANDOMIZE_FASTOWEN) {
+        v = __builtin_bitreverse32(v);
+        v ^= v * 0x3d20adeau;
+        v += hash;
+        v *= (hash >> 16) | 1u;
+        v ^= v * 0x05526c56u;
+        v ^= v * 0x53a22864u;
+        return __builtin_bitreverse32(v);
+    }
+    v ^= (hash & 1u) << 31;
+    for (uint32_t b = 1; b < 32; ++b) {
+        uint32_t prefix = v & (0xffffffffu << (32 - b));
+        if (uint32_t(mix64(uint64_t(prefix ^ hash))) & (1u << b)) v ^= 1u << (31 - b);
+    }
+    return v;
+}
+RP_DEV uint32_t sobol_word(const Scene &S, uint64_t index, uint32_t dim, uint32_t mode, uint32_t seed_bits) {
+    uint32_t v = 0;
+    for (uint64_t i = index; i; i &= i - 1) v ^= S.sobol_dir[uint32_t(__builtin_ctzll(i)) * 1024u + dim];
+    return scramble(v, dim, mode, seed_bits);
+}
+// Four consecutive dimensions 4*block..4*block+3 with one 16-byte load per index bit.
+RP_DEV uint4 sobol_block(const Scene &S, uint64_t index, uint32_t block, uint32_t mode, uint32_t seed_bits) {
+    uint32_t d = 4 * block;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    const uint4 *dir4 = reinterpret_cast<const uint4 *>(S.sobol_dir);
+    for (uint64_t i = index; i; i &= i - 1) {
+        uint4 w = dir4[(uint32_t(__builtin_ctzll(i)) * 1024u + d) >> 2];
+        v.x ^= w.x;
+        v.y ^= w.y;
+        v.z ^= w.z;
+        v.w ^= w.w;
+    }
+    return make_uint4(scramble(v.x, d, mode, seed_bits), scramble(v.y, d + 1, mode, seed_bits),
+                      scramble(v.z, d + 2, mode, seed_bits), scramble(v.w, d + 3, mode, seed_bits));
+}
+RP_DEV float sobol_uniform(uint32_t word) {
+    float u = float(word) * 0x1p-32f;
+    return u < 0x1.fffffep-1f ? u : 0x1.fffffep-1f;
+}
+RP_DEV float camera_uniform(uint32_t word, uint32_t m, uint32_t pixel) {
+    float local = sobol_uniform(word) * float(uint32_t(1) << m) - float(pixel);
+    if (local < 0.0f) return 0.0f;
+    return local < 0x1.fffffep-1f ? local : 0x1.fffffep-1f;
+}
+RP_DEV uint4 philox(uint4 c, uint2 k) {
+    const uint32_t m0 = 0xd2511f53u, m1 = 0xcd9e8d57u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(m0, c.x), lo0 = m0 * c.x;
+        uint32_t hi1 = __umulhi(m1, c.z), lo1 = m1 * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9e3779b9u;
+        k.y += 0xbb67ae85u;
+    }
+    return c;
+}
+RP_DEV float philox_u01(uint32_t w) { return float(w >> 8) * 0x1p-24f; }
+
+struct Wavelengths {
+    float lambda[4], pdf[4];
+};
+RP_DEV Wavelengths sample_wavelengths(float u, uint32_t mode) {
+    Wavelengths w;
+    if (!(u >= 0 && u < 1)) {
+        for (int k = 0; k < 4; ++k) w.lambda[k] = w.pdf[k] = __builtin_nanf("");
+        return w;
+    }
+    if (mode == 0) {
+        for (int k = 0; k < 4; ++k) {
+            float s = u + float(k) * 0.25f;
+            if (s >= 1.0f) s -= 1.0f;
+            float l = 360.0f + 470.0f * s;
+            if (l >= 830.0f) l = __uint_as_float(__float_as_uint(830.0f) - 1);
+            w.lambda[k] = l;
+            w.pdf[k] = 1.0f / 470.0f;
+        }
+        return w;
+    }
+    for (int k = 0; k < 4; ++k) {
+        uint32_t coord = uint32_t(u * 16777216.0f);
+        float s = float((coord + uint32_t(k) * 4194304u) & 0x00ffffffu) * 0x1p-24f;
+        float a = 0.85691062f - 1.82750197f * s;
+        float at = 0.5f * log_f((1.0f + a) / (1.0f - a));
+        float l = 538.0f - 138.888889f * at;
+        l = omin(omax(l, 360.0f), 830.0f);
+        float e = exp_f(0.0072f * (l - 538.0f));
+        float ch = 0.5f * (e + 1.0f / e);
+        w.lambda[k] = l;
+        w.pdf[k] = 0.0039398042f / (ch * ch);
+    }
+    return w;
+}
+
+RP_DEV float spectrum_eval(const rp_spectrum &s, float l, const float *pool) {
+    float v;
+    switch (s.kind) {
+    case RP_SPECTRUM_CONSTANT: v = s.c[0]; break;
+    case RP_SPECTRUM_SIGMOID: {
+        float x = __builtin_fmaf(__builtin_fmaf(s.c[0], l, s.c[1]), l, s.c[2]);
+        if (__builtin_isinf(x)) v = x > 0 ? 1.0f : 0.0f;
+        else v = 0.5f + x / (2.0f * __builtin_sqrtf(1.0f + x * x));
+        break;
+    }
+    case RP_SPECTRUM_TABLE: {
+        int i = int(floorf(l + 0.5f)) - 360;
+        i = i < 0 ? 0 : (i > 470 ? 470 : i);
+        v = pool[s.table + uint32_t(i)];
+        break;
+    }
+    case RP_SPECTRUM_CAUCHY: {
+        float um = l * 0.001f;
+        v = s.c[0] + s.c[1] / (um * um);
+        break;
+    }
+    case RP_SPECTRUM_LINEAR: {
+        float t = (l - 360.0f) / 470.0f;
+        v = (1.0f - t) * s.c[0] + t * s.c[1];
+        break;
+    }
+    default: v = __builtin_nanf("");
+    }
+    return v * s.scale;
+}
+
+// ================================================================ geometry
+struct Hit {
+    uint32_t prim;   // leaf-order position, 0xffffffff = miss
+    uint32_t id;     // scene triangle id
+    float t, b0, b1, b2;
+};
+
+RP_DEV bool intersect_triangle(F3 o, F3 d, float tmax, F3 p0, F3 p1, F3 p2, float &t_out, float &b0o, float &b1o,
+                               float &b2o) {
+    F3 p0t = p0 - o, p1t = p1 - o, p2t = p2 - o;
+    F3 ad = vabs(d);
+    int kz = ad.x > ad.y ? (ad.x > ad.z ? 0 : 2) : (ad.y > ad.z ? 1 : 2);
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    F3 dp = {comp(d, kx), comp(d, ky), comp(d, kz)};
+    p0t = {comp(p0t, kx), comp(p0t, ky), comp(p0t, kz)};
+    p1t = {comp(p1t, kx), comp(p1t, ky), comp(p1t, kz)};
+    p2t = {comp(p2t, kx), comp(p2t, ky), comp(p2t, kz)};
+    float sx = -dp.x / dp.z, sy = -dp.y / dp.z, sz = 1.0f / dp.z;
+    p0t.x = p0t.x + sx * p0t.z;
+    p0t.y = p0t.y + sy * p0t.z;
+    p1t.x = p1t.x + sx * p1t.z;
+    p1t.y = p1t.y + sy * p1t.z;
+    p2t.x = p2t.x + sx * p2t.z;
+    p2t.y = p2t.y + sy * p2t.z;
+    float e0 = dop(p1t.x, p2t.y, p1t.y, p2t.x);
+    float e1 = dop(p2t.x, p0t.y, p2t.y, p0t.x);
+    float e2 = dop(p0t.x, p1t.y, p0t.y, p1t.x);
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f) {
+        e0 = float(double(p2t.y) * double(p1t.x) - double(p2t.x) * double(p1t.y));
+        e1 = float(double(p0t.y) * double(p2t.x) - double(p0t.x) * double(p2t.y));
+        e2 = float(double(p1t.y) * double(p0t.x) - double(p1t.x) * double(p0t.y));
+    }
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return false;
+    float det = e0 + e1 + e2;
+    if (det == 0.0f) return false;
+    p0t.z = p0t.z * sz;
+    p1t.z = p1t.z * sz;
+    p2t.z = p2t.z * sz;
+    float ts = e0 * p0t.z + e1 * p1t.z + e2 * p2t.z;
+    if (det < 0 && (ts >= 0 || ts < tmax * det)) return false;
+    if (det > 0 && (ts <= 0 || ts > tmax * det)) return false;
+    float inv = 1.0f / det;
+    float b0 = e0 * inv, b1 = e1 * inv, b2 = e2 * inv;
+    float t = ts * inv;
+    float max_zt = omax(omax(fabsf(p0t.z), fabsf(p1t.z)), fabsf(p2t.z));
+    float delta_z = gamma_n(3) * max_zt;
+    float max_xt = omax(omax(fabsf(p0t.x), fabsf(p1t.x)), fabsf(p2t.x));
+    float max_yt = omax(omax(fabsf(p0t.y), fabsf(p1t.y)), fabsf(p2t.y));
+    float delta_x = gamma_n(5) * (max_xt + max_zt);
+    float delta_y = gamma_n(5) * (max_yt + max_zt);
+    float delta_e = 2.0f * (gamma_n(2) * max_xt * max_yt + delta_y * max_xt + delta_x * max_yt);
+    float max_e = omax(omax(fabsf(e0), fabsf(e1)), fabsf(e2));
+    float delta_t = 3.0f * (gamma_n(3) * max_e * max_zt + delta_e * max_zt + delta_z * max_e) * fabsf(inv);
+    if (t <= delta_t) return false;
+    t_out = t;
+    b0o = b0;
+    b1o = b1;
+    b2o = b2;
+    return true;
+}
+
+// ------------------------------------------------------ BVH4 traversal
+// Node layout (64 B = 4 x uint4):
+//   q0: origin.xyz, exponent bytes (x, y, z) | child_count << 24
+//   q1: qlo.x[4], qlo.y[4], qlo.z[4], qhi.x[4]  (one byte per child)
+//   q2: qhi.y[4], qhi.z[4], child[0], child[1]
+//   q3: child[2], child[3], count[4], pad
+// Leaf child word: 0x80000000 | count << 27 | first (leaf-order triangle).
+constexpr uint32_t kLeaf = 0x80000000u;
+constexpr uint32_t kEmpty = 0xffffffffu;
+constexpr int kShortStack = 16;  // per-lane LDS entries
+constexpr int kSpillStack = 48;  // per-lane overflow (scratch)
+constexpr int kMaxStack = kShortStack + kSpillStack;
+
+struct Stack {
+    uint32_t *lds_node; // [kShortStack][block]
+    float *lds_t;
+    int stride;         // block size
+    int sp;
+    uint32_t spill_node[kSpillStack];
+    float spill_t[kSpillStack];
+    RP_DEV void push(uint32_t n, float t) {
+        if (sp < kShortStack) {
+            lds_node[sp * stride] = n;
+            lds_t[sp * stride] = t;
+        } else {
+            spill_node[sp - kShortStack] = n;
+            spill_t[sp - kShortStack] = t;
+        }
+        ++sp;
+    }
+    RP_DEV void pop(uint32_t &n, float &t) {
+        --sp;
+        if (sp < kShortStack) {
+            n = lds_node[sp * stride];
+            t = lds_t[sp * stride];
+        } else {
+            n = spill_node[sp - kShortStack];
+            t = spill_t[sp - kShortStack];
+        }
+    }
+};
+
+RP_DEV float byte_of(uint32_t w, int i) { return float((w >> (8 * i)) & 0xffu); }
+
+// Tests the 4 children of a node; returns hit mask and entry distances.
+RP_DEV uint32_t test_children(const uint4 *nodes, uint32_t node, F3 o, F3 inv, float tmax, float tn[4], uint32_t child[4]) {
+    const uint4 q0 = nodes[4 * node + 0];
+    const uint4 q1 = nodes[4 * node + 1];
+    const uint4 q2 = nodes[4 * node + 2];
+    const uint4 q3 = nodes[4 * node + 3];
+    float ox = __uint_as_float(q0.x), oy = __uint_as_float(q0.y), oz = __uint_as_float(q0.z);
+    float sx = __uint_as_float((q0.w & 0xffu) << 23);
+    float sy = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
+    float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
+    child[0] = q2.z;
+    child[1] = q2.w;
+    child[2] = q3.x;
+    child[3] = q3.y;
+    const float grow = 1.0f + 2.0f * gamma_n(3);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float lx = ox + byte_of(q1.x, i) * sx, hx = ox + byte_of(q1.w, i) * sx;
+        float ly = oy + byte_of(q1.y, i) * sy, hy = oy + byte_of(q2.x, i) * sy;
+        float lz = oz + byte_of(q1.z, i) * sz, hz = oz + byte_of(q2.y, i) * sz;
+        float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+        float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+        float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+        float nx = fminf(t0x, t1x), fx = fmaxf(t0x, t1x);
+        float ny = fminf(t0y, t1y), fy = fmaxf(t0y, t1y);
+        float nz = fminf(t0z, t1z), fz = fmaxf(t0z, t1z);
+        float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+        float tfar = fminf(fminf(fx, fy), fz) * grow;
+        tfar = fminf(tfar, tmax);
+        tn[i] = tnear;
+        bool hit = child[i] != kEmpty && tnear <= tfar;
+        mask |= hit ? (1u << i) : 0u;
+    }
+    return mask;
+}
+
+RP_DEV void load_tri(const Scene &S, uint32_t p, F3 &a, F3 &b, F3 &c, uint32_t &id) {
+    float4 t0 = S.tris[3 * p], t1 = S.tris[3 * p + 1], t2 = S.tris[3 * p + 2];
+    a = {t0.x, t0.y, t0.z};
+    b = {t1.x, t1.y, t1.z};
+    c = {t2.x, t2.y, t2.z};
+    id = __float_as_uint(t0.w);
+}
+
+struct TraversalCounts {
+    uint32_t nodes, tris;
+};
+
+// Closest hit (smallest t, ties to the smallest scene triangle id).  With
+// kCount, node fetches and triangle tests are tallied (instrumented replay).
+template <bool kAny, bool kCount = false>
+RP_DEV bool traverse(const Scene &S, Stack &st, F3 o, F3 d, float tmax, Hit &h, TraversalCounts *tc = nullptr) {
+    h.prim = kEmpty;
+    h.id = kEmpty;
+    h.t = __builtin_inff();
+    if (S.node_count == 0) return false;
+    F3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    st.sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (cur & kLeaf) {
+            uint32_t first = cur & ((1u << 27) - 1), count = (cur >> 27) & 15u;
+            if (kCount) tc->tris += count;
+            for (uint32_t k = 0; k < count; ++k) {
+                F3 a, b, c;
+                uint32_t id;
+                load_tri(S, first + k, a, b, c, id);
+                float t, b0, b1, b2;
+                if (intersect_triangle(o, d, tmax, a, b, c, t, b0, b1, b2) && (t < h.t || (t == h.t && id < h.id))) {
+                    h.prim = first + k;
+                    h.id = id;
+                    h.t = t;
+                    h.b0 = b0;
+                    h.b1 = b1;
+                    h.b2 = b2;
+                    if (kAny) return true;
+                }
+            }
+        } else {
+            float tn[4];
+            uint32_t ch[4];
+            if (kCount) tc->nodes += 1;
+            uint32_t mask = test_children(S.nodes, cur, o, inv, omin(tmax, h.t), tn, ch);
+            if (mask) {
+                // order hit children by entry distance; nearest continues, others pushed far-to-near
+                float dist[4];
+                uint32_t ids[4];
+                int n = 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (mask & (1u << i)) {
+                        dist[n] = tn[i];
+                        ids[n] = ch[i];
+                        ++n;
+                    }
+                for (int i = 1; i < n; ++i) {
+                    float dv = dist[i];
+                    uint32_t iv = ids[i];
+                    int j = i - 1;
+                    while (j >= 0 && dist[j] > dv) {
+                        dist[j + 1] = dist[j];
+                        ids[j + 1] = ids[j];
+                        --j;
+                    }
+                    dist[j + 1] = dv;
+                    ids[j + 1] = iv;
+                }
+                for (int i = n - 1; i >= 1; --i) st.push(ids[i], dist[i]);
+                cur = ids[0];
+                continue;
+            }
+        }
+        // pop the next entry that can still contain a closer hit
+        bool found = false;
+        while (st.sp > 0) {
+            uint32_t n;
+            float t;
+            st.pop(n, t);
+            if (t <= h.t) {
+                cur = n;
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return h.prim != kEmpty;
+}
+
+struct SurfacePoint {
+    F3 p, n, err;
+};
+RP_DEV SurfacePoint surface_point(const Scene &S, const Hit &h) {
+    F3 a, b, c;
+    uint32_t id;
+    load_tri(S, h.prim, a, b, c, id);
+    SurfacePoint sp;
+    sp.p = h.b0 * a + h.b1 * b + h.b2 * c;
+    F3 abs_sum = vabs(h.b0 * a) + vabs(h.b1 * b) + vabs(h.b2 * c);
+    sp.err = gamma_n(7) * abs_sum;
+    sp.n = normalize(cross(a - c, b - c));
+    return sp;
+}
+RP_DEV F3 offset_origin(F3 p, F3 err, F3 n, F3 w) {
+    float dd = dot(vabs(n), err);
+    F3 off = dd * n;
+    if (dot(w, n) < 0) off = -off;
+    F3 po = p + off;
+    if (off.x > 0) po.x = next_up(po.x); else if (off.x < 0) po.x = next_down(po.x);
+    if (off.y > 0) po.y = next_up(po.y); else if (off.y < 0) po.y = next_down(po.y);
+    if (off.z > 0) po.z = next_up(po.z); else if (off.z < 0) po.z = next_down(po.z);
+    return po;
+}
+struct Frame {
+    F3 x, y, z;
+    RP_DEV F3 local(F3 v) const { return {dot(v, x), dot(v, y), dot(v, z)}; }
+    RP_DEV F3 world(F3 v) const { return x * v.x + y * v.y + z * v.z; }
+};
+RP_DEV Frame frame_from_z(F3 n) {
+    float sign = copysignf(1.0f, n.z);
+    float a = -1.0f / (sign + n.z);
+    float b = n.x * n.y * a;
+    Frame f;
+    f.x = {1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+    f.y = {b, sign + n.y * n.y * a, -n.y};
+    f.z = n;
+    return f;
+}
+
+// =================================================================== BSDF
+struct Lane {
+    uint32_t kind;
+    float r, eta, k, ax, ay;
+    bool smooth;
+};
+RP_DEV Lane prepare_lane(const Scene &S, const rp_material &m, float lambda) {
+    Lane l;
+    l.kind = m.kind;
+    l.r = 0;
+    l.eta = 1;
+    l.k = 0;
+    l.ax = l.ay = 0;
+    l.smooth = false;
+    if (m.kind == RP_MATERIAL_DIFFUSE) {
+        l.r = spectrum_eval(m.reflectance, lambda, S.spectra);
+        return l;
+    }
+    l.eta = spectrum_eval(m.eta, lambda, S.spectra);
+    if (m.kind == RP_MATERIAL_CONDUCTOR) l.k = spectrum_eval(m.k, lambda, S.spectra);
+    float u = m.roughness_u, v = m.roughness_v;
+    l.ax = m.remap_roughness ? __builtin_sqrtf(u) : u;
+    l.ay = m.remap_roughness ? __builtin_sqrtf(v) : v;
+    if (m.kind == RP_MATERIAL_DIELECTRIC && l.eta == 0.0f) l.eta = 1.0f;
+    l.smooth = omax(l.ax, l.ay) < 1e-3f || (m.kind == RP_MATERIAL_DIELECTRIC && l.eta == 1.0f);
+    if (!l.smooth) {
+        l.ax = omax(l.ax, 1e-4f);
+        l.ay = omax(l.ay, 1e-4f);
+    }
+    return l;
+}
+struct Cx {
+    float re, im;
+};
+RP_DEV Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+RP_DEV Cx cdiv(Cx a, Cx b) {
+    float scale = 1.0f / (b.re * b.re + b.im * b.im);
+    return {scale * (a.re * b.re + a.im * b.im), scale * (a.im * b.re - a.re * b.im)};
+}
+RP_DEV float cnorm(Cx a) { return a.re * a.re + a.im * a.im; }
+RP_DEV Cx csqrt(Cx z) {
+    float n = __builtin_sqrtf(cnorm(z));
+    if (n == 0.0f) return {0.0f, 0.0f};
+    float t1 = __builtin_sqrtf(0.5f * (n + fabsf(z.re)));
+    float t2 = 0.5f * z.im / t1;
+    if (z.re >= 0.0f) return {t1, t2};
+    return {fabsf(t2), copysignf(t1, z.im)};
+}
+RP_DEV float fresnel_complex(float cos_i, float eta, float k) {
+    cos_i = omin(omax(cos_i, 0.0f), 1.0f);
+    Cx e = {eta, k};
+    float sin2_i = 1.0f - cos_i * cos_i;
+    Cx sin2_t = cdiv({sin2_i, 0.0f}, cmul(e, e));
+    Cx cos_t = csqrt({1.0f - sin2_t.re, -sin2_t.im});
+    Cx ec = {e.re * cos_i, e.im * cos_i};
+    Cx r_parl = cdiv({ec.re - cos_t.re, ec.im - cos_t.im}, {ec.re + cos_t.re, ec.im + cos_t.im});
+    Cx ect = cmul(e, cos_t);
+    Cx r_perp = cdiv({cos_i - ect.re, -ect.im}, {cos_i + ect.re, ect.im});
+    return (cnorm(r_parl) + cnorm(r_perp)) * 0.5f;
+}
+RP_DEV float fresnel_dielectric(float cos_i, float eta) {
+    cos_i = omin(omax(cos_i, -1.0f), 1.0f);
+    if (cos_i < 0.0f) {
+        eta = 1.0f / eta;
+        cos_i = -cos_i;
+    }
+    float sin2_i = 1.0f - cos_i * cos_i;
+    float sin2_t = sin2_i / (eta * eta);
+    if (sin2_t >= 1.0f) return 1.0f;
+    float cos_t = __builtin_sqrtf(omax(0.0f, 1.0f - sin2_t));
+    float r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    return (r_parl * r_parl + r_perp * r_perp) * 0.5f;
+}
+RP_DEV float ggx_d(const Lane &l, F3 m) {
+    if (m.z * m.z * m.z * m.z < 1e-16f) return 0.0f;
+    F3 v = {m.x / l.ax, m.y / l.ay, m.z};
+    float t = dot(v, v);
+    return 1.0f / (kPi * l.ax * l.ay * t * t);
+}
+RP_DEV float ggx_lambda(const Lane &l, F3 w) {
+    F3 v = {1.0f, l.ax * w.x / w.z, l.ay * w.y / w.z};
+    return 0.5f * (length(v) - 1.0f);
+}
+RP_DEV F3 ggx_sample_wm(const Lane &l, F3 w, float u0, float u1) {
+    F3 wh = normalize(F3{l.ax * w.x, l.ay * w.y, w.z});
+    if (wh.z < 0) wh = -wh;
+    F3 t1 = wh.z < 0.99999f ? normalize(F3{-wh.y, wh.x, 0.0f}) : F3{1.0f, 0.0f, 0.0f};
+    F3 t2 = cross(wh, t1);
+    float r = __builtin_sqrtf(u0);
+    float sn, cs;
+    sincos_turns(u1, sn, cs);
+    float px = r * cs, py = r * sn;
+    float h = __builtin_sqrtf(1.0f - px * px);
+    float a = (1.0f + wh.z) * 0.5f;
+    py = (1.0f - a) * h + a * py;
+    float pz = __builtin_sqrtf(omax(0.0f, 1.0f - (px * px + py * py)));
+    F3 nh = px * t1 + py * t2 + pz * wh;
+    return normalize(F3{l.ax * nh.x, l.ay * nh.y, omax(1e-6f, nh.z)});
+}
+struct Eval {
+    float f, pdf;
+};
+RP_DEV Eval eval_lane(const Lane &l, F3 wo, F3 wi) {
+    Eval e = {0.0f, 0.0f};
+    if (l.kind == RP_MATERIAL_DIFFUSE) {
+        if (!(wo.z * wi.z > 0)) return e;
+        e.f = l.r * kInvPi;
+        e.pdf = fabsf(wi.z) * kInvPi;
+        return e;
+    }
+    if (l.smooth || wo.z == 0.0f || wi.z == 0.0f) return e;
+    if (l.kind == RP_MATERIAL_CONDUCTOR) {
+        if (!(wo.z * wi.z > 0)) return e;
+        F3 h = wo + wi;
+        if (dot(h, h) == 0.0f) return e;
+        F3 m = normalize(h);
+        float F = fresnel_complex(fabsf(dot(wo, m)), l.eta, l.k);
+        float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+        e.f = D * F / (4.0f * fabsf(wo.z) * fabsf(wi.z) * (1.0f + lo + li));
+        e.pdf = D / (4.0f * fabsf(wo.z) * (1.0f + lo));
+        return e;
+    }
+    bool reflect = wo.z * wi.z > 0;
+    float etap = 1.0f;
+    if (!reflect) etap = wo.z > 0 ? l.eta : 1.0f / l.eta;
+    F3 h = wi * etap + wo;
+    if (dot(h, h) == 0.0f) return e;
+    F3 m = normalize(h);
+    if (m.z < 0) m = -m;
+    float dmi = dot(m, wi), dmo = dot(m, wo);
+    if (dmi * wi.z < 0 || dmo * wo.z < 0) return e;
+    float F = fresnel_dielectric(dmo, l.eta);
+    float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+    float G = 1.0f / (1.0f + lo + li);
+    float pdf_wm = D / (fabsf(wo.z) * (1.0f + lo)) * fabsf(dmo);
+    float R = F, T = 1.0f - F;
+    if (reflect) {
+        e.f = D * G * F / fabsf(4.0f * wi.z * wo.z);
+        e.pdf = pdf_wm / (4.0f * fabsf(dmo)) * (R / (R + T));
+    } else {
+        float s = dmi + dmo / etap;
+        float denom = s * s * wi.z * wo.z;
+        float ft = D * T * G * fabsf(dmi * dmo / denom);
+        e.f = ft / (etap * etap);
+        e.pdf = pdf_wm * (fabsf(dmi) / (s * s)) * (T / (R + T));
+    }
+    return e;
+}
+struct Sampled {
+    F3 wi;
+    float pdf, eta;
+    bool delta, transmission, ok;
+};
+RP_DEV bool refract(F3 wi, F3 n, float eta, float &etap, F3 &wt) {
+    float cos_i = dot(n, wi);
+    if (cos_i < 0) {
+        eta = 1.0f / eta;
+        cos_i = -cos_i;
+        n = -n;
+    }
+    float sin2_i = omax(0.0f, 1.0f - cos_i * cos_i);
+    float sin2_t = sin2_i / (eta * eta);
+    if (sin2_t >= 1.0f) return false;
+    float cos_t = __builtin_sqrtf(omax(0.0f, 1.0f - sin2_t));
+    wt = -wi / eta + (cos_i / eta - cos_t) * n;
+    etap = eta;
+    return true;
+}
+RP_DEV F3 cosine_hemisphere(float u0, float u1) {
+    float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (ox != 0.0f || oy != 0.0f) {
+        float r, turns;
+        if (fabsf(ox) > fabsf(oy)) {
+            r = ox;
+            turns = 0.125f * (oy / ox);
+        } else {
+            r = oy;
+            turns = 0.25f - 0.125f * (ox / oy);
+        }
+        float sn, cs;
+        sincos_turns(turns, sn, cs);
+        dx = r * cs;
+        dy = r * sn;
+    }
+    float z = __builtin_sqrtf(omax(0.0f, 1.0f - dx * dx - dy * dy));
+    return {dx, dy, z};
+}
+RP_DEV Sampled sample_lane(const Lane &l, F3 wo, float uc, float u0, float u1) {
+    Sampled s;
+    s.wi = {0, 0, 0};
+    s.pdf = 0;
+    s.eta = 1;
+    s.delta = s.transmission = s.ok = false;
+    if (wo.z == 0.0f) return s;
+    if (l.kind == RP_MATERIAL_DIFFUSE) {
+        F3 wi = cosine_hemisphere(u0, u1);
+        if (wo.z < 0) wi.z = -wi.z;
+        s.wi = wi;
+        s.pdf = fabsf(wi.z) * kInvPi;
+        s.ok = s.pdf > 0;
+        return s;
+    }
+    if (l.kind == RP_MATERIAL_CONDUCTOR) {
+        if (l.smooth) {
+            s.wi = {-wo.x, -wo.y, wo.z};
+            s.pdf = 1.0f;
+            s.delta = true;
+            s.ok = true;
+            return s;
+        }
+        F3 m = ggx_sample_wm(l, wo, u0, u1);
+        F3 wi = 2.0f * dot(wo, m) * m - wo;
+        if (!(wo.z * wi.z > 0)) return s;
+        float lo = ggx_lambda(l, wo);
+        float D = ggx_d(l, m);
+        if (D == 0.0f) return s;
+        s.wi = wi;
+        s.pdf = D / (4.0f * fabsf(wo.z) * (1.0f + lo));
+        s.ok = s.pdf > 0;
+        return s;
+    }
+    if (l.smooth) {
+        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
+        if (uc < R / (R + T)) {
+            s.wi = {-wo.x, -wo.y, wo.z};
+            s.pdf = R / (R + T);
+        } else {
+            float etap;
+            F3 wt;
+            if (!refract(wo, F3{0.0f, 0.0f, 1.0f}, l.eta, etap, wt)) return s;
+            s.wi = wt;
+            s.pdf = T / (R + T);
+            s.eta = etap;
+            s.transmission = true;
+        }
+        s.delta = true;
+        s.ok = s.pdf > 0 && s.wi.z != 0.0f;
+        return s;
+    }
+    F3 m = ggx_sample_wm(l, wo, u0, u1);
+    float R = fresnel_dielectric(dot(wo, m), l.eta), T = 1.0f - R;
+    float lo = ggx_lambda(l, wo), D = ggx_d(l, m);
+    float pdf_wm = D / (fabsf(wo.z) * (1.0f + lo)) * fabsf(dot(wo, m));
+    if (uc < R / (R + T)) {
+        F3 wi = 2.0f * dot(wo, m) * m - wo;
+        if (!(wo.z * wi.z > 0)) return s;
+        s.wi = wi;
+        s.pdf = pdf_wm / (4.0f * fabsf(dot(wo, m))) * (R / (R + T));
+    } else {
+        float etap;
+        F3 wi;
+        if (!refract(wo, m, l.eta, etap, wi)) return s;
+        if (wo.z * wi.z > 0 || wi.z == 0.0f) return s;
+        float sd = dot(wi, m) + dot(wo, m) / etap;
+        float dwm_dwi = fabsf(dot(wi, m)) / (sd * sd);
+        s.wi = wi;
+        s.pdf = pdf_wm * dwm_dwi * (T / (R + T));
+        s.eta = etap;
+        s.transmission = true;
+    }
+    s.ok = s.pdf > 0 && D > 0;
+    return s;
+}
+RP_DEV void dirac_lane(const Lane &l, F3 wo, F3 wi, float u0, float u1, float &mass, float &pdf) {
+    mass = pdf = 0.0f;
+    if (wo.z == 0.0f || wi.z == 0.0f) return;
+    bool transmission = wo.z * wi.z < 0;
+    if (l.kind == RP_MATERIAL_CONDUCTOR && l.smooth) {
+        if (transmission || !(wi.x == -wo.x && wi.y == -wo.y && wi.z == wo.z)) return;
+        pdf = 1.0f;
+        mass = fresnel_complex(fabsf(wo.z), l.eta, l.k);
+        return;
+    }
+    if (l.kind == RP_MATERIAL_DIELECTRIC && l.smooth) {
+        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
+        Sampled c = sample_lane(l, wo, transmission ? R / (R + T) : 0.0f, u0, u1);
+        if (!c.ok || !(c.wi.x == wi.x && c.wi.y == wi.y && c.wi.z == wi.z)) return;
+        pdf = c.pdf;
+        float scale = transmission ? 1.0f / (c.eta * c.eta) : 1.0f;
+        mass = (transmission ? T : R) * scale;
+    }
+}
+
+// ================================================================= media
+struct MediumCoeff {
+    float st[4], ss[4];
+    float g;
+};
+RP_DEV MediumCoeff medium_coeff(const Scene &S, uint32_t m, const Wavelengths &wl) {
+    MediumCoeff c;
+    const rp_medium &md = S.media[m];
+    for (int k = 0; k < 4; ++k) {
+        float a = spectrum_eval(md.sigma_a, wl.lambda[k], S.spectra);
+        float sc = spectrum_eval(md.sigma_s, wl.lambda[k], S.spectra);
+        c.ss[k] = sc;
+        c.st[k] = a + sc;
+    }
+    c.g = omin(omax(md.g, -0.99f), 0.99f);
+    return c;
+}
+RP_DEV float hg(float cos_t, float g) {
+    float denom = 1.0f + g * g + 2.0f * g * cos_t;
+    return kInv4Pi * (1.0f - g * g) / (denom * __builtin_sqrtf(denom));
+}
+RP_DEV F3 hg_sample(F3 wo, float g, float u0, float u1, float &pdf) {
+    float cos_t;
+    if (fabsf(g) < 1e-3f) cos_t = 1.0f - 2.0f * u0;
+    else {
+        float sq = (1.0f - g * g) / (1.0f + g - 2.0f * g * u0);
+        cos_t = -1.0f / (2.0f * g) * (1.0f + g * g - sq * sq);
+    }
+    cos_t = omin(omax(cos_t, -1.0f), 1.0f);
+    float sin_t = __builtin_sqrtf(omax(0.0f, 1.0f - cos_t * cos_t));
+    float sn, cs;
+    sincos_turns(u1, sn, cs);
+    Frame f = frame_from_z(wo);
+    F3 wi = f.world(F3{sin_t * cs, sin_t * sn, cos_t});
+    pdf = hg(cos_t, g);
+    return wi;
+}
+RP_DEV uint32_t outgoing_medium(const rp_object &o, uint32_t current, F3 n, F3 w) {
+    if (o.medium_inside == o.medium_outside) return current;
+    return dot(n, w) > 0 ? o.medium_outside : o.medium_inside;
+}
+
+// ================================================================ lights
+struct LightSample {
+    F3 p, n, err, wi;
+    float le[4];
+    float pdf, dist;
+    bool delta, infinite, ok;
+};
+RP_DEV uint32_t select_light(const Scene &S, float u) {
+    uint32_t lo = 0, hi = S.light_count;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) / 2;
+        if (u < S.lights[mid].cdf) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo < S.light_count - 1 ? lo : S.light_count - 1;
+}
+RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, float ul, float u0, float u1) {
+    LightSample ls;
+    ls.ok = ls.delta = ls.infinite = false;
+    ls.pdf = 0;
+    ls.dist = 0;
+    for (int k = 0; k < 4; ++k) ls.le[k] = 0;
+    ls.p = ls.n = ls.err = ls.wi = F3{0, 0, 0};
+    if (S.light_count == 0) return ls;
+    uint32_t li = select_light(S, ul);
+    const LightEntry L = S.lights[li];
+    if (!(L.pmf > 0)) return ls;
+    if (L.kind == 0) {
+        F3 a, b, c;
+        uint32_t id;
+        load_tri(S, L.index, a, b, c, id);
+        float b0, b1;
+        if (u0 < u1) {
+            b0 = u0 * 0.5f;
+            b1 = u1 - b0;
+        } else {
+            b1 = u1 * 0.5f;
+            b0 = u0 - b1;
+        }
+        float b2 = 1.0f - b0 - b1;
+        ls.p = b0 * a + b1 * b + b2 * c;
+        ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
+        ls.n = normalize(cross(a - c, b - c));
+        F3 dv = ls.p - p;
+        float d2 = dot(dv, dv);
+        if (!(d2 > 0)) return ls;
+        ls.dist = __builtin_sqrtf(d2);
+        ls.wi = dv / ls.dist;
+        float cos_l = dot(ls.n, -ls.wi);
+        uint32_t obj = __float_as_uint(S.tris[3 * L.index + 1].w);
+        const rp_emission &em = S.emissions[S.objects[obj].emission];
+        if (!em.two_sided && !(cos_l > 0)) return ls;
+        if (cos_l == 0.0f) return ls;
+        ls.pdf = ((L.pmf * L.inv_area) * d2) / fabsf(cos_l);
+        for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(em.radiance, wl.lambda[k], S.spectra);
+        ls.ok = ls.pdf > 0 && __builtin_isfinite(ls.pdf);
+        return ls;
+    }
+    if (L.kind == 1) {
+        const rp_spot_light &sp = S.spots[L.index];
+        F3 pos = {sp.position[0], sp.position[1], sp.position[2]};
+        F3 dv = pos - p;
+        float d2 = dot(dv, dv);
+        if (!(d2 > 0)) return ls;
+        ls.dist = __builtin_sqrtf(d2);
+        ls.wi = dv / ls.dist;
+        F3 axis = {sp.direction[0], sp.direction[1], sp.direction[2]};
+        float cos_t = dot(-ls.wi, axis);
+        float t = (cos_t - sp.cos_falloff_end) / (sp.cos_falloff_start - sp.cos_falloff_end);
+        t = omin(omax(t, 0.0f), 1.0f);
+        float fall = t * t * (3.0f - 2.0f * t);
+        for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(sp.intensity, wl.lambda[k], S.spectra) * fall / d2;
+        ls.p = pos;
+        ls.pdf = L.pmf;
+        ls.delta = true;
+        ls.ok = true;
+        return ls;
+    }
+    float z = 1.0f - 2.0f * u0;
+    float r = __builtin_sqrtf(omax(0.0f, 1.0f - z * z));
+    float sn, cs;
+    sincos_turns(u1, sn, cs);
+    ls.wi = {r * cs, r * sn, z};
+    ls.infinite = true;
+    ls.pdf = L.pmf * kInv4Pi;
+    for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(S.environment, wl.lambda[k], S.spectra);
+    ls.ok = true;
+    return ls;
+}
+
+// Visibility with transmittance through null interfaces and media.
+RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uint32_t medium, const Wavelengths &wl,
+                          float tr[4], uint32_t boundary_limit) {
+    for (int k = 0; k < 4; ++k) tr[k] = 1.0f;
+    if (!S.has_interfaces) {
+        Hit h;
+        if (traverse<true>(S, st, o, d, tmax, h)) return false;
+        if (medium != RP_NONE) {
+            MediumCoeff mc = medium_coeff(S, medium, wl);
+            float len = length(d);
+            for (int k = 0; k < 4; ++k) tr[k] = tr[k] * exp_f(-mc.st[k] * (tmax * len));
+        }
+        return true;
+    }
+    uint32_t crossings = 0;
+    for (;;) {
+        Hit h;
+        bool hit = traverse<false>(S, st, o, d, tmax, h);
+        float seg = hit ? h.t : tmax;
+        if (medium != RP_NONE) {
+            MediumCoeff mc = medium_coeff(S, medium, wl);
+            float len = length(d);
+            for (int k = 0; k < 4; ++k) tr[k] = tr[k] * exp_f(-mc.st[k] * (seg * len));
+        }
+        if (!hit) return true;
+        uint32_t obj = __float_as_uint(S.tris[3 * h.prim + 1].w);
+        const rp_object &ob = S.objects[obj];
+        if (S.materials[ob.material].kind != RP_MATERIAL_INTERFACE) return false;
+        if (++crossings > boundary_limit) return false;
+        SurfacePoint sp = surface_point(S, h);
+        medium = outgoing_medium(ob, medium, sp.n, d);
+        F3 no = offset_origin(sp.p, sp.err, sp.n, d);
+        F3 target = o + tmax * d;
+        if (__builtin_isinf(tmax)) {
+            o = no;
+        } else {
+            d = target - no;
+            o = no;
+            tmax = 1.0f;
+        }
+    }
+}
+
+// ======================================================== hero bookkeeping
+RP_DEV float mixture(const float d[4], const float p[4]) { return (d[0] * p[0] + d[1] * p[1]) + (d[2] * p[2] + d[3] * p[3]); }
+RP_DEV float dsum(const float d[4]) { return (d[0] + d[1]) + (d[2] + d[3]); }
+RP_DEV void normalize_density(float d[4]) {
+    float m = omax(omax(d[0], d[1]), omax(d[2], d[3]));
+    if (!(m > 0) || !__builtin_isfinite(m)) return;
+    int e;
+    (void)frexpf(m, &e);
+    for (int k = 0; k < 4; ++k) d[k] = ldexpf(d[k], -e);
+}
+RP_DEV bool advance(float density[4], float beta[4], const float num[4], const float pdf[4]) {
+    float nd[4];
+    for (int k = 0; k < 4; ++k) nd[k] = density[k] * pdf[k];
+    float before = dsum(density), after = dsum(nd);
+    if (!(before > 0 && after > 0) || !__builtin_isfinite(after)) return false;
+    float ratio = before / after;
+    for (int k = 0; k < 4; ++k) {
+        beta[k] = (beta[k] * num[k]) * ratio;
+        if (!__builtin_isfinite(beta[k])) return false;
+        density[k] = nd[k];
+    }
+    normalize_density(density);
+    return true;
+}
+RP_DEV bool positive_normal(float v) {
+    uint32_t w = __float_as_uint(v);
+    return w >= 0x00800000u && w < 0x7f800000u;
+}
+RP_DEV bool hero_continue(float beta[4], const float eta_scale[4], bool enabled, uint32_t depth, float u) {
+    if (!enabled || depth <= 1) return true;
+    float maximum = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float v = beta[k] * eta_scale[k];
+        if (!__builtin_isfinite(v) || v < 0 || (beta[k] > 0 && !positive_normal(v))) return true;
+        maximum = omax(maximum, v);
+    }
+    if (!positive_normal(maximum) || maximum >= 1.0f) return true;
+    float q = 1.0f - maximum;
+    float survival = 1.0f - q;
+    if (!(q > 0 && q < 1) || !positive_normal(survival)) return true;
+    float surv[4];
+    for (int k = 0; k < 4; ++k) {
+        surv[k] = beta[k];
+        if (__float_as_uint(beta[k]) == 0) continue;
+        surv[k] = beta[k] / survival;
+        if (!positive_normal(surv[k])) return true;
+    }
+    if (u < q) return false;
+    for (int k = 0; k < 4; ++k) beta[k] = surv[k];
+    return true;
+}
+
+// Hero sensor: sum_k cie(lambda_k) * (L_k / pdf_k / 4).
+RP_DEV void sensor_xyz(const Scene &S, const Wavelengths &wl, const float L[4], float xyz[3]) {
+    xyz[0] = xyz[1] = xyz[2] = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        if (!(wl.pdf[k] > 0)) continue;
+        int i = int(floorf(wl.lambda[k] + 0.5f)) - 360;
+        i = i < 0 ? 0 : (i > 470 ? 470 : i);
+        float w = (L[k] / wl.pdf[k]) / 4.0f;
+        float4 c = S.cie[i];
+        xyz[0] = xyz[0] + c.x * w;
+        xyz[1] = xyz[1] + c.y * w;
+        xyz[2] = xyz[2] + c.z * w;
+    }
+}
+
+} // namespace rpd

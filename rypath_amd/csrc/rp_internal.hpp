@@ -1,0 +1,107 @@
+// Host-side internals of the rypath_amd library (not part of the C ABI).
+#pragma once
+#include "../../include/rp_api.h"
+#include "../../include/rp_synthetic.h"
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace rp {
+
+// ------------------------------------------------------------- errors
+void set_error(const std::string &message);
+const std::string &last_error();
+
+// ------------------------------------------------------------- data files
+// Directory holding sobol_tables.bin, cie_xyz_1931_2deg.csv, srgb_sigmoid.bin.
+std::string data_dir();
+struct SobolData {
+    std::vector<uint32_t> directions; // [1024][52] (PBRT layout)
+    std::vector<uint64_t> intervals;  // [25][52]
+    std::vector<uint64_t> inverse;    // [26][52]
+};
+const SobolData &sobol_data();
+const std::array<std::array<float, 3>, 471> &cie_xyz();
+
+// ------------------------------------------------------------- spectra
+struct SigmoidCoefficients {
+    bool constant = false;
+    float value = 0;          // when constant
+    std::array<float, 3> c{}; // descending powers of lambda (nm)
+};
+// PBRT v4 RGBToSpectrumTable lookup for linear sRGB in [0,1] (src/rgb_spectrum.cpp:62).
+SigmoidCoefficients srgb_sigmoid(const std::array<float, 3> &rgb);
+rp_spectrum spectrum_from_rgb(const std::array<float, 3> &rgb);
+// Blackbody radiance normalised to its Wien peak, 471 samples (src/render.cpp:2185-2200).
+std::array<float, 471> blackbody_table(double kelvin, double scale);
+float spectrum_eval_host(const rp_spectrum &s, float lambda, const float *pool);
+
+// ------------------------------------------------------------- BVH
+// 64-byte quantised 4-wide node.  Child boxes are stored as 8-bit offsets from
+// `origin` in units of 2^(e-127) per axis; the decoded box (origin + q*scale,
+// evaluated in binary32 exactly as the traversal kernel does) always contains
+// the true child box.
+struct alignas(64) QNode {
+    float origin[3];
+    uint8_t exponent[3];
+    uint8_t child_count;
+    uint8_t qlo[3][4];
+    uint8_t qhi[3][4];
+    uint32_t child[4];  // internal: node index; leaf: kLeafBit | count << 27 | first (leaf order)
+    uint8_t count[4];   // leaf triangle count (0 for internal children)
+    uint32_t pad;
+};
+static_assert(sizeof(QNode) == 64, "QNode must be one 64-byte line");
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kLeafCountShift = 27;
+constexpr uint32_t kLeafFirstMask = (1u << 27) - 1;
+constexpr uint32_t kEmptyChild = 0xffffffffu;
+
+struct Bvh4 {
+    std::vector<QNode> nodes;
+    std::vector<uint32_t> tri_order; // leaf-order position -> scene triangle id
+    uint32_t depth = 0;              // max node depth (root = 1)
+    uint32_t max_stack = 0;          // worst-case traversal stack entries
+    uint64_t leaf_count = 0;
+    double seconds = 0;
+};
+Bvh4 build_bvh4(const float *positions, const uint32_t *indices, uint32_t triangle_count, uint32_t leaf_size,
+                uint32_t bins, uint32_t threads);
+inline float decode_lo(float origin, uint8_t e, uint8_t q) {
+    float s;
+    uint32_t bits = uint32_t(e) << 23;
+    static_assert(sizeof(s) == 4);
+    __builtin_memcpy(&s, &bits, 4);
+    return origin + float(q) * s;
+}
+
+// ------------------------------------------------------------- lights
+struct LightEntry {
+    uint32_t kind;  // 0 triangle, 1 spot, 2 environment
+    uint32_t index; // triangle id / spot index / 0
+    float pmf, cdf, inv_area;
+    uint32_t pad[3];
+};
+// Power-proportional light table (PBRT PowerLightSampler semantics, fixed order:
+// emissive triangles by id, then spots, then the environment).
+std::vector<LightEntry> build_light_table(const rp_scene_desc &d, float scene_radius, std::vector<uint32_t> &tri_light);
+float scene_bounding_radius(const rp_scene_desc &d);
+
+// ------------------------------------------------------------- owned scene copy
+struct HostScene {
+    rp_scene_desc desc{};
+    std::vector<float> positions, spectra;
+    std::vector<uint32_t> indices, tri_object;
+    std::vector<rp_object> objects;
+    std::vector<rp_material> materials;
+    std::vector<rp_emission> emissions;
+    std::vector<rp_medium> media;
+    std::vector<rp_spot_light> spots;
+    void copy_from(const rp_scene_desc &d);
+};
+std::string validate_desc(const rp_scene_desc &d);
+
+} // namespace rp

@@ -1,0 +1,411 @@
+// Wavefront form of the path tracer: generate -> extend -> (sort) -> shade ->
+// shadow -> accumulate, with path state in HBM (structure of arrays, 16-byte
+// fields) and queues compacted with wave ballots.  The per-event arithmetic is
+// rp_path.h's, so a wavefront frame equals the megakernel frame bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rp_path.h"
+
+namespace rpd {
+
+// Plain view of the device buffers (passed by value to kernels).
+struct WfState {
+    // path state
+    float4 *o_t, *d, *beta, *density, *prev_density, *prev_pdf, *eta, *L, *prev_p;
+    uint4 *id, *meta, *meta2;
+    // hits
+    float4 *hit_t;
+    uint2 *hit_id;
+    // shadow requests
+    float4 *sh_o, *sh_d, *sh_val;
+    uint2 *sh_meta; // (medium, no-MIS flag)
+    // queues
+    uint32_t *qa, *qb, *qs, *qsort;
+    uint32_t *qcount; // [0] next, [1] shadow
+    uint32_t *hist;   // [256] histogram, [256] cursors
+};
+
+// Owner of the wavefront buffers (host side).
+struct WavefrontBuffers {
+    WfState v{};
+    uint64_t capacity = 0;
+    uint32_t *host_count = nullptr; // pinned
+    WavefrontBuffers() = default;
+    WavefrontBuffers(const WavefrontBuffers &) = delete;
+    WavefrontBuffers &operator=(const WavefrontBuffers &) = delete;
+    void release();
+    void ensure(uint64_t n);
+    ~WavefrontBuffers() { release(); }
+};
+
+// ------------------------------------------------------------ state I/O
+RP_DEV void load_state(const WfState &B, uint32_t i, PathState &ps) {
+    float4 a = B.o_t[i], b = B.d[i];
+    ps.o = {a.x, a.y, a.z};
+    ps.tmax = a.w;
+    ps.d = {b.x, b.y, b.z};
+    float4 v;
+    v = B.beta[i];
+    ps.beta[0] = v.x, ps.beta[1] = v.y, ps.beta[2] = v.z, ps.beta[3] = v.w;
+    v = B.density[i];
+    ps.density[0] = v.x, ps.density[1] = v.y, ps.density[2] = v.z, ps.density[3] = v.w;
+    v = B.prev_density[i];
+    ps.prev_density[0] = v.x, ps.prev_density[1] = v.y, ps.prev_density[2] = v.z, ps.prev_density[3] = v.w;
+    v = B.prev_pdf[i];
+    ps.prev_pdf[0] = v.x, ps.prev_pdf[1] = v.y, ps.prev_pdf[2] = v.z, ps.prev_pdf[3] = v.w;
+    v = B.eta[i];
+    ps.eta_scale[0] = v.x, ps.eta_scale[1] = v.y, ps.eta_scale[2] = v.z, ps.eta_scale[3] = v.w;
+    v = B.L[i];
+    ps.L[0] = v.x, ps.L[1] = v.y, ps.L[2] = v.z, ps.L[3] = v.w;
+    v = B.prev_p[i];
+    ps.prev_p = {v.x, v.y, v.z};
+    ps.primary = __float_as_uint(v.w);
+    uint4 u = B.id[i];
+    ps.pixel = u.x;
+    ps.sample = uint64_t(u.y) | (uint64_t(u.z) << 32);
+    ps.w2 = u.w;
+    u = B.meta[i];
+    ps.index = uint64_t(u.x) | (uint64_t(u.y) << 32);
+    ps.count = u.z;
+    ps.crossings = u.w;
+    u = B.meta2[i];
+    ps.medium = u.x;
+    ps.mdraw = u.y;
+    ps.flags = u.z;
+    ps.status = u.w;
+}
+RP_DEV void store_state(const WfState &B, uint32_t i, const PathState &ps) {
+    B.o_t[i] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.tmax);
+    B.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+    B.beta[i] = make_float4(ps.beta[0], ps.beta[1], ps.beta[2], ps.beta[3]);
+    B.density[i] = make_float4(ps.density[0], ps.density[1], ps.density[2], ps.density[3]);
+    B.prev_density[i] = make_float4(ps.prev_density[0], ps.prev_density[1], ps.prev_density[2], ps.prev_density[3]);
+    B.prev_pdf[i] = make_float4(ps.prev_pdf[0], ps.prev_pdf[1], ps.prev_pdf[2], ps.prev_pdf[3]);
+    B.eta[i] = make_float4(ps.eta_scale[0], ps.eta_scale[1], ps.eta_scale[2], ps.eta_scale[3]);
+    B.L[i] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+    B.prev_p[i] = make_float4(ps.prev_p.x, ps.prev_p.y, ps.prev_p.z, __uint_as_float(ps.primary));
+    B.id[i] = make_uint4(ps.pixel, uint32_t(ps.sample), uint32_t(ps.sample >> 32), ps.w2);
+    B.meta[i] = make_uint4(uint32_t(ps.index), uint32_t(ps.index >> 32), ps.count, ps.crossings);
+    B.meta2[i] = make_uint4(ps.medium, ps.mdraw, ps.flags, ps.status);
+}
+
+// Wave-aggregated queue append: one atomic per wave.
+RP_DEV uint32_t wave_append(bool pred, uint32_t *counter) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    uint32_t lane = threadIdx.x & 63;
+    uint32_t leader = uint32_t(__builtin_ctzll(mask));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, uint32_t(__builtin_popcountll(mask)));
+    base = __shfl(base, int(leader));
+    uint64_t below = mask & ((uint64_t(1) << lane) - 1);
+    return base + uint32_t(__builtin_popcountll(below));
+}
+
+RP_DEV void wave_count(unsigned long long *counter, uint32_t v) {
+    unsigned long long e = v;
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off);
+    if ((threadIdx.x & 63) == 0 && e) atomicAdd(counter, e);
+}
+
+constexpr int kWfBlock = 128;
+
+__global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint32_t npx, uint64_t sample_begin,
+                                                       uint64_t n, WfState B) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PathState ps;
+    uint32_t w0, w1;
+    path_begin(S, P, uint32_t(i % npx), sample_begin + i / npx, ps, w0, w1);
+    store_state(B, uint32_t(i), ps);
+    B.qa[i] = uint32_t(i);
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState B, const uint32_t *q, uint32_t qn,
+                                                          unsigned long long *counters) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    Stack st;
+    st.lds_node = s_node + threadIdx.x;
+    st.lds_t = s_t + threadIdx.x;
+    st.stride = kWfBlock;
+    st.sp = 0;
+    uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    uint32_t rays = 0, nodes = 0, tris = 0;
+    if (i < qn) {
+        uint32_t slot = q[i];
+        float4 a = B.o_t[slot], b = B.d[slot];
+        Hit h;
+        TraversalCounts tc = {0, 0};
+        traverse<false, kCount>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h, &tc);
+        B.hit_t[slot] = make_float4(h.t, h.b0, h.b1, h.b2);
+        B.hit_id[slot] = make_uint2(h.prim, h.id);
+        rays = 1;
+        if (kCount) {
+            nodes = tc.nodes;
+            tris = tc.tris;
+        }
+    }
+    wave_count(&counters[0], rays);
+    if (kCount) {
+        wave_count(&counters[3], nodes);
+        wave_count(&counters[4], tris);
+    }
+}
+
+RP_DEV uint32_t material_key(const Scene &S, const WfState &B, uint32_t slot, uint32_t bits) {
+    uint32_t prim = B.hit_id[slot].x;
+    if (prim == kEmpty) return 0;
+    uint32_t obj = __float_as_uint(S.tris[3 * prim + 1].w);
+    uint32_t m = S.objects[obj].material;
+    uint32_t kind = S.materials[m].kind & 3u;
+    uint32_t K = 1u << bits;
+    if (bits <= 2) return 1 + kind < K ? 1 + kind : K - 1;
+    uint32_t key = 1 + (kind << (bits - 2)) + (m & ((1u << (bits - 2)) - 1));
+    return key < K ? key : K - 1;
+}
+
+__global__ void __launch_bounds__(256) k_sort_hist(Scene S, WfState B, const uint32_t *q, uint32_t qn,
+                                                     uint32_t bits, uint32_t *hist) {
+    __shared__ uint32_t h[256];
+    uint32_t K = 1u << bits;
+    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < qn) atomicAdd(&h[material_key(S, B, q[i], bits)], 1u);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+        if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+__global__ void k_sort_scan(uint32_t bits, uint32_t *hist) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t K = 1u << bits, acc = 0;
+    for (uint32_t k = 0; k < K; ++k) {
+        uint32_t c = hist[k];
+        hist[256 + k] = acc; // cursor
+        acc += c;
+    }
+}
+__global__ void __launch_bounds__(256) k_sort_scatter(Scene S, WfState B, const uint32_t *q, uint32_t qn,
+                                                        uint32_t bits, uint32_t *hist, uint32_t *out) {
+    __shared__ uint32_t h[256], base[256];
+    uint32_t K = 1u << bits;
+    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t key = 0, rank = 0, slot = 0;
+    if (i < qn) {
+        slot = q[i];
+        key = material_key(S, B, slot, bits);
+        rank = atomicAdd(&h[key], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) base[k] = h[k] ? atomicAdd(&hist[256 + k], h[k]) : 0;
+    __syncthreads();
+    if (i < qn) out[base[key] + rank] = slot;
+}
+__global__ void k_wf_reset(uint32_t *qcount, uint32_t *hist, uint32_t bits) {
+    uint32_t t = threadIdx.x;
+    if (t < 4) qcount[t] = 0;
+    uint32_t K = 1u << bits;
+    for (uint32_t k = t; k < 512; k += blockDim.x)
+        if (k < K || (k >= 256 && k < 256 + K)) hist[k] = 0;
+}
+
+__global__ void __launch_bounds__(kWfBlock) k_wf_shade(Scene S, PathParams P, WfState B, const uint32_t *q,
+                                                         uint32_t qn, uint32_t *qnext) {
+    uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    bool cont = false, shadow = false;
+    uint32_t slot = 0;
+    ShadowReq sr;
+    sr.active = false;
+    if (i < qn) {
+        slot = q[i];
+        PathState ps;
+        load_state(B, slot, ps);
+        float4 ht = B.hit_t[slot];
+        uint2 hi = B.hit_id[slot];
+        Hit h;
+        h.prim = hi.x;
+        h.id = hi.y;
+        h.t = ht.x;
+        h.b0 = ht.y;
+        h.b1 = ht.z;
+        h.b2 = ht.w;
+        cont = path_event(S, P, ps, h, sr);
+        shadow = sr.active;
+        cont = cont && path_alive(S, ps);
+        store_state(B, slot, ps);
+        if (shadow) {
+            B.sh_o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
+            B.sh_d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
+            B.sh_val[slot] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
+            B.sh_meta[slot] = make_uint2(sr.medium, sr.delta_or_nomis ? 1u : 0u);
+        }
+    }
+    uint32_t pos = wave_append(cont, &B.qcount[0]);
+    if (cont) qnext[pos] = slot;
+    uint32_t spos = wave_append(shadow, &B.qcount[1]);
+    if (shadow) B.qs[spos] = slot;
+}
+
+__global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, WfState B, uint32_t sn,
+                                                          unsigned long long *counters) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    Stack st;
+    st.lds_node = s_node + threadIdx.x;
+    st.lds_t = s_t + threadIdx.x;
+    st.stride = kWfBlock;
+    st.sp = 0;
+    uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    uint32_t rays = 0;
+    if (i < sn) {
+        uint32_t slot = B.qs[i];
+        float4 o = B.sh_o[slot], d = B.sh_d[slot], v = B.sh_val[slot];
+        uint2 meta = B.sh_meta[slot];
+        ShadowReq sr;
+        sr.o = {o.x, o.y, o.z};
+        sr.tmax = o.w;
+        sr.d = {d.x, d.y, d.z};
+        sr.w = d.w;
+        sr.val[0] = v.x, sr.val[1] = v.y, sr.val[2] = v.z, sr.val[3] = v.w;
+        sr.medium = meta.x;
+        sr.delta_or_nomis = meta.y != 0;
+        uint32_t w2 = B.id[slot].w;
+        Wavelengths wl = path_wavelengths(S, w2);
+        float tr[4];
+        bool vis = transmittance(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
+        if (vis) {
+            float4 L = B.L[slot];
+            PathState ps;
+            ps.L[0] = L.x, ps.L[1] = L.y, ps.L[2] = L.z, ps.L[3] = L.w;
+            shadow_apply(ps, sr, vis, tr);
+            B.L[slot] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+        }
+        rays = 1;
+    }
+    wave_count(&counters[1], rays);
+}
+
+__global__ void __launch_bounds__(256) k_wf_finish(Scene S, WfState B, uint64_t n, float *xyz,
+                                                     unsigned long long *counters) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint32_t failed = 0;
+    if (i < n) {
+        float4 L = B.L[i];
+        uint32_t status = B.meta2[i].w;
+        Wavelengths wl = path_wavelengths(S, B.id[i].w);
+        float Lv[4] = {L.x, L.y, L.z, L.w};
+        float v[3] = {0, 0, 0};
+        if (status == RP_PATH_SUCCESS) sensor_xyz(S, wl, Lv, v);
+        else failed = 1;
+        xyz[i] = v[0];
+        xyz[n + i] = v[1];
+        xyz[2 * n + i] = v[2];
+    }
+    wave_count(&counters[2], failed);
+}
+
+#define WF_CHECK(expr)                                                                                        \
+    do {                                                                                                       \
+        hipError_t err_ = (expr);                                                                              \
+        if (err_ != hipSuccess)                                                                                \
+            throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(err_));                     \
+    } while (0)
+
+inline void WavefrontBuffers::release() {
+    void *ptrs[] = {v.o_t, v.d, v.beta, v.density, v.prev_density, v.prev_pdf, v.eta, v.L, v.prev_p, v.id,
+                    v.meta, v.meta2, v.hit_t, v.hit_id, v.sh_o, v.sh_d, v.sh_val, v.sh_meta, v.qa, v.qb,
+                    v.qs, v.qsort, v.qcount, v.hist};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (host_count) (void)hipHostFree(host_count);
+    v = WfState{};
+    capacity = 0;
+    host_count = nullptr;
+}
+
+template <typename T> inline void wf_alloc(T *&p, uint64_t n) { WF_CHECK(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T))); }
+
+inline void WavefrontBuffers::ensure(uint64_t n) {
+    if (n <= capacity) return;
+    release();
+    wf_alloc(v.o_t, n), wf_alloc(v.d, n), wf_alloc(v.beta, n), wf_alloc(v.density, n), wf_alloc(v.prev_density, n);
+    wf_alloc(v.prev_pdf, n), wf_alloc(v.eta, n), wf_alloc(v.L, n), wf_alloc(v.prev_p, n);
+    wf_alloc(v.id, n), wf_alloc(v.meta, n), wf_alloc(v.meta2, n);
+    wf_alloc(v.hit_t, n), wf_alloc(v.hit_id, n);
+    wf_alloc(v.sh_o, n), wf_alloc(v.sh_d, n), wf_alloc(v.sh_val, n), wf_alloc(v.sh_meta, n);
+    wf_alloc(v.qa, n), wf_alloc(v.qb, n), wf_alloc(v.qs, n), wf_alloc(v.qsort, n);
+    wf_alloc(v.qcount, 4);
+    wf_alloc(v.hist, 512);
+    WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 4 * sizeof(uint32_t), hipHostMallocDefault));
+    capacity = n;
+}
+
+// Runs n paths (path i: pixel i % npx, sample sample_begin + i / npx) to
+// completion and writes planar per-path XYZ.  Returns device seconds spent in
+// the closest-hit (extend) kernels.
+inline double wavefront_render(const Scene &S, const PathParams &P, WavefrontBuffers &WB, const rp_frame_config &cfg,
+                               uint32_t npx, uint64_t sample_begin, uint64_t n, float *xyz,
+                               unsigned long long *counters, hipStream_t stream, uint64_t &launches) {
+    if (n > 0xffffffffull) throw std::runtime_error("wavefront chunk too large");
+    WB.ensure(n);
+    const WfState &B = WB.v;
+    const bool sorted = cfg.mode == RP_MODE_WAVEFRONT_SORTED;
+    uint32_t bits = cfg.sort_bits ? cfg.sort_bits : 2;
+    if (bits < 1 || bits > 8) throw std::runtime_error("sort_bits must be in [1, 8]");
+    hipLaunchKernelGGL(k_wf_generate, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n, B);
+    uint32_t *q = B.qa, *qnext = B.qb;
+    uint32_t qn = uint32_t(n);
+    hipEvent_t e0, e1;
+    WF_CHECK(hipEventCreate(&e0));
+    WF_CHECK(hipEventCreate(&e1));
+    double trace = 0;
+    while (qn > 0) {
+        WF_CHECK(hipEventRecord(e0, stream));
+        if (cfg.flags & RP_FRAME_COUNT_TRAFFIC)
+            hipLaunchKernelGGL(k_wf_extend<true>, dim3((qn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, B,
+                               q, qn, counters);
+        else
+            hipLaunchKernelGGL(k_wf_extend<false>, dim3((qn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, B,
+                               q, qn, counters);
+        WF_CHECK(hipEventRecord(e1, stream));
+        ++launches;
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, B.qcount, B.hist, bits);
+        const uint32_t *shade_q = q;
+        if (sorted) {
+            dim3 g((qn + 255) / 256);
+            hipLaunchKernelGGL(k_sort_hist, g, dim3(256), 0, stream, S, B, q, qn, bits, B.hist);
+            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(64), 0, stream, bits, B.hist);
+            hipLaunchKernelGGL(k_sort_scatter, g, dim3(256), 0, stream, S, B, q, qn, bits, B.hist, B.qsort);
+            shade_q = B.qsort;
+        }
+        hipLaunchKernelGGL(k_wf_shade, dim3((qn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, P, B, shade_q,
+                           qn, qnext);
+        WF_CHECK(hipMemcpyAsync(WB.host_count, B.qcount, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        WF_CHECK(hipStreamSynchronize(stream));
+        float ms = 0;
+        WF_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        trace += ms * 1e-3;
+        uint32_t next = WB.host_count[0], sn = WB.host_count[1];
+        if (sn)
+            hipLaunchKernelGGL(k_wf_shadow, dim3((sn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, P, B, sn,
+                               counters);
+        std::swap(q, qnext);
+        qn = next;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    hipLaunchKernelGGL(k_wf_finish, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, B, n, xyz, counters);
+    WF_CHECK(hipGetLastError());
+    return trace;
+}
+
+} // namespace rpd

@@ -1,0 +1,150 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Bars: bit-exact Sobol indices, camera words, wavelengths and primary-hit
+triangle ids; per-path radiance and per-pixel XYZ within 1e-3 relative on
+>= 99.9% of paths/pixels; megakernel, wavefront and sorted wavefront films
+bit-identical to each other.
+"""
+import numpy as np
+import pytest
+
+from tests.parity_util import AGREE_FRACTION, agreement, camera_rays, make_rays
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cornell(rp, gpu_available):
+    s = rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=64, height=64)
+    return s, rp.Scene(s.desc)
+
+
+def _records_parity(rp, orc, synth, scene, req):
+    print(f"[parity] pixels {req.pixel_begin}+{req.pixel_count} samples {req.sample_begin}+{req.sample_count}",
+          flush=True)
+    gpu = scene.sample(req)
+    cpu = orc.scene(synth.desc).sample(req)
+    assert np.array_equal(gpu["pixel"], cpu["pixel"])
+    assert np.array_equal(gpu["sample"], cpu["sample"])
+    assert np.array_equal(gpu["sobol_index"], cpu["sobol_index"])
+    assert np.array_equal(gpu["camera_words"], cpu["camera_words"])
+    assert np.array_equal(gpu["wavelengths"].view(np.uint32), cpu["wavelengths"].view(np.uint32))
+    assert np.array_equal(gpu["primary_triangle"], cpu["primary_triangle"])
+    assert (gpu["status"] == cpu["status"]).mean() >= AGREE_FRACTION
+    frac = agreement(gpu["radiance"], cpu["radiance"])
+    frac_xyz = agreement(gpu["xyz"], cpu["xyz"])
+    exact = float(np.mean(np.all(gpu["radiance"] == cpu["radiance"], axis=1)))
+    print(f"[parity] radiance agreement {frac:.6f} xyz {frac_xyz:.6f} bit-exact {exact:.6f}", flush=True)
+    assert frac >= AGREE_FRACTION
+    assert frac_xyz >= AGREE_FRACTION
+    return gpu, cpu
+
+
+def test_cornell_records(rp, orc, cornell):
+    synth, scene = cornell
+    req = rp.request(seed=1, pixel_begin=0, pixel_count=64 * 64, sample_begin=0, sample_count=8)
+    gpu, _ = _records_parity(rp, orc, synth, scene, req)
+    assert (gpu["status"] == 0).all()
+
+
+def test_cornell_records_offset_range(rp, orc, cornell):
+    synth, scene = cornell
+    req = rp.request(seed=7, pixel_begin=1000, pixel_count=777, sample_begin=1000, sample_count=3)
+    _records_parity(rp, orc, synth, scene, req)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_cornell_randomizations(rp, orc, cornell, mode):
+    synth, scene = cornell
+    req = rp.request(seed=3, pixel_begin=0, pixel_count=4096, sample_count=2, randomization=mode)
+    _records_parity(rp, orc, synth, scene, req)
+
+
+def test_cornell_closest_hits(rp, orc, cornell):
+    synth, scene = cornell
+    rng = np.random.default_rng(5)
+    o, d = camera_rays(rng, 200000, (278, 273, -800), (278, 273, 280), 0.6)
+    rays = make_rays(rp, o, d)
+    hits, _ = scene.trace_closest(rays)
+    ref = orc.scene(synth.desc).trace_closest(rays)
+    assert np.array_equal(hits["triangle"], ref["triangle"])
+    assert np.array_equal(hits["t"].view(np.uint32), ref["t"].view(np.uint32))
+
+
+def test_frame_modes_identical(rp, cornell):
+    _, scene = cornell
+    films = []
+    for mode in (rp.MODE_MEGAKERNEL, rp.MODE_WAVEFRONT, rp.MODE_WAVEFRONT_SORTED):
+        img, st = scene.render_frame(rp.frame_config(8, mode=mode, seed=1))
+        print(f"[modes] mode {mode}: {st}", flush=True)
+        assert st["failed_paths"] == 0
+        films.append(img)
+    assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))
+    assert np.array_equal(films[0].view(np.uint32), films[2].view(np.uint32))
+
+
+def test_frame_vs_oracle(rp, orc, cornell):
+    synth, scene = cornell
+    img, _ = scene.render_frame(rp.frame_config(16, mode=rp.MODE_WAVEFRONT_SORTED, seed=1))
+    ref, _ = orc.scene(synth.desc).render_frame(16, seed=1)
+    frac = agreement(img.reshape(-1, 3), ref.reshape(-1, 3))
+    print(f"[frame] per-pixel agreement {frac:.6f}", flush=True)
+    assert frac >= AGREE_FRACTION
+    assert abs(img.mean() / ref.mean() - 1) < 5e-3
+
+
+def test_frame_sample_ranges_compose(rp, cornell):
+    # two 4-spp frames at sample offsets 0 and 4 average to the 8-spp frame
+    _, scene = cornell
+    a, _ = scene.render_frame(rp.frame_config(4, sample_begin=0))
+    b, _ = scene.render_frame(rp.frame_config(4, sample_begin=4))
+    c, _ = scene.render_frame(rp.frame_config(8, sample_begin=0))
+    assert np.allclose((a + b) / 2, c, rtol=1e-5, atol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def terrain(rp, gpu_available):
+    s = rp.Synthetic(rp.SYNTH_TERRAIN, seed=1, width=192, height=108)
+    return s, rp.Scene(s.desc)
+
+
+def test_terrain_closest_hits(rp, orc, terrain):
+    synth, scene = terrain
+    rng = np.random.default_rng(11)
+    o, d = camera_rays(rng, 100000, (0, 0.85, -1.9), (0, 0, 0.05), 0.45)
+    rays = make_rays(rp, o, d)
+    hits, sec = scene.trace_closest(rays)
+    ref = orc.scene(synth.desc).trace_closest(rays)
+    print(f"[terrain] hit fraction {(hits['triangle'] != rp.NONE).mean():.3f}", flush=True)
+    assert np.array_equal(hits["triangle"], ref["triangle"])
+    assert np.array_equal(hits["t"].view(np.uint32), ref["t"].view(np.uint32))
+
+
+def test_terrain_records(rp, orc, terrain):
+    synth, scene = terrain
+    req = rp.request(seed=1, pixel_begin=0, pixel_count=192 * 108, sample_count=2)
+    _records_parity(rp, orc, synth, scene, req)
+
+
+def test_terrain_modes_identical(rp, terrain):
+    _, scene = terrain
+    a, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_MEGAKERNEL))
+    b, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_WAVEFRONT_SORTED, sort_bits=4))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_volume_records(rp, orc, gpu_available):
+    synth = rp.Synthetic(rp.SYNTH_VOLUME, seed=3, width=48, height=48, volume_subdivision=4, hg_g=0.3)
+    scene = rp.Scene(synth.desc)
+    req = rp.request(seed=1, pixel_begin=0, pixel_count=48 * 48, sample_count=4)
+    _records_parity(rp, orc, synth, scene, req)
+
+
+def test_inverse_scene_records(rp, orc, gpu_available):
+    synth = rp.Synthetic(rp.SYNTH_INVERSE, seed=2, width=48, height=48)
+    scene = rp.Scene(synth.desc)
+    req = rp.request(seed=1, pixel_begin=0, pixel_count=48 * 48, sample_count=4)
+    _records_parity(rp, orc, synth, scene, req)
+    a, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_MEGAKERNEL))
+    b, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_WAVEFRONT))
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
