@@ -36,7 +36,19 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="mega,wave,sorted")
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume"])
+    ap.add_argument("--profile", action="store_true",
+                    help="profiling run: frames only (no roofline replay, no CPU leg), then a 1 GiB device "
+                         "copy used to calibrate FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
+
+
+WORKLOADS = {
+    # name: (synthetic kind, width, height, spp, options, workload label)
+    "terrain": (1, 1920, 1080, 16, {}, "terrain-1080p-16spp-depth8"),
+    "cornell": (0, 256, 256, 64, {}, "cornell-256-64spp-depth5"),
+    "volume": (3, 512, 512, 32, {"hg_g": 0.3}, "volume-512-32spp-depth32-g0.3"),
+}
 
 
 def cpu_baseline(synth, width, height, budget):
@@ -81,7 +93,10 @@ def main():
     import rypath_amd as rp
 
     torch.cuda.init()
-    synth = rp.Synthetic(rp.SYNTH_TERRAIN, seed=1, width=args.width, height=args.height)
+    kind, w, h, spp, opts, label = WORKLOADS[args.workload]
+    if args.workload != "terrain":
+        args.width, args.height, args.spp = w, h, spp
+    synth = rp.Synthetic(kind, seed=1, width=args.width, height=args.height, **opts)
     t0 = time.time()
     scene = rp.Scene(synth.desc)
     info = scene.info()
@@ -104,6 +119,14 @@ def main():
         results[name] = {"ms_per_step": dt * 1e3, "mpaths": paths / dt / 1e6, "stats": stats[-1]}
         print(f"[bench] {name}: {dt * 1e3:.1f} ms/frame, {paths / dt / 1e6:.1f} M paths/s, "
               f"rays {stats[-1]['extension_rays'] + stats[-1]['shadow_rays']}", file=sys.stderr, flush=True)
+    if args.profile:
+        src = torch.empty(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("[bench] calibration copy: 1 GiB read + 1 GiB written", file=sys.stderr, flush=True)
+        return
     best = max(results, key=lambda k: results[k]["mpaths"])
     headline = "sorted" if "sorted" in results else best
     r = results[headline]
@@ -124,11 +147,12 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(synth, args.width, args.height, args.cpu_seconds)
     st = r["stats"]
     line = {
-        "metric": "M paths/s @1080p 16spp", "value": round(r["mpaths"], 2), "unit": "M paths/s", "n_gpus": 1,
+        "metric": "M paths/s @1080p 16spp" if args.workload == "terrain" else f"M paths/s ({label})", "value": round(r["mpaths"], 2), "unit": "M paths/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(r["ms_per_step"], 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "terrain-1080p-16spp-depth8", "width": args.width, "height": args.height,
-                   "spp": args.spp, "max_depth": 8, "triangles": synth.d.triangle_count, "mode": headline},
+        "config": {"workload": label, "width": args.width, "height": args.height,
+                   "spp": args.spp, "max_depth": synth.d.max_depth, "triangles": synth.d.triangle_count,
+                   "mode": headline},
         "modes": {k: {"M_paths_s": round(v["mpaths"], 2), "ms_per_frame": round(v["ms_per_step"], 2),
                       "G_rays_s": round((v["stats"]["extension_rays"] + v["stats"]["shadow_rays"]) /
                                         (v["ms_per_step"] * 1e-3) / 1e9, 3)} for k, v in results.items()},
