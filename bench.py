@@ -35,8 +35,9 @@ def parse():
     ap.add_argument("--spp", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--modes", default="mega,wave,sorted")
-    ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume"])
+    ap.add_argument("--modes", default="mega,wave,sorted,pool")
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume", "batch"])
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: frames only (no roofline replay, no CPU leg), then a 1 GiB device "
                          "copy used to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -82,6 +83,74 @@ def cpu_baseline(synth, width, height, budget):
                       f"{build:.1f}s excluded"}
 
 
+def batch_workload(args):
+    """64 seeded uneven scenes in one pool vs the same scenes rendered one after another."""
+    import numpy as np
+    import torch
+
+    import rypath_amd as rp
+
+    rng = np.random.default_rng(2026)
+    synths, items = [], []
+    t0 = time.time()
+    for i in range(args.batch_scenes):
+        tris = int(np.exp(rng.uniform(np.log(5e3), np.log(5e5))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        spp = int(rng.choice([4, 16, 64]))
+        s = rp.Synthetic(rp.SYNTH_BATCH, seed=1000 + i, width=res, height=res, target_triangles=tris)
+        synths.append(s)
+        items.append((rp.Scene(s.desc), spp, 1 + i))
+    build = time.time() - t0
+    total = sum(sc.width * sc.height * spp for sc, spp, _ in items)
+    tris = sum(s.d.triangle_count for s in synths)
+    films = {sc: torch.empty((sc.height, sc.width, 3), dtype=torch.float32, device="cuda") for sc, _, _ in items}
+    out = {}
+
+    def sequential(mode):
+        for _ in range(args.warmup):
+            for sc, spp, seed in items:
+                sc.render_frame_device(rp.frame_config(spp, mode=mode, seed=seed), films[sc].data_ptr())
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            for sc, spp, seed in items:
+                sc.render_frame_device(rp.frame_config(spp, mode=mode, seed=seed), films[sc].data_ptr())
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / args.steps
+
+    for name, mode in (("sequential_plain", rp.MODE_MEGAKERNEL), ("sequential_wavefront", rp.MODE_WAVEFRONT)):
+        dt = sequential(mode)
+        out[name] = {"ms": round(dt * 1e3, 2), "M_paths_s": round(total / dt / 1e6, 2)}
+        print(f"[bench] {name}: {dt * 1e3:.1f} ms, {total / dt / 1e6:.1f} M paths/s", file=sys.stderr, flush=True)
+    batch = rp.Batch(items)
+    t = time.time()
+    cfg, _ = batch.autotune()
+    tune = time.time() - t
+    for name, c in (("batched_persistent", rp.Batch.config(scheduler=rp.SCHED_PERSISTENT, group_size=128)),
+                    ("batched_autotuned", cfg)):
+        for _ in range(args.warmup):
+            batch.render(c, films=False)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            _, st = batch.render(c, films=False)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        out[name] = {"ms": round(dt * 1e3, 2), "M_paths_s": round(total / dt / 1e6, 2), "config": rp.config_dict(c)}
+        print(f"[bench] {name}: {dt * 1e3:.1f} ms, {total / dt / 1e6:.1f} M paths/s", file=sys.stderr, flush=True)
+    best = max(("batched_persistent", "batched_autotuned"), key=lambda k: out[k]["M_paths_s"])
+    line = {
+        "metric": "M paths/s (64 uneven scenes, one pool)", "value": out[best]["M_paths_s"], "unit": "M paths/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": out[best]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch-64-uneven", "scenes": len(items), "paths": total, "triangles": tris,
+                   "max_depth": 8},
+        "modes": out, "batched_vs_sequential_plain": round(out[best]["M_paths_s"] / out["sequential_plain"]["M_paths_s"], 3),
+        "bvh_build_s": round(build, 2), "autotune_s": round(tune, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -93,6 +162,8 @@ def main():
     import rypath_amd as rp
 
     torch.cuda.init()
+    if args.workload == "batch":
+        return batch_workload(args)
     kind, w, h, spp, opts, label = WORKLOADS[args.workload]
     if args.workload != "terrain":
         args.width, args.height, args.spp = w, h, spp
@@ -105,6 +176,23 @@ def main():
     modes = {"mega": rp.MODE_MEGAKERNEL, "wave": rp.MODE_WAVEFRONT, "sorted": rp.MODE_WAVEFRONT_SORTED}
     results = {}
     for name in args.modes.split(","):
+        if name == "pool":
+            batch = rp.Batch([(scene, args.spp, 1)])
+            bcfg, _ = batch.autotune()
+            for _ in range(args.warmup):
+                batch.render(bcfg, films=False)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                _, bst = batch.render(bcfg, films=False)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / args.steps
+            results[name] = {"ms_per_step": dt * 1e3, "mpaths": paths / dt / 1e6,
+                             "stats": dict(bst, trace_seconds=0.0, extend_launches=0),
+                             "config": rp.config_dict(bcfg)}
+            print(f"[bench] pool {rp.config_dict(bcfg)}: {dt * 1e3:.1f} ms/frame, {paths / dt / 1e6:.1f} M paths/s",
+                  file=sys.stderr, flush=True)
+            continue
         mode = modes[name]
         cfg = rp.frame_config(args.spp, mode=mode, seed=1)
         for _ in range(args.warmup):
@@ -128,18 +216,19 @@ def main():
         print("[bench] calibration copy: 1 GiB read + 1 GiB written", file=sys.stderr, flush=True)
         return
     best = max(results, key=lambda k: results[k]["mpaths"])
-    headline = "sorted" if "sorted" in results else best
+    headline = best
+    frame_mode = max((k for k in results if k in ("wave", "sorted")), key=lambda k: results[k]["mpaths"], default=None)
     r = results[headline]
     roofline = None
-    if headline in ("wave", "sorted"):
-        cfg = rp.frame_config(args.spp, mode=modes[headline], seed=1, flags=rp.FRAME_COUNT_TRAFFIC)
+    if frame_mode is not None:
+        cfg = rp.frame_config(args.spp, mode=modes[frame_mode], seed=1, flags=rp.FRAME_COUNT_TRAFFIC)
         cnt = scene.render_frame_device(cfg, film.data_ptr())
-        st = r["stats"]
+        st = results[frame_mode]["stats"]
         alg = RAY_BYTES * cnt["extension_rays"] + NODE_BYTES * cnt["node_fetches"] + TRI_BYTES * cnt["triangle_tests"]
         achieved = alg / st["trace_seconds"] / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_wf_extend (closest hit)",
+                    "kernel": f"k_wf_extend (closest hit, {frame_mode} frame)",
                     "per_ray_bytes": round(alg / cnt["extension_rays"], 1),
                     "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),
                     "tris_per_ray": round(cnt["triangle_tests"] / cnt["extension_rays"], 2),
@@ -155,7 +244,10 @@ def main():
                    "mode": headline},
         "modes": {k: {"M_paths_s": round(v["mpaths"], 2), "ms_per_frame": round(v["ms_per_step"], 2),
                       "G_rays_s": round((v["stats"]["extension_rays"] + v["stats"]["shadow_rays"]) /
-                                        (v["ms_per_step"] * 1e-3) / 1e9, 3)} for k, v in results.items()},
+                                        (v["ms_per_step"] * 1e-3) / 1e9, 3),
+                      **({"autotuned": v["config"]} if "config" in v else {})} for k, v in results.items()},
+        "batched_vs_plain": (round(results["pool"]["mpaths"] / results["mega"]["mpaths"], 3)
+                             if "pool" in results and "mega" in results else None),
         "bvh_build_s": round(info["build_seconds"], 2), "upload_s": round(info["upload_seconds"], 2),
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -275,13 +275,18 @@ typedef struct rp_batch_item {
     uint64_t seed;
 } rp_batch_item;
 
+typedef enum rp_scheduler {
+    RP_SCHED_PERSISTENT = 0, /* one persistent megakernel launch pulling paths from a device work counter */
+    RP_SCHED_WAVEFRONT = 1   /* device-resident wavefront pool: refill/extend/shade/shadow/retire */
+} rp_scheduler;
+
 typedef struct rp_batch_config {
-    uint32_t capacity;    /* paths in flight, 0 = autotuned/default */
-    uint32_t group_size;  /* 0 = autotuned/default */
-    uint32_t sort_bits;   /* 0 = autotuned/default */
-    uint32_t use_graph;   /* nonzero: replay the bounce loop as a captured HIP graph */
+    uint32_t capacity;    /* wavefront paths in flight, 0 = default */
+    uint32_t group_size;  /* threads per workgroup (64, 128, 256), 0 = default */
+    uint32_t sort_bits;   /* material sort-key width (0 = no sort) */
+    uint32_t use_graph;   /* nonzero: the wavefront bounce loop is replayed as one captured HIP graph */
     uint32_t randomization;
-    uint32_t pad;
+    uint32_t scheduler;   /* rp_scheduler */
 } rp_batch_config;
 
 typedef struct rp_batch_stats {

@@ -753,9 +753,10 @@ static uint32_t outgoing_medium(const rp_object &o, uint32_t current, V3 n, V3 w
 // Visibility with transmittance through null interfaces and homogeneous media.
 // Returns false when blocked by a scattering surface.
 static bool transmittance(const Ctx &c, V3 o, V3 d, float tmax, uint32_t medium, const Wavelengths &wl,
-                          float tr[4], uint32_t &crossings) {
+                          float tr[4], uint32_t &crossings, float *mlen = nullptr, uint32_t mlen_medium = RP_NONE) {
     const Scene &s = *c.s;
     for (int k = 0; k < 4; ++k) tr[k] = 1.0f;
+    if (mlen) *mlen = 0.0f;
     for (;;) {
         Hit h = closest(s, o, d, tmax);
         float seg = h.tri == RP_NONE ? tmax : h.t;
@@ -763,6 +764,7 @@ static bool transmittance(const Ctx &c, V3 o, V3 d, float tmax, uint32_t medium,
             MediumCoeff mc = medium_coeff(s, medium, wl);
             float len = length(d);
             for (int k = 0; k < 4; ++k) tr[k] = tr[k] * exp_f(-mc.st[k] * (seg * len));
+            if (mlen && medium == mlen_medium) *mlen = *mlen + seg * len;
         }
         if (h.tri == RP_NONE) return true;
         const auto &obj = s.objects[s.tri_object[h.tri]];
@@ -890,9 +892,153 @@ static float env_pdf(const Scene &s) {
     return s.lights.back().pmf * kInv4Pi;
 }
 
+// Forward-mode derivative of the detached estimator (independent of the
+// device's reverse-mode replay): d log(beta_k) / d theta_p is carried along
+// the path and every contribution C_k adds a_k * C_k * d log C_k / d theta_p.
+struct NoTrack {
+    static constexpr bool on = false;
+    void begin(const Wavelengths &) {}
+    void medium_factor(uint32_t, const Wavelengths &, float, const float *) {}
+    void medium_pass(uint32_t, const Wavelengths &, float) {}
+    void add(const float *) {}
+    void save_direct() {}
+    void continuation(const rp_material &, uint32_t, const Lane *, V3, V3, const Wavelengths &) {}
+    void nee(const rp_material &, uint32_t, const Lane *, V3, V3, const Wavelengths &, const float *, float) {}
+    void nee_medium(const float *, float) {}
+};
+
+struct GradSpec {
+    std::vector<int32_t> refl_base, rough, sigma_base;
+    std::vector<uint32_t> refl_bins, sigma_bins;
+    uint32_t nparams = 0;
+    uint32_t sigma_medium = RP_NONE;
+};
+
+static uint32_t bin_of(float lambda, uint32_t bins) {
+    int i = int(std::floor(lambda + 0.5f)) - 360;
+    i = std::min(std::max(i, 0), 470);
+    return std::min(bins - 1, uint32_t(i) * bins / 471u);
+}
+
+// d log f / d r of an isotropic rough conductor, alpha = sqrt(r) (remapped) or r.
+static double conductor_dlogf(const Lane &l, const rp_material &m, V3 wo, V3 wi) {
+    if (l.smooth || !(wo.z * wi.z > 0)) return 0.0;
+    double r = m.roughness_u;
+    double a_raw = m.remap_roughness ? std::sqrt(r) : r;
+    if (a_raw < 1e-4) return 0.0;
+    double a = l.ax;
+    double hx = double(wo.x) + wi.x, hy = double(wo.y) + wi.y, hz = double(wo.z) + wi.z;
+    double hn = std::sqrt(hx * hx + hy * hy + hz * hz);
+    hx /= hn, hy /= hn, hz /= hn;
+    // D = 1 / (pi a^2 (s/a^2 + c^2)^2),  s = sin^2, c = cos of the half vector
+    double sn = hx * hx + hy * hy, t = sn / (a * a) + hz * hz;
+    double dlogD = -2.0 / a + 4.0 * sn / (a * a * a * t);
+    auto lam = [&](V3 w, double &dl) {
+        double tan2 = (double(w.x) * w.x + double(w.y) * w.y) / (double(w.z) * w.z);
+        double root = std::sqrt(1.0 + a * a * tan2);
+        dl = 0.5 * a * tan2 / root;
+        return 0.5 * (root - 1.0);
+    };
+    double dlo, dli;
+    double lo = lam(wo, dlo), li = lam(wi, dli);
+    double dlogf = dlogD - (dlo + dli) / (1.0 + lo + li);
+    return dlogf * (m.remap_roughness ? 0.5 / a : 1.0);
+}
+
+struct ForwardGrad {
+    static constexpr bool on = true;
+    const GradSpec *g = nullptr;
+    double a[4] = {0, 0, 0, 0};
+    std::vector<double> dlb, ddlb; // [4][P] current and pre-continuation log-derivatives
+    std::vector<double> *grad = nullptr;
+    // pending NEE local derivative
+    int32_t nee_idx[4];
+    double nee_dlog[4];
+    uint32_t P() const { return g->nparams; }
+    void begin(const Wavelengths &) {
+        dlb.assign(4 * size_t(P()), 0.0);
+        ddlb = dlb;
+    }
+    void bump(int k, int32_t idx, double v) {
+        if (idx >= 0) dlb[size_t(k) * P() + idx] += v;
+    }
+    void medium_factor(uint32_t m, const Wavelengths &wl, float t, const float *ss) {
+        if (g->sigma_base[m] < 0) return;
+        for (int k = 0; k < 4; ++k)
+            if (ss[k] > 0) bump(k, g->sigma_base[m] + int32_t(bin_of(wl.lambda[k], g->sigma_bins[m])), 1.0 / ss[k] - t);
+    }
+    void medium_pass(uint32_t m, const Wavelengths &wl, float d) {
+        if (g->sigma_base[m] < 0) return;
+        for (int k = 0; k < 4; ++k) bump(k, g->sigma_base[m] + int32_t(bin_of(wl.lambda[k], g->sigma_bins[m])), -double(d));
+    }
+    void add(const float *v) { // contribution carried by the current throughput
+        for (int k = 0; k < 4; ++k)
+            if (v[k] != 0.0f)
+                for (uint32_t p = 0; p < P(); ++p) (*grad)[p] += a[k] * double(v[k]) * dlb[size_t(k) * P() + p];
+    }
+    void save_direct() { ddlb = dlb; }
+    void local(const rp_material &mat, uint32_t mid, const Lane *lanes, V3 wo, V3 wi, const Wavelengths &wl,
+               int32_t idx[4], double dl[4]) {
+        for (int k = 0; k < 4; ++k) idx[k] = -1, dl[k] = 0.0;
+        if (mat.kind == RP_MATERIAL_DIFFUSE && g->refl_base[mid] >= 0) {
+            for (int k = 0; k < 4; ++k)
+                if (lanes[k].r > 0 && wo.z * wi.z > 0) {
+                    idx[k] = g->refl_base[mid] + int32_t(bin_of(wl.lambda[k], g->refl_bins[mid]));
+                    dl[k] = 1.0 / lanes[k].r;
+                }
+        } else if (mat.kind == RP_MATERIAL_CONDUCTOR && g->rough[mid] >= 0) {
+            for (int k = 0; k < 4; ++k) {
+                idx[k] = g->rough[mid];
+                dl[k] = conductor_dlogf(lanes[k], mat, wo, wi);
+            }
+        }
+    }
+    void continuation(const rp_material &mat, uint32_t mid, const Lane *lanes, V3 wo, V3 wi, const Wavelengths &wl) {
+        int32_t idx[4];
+        double dl[4];
+        local(mat, mid, lanes, wo, wi, wl, idx, dl);
+        for (int k = 0; k < 4; ++k) bump(k, idx[k], dl[k]);
+    }
+    // NEE contribution from a surface: pre-continuation derivatives + BSDF at the light direction + shadow medium.
+    void nee(const rp_material &mat, uint32_t mid, const Lane *lanes, V3 wo, V3 wi, const Wavelengths &wl,
+             const float *c, float mlen) {
+        int32_t idx[4];
+        double dl[4];
+        local(mat, mid, lanes, wo, wi, wl, idx, dl);
+        finish_nee(wl, c, mlen, idx, dl, ddlb);
+    }
+    void nee_medium(const float *c, float mlen) {
+        int32_t idx[4] = {-1, -1, -1, -1};
+        double dl[4] = {0, 0, 0, 0};
+        Wavelengths wl = cur_wl;
+        finish_nee(wl, c, mlen, idx, dl, dlb);
+    }
+    void finish_nee(const Wavelengths &wl, const float *c, float mlen, const int32_t idx[4], const double dl[4],
+                    const std::vector<double> &base) {
+        for (int k = 0; k < 4; ++k) {
+            if (c[k] == 0.0f) continue;
+            double ak = a[k] * double(c[k]);
+            for (uint32_t p = 0; p < P(); ++p) (*grad)[p] += ak * base[size_t(k) * P() + p];
+            if (idx[k] >= 0) (*grad)[idx[k]] += ak * dl[k];
+            if (g->sigma_medium != RP_NONE && mlen > 0) {
+                int32_t si = g->sigma_base[g->sigma_medium] + int32_t(bin_of(wl.lambda[k], g->sigma_bins[g->sigma_medium]));
+                (*grad)[si] += ak * (-double(mlen));
+            }
+        }
+    }
+    Wavelengths cur_wl{};
+};
+
 // One complete hero path: src/render.metal:238-312 (begin_sample, primary_ray)
 // followed by the event loop of shaders/packet_pt_event_body.inc.
+template <class T>
+static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &trk);
 static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
+    NoTrack nt;
+    return trace_path_t(c, pixel, sample, nt);
+}
+template <class T>
+static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &trk) {
     const Scene &s = *c.s;
     PathOut out;
     const uint32_t W = s.d.camera.width, H = s.d.camera.height;
@@ -906,6 +1052,10 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
     out.w1 = w1;
     Wavelengths wl = sample_wavelengths(sobol_uniform(w2), s.d.hero_wavelengths);
     out.wl = wl;
+    trk.begin(wl);
+    if constexpr (T::on) trk.cur_wl = wl;
+    uint32_t mlen_medium = RP_NONE;
+    if constexpr (T::on) mlen_medium = trk.g->sigma_medium;
     float ux = sobol_camera_uniform(w0, s.m, px), uy = sobol_camera_uniform(w1, s.m, py);
     float fx = (float(px) + ux) / float(W), fy = (float(py) + uy) / float(H);
     const rp_camera &cam = s.d.camera;
@@ -950,6 +1100,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                     pdf[k] = mc.st[k] * tr;
                 }
                 if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                trk.medium_factor(medium, wl, tf, mc.ss);
                 for (int k = 0; k < 4; ++k) prev_pdf[k] = prev_pdf[k] * pdf[k];
                 V3 mp = o + tf * d;
                 ++count;
@@ -996,7 +1147,8 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                             tm = 1.0f - 1e-4f;
                         }
                         ++out.shadow_rays;
-                        if (transmittance(c, mp, sd, tm, medium, wl, tr, cr)) {
+                        float mlen = 0.0f;
+                        if (transmittance(c, mp, sd, tm, medium, wl, tr, cr, &mlen, mlen_medium)) {
                             for (int k = 0; k < 4; ++k) val[k] = val[k] * tr[k];
                             if (!ls.delta) {
                                 float lp[4] = {ls.pdf, ls.pdf, ls.pdf, ls.pdf};
@@ -1006,6 +1158,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                                 for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
                             }
                             add(val);
+                            trk.nee_medium(val, mlen);
                         }
                     }
                 }
@@ -1023,6 +1176,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                     pdf[k] = tr;
                 }
                 if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                trk.medium_pass(medium, wl, seg);
                 for (int k = 0; k < 4; ++k) prev_pdf[k] = prev_pdf[k] * pdf[k];
             } else {
                 break; // infinite medium, nothing reached
@@ -1042,6 +1196,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                     for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
                 }
                 add(val);
+                trk.add(val);
             }
             break;
         }
@@ -1076,6 +1231,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                     for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
                 }
                 add(val);
+                trk.add(val);
             }
         }
         if (count == capacity) break;
@@ -1090,6 +1246,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
         std::memcpy(direct_density, density, sizeof density);
         bool direct_allowed = hero_continue(beta, eta_scale, rr_enabled, count - 1, sobol_uniform(dr[3]));
         std::memcpy(direct_beta, beta, sizeof beta);
+        trk.save_direct();
         bool cont = direct_allowed;
         if (cont) {
             float uc = sobol_uniform(dr[0]), u0 = sobol_uniform(dr[1]), u1 = sobol_uniform(dr[2]);
@@ -1121,6 +1278,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                     out.status = RP_PATH_INVALID_ARITHMETIC;
                     cont = false;
                 } else {
+                    if (!hs.delta) trk.continuation(mat, obj.material, lanes, wol, hs.wi, wl);
                     std::memcpy(prev_pdf, pdf, sizeof pdf);
                     if (hs.transmission && rr_enabled)
                         for (int k = 0; k < 4; ++k) eta_scale[k] = eta_scale[k] * (etas[k] * etas[k]);
@@ -1164,7 +1322,8 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                     uint32_t cr = 0;
                     uint32_t nm = outgoing_medium(obj, medium, sp.n, ls.wi);
                     ++out.shadow_rays;
-                    if (transmittance(c, from, sd, tm, nm, wl, tr, cr)) {
+                    float mlen = 0.0f;
+                    if (transmittance(c, from, sd, tm, nm, wl, tr, cr, &mlen, mlen_medium)) {
                         for (int k = 0; k < 4; ++k) val[k] = val[k] * tr[k];
                         if (!ls.delta) {
                             float lp[4] = {ls.pdf, ls.pdf, ls.pdf, ls.pdf};
@@ -1173,6 +1332,7 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
                             for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
                         }
                         add(val);
+                        trk.nee(mat, obj.material, lanes, wol, wil, wl, val, mlen);
                     }
                 }
             }
@@ -1266,6 +1426,12 @@ void or_hg_sample(const float wo[3], float g, float u0, float u1, float wi[3], f
     wi[1] = r.y;
     wi[2] = r.z;
 }
+double or_conductor_dlogf(const rp_material *m, float lambda, const float wo[3], const float wi[3]) {
+    Scene s;
+    Lane l = prepare_lane(s, *m, lambda);
+    return conductor_dlogf(l, *m, {wo[0], wo[1], wo[2]}, {wi[0], wi[1], wi[2]});
+}
+
 int or_intersect_triangle(const float o[3], const float d[3], float tmax, const float p[9], float out[4]) {
     float t, b0, b1, b2;
     bool hit = intersect_triangle({o[0], o[1], o[2]}, {d[0], d[1], d[2]}, tmax, {p[0], p[1], p[2]},
@@ -1421,6 +1587,76 @@ int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sa
     work();
     for (auto &t : pool) t.join();
     if (rays) *rays = ray_count.load();
+    return 0;
+}
+
+// Reference gradient of sum_pixels dl_dxyz . XYZ (forward-mode, double accumulation).
+// Parameter layout as rp_backward: binned spectra consecutive with bin 0 first.
+int or_gradient(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                const rp_param *params, uint32_t count, const float *dl_dxyz, double *grad_out, uint32_t threads) {
+    auto *s = static_cast<Scene *>(sp);
+    GradSpec g;
+    g.nparams = count;
+    g.refl_base.assign(s->materials.size() + 1, -1);
+    g.rough.assign(s->materials.size() + 1, -1);
+    g.refl_bins.assign(s->materials.size() + 1, 0);
+    g.sigma_base.assign(s->media.size() + 1, -1);
+    g.sigma_bins.assign(s->media.size() + 1, 0);
+    for (uint32_t i = 0; i < count; ++i) {
+        const rp_param &p = params[i];
+        if (p.kind == RP_PARAM_ROUGHNESS) g.rough[p.target] = int32_t(i);
+        else if (p.kind == RP_PARAM_REFLECTANCE_BIN && p.bin == 0) g.refl_base[p.target] = int32_t(i), g.refl_bins[p.target] = p.bins;
+        else if (p.kind == RP_PARAM_SIGMA_S_BIN && p.bin == 0) {
+            g.sigma_base[p.target] = int32_t(i), g.sigma_bins[p.target] = p.bins;
+            g.sigma_medium = p.target;
+        }
+    }
+    Ctx c{s, seed, randomization, 1024u, 4096u};
+    const uint32_t npx = s->d.camera.width * s->d.camera.height;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::vector<double>> partial(threads, std::vector<double>(count, 0.0));
+    std::atomic<uint32_t> next{0};
+    auto work = [&](uint32_t tid) {
+        ForwardGrad fg;
+        fg.g = &g;
+        fg.grad = &partial[tid];
+        for (;;) {
+            uint32_t b = next.fetch_add(16);
+            if (b >= npx) break;
+            for (uint32_t pixel = b; pixel < std::min(npx, b + 16); ++pixel) {
+                for (uint32_t k = 0; k < spp; ++k) {
+                    // dLoss/dL_k for this path needs its wavelengths: trace once to get them
+                    uint64_t sample = sample_begin + k;
+                    uint32_t px = pixel % s->d.camera.width, py = pixel / s->d.camera.width;
+                    uint64_t index = sobol_index(s->m, px, py, sample);
+                    uint32_t w2 = sobol_word(index, 2, randomization, uint32_t(seed));
+                    Wavelengths wl = sample_wavelengths(sobol_uniform(w2), s->d.hero_wavelengths);
+                    for (int l = 0; l < 4; ++l) {
+                        fg.a[l] = 0.0;
+                        if (wl.pdf[l] > 0) {
+                            int i = std::min(std::max(int(std::floor(wl.lambda[l] + 0.5f)) - 360, 0), 470);
+                            const auto &cie = cie_table().xyz[i];
+                            double d = double(dl_dxyz[3 * size_t(pixel)]) * cie[0] + double(dl_dxyz[3 * size_t(pixel) + 1]) * cie[1] +
+                                       double(dl_dxyz[3 * size_t(pixel) + 2]) * cie[2];
+                            fg.a[l] = d / double(wl.pdf[l]) / (4.0 * spp);
+                        }
+                    }
+                    std::vector<double> before = partial[tid];
+                    PathOut po = trace_path_t(c, pixel, sample, fg);
+                    if (po.status != RP_PATH_SUCCESS) partial[tid] = before; // failed paths contribute nothing
+                }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &t : pool) t.join();
+    for (uint32_t p = 0; p < count; ++p) {
+        double acc = 0;
+        for (uint32_t t = 0; t < threads; ++t) acc += partial[t][p];
+        grad_out[p] = acc;
+    }
     return 0;
 }
 

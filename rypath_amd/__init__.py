@@ -25,6 +25,7 @@ RANDOMIZE_NONE, RANDOMIZE_PERMUTE, RANDOMIZE_FASTOWEN, RANDOMIZE_OWEN = range(4)
 MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_WAVEFRONT_SORTED = range(3)
 SYNTH_CORNELL, SYNTH_TERRAIN, SYNTH_BATCH, SYNTH_VOLUME, SYNTH_INVERSE = range(5)
 PARAM_REFLECTANCE_BIN, PARAM_ROUGHNESS, PARAM_SIGMA_S_BIN = range(3)
+SCHED_PERSISTENT, SCHED_WAVEFRONT = range(2)
 u32, u64, f32, f64 = C.c_uint32, C.c_uint64, C.c_float, C.c_double
 
 
@@ -116,7 +117,7 @@ class BatchItem(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("capacity", u32), ("group_size", u32), ("sort_bits", u32), ("use_graph", u32),
-                ("randomization", u32), ("pad", u32)]
+                ("randomization", u32), ("scheduler", u32)]
 
 
 class BatchStats(C.Structure):
@@ -291,6 +292,24 @@ class Scene:
                                                                       C.byref(st)))
         return {f: getattr(st, f) for f, _ in FrameStats._fields_}
 
+    def set_params(self, params, values):
+        """Writes parameter values (rp_set_params); params: list of (kind, target, bin, bins)."""
+        arr = params_array(params)
+        vals = np.ascontiguousarray(values, dtype=np.float32)
+        assert len(vals) == len(arr)
+        _check("rp_set_params", _lib.rp_set_params(self._h, arr, len(arr), vals.ctypes.data_as(C.POINTER(f32))))
+
+    def backward(self, cfg, params, dl_dxyz):
+        """Path-replay adjoint: d(sum dl_dxyz * XYZ)/d params for the frame `cfg` (rp_backward)."""
+        arr = params_array(params)
+        dl = np.ascontiguousarray(dl_dxyz, dtype=np.float32).reshape(-1)
+        assert dl.size == self.width * self.height * 3
+        g = np.zeros(len(arr), np.float32)
+        st = FrameStats()
+        _check("rp_backward", _lib.rp_backward(self._h, C.byref(cfg), arr, len(arr), dl.ctypes.data_as(C.POINTER(f32)),
+                                                g.ctypes.data_as(C.POINTER(f32)), C.byref(st)))
+        return g, {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
     def trace_closest(self, rays):
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         hits = np.zeros(len(rays), dtype=HIT_DTYPE)
@@ -306,3 +325,82 @@ class Scene:
         _check("rp_trace_any", _lib.rp_trace_any(self._h, rays.ctypes.data_as(C.c_void_p), len(rays),
                                                   occ.ctypes.data_as(C.c_void_p), C.byref(sec)))
         return occ, sec.value
+
+
+class Batch:
+    """Uneven multi-scene work pool (rp_batch_*): one device-resident pool for all items."""
+
+    def __init__(self, items):
+        self.items = list(items)  # (Scene, spp, seed)
+        arr = (BatchItem * len(self.items))()
+        for i, (scene, spp, seed) in enumerate(self.items):
+            arr[i].scene, arr[i].spp, arr[i].seed = scene.handle, spp, seed
+        h = C.c_void_p()
+        _check("rp_batch_create", _lib.rp_batch_create(arr, len(self.items), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rp_batch_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def config(scheduler=SCHED_WAVEFRONT, capacity=0, group_size=0, sort_bits=0, use_graph=1,
+               randomization=RANDOMIZE_FASTOWEN):
+        c = BatchConfig()
+        c.scheduler, c.capacity, c.group_size, c.sort_bits = scheduler, capacity, group_size, sort_bits
+        c.use_graph, c.randomization = use_graph, randomization
+        return c
+
+    def render(self, cfg, films=True):
+        outs = [np.zeros((s.height, s.width, 3), np.float32) for s, _, _ in self.items] if films else None
+        ptrs = (C.c_void_p * len(self.items))()
+        if films:
+            for i, o in enumerate(outs):
+                ptrs[i] = o.ctypes.data
+        st = BatchStats()
+        _check("rp_batch_render", _lib.rp_batch_render(self._h, C.byref(cfg), ptrs, C.byref(st)))
+        return outs, {f: getattr(st, f) for f, _ in BatchStats._fields_ if f != "pad"}
+
+    def autotune(self, randomization=RANDOMIZE_FASTOWEN):
+        c = BatchConfig()
+        c.randomization = randomization
+        sec = f64()
+        _check("rp_autotune", _lib.rp_autotune(self._h, C.byref(c), C.byref(sec)))
+        return c, sec.value
+
+
+def config_dict(cfg):
+    return {f: getattr(cfg, f) for f, _ in cfg._fields_ if f != "pad"}
+
+
+def params_array(params):
+    arr = (Param * max(len(params), 1))()
+    for i, (kind, target, b, bins) in enumerate(params):
+        arr[i].kind, arr[i].target, arr[i].bin, arr[i].bins = kind, target, b, bins
+    return arr
+
+
+def inverse_params(synth, bins=16, sigma_bins=8):
+    """The 58 parameters of the inverse-rendering Cornell box (RP_SYNTH_INVERSE):
+    reflectance bins of the three wall materials, roughness of the two box
+    conductors, sigma_s bins of the fog."""
+    d = synth.d
+    params = []
+    for m in range(3):
+        params += [(PARAM_REFLECTANCE_BIN, m, b, bins) for b in range(bins)]
+    params += [(PARAM_ROUGHNESS, 3, 0, 0), (PARAM_ROUGHNESS, 4, 0, 0)]
+    params += [(PARAM_SIGMA_S_BIN, 0, b, sigma_bins) for b in range(sigma_bins)]
+    values = []
+    spectra = np.ctypeslib.as_array(d.spectra, shape=(d.spectra_count,))
+    for m in range(3):
+        t = d.materials[m].reflectance.table
+        for b in range(bins):
+            i = next(i for i in range(471) if i * bins // 471 == b)
+            values.append(float(spectra[t + i]))
+    values += [d.materials[3].roughness_u, d.materials[4].roughness_u]
+    t = d.media[0].sigma_s.table
+    for b in range(sigma_bins):
+        i = next(i for i in range(471) if i * sigma_bins // 471 == b)
+        values.append(float(spectra[t + i]))
+    return params, np.array(values, np.float32)

@@ -1,0 +1,33 @@
+"""PyTorch autograd harness over the C ABI's forward render and path-replay adjoint.
+
+RenderXYZ.apply(values, scene, params, cfg) renders the raw XYZ film (H, W, 3)
+on the GPU with the parameter values written into the scene (rp_set_params +
+rp_render_frame_device); its backward calls rp_backward with the incoming
+dLoss/dXYZ image and returns dLoss/dvalues.
+"""
+import numpy as np
+import torch
+
+import rypath_amd as rp
+
+
+class RenderXYZ(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, values, scene, params, cfg):
+        scene.set_params(params, values.detach().cpu().numpy().astype(np.float32))
+        film = torch.empty((scene.height, scene.width, 3), dtype=torch.float32, device="cuda")
+        scene.render_frame_device(cfg, film.data_ptr())
+        torch.cuda.synchronize()
+        ctx.scene, ctx.params, ctx.cfg = scene, params, cfg
+        ctx.device = values.device
+        return film
+
+    @staticmethod
+    def backward(ctx, grad_film):
+        dl = grad_film.detach().float().contiguous().cpu().numpy()
+        g, _ = ctx.scene.backward(ctx.cfg, ctx.params, dl)
+        return torch.from_numpy(g).to(ctx.device), None, None, None
+
+
+def render(values, scene, params, cfg):
+    return RenderXYZ.apply(values, scene, params, cfg)

@@ -987,9 +987,12 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, flo
 }
 
 // Visibility with transmittance through null interfaces and media.
+// With `mlen` the length travelled inside media is accumulated (adjoint).
 RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uint32_t medium, const Wavelengths &wl,
-                          float tr[4], uint32_t boundary_limit) {
+                          float tr[4], uint32_t boundary_limit, float *mlen = nullptr,
+                          uint32_t mlen_medium = RP_NONE) {
     for (int k = 0; k < 4; ++k) tr[k] = 1.0f;
+    if (mlen) *mlen = 0.0f;
     if (!S.has_interfaces) {
         Hit h;
         if (traverse<true>(S, st, o, d, tmax, h)) return false;
@@ -997,6 +1000,7 @@ RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uin
             MediumCoeff mc = medium_coeff(S, medium, wl);
             float len = length(d);
             for (int k = 0; k < 4; ++k) tr[k] = tr[k] * exp_f(-mc.st[k] * (tmax * len));
+            if (mlen && medium == mlen_medium) *mlen = *mlen + tmax * len;
         }
         return true;
     }
@@ -1009,6 +1013,7 @@ RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uin
             MediumCoeff mc = medium_coeff(S, medium, wl);
             float len = length(d);
             for (int k = 0; k < 4; ++k) tr[k] = tr[k] * exp_f(-mc.st[k] * (seg * len));
+            if (mlen && medium == mlen_medium) *mlen = *mlen + seg * len;
         }
         if (!hit) return true;
         uint32_t obj = __float_as_uint(S.tris[3 * h.prim + 1].w);

@@ -13,6 +13,8 @@
 #include "rp_internal.hpp"
 #include "rp_path.h"
 #include "rp_wavefront.h"
+#include "rp_batch.h"
+#include "rp_adjoint.h"
 
 namespace rp {
 
@@ -605,30 +607,497 @@ int rp_trace_any(rp_scene *s, const rp_ray *rays, uint64_t count, uint8_t *occlu
 
 } // extern "C"
 
-// Entry points whose implementation lives in later translation units are
-// declared in include/rp_api.h; the ones below are not available yet.
+// ============================================================== batches
+struct rp_batch {
+    std::vector<rp_batch_item> items;
+    std::vector<uint32_t> order;      // issue order (largest estimated cost first)
+    std::vector<rpd::BatchScene> host_scenes;
+    std::vector<uint64_t> offsets;    // per item offset (floats) into xyz
+    uint64_t total = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<rpd::BatchScene> scenes;
+    DevBuf<float> xyz;
+    DevBuf<double> film;
+    DevBuf<float> film_out;
+    DevBuf<unsigned long long> work;
+    // wavefront pool
+    rpd::WavefrontBuffers wf;
+    DevBuf<uint32_t> sid, q0, q1, qsorted, ring, retire;
+    DevBuf<rpd::PoolCounters> counters;
+    rpd::PoolCounters *host_counters = nullptr;
+    uint32_t capacity = 0;
+    hipGraphExec_t graph = nullptr;
+    uint64_t graph_key = ~0ull;
+    ~rp_batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (host_counters) (void)hipHostFree(host_counters);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int cu_count(int device) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    return n;
+}
+
+template <int BS> void launch_persistent(rp_batch *b, dim3 grid) {
+    hipLaunchKernelGGL(rpd::k_batch_persistent<BS>, grid, dim3(BS), 0, b->stream, b->scenes.p,
+                       uint32_t(b->host_scenes.size()), b->work.p, b->total);
+}
+
+rpd::PoolView pool_view(rp_batch *b) {
+    rpd::PoolView V{};
+    V.B = b->wf.v;
+    V.scenes = b->scenes.p;
+    V.nscenes = uint32_t(b->host_scenes.size());
+    V.capacity = b->capacity;
+    V.sid = b->sid.p;
+    V.q[0] = b->q0.p;
+    V.q[1] = b->q1.p;
+    V.qsorted = b->qsorted.p;
+    V.free_ring = b->ring.p;
+    V.retire = b->retire.p;
+    V.c = b->counters.p;
+    return V;
+}
+
+template <int BS> void pool_iteration(rp_batch *b, const rpd::PoolView &V, uint32_t p, uint32_t bits, dim3 grid) {
+    hipStream_t st = b->stream;
+    hipLaunchKernelGGL(rpd::k_pool_refill, grid, dim3(256), 0, st, V, p);
+    hipLaunchKernelGGL(rpd::k_pool_commit, dim3(1), dim3(256), 0, st, V, p, b->wf.v.hist, bits);
+    hipLaunchKernelGGL(rpd::k_pool_extend<BS>, grid, dim3(BS), 0, st, V, p);
+    if (bits) {
+        hipLaunchKernelGGL(rpd::k_pool_sort_hist, grid, dim3(256), 0, st, V, p, bits, b->wf.v.hist);
+        hipLaunchKernelGGL(rpd::k_sort_scan, dim3(1), dim3(64), 0, st, bits, b->wf.v.hist);
+        hipLaunchKernelGGL(rpd::k_pool_sort_scatter, grid, dim3(256), 0, st, V, p, bits, b->wf.v.hist);
+    }
+    hipLaunchKernelGGL(rpd::k_pool_shade<BS>, grid, dim3(BS), 0, st, V, p, bits ? 1u : 0u);
+    hipLaunchKernelGGL(rpd::k_pool_shadow<BS>, grid, dim3(BS), 0, st, V);
+    hipLaunchKernelGGL(rpd::k_pool_retire, grid, dim3(256), 0, st, V);
+}
+
+template <int BS> void pool_block(rp_batch *b, const rpd::PoolView &V, uint32_t bits, dim3 grid, int pairs) {
+    for (int k = 0; k < pairs; ++k) {
+        pool_iteration<BS>(b, V, 0, bits, grid);
+        pool_iteration<BS>(b, V, 1, bits, grid);
+    }
+}
+
+void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches) {
+    uint32_t cap = cfg.capacity ? cfg.capacity : (1u << 24);
+    if (cap & (cap - 1)) throw std::runtime_error("wavefront capacity must be a power of two");
+    if (cap > (1u << 28)) throw std::runtime_error("wavefront capacity too large");
+    const uint32_t bs = cfg.group_size ? cfg.group_size : 128;
+    if (bs != 64 && bs != 128 && bs != 256) throw std::runtime_error("group_size must be 64, 128 or 256");
+    const uint32_t bits = cfg.sort_bits;
+    if (bits > 8) throw std::runtime_error("sort_bits must be in [0, 8]");
+    if (cap != b->capacity) {
+        b->wf.ensure(cap);
+        b->sid.alloc(cap), b->q0.alloc(cap), b->q1.alloc(cap), b->qsorted.alloc(cap), b->ring.alloc(cap),
+            b->retire.alloc(cap);
+        b->counters.alloc(1);
+        b->capacity = cap;
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+    }
+    if (!b->host_counters)
+        HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&b->host_counters), sizeof(rpd::PoolCounters),
+                                hipHostMallocDefault));
+    rpd::PoolView V = pool_view(b);
+    const dim3 grid(uint32_t(cu_count(b->device)) * 8);
+    hipLaunchKernelGGL(rpd::k_pool_init, dim3((cap + 255) / 256), dim3(256), 0, b->stream, V, b->total);
+    const int pairs = 4; // 8 bounce iterations per submission
+    auto submit = [&] {
+        if (bs == 64) pool_block<64>(b, V, bits, grid, pairs);
+        else if (bs == 128) pool_block<128>(b, V, bits, grid, pairs);
+        else pool_block<256>(b, V, bits, grid, pairs);
+    };
+    const uint64_t key = (uint64_t(cap) << 20) | (uint64_t(bs) << 8) | bits;
+    if (cfg.use_graph && (b->graph == nullptr || b->graph_key != key)) {
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+        hipGraph_t g;
+        HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        submit();
+        HIP_CHECK(hipStreamEndCapture(b->stream, &g));
+        HIP_CHECK(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        b->graph_key = key;
+    }
+    graph_launches = 0;
+    for (uint64_t guard = 0;; ++guard) {
+        if (cfg.use_graph) {
+            HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
+            ++graph_launches;
+        } else {
+            submit();
+        }
+        HIP_CHECK(hipMemcpyAsync(b->host_counters, b->counters.p, sizeof(rpd::PoolCounters), hipMemcpyDeviceToHost,
+                                 b->stream));
+        HIP_CHECK(hipStreamSynchronize(b->stream));
+        if (b->host_counters->done >= b->total) break;
+        if (guard > (1ull << 22)) throw std::runtime_error("wavefront pool did not drain");
+    }
+}
+
+double batch_render_impl(rp_batch *b, const rp_batch_config &cfg, float **xyz_host, rp_batch_stats *stats) {
+    HIP_CHECK(hipSetDevice(b->device));
+    if (cfg.randomization > RP_RANDOMIZE_OWEN) throw std::runtime_error("unknown randomization");
+    if (cfg.scheduler > RP_SCHED_WAVEFRONT) throw std::runtime_error("unknown scheduler");
+    for (auto &bs : b->host_scenes) bs.P.mode = cfg.randomization;
+    b->scenes.upload(b->host_scenes);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, b->stream));
+    uint32_t launches = 0;
+    unsigned long long rays[3] = {0, 0, 0};
+    if (cfg.scheduler == RP_SCHED_PERSISTENT) {
+        b->work.alloc(4);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, b->stream, b->work.p, 4u);
+        const uint32_t bs = cfg.group_size ? cfg.group_size : 128;
+        const dim3 grid(uint32_t(cu_count(b->device)) * (bs == 64 ? 16 : bs == 128 ? 8 : 4));
+        if (bs == 64) launch_persistent<64>(b, grid);
+        else if (bs == 128) launch_persistent<128>(b, grid);
+        else if (bs == 256) launch_persistent<256>(b, grid);
+        else throw std::runtime_error("group_size must be 64, 128 or 256");
+        HIP_CHECK(hipGetLastError());
+        launches = 1;
+    } else {
+        run_pool(b, cfg, launches);
+    }
+    // per-scene films (atomic-free, samples in increasing order)
+    for (size_t i = 0; i < b->host_scenes.size(); ++i) {
+        const rpd::BatchScene &bs = b->host_scenes[i];
+        const uint32_t npx = bs.npx, spp = uint32_t(bs.paths / npx);
+        hipLaunchKernelGGL(k_film_zero, dim3(uint32_t((6 * uint64_t(npx) + 255) / 256)), dim3(256), 0, b->stream,
+                           6 * uint64_t(npx), b->film.p);
+        hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, b->stream, npx, spp, bs.paths, bs.xyz,
+                           b->film.p);
+        hipLaunchKernelGGL(k_film_finish, dim3((npx + 255) / 256), dim3(256), 0, b->stream, npx, spp, b->film.p,
+                           b->film_out.p);
+        uint32_t item = b->order[i];
+        if (xyz_host && xyz_host[item])
+            HIP_CHECK(hipMemcpyAsync(xyz_host[item], b->film_out.p, 3 * size_t(npx) * sizeof(float),
+                                     hipMemcpyDeviceToHost, b->stream));
+    }
+    HIP_CHECK(hipEventRecord(e1, b->stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (cfg.scheduler == RP_SCHED_PERSISTENT) {
+        unsigned long long w[4];
+        HIP_CHECK(hipMemcpy(w, b->work.p, sizeof w, hipMemcpyDeviceToHost));
+        rays[0] = w[1], rays[1] = w[2], rays[2] = w[3];
+    } else {
+        rays[0] = b->host_counters->rays[0], rays[1] = b->host_counters->rays[1], rays[2] = b->host_counters->rays[2];
+    }
+    if (stats) {
+        stats->seconds = ms * 1e-3;
+        stats->paths = b->total;
+        stats->extension_rays = rays[0];
+        stats->shadow_rays = rays[1];
+        stats->failed_paths = rays[2];
+        stats->graph_launches = launches;
+        stats->pad = 0;
+    }
+    return ms * 1e-3;
+}
+
+} // namespace
+
 extern "C" {
-int rp_batch_create(const rp_batch_item *, uint32_t, rp_batch **out) {
-    if (out) *out = nullptr;
-    set_error("rp_batch_create: not available in this build");
-    return RP_NOT_BUILT;
+
+int rp_batch_create(const rp_batch_item *items, uint32_t count, rp_batch **out) {
+    if (!out || (!items && count) || count == 0) {
+        set_error("rp_batch_create: null argument or empty batch");
+        return RP_INVALID;
+    }
+    *out = nullptr;
+    try {
+        auto b = std::make_unique<rp_batch>();
+        b->items.assign(items, items + count);
+        b->device = items[0].scene ? items[0].scene->device : 0;
+        std::vector<double> cost(count);
+        uint32_t max_npx = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            const rp_batch_item &it = items[i];
+            if (!it.scene || it.spp == 0) throw std::runtime_error("batch item without scene or spp");
+            if (it.scene->device != b->device) throw std::runtime_error("batch scenes on different devices");
+            const rp_camera &cam = it.scene->host.desc.camera;
+            uint64_t paths = uint64_t(cam.width) * cam.height * it.spp;
+            cost[i] = double(paths) * std::log2(2.0 + double(it.scene->host.desc.triangle_count));
+            max_npx = std::max(max_npx, cam.width * cam.height);
+        }
+        b->order.resize(count);
+        for (uint32_t i = 0; i < count; ++i) b->order[i] = i;
+        std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
+        uint64_t floats = 0;
+        for (uint32_t k = 0; k < count; ++k) {
+            const rp_batch_item &it = items[b->order[k]];
+            const rp_camera &cam = it.scene->host.desc.camera;
+            rpd::BatchScene bs{};
+            bs.S = it.scene->dev;
+            bs.P = path_params(it.seed, RP_RANDOMIZE_FASTOWEN, 0, 0);
+            bs.npx = cam.width * cam.height;
+            bs.paths = uint64_t(bs.npx) * it.spp;
+            bs.path_begin = b->total;
+            bs.sample_begin = 0;
+            b->total += bs.paths;
+            b->offsets.push_back(floats);
+            floats += 3 * bs.paths;
+            b->host_scenes.push_back(bs);
+        }
+        HIP_CHECK(hipSetDevice(b->device));
+        HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->xyz.alloc(floats);
+        for (uint32_t k = 0; k < count; ++k) b->host_scenes[k].xyz = b->xyz.p + b->offsets[k];
+        b->film.alloc(6 * size_t(max_npx));
+        b->film_out.alloc(3 * size_t(max_npx));
+        *out = b.release();
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_batch_create", e);
+    }
 }
-int rp_batch_render(rp_batch *, const rp_batch_config *, float **, rp_batch_stats *) {
-    set_error("rp_batch_render: not available in this build");
-    return RP_NOT_BUILT;
+
+int rp_batch_render(rp_batch *b, const rp_batch_config *cfg, float **xyz_host, rp_batch_stats *stats) {
+    if (!b || !cfg) {
+        set_error("rp_batch_render: null argument");
+        return RP_INVALID;
+    }
+    try {
+        batch_render_impl(b, *cfg, xyz_host, stats);
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_batch_render", e);
+    }
 }
-void rp_batch_destroy(rp_batch *) {}
-int rp_autotune(rp_batch *, rp_batch_config *, double *) {
-    set_error("rp_autotune: not available in this build");
-    return RP_NOT_BUILT;
+
+void rp_batch_destroy(rp_batch *b) { delete b; }
+
+// Coordinate-descent autotuner over scheduler, workgroup size, wavefront
+// capacity and sort-key width; every candidate renders the whole batch.
+int rp_autotune(rp_batch *b, rp_batch_config *best, double *best_seconds) {
+    if (!b || !best) {
+        set_error("rp_autotune: null argument");
+        return RP_INVALID;
+    }
+    try {
+        const uint32_t randomization = best->randomization;
+        double best_t = 1e300;
+        rp_batch_config best_cfg{};
+        auto trial = [&](rp_batch_config c) {
+            c.randomization = randomization;
+            double t = batch_render_impl(b, c, nullptr, nullptr);
+            if (t < best_t) {
+                best_t = t;
+                best_cfg = c;
+            }
+        };
+        for (uint32_t g : {64u, 128u, 256u}) trial({0, g, 0, 0, 0, RP_SCHED_PERSISTENT});
+        const uint32_t g0 = 128;
+        for (uint32_t cap : {1u << 22, 1u << 24, 1u << 25}) trial({cap, g0, 0, 1, 0, RP_SCHED_WAVEFRONT});
+        rp_batch_config wf = best_cfg.scheduler == RP_SCHED_WAVEFRONT ? best_cfg : rp_batch_config{1u << 24, g0, 0, 1, 0, RP_SCHED_WAVEFRONT};
+        for (uint32_t g : {64u, 256u}) {
+            rp_batch_config c = wf;
+            c.group_size = g;
+            trial(c);
+        }
+        for (uint32_t bits : {2u, 4u}) {
+            rp_batch_config c = best_cfg.scheduler == RP_SCHED_WAVEFRONT ? best_cfg : wf;
+            c.sort_bits = bits;
+            trial(c);
+        }
+        *best = best_cfg;
+        best->randomization = randomization;
+        if (best_seconds) *best_seconds = best_t;
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_autotune", e);
+    }
 }
-int rp_set_params(rp_scene *, const rp_param *, uint32_t, const float *) {
-    set_error("rp_set_params: not available in this build");
-    return RP_NOT_BUILT;
+
+} // extern "C"
+
+// ============================================================== adjoint
+namespace {
+
+// Writes one parameter value into the owned scene description.
+void set_param(rp_scene *s, const rp_param &p, float v) {
+    HostScene &h = s->host;
+    if (p.kind == RP_PARAM_REFLECTANCE_BIN || p.kind == RP_PARAM_SIGMA_S_BIN) {
+        const rp_spectrum *sp = nullptr;
+        if (p.kind == RP_PARAM_REFLECTANCE_BIN) {
+            if (p.target >= h.materials.size() || h.materials[p.target].kind != RP_MATERIAL_DIFFUSE)
+                throw std::runtime_error("reflectance parameter needs a diffuse material");
+            sp = &h.materials[p.target].reflectance;
+        } else {
+            if (p.target >= h.media.size()) throw std::runtime_error("sigma_s parameter: medium out of range");
+            sp = &h.media[p.target].sigma_s;
+        }
+        if (sp->kind != RP_SPECTRUM_TABLE || sp->scale != 1.0f)
+            throw std::runtime_error("binned parameters need an unscaled 471-sample table spectrum");
+        if (p.bins == 0 || p.bins > 471 || p.bin >= p.bins) throw std::runtime_error("bad parameter bin");
+        for (uint32_t i = 0; i < 471; ++i)
+            if (i * p.bins / 471u == p.bin) h.spectra[sp->table + i] = v;
+    } else if (p.kind == RP_PARAM_ROUGHNESS) {
+        if (p.target >= h.materials.size() || h.materials[p.target].kind == RP_MATERIAL_DIFFUSE)
+            throw std::runtime_error("roughness parameter needs a conductor or dielectric");
+        if (!(v >= 0)) throw std::runtime_error("roughness must be nonnegative");
+        h.materials[p.target].roughness_u = v;
+        h.materials[p.target].roughness_v = v;
+    } else {
+        throw std::runtime_error("unknown parameter kind");
+    }
 }
-int rp_backward(rp_scene *, const rp_frame_config *, const rp_param *, uint32_t, const float *, float *,
-                rp_frame_stats *) {
-    set_error("rp_backward: not available in this build");
-    return RP_NOT_BUILT;
+
+struct GradBuffers {
+    DevBuf<int32_t> refl_base, rough, sigma_base;
+    DevBuf<uint32_t> refl_bins, sigma_bins;
+};
+
+rpd::GradTables grad_tables(rp_scene *s, const rp_param *params, uint32_t count, GradBuffers &gb) {
+    const HostScene &h = s->host;
+    std::vector<int32_t> refl_base(h.materials.size() + 1, -1), rough(h.materials.size() + 1, -1),
+        sigma_base(h.media.size() + 1, -1);
+    std::vector<uint32_t> refl_bins(h.materials.size() + 1, 0), sigma_bins(h.media.size() + 1, 0);
+    uint32_t sigma_medium = RP_NONE;
+    for (uint32_t i = 0; i < count; ++i) {
+        const rp_param &p = params[i];
+        if (p.kind == RP_PARAM_ROUGHNESS) {
+            if (p.target >= h.materials.size() || h.materials[p.target].kind != RP_MATERIAL_CONDUCTOR)
+                throw std::runtime_error("roughness gradients are supported for conductors");
+            rough[p.target] = int32_t(i);
+            continue;
+        }
+        if (p.kind != RP_PARAM_REFLECTANCE_BIN && p.kind != RP_PARAM_SIGMA_S_BIN)
+            throw std::runtime_error("unknown parameter kind");
+        if (p.bins == 0 || p.bins > 471 || p.bin >= p.bins) throw std::runtime_error("bad parameter bin");
+        std::vector<int32_t> &base = p.kind == RP_PARAM_REFLECTANCE_BIN ? refl_base : sigma_base;
+        std::vector<uint32_t> &bins = p.kind == RP_PARAM_REFLECTANCE_BIN ? refl_bins : sigma_bins;
+        size_t limit = p.kind == RP_PARAM_REFLECTANCE_BIN ? h.materials.size() : h.media.size();
+        if (p.target >= limit) throw std::runtime_error("parameter target out of range");
+        // bins of one spectrum must be consecutive parameters in bin order
+        if (p.bin == 0) {
+            base[p.target] = int32_t(i);
+            bins[p.target] = p.bins;
+        } else if (base[p.target] < 0 || bins[p.target] != p.bins || base[p.target] + int32_t(p.bin) != int32_t(i)) {
+            throw std::runtime_error("binned parameters of one spectrum must be consecutive, bin 0 first");
+        }
+        if (p.kind == RP_PARAM_SIGMA_S_BIN) {
+            if (sigma_medium != RP_NONE && sigma_medium != p.target)
+                throw std::runtime_error("sigma_s gradients are supported for one medium");
+            sigma_medium = p.target;
+        }
+        if (p.kind == RP_PARAM_REFLECTANCE_BIN && h.materials[p.target].kind != RP_MATERIAL_DIFFUSE)
+            throw std::runtime_error("reflectance gradients need a diffuse material");
+    }
+    gb.refl_base.upload(refl_base);
+    gb.rough.upload(rough);
+    gb.sigma_base.upload(sigma_base);
+    gb.refl_bins.upload(refl_bins);
+    gb.sigma_bins.upload(sigma_bins);
+    rpd::GradTables G{};
+    G.refl_base = gb.refl_base.p;
+    G.refl_bins = gb.refl_bins.p;
+    G.rough = gb.rough.p;
+    G.sigma_base = gb.sigma_base.p;
+    G.sigma_bins = gb.sigma_bins.p;
+    G.nparams = count;
+    G.sigma_medium = sigma_medium;
+    return G;
 }
+
+} // namespace
+
+extern "C" {
+
+int rp_set_params(rp_scene *s, const rp_param *params, uint32_t count, const float *values) {
+    if (!s || (count && (!params || !values))) {
+        set_error("rp_set_params: null argument");
+        return RP_INVALID;
+    }
+    try {
+        for (uint32_t i = 0; i < count; ++i) set_param(s, params[i], values[i]);
+        HIP_CHECK(hipSetDevice(s->device));
+        s->spectra.upload(s->host.spectra);
+        s->materials.upload(s->host.materials);
+        s->media.upload(s->host.media);
+        HIP_CHECK(hipDeviceSynchronize());
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_set_params", e);
+    }
 }
+
+int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params, uint32_t count,
+                const float *dl_dxyz, float *grad, rp_frame_stats *stats) {
+    if (!s || !cfg || (count && !params) || !dl_dxyz || !grad) {
+        set_error("rp_backward: null argument");
+        return RP_INVALID;
+    }
+    if (count > rpd::kMaxParams) {
+        set_error("rp_backward: at most 256 parameters");
+        return RP_INVALID;
+    }
+    try {
+        HIP_CHECK(hipSetDevice(s->device));
+        if (cfg->spp == 0) throw std::runtime_error("spp must be positive");
+        GradBuffers gb;
+        rpd::GradTables G = grad_tables(s, params, count, gb);
+        const rp_camera &cam = s->host.desc.camera;
+        const uint32_t npx = cam.width * cam.height;
+        DevBuf<float> dl, g;
+        dl.upload(dl_dxyz, 3 * size_t(npx));
+        g.alloc(std::max<uint32_t>(count, 1));
+        std::vector<float> zeros(std::max<uint32_t>(count, 1), 0.0f);
+        HIP_CHECK(hipMemcpy(g.p, zeros.data(), zeros.size() * sizeof(float), hipMemcpyHostToDevice));
+        rpd::PathParams P = path_params(cfg->seed, cfg->randomization, 0, 0);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, s->counters.p, 8u);
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, s->stream));
+        const uint64_t total = uint64_t(npx) * cfg->spp;
+        const uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t(1) << 26) / npx); // whole samples per launch
+        for (uint64_t s0 = 0; s0 < cfg->spp; s0 += per_chunk) {
+            uint64_t ns = std::min<uint64_t>(per_chunk, cfg->spp - s0);
+            uint64_t n = ns * npx;
+            hipLaunchKernelGGL(rpd::k_adjoint<128>, dim3(uint32_t((n + 127) / 128)), dim3(128), 0, s->stream, s->dev, P,
+                               G, npx, cfg->sample_begin + s0, cfg->spp, n, dl.p, g.p, s->counters.p);
+            HIP_CHECK(hipGetLastError());
+        }
+        HIP_CHECK(hipEventRecord(e1, s->stream));
+        HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_CHECK(hipMemcpy(grad, g.p, count * sizeof(float), hipMemcpyDeviceToHost));
+        if (stats) {
+            unsigned long long c[8];
+            HIP_CHECK(hipMemcpy(c, s->counters.p, sizeof c, hipMemcpyDeviceToHost));
+            std::memset(stats, 0, sizeof *stats);
+            stats->seconds = ms * 1e-3;
+            stats->paths = total;
+            stats->extension_rays = c[0];
+        }
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_backward", e);
+    }
+}
+
+} // extern "C"

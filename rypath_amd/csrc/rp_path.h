@@ -39,6 +39,19 @@ struct ShadowReq {
     float w;
 };
 
+// Event hooks for the path-replay adjoint.  The forward renderer uses NoHook,
+// whose empty members compile away; the adjoint receives every contribution
+// added to the path radiance and the log-derivatives of every factor.
+struct NoHook {
+    static constexpr bool active = false;
+    RP_DEV void add(const float *) {}
+    RP_DEV void medium_factor(uint32_t, const Wavelengths &, float, const MediumCoeff &) {}
+    RP_DEV void medium_pass(uint32_t, const Wavelengths &, float) {}
+    RP_DEV void continuation(const rp_material &, uint32_t, const Lane *, F3, F3, const Wavelengths &) {}
+    RP_DEV void nee(const rp_material &, uint32_t, const Lane *, F3, F3, const Wavelengths &) {}
+    RP_DEV void nee_medium() {}
+};
+
 RP_DEV Wavelengths path_wavelengths(const Scene &S, uint32_t w2) { return sample_wavelengths(sobol_uniform(w2), S.hero_mode); }
 
 // Camera sample: Sobol index, camera words, wavelengths, primary ray
@@ -108,7 +121,8 @@ RP_DEV void shadow_setup(const LightSample &ls, F3 from, uint32_t medium, Shadow
 }
 
 // One path event.  Returns true when the path continues with ps.o/ps.d.
-RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const Hit &h, ShadowReq &sr) {
+template <class Hook>
+RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const Hit &h, ShadowReq &sr, Hook &hk) {
     sr.active = false;
     const Wavelengths wl = path_wavelengths(S, ps.w2);
     const uint32_t seed_bits = uint32_t(P.seed);
@@ -133,6 +147,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 pdf[k] = mc.st[k] * tr;
             }
             if (!advance(ps.density, ps.beta, num, pdf)) { ps.status = RP_PATH_INVALID_ARITHMETIC; return false; }
+            if (Hook::active) hk.medium_factor(ps.medium, wl, tf, mc);
             for (int k = 0; k < 4; ++k) ps.prev_pdf[k] = ps.prev_pdf[k] * pdf[k];
             F3 mp = ps.o + tf * ps.d;
             ++ps.count;
@@ -165,6 +180,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 }
                 if (any) {
                     shadow_setup(ls, mp, ps.medium, sr);
+                    if (Hook::active) hk.nee_medium();
                     sr.delta_or_nomis = ls.delta;
                     sr.w = 1.0f;
                     if (!ls.delta) {
@@ -188,6 +204,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 pdf[k] = tr;
             }
             if (!advance(ps.density, ps.beta, num, pdf)) { ps.status = RP_PATH_INVALID_ARITHMETIC; return false; }
+            if (Hook::active) hk.medium_pass(ps.medium, wl, seg);
             for (int k = 0; k < 4; ++k) ps.prev_pdf[k] = ps.prev_pdf[k] * pdf[k];
         } else {
             return false;
@@ -206,6 +223,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
             }
             add4(ps.L, val);
+            if (Hook::active) hk.add(val);
         }
         return false;
     }
@@ -241,6 +259,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 for (int k = 0; k < 4; ++k) val[k] = val[k] * w;
             }
             add4(ps.L, val);
+            if (Hook::active) hk.add(val);
         }
     }
     if (ps.count == capacity) return false;
@@ -286,6 +305,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 ps.status = RP_PATH_INVALID_ARITHMETIC;
                 cont = false;
             } else {
+                if (Hook::active && !hs.delta) hk.continuation(mat, obj.material, lanes, wol, hs.wi, wl);
                 for (int k = 0; k < 4; ++k) ps.prev_pdf[k] = pdf[k];
                 if (hs.transmission && rr_enabled)
                     for (int k = 0; k < 4; ++k) ps.eta_scale[k] = ps.eta_scale[k] * (etas[k] * etas[k]);
@@ -317,6 +337,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
             if (any) {
                 F3 from = offset_origin(sp.p, sp.err, sp.n, ls.wi);
                 shadow_setup(ls, from, outgoing_medium(obj, ps.medium, sp.n, ls.wi), sr);
+                if (Hook::active) hk.nee(mat, obj.material, lanes, wol, wil, wl);
                 sr.delta_or_nomis = ls.delta;
                 sr.w = 1.0f;
                 if (!ls.delta) {
@@ -361,7 +382,8 @@ RP_DEV void trace_path(const Scene &S, const PathParams &P, Stack &st, uint32_t 
         traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
         ++out.ext_rays;
         ShadowReq sr;
-        bool cont = path_event(S, P, ps, h, sr);
+        NoHook nh;
+        bool cont = path_event(S, P, ps, h, sr, nh);
         if (sr.active) {
             float tr[4];
             ++out.shadow_rays;

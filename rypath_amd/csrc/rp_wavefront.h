@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shade(Scene S, PathParams P, Wf
         h.b0 = ht.y;
         h.b1 = ht.z;
         h.b2 = ht.w;
-        cont = path_event(S, P, ps, h, sr);
+        NoHook nh;
+        cont = path_event(S, P, ps, h, sr, nh);
         shadow = sr.active;
         cont = cont && path_alive(S, ps);
         store_state(B, slot, ps);

@@ -128,3 +128,19 @@ class OracleScene:
 
 def sample(orc, scene, req):
     return scene.sample(req)
+
+
+def oracle_gradient(orc, oscene, params, dl_dxyz, spp, seed=1, randomization=2, sample_begin=0, threads=0):
+    import ctypes as C
+
+    rp = orc.rp
+    arr = rp.params_array(params)
+    L = orc.L
+    L.or_gradient.restype = C.c_int
+    L.or_gradient.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.c_void_p, C.c_uint32,
+                              C.c_void_p, C.c_void_p, C.c_uint32]
+    dl = np.ascontiguousarray(dl_dxyz, np.float32).reshape(-1)
+    out = np.zeros(len(params), np.float64)
+    L.or_gradient(oscene.h, seed, randomization, sample_begin, spp, C.cast(arr, C.c_void_p), len(params),
+                  dl.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), threads)
+    return out

@@ -1,0 +1,67 @@
+"""Batched work pool: every scene's film equals rendering it alone, bit for bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def uneven(rp, gpu_available):
+    items = []
+    specs = [(rp.SYNTH_BATCH, 40, 32, 3, dict(target_triangles=6000)),
+             (rp.SYNTH_BATCH, 64, 48, 1, dict(target_triangles=60000)),
+             (rp.SYNTH_CORNELL, 32, 32, 5, {}),
+             (rp.SYNTH_INVERSE, 24, 24, 2, {}),
+             (rp.SYNTH_BATCH, 16, 16, 7, dict(target_triangles=20000))]
+    keep = []
+    for i, (kind, w, h, spp, opts) in enumerate(specs):
+        s = rp.Synthetic(kind, seed=100 + i, width=w, height=h, **opts)
+        scene = rp.Scene(s.desc)
+        keep.append(s)
+        items.append((scene, spp, 1 + i))
+    return keep, items
+
+
+def _alone(rp, items):
+    return [scene.render_frame(rp.frame_config(spp, mode=rp.MODE_MEGAKERNEL, seed=seed))[0]
+            for scene, spp, seed in items]
+
+
+@pytest.mark.parametrize("cfg", [
+    dict(scheduler=0, group_size=128),
+    dict(scheduler=0, group_size=64),
+    dict(scheduler=1, capacity=1 << 12, group_size=128, sort_bits=0, use_graph=0),
+    dict(scheduler=1, capacity=1 << 12, group_size=256, sort_bits=2, use_graph=1),
+    dict(scheduler=1, capacity=1 << 16, group_size=64, sort_bits=4, use_graph=1),
+], ids=["persistent128", "persistent64", "pool-nograph", "pool-graph-sort2", "pool-graph-sort4"])
+def test_batch_films_equal_single_frames(rp, uneven, cfg):
+    _, items = uneven
+    batch = rp.Batch(items)
+    films, st = batch.render(rp.Batch.config(**cfg))
+    print(f"[batch] {cfg}: {st}", flush=True)
+    assert st["failed_paths"] == 0
+    assert st["paths"] == sum(s.width * s.height * spp for s, spp, _ in items)
+    for got, want in zip(films, _alone(rp, items)):
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_batch_graph_replays_repeatable(rp, uneven):
+    _, items = uneven
+    batch = rp.Batch(items)
+    cfg = rp.Batch.config(scheduler=1, capacity=1 << 10, group_size=128, use_graph=1)
+    a, sa = batch.render(cfg)
+    b, sb = batch.render(cfg)
+    assert sa["graph_launches"] >= 1
+    for x, y in zip(a, b):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_autotune_returns_valid_config(rp, uneven):
+    _, items = uneven
+    batch = rp.Batch(items)
+    cfg, sec = batch.autotune()
+    print(f"[autotune] {rp.config_dict(cfg)} {sec * 1e3:.2f} ms", flush=True)
+    assert sec > 0 and cfg.scheduler in (0, 1) and cfg.group_size in (64, 128, 256)
+    films, _ = batch.render(cfg)
+    for got, want in zip(films, _alone(rp, items)):
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
