@@ -40,7 +40,9 @@ typedef enum rp_status {
     RP_NOT_BUILT = 1,   /* library built without a device backend */
     RP_NO_DEVICE = 2,   /* no HIP device visible */
     RP_FAILURE = 3,     /* invalid input or device error; see rp_last_error */
-    RP_INVALID = 4      /* invalid argument (null pointer, bad size, bad enum) */
+    RP_INVALID = 4,     /* invalid argument (null pointer, bad size, bad enum) */
+    RP_BUSY = 5         /* rp_frame_submit: both frame slots retained (FrameSubmitResult::busy,
+                           src/frame_session.hpp:176-181) */
 } rp_status;
 
 /* ---- per-path status (render_wire::Status, src/render_wire.h:20-26) ---- */

@@ -1037,6 +1037,12 @@ static PathOut trace_path(const Ctx &c, uint32_t pixel, uint64_t sample) {
     NoTrack nt;
     return trace_path_t(c, pixel, sample, nt);
 }
+// Test switch (or_set_roulette): with roulette off, a path's sampling no
+// longer depends on its throughput, so finite differences of the renderer
+// with common random numbers are smooth in reflectance parameters and pin
+// the forward-mode derivative exactly.  The GPU renderer has no such switch.
+static std::atomic<bool> g_roulette{true};
+
 template <class T>
 static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &trk) {
     const Scene &s = *c.s;
@@ -1069,7 +1075,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
     float beta[4] = {1, 1, 1, 1}, density[4] = {1, 1, 1, 1};
     float prev_density[4] = {1, 1, 1, 1}, prev_pdf[4] = {0, 0, 0, 0};
     float eta_scale[4] = {1, 1, 1, 1};
-    bool rr_enabled = true, prev_delta = false;
+    bool rr_enabled = g_roulette.load(std::memory_order_relaxed), prev_delta = false;
     V3 prev_p = o;
     uint32_t count = 1, crossings = 0, medium = cam.medium, mdraw = 0;
     const uint32_t capacity = s.d.max_depth + 2;
@@ -1366,6 +1372,8 @@ extern "C" {
 int or_init(const char *sobol_path, const char *cie_path) {
     return (load_sobol_tables(sobol_path) && load_cie_table(cie_path)) ? 0 : -1;
 }
+
+void or_set_roulette(int enabled) { g_roulette.store(enabled != 0); }
 
 uint64_t or_sobol_index(uint32_t m, uint32_t x, uint32_t y, uint64_t sample) { return sobol_index(m, x, y, sample); }
 uint32_t or_sobol_word(uint64_t index, uint32_t dim, uint32_t mode, uint32_t seed) {

@@ -15,10 +15,18 @@ _LIB_PATH = _HERE / "librypath_amd.so"
 if not _LIB_PATH.exists():
     raise ImportError(f"{_LIB_PATH} is missing: build it with `python __graft_entry__.py` (hipcc, gfx950)")
 
+# PyTorch wheels ship their own HIP runtime.  Loading torch first makes this
+# library bind to that same runtime, so one process holds one HIP runtime
+# (a second runtime initialised after ours leaves torch without devices).
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 _lib = C.CDLL(str(_LIB_PATH))
 
 NONE = 0xFFFFFFFF
-SUCCESS, NOT_BUILT, NO_DEVICE, FAILURE, INVALID = range(5)
+SUCCESS, NOT_BUILT, NO_DEVICE, FAILURE, INVALID, BUSY = range(6)
 SPECTRUM_CONSTANT, SPECTRUM_SIGMOID, SPECTRUM_TABLE, SPECTRUM_CAUCHY, SPECTRUM_LINEAR = range(5)
 MATERIAL_DIFFUSE, MATERIAL_CONDUCTOR, MATERIAL_DIELECTRIC, MATERIAL_INTERFACE = range(4)
 RANDOMIZE_NONE, RANDOMIZE_PERMUTE, RANDOMIZE_FASTOWEN, RANDOMIZE_OWEN = range(4)
@@ -168,11 +176,31 @@ _sig("rp_synthetic_create", C.c_int, [u32, u64, C.POINTER(SyntheticOptions), C.P
 _sig("rp_synthetic_desc", C.POINTER(SceneDesc), [C.c_void_p])
 _sig("rp_synthetic_destroy", None, [C.c_void_p])
 
+
+class PbrtInfo(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("spp", u32), ("randomization", u32), ("seed", u64),
+                ("max_depth", u32), ("volumetric", u32), ("output", C.c_char * 256)]
+
+
+_sig("rp_pbrt_load", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)])
+_sig("rp_pbrt_load_string", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)])
+_sig("rp_pbrt_desc", C.POINTER(SceneDesc), [C.c_void_p])
+_sig("rp_pbrt_get_info", C.c_int, [C.c_void_p, C.POINTER(PbrtInfo)])
+_sig("rp_pbrt_destroy", None, [C.c_void_p])
+_sig("rp_write_exr", C.c_int, [C.c_char_p, u32, u32, C.POINTER(f32), f64, u32, C.POINTER(C.c_char_p),
+                               C.POINTER(C.c_char_p), u32])
+_sig("rp_frame_submit", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.POINTER(C.c_void_p)])
+_sig("rp_frame_poll", C.c_int, [C.c_void_p, C.POINTER(u32)])
+_sig("rp_frame_wait", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FrameStats)])
+_sig("rp_frame_release", None, [C.c_void_p])
+
 EXPORTED = [
     "rp_api_version", "rp_device_count", "rp_last_error", "rp_scene_create", "rp_scene_info", "rp_scene_destroy",
     "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_trace_closest", "rp_trace_any",
     "rp_batch_create", "rp_batch_render", "rp_batch_destroy", "rp_autotune", "rp_set_params", "rp_backward",
     "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
+    "rp_pbrt_load", "rp_pbrt_load_string", "rp_pbrt_desc", "rp_pbrt_get_info", "rp_pbrt_destroy",
+    "rp_write_exr", "rp_frame_submit", "rp_frame_poll", "rp_frame_wait", "rp_frame_release",
 ]
 
 
@@ -216,7 +244,9 @@ class Synthetic:
 
     @property
     def d(self):
-        return self.desc.contents
+        c = self.desc.contents
+        c._owner = self  # the description points into this object's storage
+        return c
 
     def triangles(self):
         d = self.d
@@ -325,6 +355,84 @@ class Scene:
         _check("rp_trace_any", _lib.rp_trace_any(self._h, rays.ctypes.data_as(C.c_void_p), len(rays),
                                                   occ.ctypes.data_as(C.c_void_p), C.byref(sec)))
         return occ, sec.value
+
+    def submit(self, cfg):
+        """Asynchronous frame (rp_frame_submit); returns a Frame, or raises with code BUSY
+        when two frames are already retained."""
+        h = C.c_void_p()
+        _check("rp_frame_submit", _lib.rp_frame_submit(self._h, C.byref(cfg), C.byref(h)))
+        return Frame(h, self.width, self.height)
+
+
+class Frame:
+    """A submitted frame (FrameSession::submit result): poll, wait, release."""
+
+    def __init__(self, handle, width, height):
+        self._h, self.width, self.height = handle, width, height
+
+    def poll(self):
+        done = u32()
+        _check("rp_frame_poll", _lib.rp_frame_poll(self._h, C.byref(done)))
+        return bool(done.value)
+
+    def wait(self, film=True):
+        out = np.zeros((self.height, self.width, 3), np.float32) if film else None
+        st = FrameStats()
+        ptr = out.ctypes.data_as(C.c_void_p) if film else None
+        _check("rp_frame_wait", _lib.rp_frame_wait(self._h, ptr, C.byref(st)))
+        return out, {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
+    def release(self):
+        if getattr(self, "_h", None):
+            _lib.rp_frame_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.release()
+
+
+class PbrtScene:
+    """PBRT-v4 scene-file subset translated to a scene description (rp_pbrt_load)."""
+
+    def __init__(self, path=None, text=None, base_dir="."):
+        h = C.c_void_p()
+        if text is not None:
+            _check("rp_pbrt_load_string", _lib.rp_pbrt_load_string(text.encode(), str(base_dir).encode(), C.byref(h)))
+        else:
+            _check("rp_pbrt_load", _lib.rp_pbrt_load(str(path).encode(), C.byref(h)))
+        self._h = h
+        self.desc = _lib.rp_pbrt_desc(h)
+        i = PbrtInfo()
+        _check("rp_pbrt_get_info", _lib.rp_pbrt_get_info(h, C.byref(i)))
+        self.info = {f: getattr(i, f) for f, _ in PbrtInfo._fields_}
+        self.info["output"] = i.output.decode()
+
+    @property
+    def d(self):
+        c = self.desc.contents
+        c._owner = self  # the description points into this object's storage
+        return c
+
+    def frame_config(self, mode=MODE_WAVEFRONT_SORTED, sample_begin=0):
+        i = self.info
+        return frame_config(i["spp"], mode=mode, sample_begin=sample_begin, seed=i["seed"],
+                            randomization=i["randomization"])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rp_pbrt_destroy(self._h)
+            self._h = None
+
+
+def write_exr(path, xyz, exposure=1.0, half=False, metadata=None):
+    """Writes a (H, W, 3) XYZ film as a linear-sRGB OpenEXR file (rp_write_exr)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32)
+    h, w, _ = xyz.shape
+    meta = dict(metadata or {})
+    keys = (C.c_char_p * max(len(meta), 1))(*[k.encode() for k in meta])
+    vals = (C.c_char_p * max(len(meta), 1))(*[str(v).encode() for v in meta.values()])
+    _check("rp_write_exr", _lib.rp_write_exr(str(path).encode(), w, h, xyz.ctypes.data_as(C.POINTER(f32)),
+                                              float(exposure), 1 if half else 0, keys, vals, len(meta)))
 
 
 class Batch:

@@ -72,6 +72,27 @@ const std::array<std::array<float, 3>, 471> &cie_xyz() {
     return table;
 }
 
+const std::array<float, 471> &d65() {
+    static std::array<float, 471> table{};
+    static std::once_flag once;
+    std::call_once(once, [] {
+        std::string path = data_dir() + "/d65.bin";
+        std::FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) throw std::runtime_error("cannot open " + path);
+        bool ok = std::fread(table.data(), 4, 471, f) == 471;
+        std::fclose(f);
+        if (!ok) throw std::runtime_error("short read " + path);
+    });
+    return table;
+}
+
+double y_inner_product(const std::array<float, 471> &s) {
+    const auto &cie = cie_xyz();
+    double sum = 0;
+    for (size_t i = 0; i < 471; ++i) sum += double(s[i]) * double(cie[i][1]);
+    return sum;
+}
+
 // ------------------------------------------------------------- sRGB sigmoid
 namespace {
 struct SigmoidTable {

@@ -25,6 +25,10 @@ struct SobolData {
 };
 const SobolData &sobol_data();
 const std::array<std::array<float, 3>, 471> &cie_xyz();
+// PBRT stdillum-D65 at 360..830 nm (d65.bin), the sRGB colour space's illuminant.
+const std::array<float, 471> &d65();
+// sum_i s[i] * ybar[i] over the 471 one-nanometre samples (PBRT SpectrumToPhotometric).
+double y_inner_product(const std::array<float, 471> &s);
 
 // ------------------------------------------------------------- spectra
 struct SigmoidCoefficients {

@@ -2,14 +2,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "../../include/rp_io.h"
 #include "rp_internal.hpp"
 #include "rp_path.h"
 #include "rp_wavefront.h"
@@ -51,6 +57,8 @@ template <typename T> struct DevBuf {
     void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
 };
 
+struct FrameSession;
+
 } // namespace rp
 
 using namespace rp;
@@ -83,9 +91,11 @@ struct rp_scene {
     DevBuf<uint8_t> occluded;
     hipStream_t stream = nullptr;
     rpd::WavefrontBuffers wf;
-    ~rp_scene() {
-        if (stream) (void)hipStreamDestroy(stream);
-    }
+    // Serialises device work on this scene between API calls and the
+    // asynchronous frame worker.
+    std::mutex exec;
+    std::shared_ptr<rp::FrameSession> frames;
+    ~rp_scene();
 };
 
 namespace {
@@ -423,6 +433,89 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
 
 } // namespace
 
+// ------------------------------------------------------------- asynchronous frames
+// FrameSession::submit / Frame::poll / Frame::wait (src/frame_session.hpp:145-212):
+// one worker thread per scene renders submitted frames in order on the
+// scene's stream; a retained frame holds one of two slots until released,
+// and frames stay valid (film included) after the scene is destroyed.
+namespace rp {
+
+struct FrameJob {
+    rp_frame_config cfg{};
+    uint32_t width = 0, height = 0;
+    DevBuf<float> film;
+    std::mutex m;
+    std::condition_variable cv;
+    uint32_t state = 0; // 0 pending, 1 complete, 2 failed
+    rp_frame_stats stats{};
+    std::string error;
+    std::shared_ptr<FrameSession> session;
+};
+
+struct FrameSession {
+    rp_scene *scene;
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<FrameJob>> queue;
+    uint32_t retained = 0;
+    bool stopping = false;
+    std::thread worker;
+
+    explicit FrameSession(rp_scene *s) : scene(s) {
+        worker = std::thread([this] { run(); });
+    }
+    void run() {
+        for (;;) {
+            std::shared_ptr<FrameJob> job;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return stopping || !queue.empty(); });
+                if (queue.empty()) return;
+                job = queue.front();
+                queue.pop_front();
+            }
+            uint32_t state = 1;
+            std::string error;
+            rp_frame_stats st{};
+            try {
+                std::lock_guard<std::mutex> g(scene->exec);
+                render_frame_impl(scene, job->cfg, job->film.p, &st);
+            } catch (const std::exception &e) {
+                state = 2;
+                error = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(job->m);
+                job->stats = st;
+                job->error = error;
+                job->state = state;
+            }
+            job->cv.notify_all();
+        }
+    }
+    // Renders every queued frame, then stops the worker.
+    void shutdown() {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            stopping = true;
+        }
+        cv.notify_all();
+        if (worker.joinable()) worker.join();
+        scene = nullptr;
+    }
+};
+
+} // namespace rp
+
+rp_scene::~rp_scene() {
+    if (frames) frames->shutdown();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+struct rp_frame {
+    std::shared_ptr<rp::FrameJob> job;
+};
+
 extern "C" {
 
 uint32_t rp_api_version(void) { return RP_API_VERSION; }
@@ -516,6 +609,7 @@ int rp_sample(rp_scene *s, const rp_request *rq, rp_path_record *out, uint64_t c
         return RP_INVALID;
     }
     try {
+        std::lock_guard<std::mutex> lock(s->exec);
         HIP_CHECK(hipSetDevice(s->device));
         s->records.alloc(n);
         rpd::PathParams P = path_params(rq->seed, rq->randomization, rq->emergency_limit, rq->boundary_limit);
@@ -536,6 +630,7 @@ int rp_render_frame_device(rp_scene *s, const rp_frame_config *cfg, float *xyz_d
         return RP_INVALID;
     }
     try {
+        std::lock_guard<std::mutex> lock(s->exec);
         render_frame_impl(s, *cfg, xyz_device, stats);
         return RP_SUCCESS;
     } catch (const std::exception &e) {
@@ -549,6 +644,7 @@ int rp_render_frame(rp_scene *s, const rp_frame_config *cfg, float *xyz_host, rp
         return RP_INVALID;
     }
     try {
+        std::lock_guard<std::mutex> lock(s->exec);
         HIP_CHECK(hipSetDevice(s->device));
         const uint32_t npx = s->host.desc.camera.width * s->host.desc.camera.height;
         DevBuf<float> out;
@@ -561,6 +657,96 @@ int rp_render_frame(rp_scene *s, const rp_frame_config *cfg, float *xyz_host, rp
     }
 }
 
+int rp_frame_submit(rp_scene *s, const rp_frame_config *cfg, rp_frame **out) {
+    if (!s || !cfg || !out) {
+        set_error("rp_frame_submit: null argument");
+        return RP_INVALID;
+    }
+    *out = nullptr;
+    if (cfg->spp == 0 || cfg->mode > RP_MODE_WAVEFRONT_SORTED || cfg->randomization > RP_RANDOMIZE_OWEN) {
+        set_error("rp_frame_submit: spp must be positive and mode/randomization known");
+        return RP_INVALID;
+    }
+    try {
+        if (!s->frames) s->frames = std::make_shared<FrameSession>(s);
+        FrameSession &fs = *s->frames;
+        {
+            std::lock_guard<std::mutex> lk(fs.m);
+            if (fs.retained >= 2) {
+                set_error("rp_frame_submit: both frame slots are retained; release a frame first");
+                return RP_BUSY;
+            }
+        }
+        auto job = std::make_shared<FrameJob>();
+        job->cfg = *cfg;
+        job->width = s->host.desc.camera.width;
+        job->height = s->host.desc.camera.height;
+        HIP_CHECK(hipSetDevice(s->device));
+        job->film.alloc(3 * size_t(job->width) * job->height);
+        job->session = s->frames;
+        {
+            std::lock_guard<std::mutex> lk(fs.m);
+            ++fs.retained;
+            fs.queue.push_back(job);
+        }
+        fs.cv.notify_all();
+        *out = new rp_frame{job};
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_frame_submit", e);
+    }
+}
+
+int rp_frame_poll(const rp_frame *f, uint32_t *done) {
+    if (!f || !done) {
+        set_error("rp_frame_poll: null argument");
+        return RP_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(f->job->m);
+    *done = f->job->state != 0;
+    return f->job->state == 2 ? RP_FAILURE : RP_SUCCESS;
+}
+
+int rp_frame_wait(rp_frame *f, float *xyz_host, rp_frame_stats *stats) {
+    if (!f) {
+        set_error("rp_frame_wait: null argument");
+        return RP_INVALID;
+    }
+    FrameJob &job = *f->job;
+    {
+        std::unique_lock<std::mutex> lk(job.m);
+        job.cv.wait(lk, [&] { return job.state != 0; });
+        if (job.state == 2) {
+            set_error("rp_frame_wait: " + job.error);
+            return RP_FAILURE;
+        }
+    }
+    try {
+        if (xyz_host)
+            HIP_CHECK(hipMemcpy(xyz_host, job.film.p, 3 * size_t(job.width) * job.height * sizeof(float),
+                                hipMemcpyDeviceToHost));
+        if (stats) *stats = job.stats;
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_frame_wait", e);
+    }
+}
+
+void rp_frame_release(rp_frame *f) {
+    if (!f) return;
+    {
+        // the worker may still be rendering this frame: wait, then free the slot
+        std::unique_lock<std::mutex> lk(f->job->m);
+        f->job->cv.wait(lk, [&] { return f->job->state != 0; });
+    }
+    if (auto fs = f->job->session) {
+        std::lock_guard<std::mutex> lk(fs->m);
+        --fs->retained;
+    }
+    f->job->session.reset();
+    delete f;
+}
+
 static int trace_common(rp_scene *s, const rp_ray *rays, uint64_t count, void *out, bool any, double *seconds) {
     if (!s || (!rays && count) || (!out && count)) {
         set_error("rp_trace: null argument");
@@ -568,6 +754,7 @@ static int trace_common(rp_scene *s, const rp_ray *rays, uint64_t count, void *o
     }
     if (count == 0) return RP_SUCCESS;
     try {
+        std::lock_guard<std::mutex> lock(s->exec);
         HIP_CHECK(hipSetDevice(s->device));
         s->rays.upload(rays, count);
         hipEvent_t e0, e1;
@@ -733,6 +920,7 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
         b->graph_key = key;
     }
     graph_launches = 0;
+    const auto t_start = std::chrono::steady_clock::now();
     for (uint64_t guard = 0;; ++guard) {
         if (cfg.use_graph) {
             HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
@@ -744,7 +932,11 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
                                  b->stream));
         HIP_CHECK(hipStreamSynchronize(b->stream));
         if (b->host_counters->done >= b->total) break;
-        if (guard > (1ull << 22)) throw std::runtime_error("wavefront pool did not drain");
+        const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        if (guard > 200000 || waited > 120.0)
+            throw std::runtime_error("wavefront pool did not drain: done " + std::to_string(b->host_counters->done) +
+                                     " of " + std::to_string(b->total) + " after " +
+                                     std::to_string(guard + 1) + " submissions");
     }
 }
 
@@ -1030,6 +1222,7 @@ int rp_set_params(rp_scene *s, const rp_param *params, uint32_t count, const flo
         return RP_INVALID;
     }
     try {
+        std::lock_guard<std::mutex> lock(s->exec);
         for (uint32_t i = 0; i < count; ++i) set_param(s, params[i], values[i]);
         HIP_CHECK(hipSetDevice(s->device));
         s->spectra.upload(s->host.spectra);
@@ -1053,6 +1246,7 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
         return RP_INVALID;
     }
     try {
+        std::lock_guard<std::mutex> lock(s->exec);
         HIP_CHECK(hipSetDevice(s->device));
         if (cfg->spp == 0) throw std::runtime_error("spp must be positive");
         GradBuffers gb;

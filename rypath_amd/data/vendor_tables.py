@@ -10,10 +10,12 @@ they were produced.  Inputs (paths under the reference checkout):
   (64 scale nodes + 3 x 64^3 x 3 binary32, little endian).  Apache-2.0.
 * third_party/cie/CIE_xyz_1931_2deg.csv -- CIE 1931 2-degree observer,
   1 nm, 360..830 nm.  CC BY-SA 4.0 (CIE 2019, DOI 10.25039/CIE.DS.xvudnb9b).
+* third_party/pbrt/d65.inc -- PBRT v4 stdillum-D65 (CIE standard illuminant
+  D65, normalised), 471 binary32 values at 360..830 nm.  Apache-2.0.
 
 Outputs (this directory): sobol_tables.bin (234208 B: directions u32,
 intervals u64, inverse u64, little endian), srgb_sigmoid.bin (verbatim copy),
-cie_xyz_1931_2deg.csv (verbatim copy).
+cie_xyz_1931_2deg.csv (verbatim copy), d65.bin (471 binary32, little endian).
 """
 import hashlib
 import re
@@ -34,6 +36,11 @@ def parse_sobol(text):
     return vals
 
 
+def parse_hex_floats(text):
+    body = "\n".join(line for line in text.splitlines() if not line.lstrip().startswith("//"))
+    return [float.fromhex(tok) for tok in re.findall(r"(-?0x[0-9a-fA-F.]+p[-+]?\d+)f?", body)]
+
+
 def main(ref=Path("/root/reference")):
     inc = (ref / "third_party/pbrt/sobol_tables.inc").read_bytes()
     sha = hashlib.sha256(inc).hexdigest()
@@ -49,7 +56,10 @@ def main(ref=Path("/root/reference")):
     (HERE / "sobol_tables.bin").write_bytes(blob)
     shutil.copyfile(ref / "third_party/pbrt/srgb.bin", HERE / "srgb_sigmoid.bin")
     shutil.copyfile(ref / "third_party/cie/CIE_xyz_1931_2deg.csv", HERE / "cie_xyz_1931_2deg.csv")
-    for name in ("sobol_tables.bin", "srgb_sigmoid.bin", "cie_xyz_1931_2deg.csv"):
+    d65 = parse_hex_floats((ref / "third_party/pbrt/d65.inc").read_text())
+    assert len(d65) == 471, len(d65)
+    (HERE / "d65.bin").write_bytes(struct.pack("<471f", *d65))
+    for name in ("sobol_tables.bin", "srgb_sigmoid.bin", "cie_xyz_1931_2deg.csv", "d65.bin"):
         print(name, hashlib.sha256((HERE / name).read_bytes()).hexdigest())
 
 

@@ -1,0 +1,51 @@
+"""Shared helpers for derivative checks of the inverse-rendering scene: write
+parameter values into a scene description's tables (the host-side meaning of
+rp_set_params) and estimate directional derivatives by central differences
+over several independent sample sets."""
+import numpy as np
+
+
+class DescParams:
+    """Applies parameter vectors to an owned scene description (Synthetic)."""
+
+    def __init__(self, rp, synth, params, values):
+        self.rp, self.synth, self.params, self.values = rp, synth, params, np.asarray(values)
+        d = synth.d
+        self.d = d
+        self.pool = np.ctypeslib.as_array(d.spectra, shape=(d.spectra_count,))
+        self.orig_pool = self.pool.copy()
+        self.orig_rough = {t: (d.materials[t].roughness_u, d.materials[t].roughness_v)
+                           for k, t, _, _ in params if k == rp.PARAM_ROUGHNESS}
+
+    def apply(self, vals):
+        rp, d = self.rp, self.d
+        self.pool[:] = self.orig_pool
+        for t, (u, v) in self.orig_rough.items():
+            d.materials[t].roughness_u, d.materials[t].roughness_v = u, v
+        for (kind, target, b, bins), v in zip(self.params, vals):
+            if kind == rp.PARAM_ROUGHNESS:
+                d.materials[target].roughness_u = d.materials[target].roughness_v = float(v)
+                continue
+            spec = d.materials[target].reflectance if kind == rp.PARAM_REFLECTANCE_BIN else d.media[target].sigma_s
+            for i in range(471):
+                if i * bins // 471 == b:
+                    self.pool[spec.table + i] = v
+
+    def restore(self):
+        self.apply(self.values)
+
+
+GROUPS = {"walls": slice(0, 48), "roughness": slice(48, 50), "sigma_s": slice(50, 58)}
+
+
+def direction(values, group, seed):
+    rng = np.random.default_rng(seed)
+    sl = GROUPS[group]
+    v = np.zeros(len(values))
+    v[sl] = rng.normal(0, 1, sl.stop - sl.start) * np.abs(values[sl])
+    return v
+
+
+def mean_se(x):
+    x = np.asarray(x, np.float64)
+    return float(x.mean()), float(x.std(ddof=1) / np.sqrt(len(x)))

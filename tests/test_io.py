@@ -1,0 +1,368 @@
+"""Scene and image I/O on the host: the PBRT-v4 subset translator
+(rp_pbrt_load, counterpart of src/pbrt_render.cpp translate), the EXR writer
+(rp_write_exr, counterpart of src/exr_output.cpp:62-120) and the `rypath`
+command line (src/main.cpp).  No GPU is needed."""
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import rypath_amd as rp
+from tests import exr_util
+
+ROOT = Path(__file__).resolve().parents[1]
+CORNELL = ROOT / "rypath_amd" / "scenes" / "cornell.pbrt"
+CLI = ROOT / "rypath_amd" / "rypath"
+CIE = np.loadtxt(ROOT / "rypath_amd" / "data" / "cie_xyz_1931_2deg.csv", delimiter=",")
+assert CIE.shape == (471, 4) and CIE[0, 0] == 360
+YBAR = CIE[:, 2]
+LAMBDA = np.arange(360, 831, dtype=np.float64)
+
+
+def table(d, spec):
+    assert spec.kind == rp.SPECTRUM_TABLE
+    pool = np.ctypeslib.as_array(d.spectra, shape=(d.spectra_count,))
+    return pool[spec.table:spec.table + 471].astype(np.float64)
+
+
+def planck(nm, kelvin):
+    # Planck's law (SI constants); only ratios are used
+    c, h, kb = 299792458.0, 6.62607015e-34, 1.380649e-23
+    lam = nm * 1e-9
+    return 2 * h * c * c / lam ** 5 / np.expm1(h * c / (lam * kb * kelvin))
+
+
+def positions(d):
+    return np.ctypeslib.as_array(d.positions, shape=(d.vertex_count, 3)).astype(np.float64)
+
+
+def triangles(d):
+    p = positions(d)
+    idx = np.ctypeslib.as_array(d.indices, shape=(d.triangle_count, 3))
+    return p[idx]
+
+
+def load(text, base_dir="."):
+    return rp.PbrtScene(text=text, base_dir=base_dir)
+
+
+HEADER = """LookAt 0 0 -5  0 0 0  0 1 0
+Camera "perspective" "float fov" [45]
+Film "rgb" "integer xresolution" [64] "integer yresolution" [32]
+Sampler "sobol" "integer pixelsamples" [8] "integer seed" [3]
+Integrator "path" "integer maxdepth" [4]
+WorldBegin
+"""
+TRI = 'Shape "trianglemesh" "integer indices" [0 1 2] "point3 P" [0 0 0  1 0 0  0 1 0]\n'
+
+
+# ------------------------------------------------------------------ PBRT
+
+
+def test_cornell_file_matches_demo_scene():
+    """The shipped Cornell file translates to the demo scene's camera, geometry
+    and reflectances; its blackbody emitter is photometrically normalised."""
+    p = rp.PbrtScene(CORNELL)
+    s = rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=256, height=256)
+    a, b = p.d, s.d
+    assert p.info["width"] == 256 and p.info["spp"] == 64 and p.info["max_depth"] == 5 and p.info["seed"] == 1
+    assert p.info["randomization"] == rp.RANDOMIZE_FASTOWEN and p.info["output"] == "cornell.exr"
+    for f in ("origin", "corner", "dx", "dy"):
+        np.testing.assert_allclose(list(getattr(a.camera, f)), list(getattr(b.camera, f)), rtol=1e-6, atol=1e-5)
+    assert a.triangle_count == b.triangle_count == 36
+    ta, tb = triangles(a), triangles(b)
+    # same triangles up to order: match by centroid, then compare vertex sets
+    ca, cb = ta.mean(axis=1), tb.mean(axis=1)
+    for i in range(36):
+        j = int(np.argmin(np.linalg.norm(cb - ca[i], axis=1)))
+        np.testing.assert_allclose(np.sort(ta[i], axis=0), np.sort(tb[j], axis=0), atol=2e-3)
+    # reflectances: the same RGB reconstruction as the demo scene
+    ma = {(m.reflectance.kind, tuple(m.reflectance.c)) for m in a.materials[1:a.material_count]}
+    mb = {(m.reflectance.kind, tuple(m.reflectance.c)) for m in b.materials[0:b.material_count]}
+    assert ma == mb
+    # emission: L = B_6500 * 17 / <B_6500, ybar>  =>  <L, ybar> = 17 and L ~ Planck
+    assert a.emission_count == 1
+    L = table(a, a.emissions[0].radiance)
+    assert abs(L @ YBAR - 17.0) < 17.0 * 1e-5
+    ref = planck(LAMBDA, 6500.0)
+    np.testing.assert_allclose(L / L[200], ref / ref[200], rtol=2e-5)
+    # the emitter faces down (-y), as in the demo scene
+    emitter = [i for i in range(36) if a.objects[a.tri_object[i]].emission != rp.NONE]
+    assert len(emitter) == 2
+    for i in emitter:
+        p0, p1, p2 = ta[i]
+        n = np.cross(p0 - p2, p1 - p2)
+        assert n[1] < 0 and abs(n[0]) < 1e-6 and abs(n[2]) < 1e-6
+
+
+def test_rgb_emission_is_d65_based():
+    """RGB emission is D65 * sigmoid(rgb / 2m) * 2m, normalised by D65's CIE Y
+    inner product: white light has <L, ybar> = scale and the D65 shape
+    (CIE D65 relative SPD: 460 nm 117.812, 500 nm 109.354, 560 nm 100.0,
+    600 nm 90.0062, 700 nm 71.6091)."""
+    sc = load(HEADER + 'AttributeBegin\nAreaLightSource "diffuse" "rgb L" [1 1 1] "float scale" [3]\n'
+              + TRI + 'AttributeEnd\n')
+    d = sc.d
+    L = table(d, d.emissions[0].radiance)
+    assert abs(L @ YBAR - 3.0) < 3e-5
+    for nm, v in [(460, 117.812), (500, 109.354), (600, 90.0062), (700, 71.6091)]:
+        assert abs(L[nm - 360] / L[560 - 360] - v / 100.0) < 2e-3, nm
+    # a coloured light keeps <L, ybar> = scale * <D65 * s(rgb/2m) * 2m, ybar> / <D65, ybar>
+    red = load(HEADER + 'AttributeBegin\nAreaLightSource "diffuse" "rgb L" [4 1 0.5]\n' + TRI + 'AttributeEnd\n')
+    Lr = table(red.d, red.d.emissions[0].radiance)
+    assert Lr[700 - 360] / Lr[450 - 360] > 4  # red-heavy
+    # default L is the colour space illuminant, normalised to one nit
+    dflt = load(HEADER + 'AttributeBegin\nAreaLightSource "diffuse"\n' + TRI + 'AttributeEnd\n')
+    assert abs(table(dflt.d, dflt.d.emissions[0].radiance) @ YBAR - 1.0) < 1e-5
+
+
+def test_spectrum_knots_emission_and_reflectance():
+    text = (HEADER + 'Material "diffuse" "spectrum reflectance" [400 0.2 500 0.6 700 0.1]\n'
+            'AttributeBegin\nAreaLightSource "diffuse" "spectrum L" [380 1 780 1] "float scale" [2]\n'
+            + TRI + 'AttributeEnd\n' + TRI)
+    d = load(text).d
+    R = table(d, d.materials[1].reflectance)
+    assert R[0] == 0 and R[400 - 360] == pytest.approx(0.2) and R[450 - 360] == pytest.approx(0.4)
+    assert R[600 - 360] == pytest.approx(0.35) and R[700 - 360] == pytest.approx(0.1) and R[701 - 360] == 0
+    L = table(d, d.emissions[0].radiance)
+    assert L[379 - 360] == 0 and L[781 - 360] == 0
+    assert abs(L @ YBAR - 2.0) < 2e-5
+    box = np.zeros(471)
+    box[380 - 360:780 - 360 + 1] = 1
+    np.testing.assert_allclose(L, box * 2.0 / (box @ YBAR), rtol=1e-6)
+
+
+def test_transforms_compose_like_matrices():
+    text = HEADER + ('AttributeBegin\nTranslate 1 2 3\nRotate 30 0 0 1\nScale 2 1 1\n' + TRI + 'AttributeEnd\n'
+                     'AttributeBegin\nConcatTransform [1 0 0 0  0 1 0 0  0 0 1 0  5 6 7 1]\n' + TRI + 'AttributeEnd\n')
+    d = load(text).d
+    p = positions(d)
+    a = np.radians(30)
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    M = R @ np.diag([2.0, 1, 1])
+    src = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    np.testing.assert_allclose(p[:3], src @ M.T + [1, 2, 3], atol=1e-6)
+    np.testing.assert_allclose(p[3:6], src + [5, 6, 7], atol=1e-6)
+
+
+def test_orientation_flips_winding():
+    """ReverseOrientation, and transforms that swap handedness, flip the
+    emitting side (PBRT's area-light convention)."""
+    def normal(text):
+        d = load(HEADER + text).d
+        t = triangles(d)[0]
+        return np.cross(t[0] - t[2], t[1] - t[2])
+
+    base = normal('AreaLightSource "diffuse"\n' + TRI)
+    assert base[2] > 0
+    assert normal('ReverseOrientation\nAreaLightSource "diffuse"\n' + TRI)[2] < 0
+    # the object-space normal +z maps to -z under the mirror, although the
+    # mirrored vertices (all at z = 0) are unchanged
+    assert normal('Scale 1 1 -1\nAreaLightSource "diffuse"\n' + TRI)[2] < 0
+    assert normal('Scale 1 1 -1\nReverseOrientation\nAreaLightSource "diffuse"\n' + TRI)[2] > 0
+
+
+def test_media_lights_and_camera_medium():
+    text = """LookAt 0 0 -5  0 0 0  0 1 0
+MakeNamedMedium "fog" "string type" "homogeneous" "rgb sigma_a" [0.5 0.5 0.5] "float sigma_s" [2] "float scale" [3] "float g" [0.4]
+MediumInterface "" "fog"
+Camera "perspective" "float fov" [30]
+Integrator "volpath" "integer maxdepth" [16]
+WorldBegin
+LightSource "spot" "point3 from" [0 4 0] "point3 to" [0 0 0] "float coneangle" [30] "float conedelta" [10] "blackbody I" [4000] "float scale" [5]
+LightSource "infinite" "float L" [0.25]
+AttributeBegin
+MediumInterface "fog" ""
+Material "interface"
+""" + TRI + "AttributeEnd\n"
+    sc = load(text)
+    d = sc.d
+    assert sc.info["volumetric"] == 1 and sc.info["max_depth"] == 16
+    assert d.medium_count == 1 and d.camera.medium == 0
+    m = d.media[0]
+    assert m.g == pytest.approx(0.4)
+    assert m.sigma_a.kind == rp.SPECTRUM_CONSTANT and m.sigma_a.c[0] * m.sigma_a.scale == pytest.approx(1.5)
+    assert m.sigma_s.c[0] * m.sigma_s.scale == pytest.approx(6.0)
+    o = d.objects[0]
+    assert (o.medium_inside, o.medium_outside) == (0, rp.NONE)
+    assert d.materials[o.material].kind == rp.MATERIAL_INTERFACE
+    sp = d.spots[0]
+    np.testing.assert_allclose(list(sp.position), [0, 4, 0])
+    np.testing.assert_allclose(list(sp.direction), [0, -1, 0], atol=1e-7)
+    assert sp.cos_falloff_end == pytest.approx(np.cos(np.radians(30)))
+    assert sp.cos_falloff_start == pytest.approx(np.cos(np.radians(20)))
+    assert abs(table(d, sp.intensity) @ YBAR - 5.0) < 5e-5
+    assert d.has_environment == 1
+    env = table(d, d.environment)
+    assert abs(env @ YBAR - 1.0) < 1e-5 and np.ptp(env) == 0  # constant 0.25 normalised to one nit
+
+
+def test_conductor_and_dielectric_parameters():
+    text = HEADER + ('Material "conductor" "spectrum eta" [400 0.2 700 0.3] "rgb k" [3 2.5 2] "float roughness" [0.1]'
+                     ' "bool remaproughness" false\n' + TRI
+                     + 'Material "dielectric" "float eta" [1.33] "float uroughness" [0.2] "float vroughness" [0.3]\n'
+                     + TRI)
+    d = load(text).d
+    c = d.materials[1]
+    assert c.kind == rp.MATERIAL_CONDUCTOR and c.remap_roughness == 0
+    assert c.roughness_u == pytest.approx(0.1) and c.roughness_v == pytest.approx(0.1)
+    assert c.eta.kind == rp.SPECTRUM_TABLE and c.k.kind == rp.SPECTRUM_SIGMOID and c.k.scale == pytest.approx(6.0)
+    g = d.materials[2]
+    assert g.kind == rp.MATERIAL_DIELECTRIC and g.eta.c[0] == pytest.approx(1.33) and g.remap_roughness == 1
+    assert (g.roughness_u, g.roughness_v) == (pytest.approx(0.2), pytest.approx(0.3))
+
+
+def test_plymesh_ascii_and_binary(tmp_path):
+    verts = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], np.float32)
+    ascii = ("ply\nformat ascii 1.0\nelement vertex 4\nproperty float x\nproperty float y\nproperty float z\n"
+             "element face 1\nproperty list uchar int vertex_indices\nend_header\n"
+             + "".join(f"{a} {b} {c}\n" for a, b, c in verts) + "4 0 1 2 3\n")
+    (tmp_path / "a.ply").write_text(ascii)
+    head = ("ply\nformat binary_little_endian 1.0\nelement vertex 4\nproperty float x\nproperty float y\n"
+            "property float z\nproperty float nx\nelement face 1\nproperty list uchar uint vertex_indices\n"
+            "end_header\n").encode()
+    body = np.concatenate([verts, np.zeros((4, 1), np.float32)], axis=1).tobytes()
+    body += bytes([4]) + np.array([0, 1, 2, 3], "<u4").tobytes()
+    (tmp_path / "b.ply").write_bytes(head + body)
+    for name in ("a.ply", "b.ply"):
+        sc = rp.PbrtScene(text=HEADER + f'Shape "plymesh" "string filename" "{name}"\n', base_dir=tmp_path)
+        d = sc.d
+        assert d.triangle_count == 2
+        np.testing.assert_array_equal(positions(d), verts)
+        np.testing.assert_array_equal(np.ctypeslib.as_array(d.indices, shape=(6,)), [0, 1, 2, 0, 2, 3])
+
+
+def test_include_and_named_materials(tmp_path):
+    (tmp_path / "geo.pbrt").write_text('NamedMaterial "m"\n' + TRI)
+    text = HEADER + 'MakeNamedMaterial "m" "string type" "diffuse" "rgb reflectance" [0.2 0.4 0.6]\nInclude "geo.pbrt"\n'
+    d = rp.PbrtScene(text=text, base_dir=tmp_path).d
+    assert d.triangle_count == 1 and d.objects[0].material == 1
+    assert d.materials[1].reflectance.kind == rp.SPECTRUM_SIGMOID
+
+
+@pytest.mark.parametrize("text,needle", [
+    (HEADER + 'Shape "sphere" "float radius" [1]\n', "unsupported shape 'sphere'"),
+    (HEADER + 'Material "coateddiffuse"\n', "unsupported material 'coateddiffuse'"),
+    (HEADER + 'LightSource "point"\n', "unsupported light 'point'"),
+    (HEADER + 'Texture "t" "spectrum" "imagemap"\n', "unsupported directive 'Texture'"),
+    (HEADER.replace('"sobol"', '"halton"'), "unsupported sampler 'halton'"),
+    (HEADER + 'NamedMaterial "nope"\n', "unknown material 'nope'"),
+    (HEADER + 'AttributeEnd\n', "unmatched AttributeEnd"),
+    (HEADER + 'Material "diffuse" "rgb reflectance" [2 0 0]\n', "reflectance rgb above 1"),
+    (HEADER + 'Material "conductor"\n', "explicit eta and k"),
+    ('Scale 2 2 2\n' + HEADER, "Camera transform must be rigid"),
+    ('WorldBegin\n' + TRI, "no Camera"),
+    (HEADER + 'Shape "trianglemesh" "integer indices" [0 1 5] "point3 P" [0 0 0 1 0 0 0 1 0]\n', "out of range"),
+])
+def test_unsupported_input_is_an_error(text, needle):
+    with pytest.raises(rp.RypathError) as e:
+        load(text)
+    assert needle in str(e.value)
+    assert e.value.code == rp.FAILURE
+
+
+def test_error_names_line():
+    with pytest.raises(rp.RypathError, match=r"<scene>:7: unsupported shape 'disk'"):
+        load(HEADER + 'Shape "disk"\n')
+
+
+# ------------------------------------------------------------------ EXR
+
+
+def film_2x1():
+    # mean XYZ (1.25, 0, 0) and (0, 1, 0)
+    return np.array([[[1.25, 0, 0], [0, 1, 0]]], np.float32)
+
+
+# linear-sRGB projection with exposure 2 (tests/exr_output_tests.cpp:164-170)
+EXPECTED_2X1 = np.array([8.1011107285475, -2.4231668814, 0.13910879918,
+                         -3.07426766109, 3.75202448596, -0.408052477596])
+
+
+@pytest.mark.parametrize("half,tol", [(False, 1e-6), (True, 0.004)])
+def test_exr_projection_golden(tmp_path, half, tol):
+    path = tmp_path / "result.exr"
+    rp.write_exr(path, film_2x1(), exposure=2.0, half=half, metadata={"rypath.test": "film-output"})
+    rgb, e = exr_util.read_rgb(path)
+    assert (e["width"], e["height"]) == (2, 1)
+    assert [c for c, _ in e["channels"]] == ["B", "G", "R"]
+    assert {t for _, t in e["channels"]} == {1 if half else 2}
+    np.testing.assert_allclose(rgb.reshape(-1), EXPECTED_2X1, atol=tol)
+    assert e["strings"]["rypath.test"] == "film-output"
+    assert not [p for p in os.listdir(tmp_path) if p.startswith(".rypath-exr")]
+
+
+def test_exr_rejections_keep_destination(tmp_path):
+    path = tmp_path / "result.exr"
+    path.write_text("existing")
+    film = film_2x1()
+    bad = film.copy()
+    bad[0, 0, 0] = np.inf
+    cases = [
+        dict(xyz=bad), dict(exposure=float("inf")), dict(exposure=1e308),
+        dict(xyz=film * 1e6, half=True), dict(metadata={"channels": "x"}), dict(metadata={"k" * 256: "v"}),
+    ]
+    for c in cases:
+        kw = dict(xyz=film, exposure=1.0, half=False, metadata=None)
+        kw.update(c)
+        with pytest.raises(rp.RypathError):
+            rp.write_exr(path, kw["xyz"], exposure=kw["exposure"], half=kw["half"], metadata=kw["metadata"])
+        assert path.read_text() == "existing"
+    with pytest.raises(rp.RypathError):
+        rp.write_exr(tmp_path / "missing" / "x.exr", film)
+    target = tmp_path / "directory.exr"
+    target.mkdir()
+    with pytest.raises(rp.RypathError):
+        rp.write_exr(target, film)
+    assert target.is_dir()
+    assert not [p for p in os.listdir(tmp_path) if p.startswith(".rypath-exr")]
+
+
+@pytest.mark.parametrize("length", [31, 32, 255])
+def test_exr_long_metadata_names(tmp_path, length):
+    path = tmp_path / "m.exr"
+    key = "k" * length
+    rp.write_exr(path, film_2x1(), metadata={key: "complete attribute value"})
+    e = exr_util.parse(path.read_bytes())
+    assert bool(e["version"] & 0x400) == (length > 31)
+    assert e["strings"][key] == "complete attribute value"
+
+
+def test_exr_wide_image(tmp_path):
+    path = tmp_path / "wide.exr"
+    xyz = np.zeros((1, 4097, 3), np.float32)
+    xyz[0, :, 1] = np.linspace(0, 1, 4097)
+    rp.write_exr(path, xyz)
+    rgb, e = exr_util.read_rgb(path)
+    assert e["width"] == 4097
+    np.testing.assert_allclose(rgb[0, :, 1], xyz[0, :, 1] * 1.876012242980, rtol=1e-6, atol=1e-7)
+
+
+# ------------------------------------------------------------------ CLI
+
+
+def run_cli(*args):
+    return subprocess.run([str(CLI), *map(str, args)], capture_output=True, text=True, timeout=60)
+
+
+def test_cli_usage_and_checks(tmp_path):
+    assert run_cli().returncode == 1
+    assert run_cli("--help").returncode == 0
+    r = run_cli("--check-pbrt", CORNELL)
+    assert r.returncode == 0 and "36 triangles" in r.stdout
+    bad = tmp_path / "bad.pbrt"
+    bad.write_text(HEADER + 'Shape "sphere"\n')
+    r = run_cli("--check-pbrt", bad)
+    assert r.returncode == 1 and "unsupported shape 'sphere'" in r.stderr
+    assert run_cli("--render-demo").returncode == 1
+    assert run_cli("--render-pbrt", CORNELL, "--output", tmp_path / "x.png").returncode == 1
+    assert run_cli("--render-demo", "--output", tmp_path / "x.exr", "--samples", "0").returncode == 1
+
+
+@pytest.mark.skipif(rp.device_count() > 0, reason="exercises the no-device exit status")
+def test_cli_without_device_exits_2(tmp_path):
+    r = run_cli("--render-demo", "--output", tmp_path / "x.exr")
+    assert r.returncode == 2
+    assert not (tmp_path / "x.exr").exists()
