@@ -52,6 +52,24 @@ WORKLOADS = {
 }
 
 
+PMC_SUMMARY = ROOT / "profiles" / "terrain_1080p_hbm_traffic.md"
+
+
+def pmc_traffic():
+    """HBM bytes per closest-hit launch of the 1080p terrain wavefront frame,
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by
+    its copy calibration).  None when the summary is absent."""
+    try:
+        for line in PMC_SUMMARY.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if cells and cells[0] == "void rpd::k_wf_extend<false>":
+                launches, read_mib, write_mib = int(cells[1]), float(cells[4]), float(cells[5])
+                return round((read_mib + write_mib) * 2 ** 20 / launches)
+    except (OSError, ValueError, IndexError):
+        return None
+    return None
+
+
 def cpu_baseline(synth, width, height, budget):
     """Oracle (scalar C++ restatement) on a bounded band of pixels, all host threads."""
     import rypath_amd as rp
@@ -226,8 +244,12 @@ def main():
         st = results[frame_mode]["stats"]
         alg = RAY_BYTES * cnt["extension_rays"] + NODE_BYTES * cnt["node_fetches"] + TRI_BYTES * cnt["triangle_tests"]
         achieved = alg / st["trace_seconds"] / 1e9
+        traffic = pmc_traffic() if (args.workload == "terrain" and (args.width, args.height, args.spp) ==
+                                    (1920, 1080, 16) and frame_mode == "wave") else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_source": PMC_SUMMARY.relative_to(ROOT).as_posix() if traffic else None,
+                    "algorithmic_bytes_per_launch": round(alg / max(st["extend_launches"], 1)),
                     "kernel": f"k_wf_extend (closest hit, {frame_mode} frame)",
                     "per_ray_bytes": round(alg / cnt["extension_rays"], 1),
                     "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),

@@ -9,6 +9,10 @@
  *   rp_frame_*     asynchronous frame submission with two retained slots
  *                  (counterpart of FrameSession::submit / Frame::poll / wait,
  *                  src/frame_session.hpp:145-212)
+ *   rp_film_*, rp_checkpoint_*
+ *                  running films, checkpoint and resume (counterpart of
+ *                  Film + write_checkpoint / read_checkpoint,
+ *                  src/render.hpp:317-362, src/checkpoint.hpp:8-24)
  */
 #ifndef RP_IO_H
 #define RP_IO_H
@@ -45,6 +49,42 @@ void rp_pbrt_destroy(rp_pbrt *scene);
  * temporary and renamed into place. */
 int rp_write_exr(const char *path, uint32_t width, uint32_t height, const float *xyz, double exposure,
                  uint32_t half, const char *const *keys, const char *const *values, uint32_t count);
+
+/* Running films and checkpoints (counterpart of the progressive Film with
+ * write_checkpoint / read_checkpoint, src/render.hpp:317-362,
+ * src/checkpoint.hpp:8-24).  A film holds the compensated per-pixel sums of
+ * the samples [0, samples) of one scene, seed and randomization.
+ * rp_film_accumulate renders the next config->spp samples; config->sample_begin
+ * must equal the film's sample count, and after the first accumulation the
+ * seed, randomization and scene identity must match the film's.  A render
+ * resumed from a checkpoint equals the uninterrupted render bit for bit.
+ *
+ * Checkpoint file (format 1, little endian):
+ *   "RPAFILM\n", u32 format = 1, u32 width, u32 height, u32 randomization,
+ *   u64 seed, u64 samples, u8[32] scene identity, u32 segment count,
+ *   segments { u64 begin, u64 end, u32 n, n bytes device }, then
+ *   6 * width * height f64 planes (sum X, corr X, sum Y, corr Y, sum Z, corr Z),
+ *   then the SHA-256 of all preceding bytes.
+ * Files are published through a sibling temporary, fsync and rename. */
+typedef struct rp_film rp_film;
+typedef struct rp_film_info {
+    uint32_t width, height;
+    uint64_t samples, seed;
+    uint32_t randomization, segments;
+    uint8_t scene_identity[32]; /* SHA-256 of the scene description (zero until bound) */
+} rp_film_info;
+int rp_film_create(uint32_t width, uint32_t height, rp_film **out);
+int rp_film_accumulate(rp_scene *scene, rp_film *film, const rp_frame_config *config, float *xyz_host,
+                       rp_frame_stats *stats);
+int rp_film_resolve(const rp_film *film, float *xyz_host);
+int rp_film_get_info(const rp_film *film, rp_film_info *info);
+int rp_checkpoint_write(const rp_film *film, const char *path);
+int rp_checkpoint_read(const char *path, rp_film **out);
+void rp_film_destroy(rp_film *film);
+/* SHA-256 identity of the scene's description (the value films bind to). */
+int rp_scene_identity(rp_scene *scene, uint8_t identity[32]);
+/* SHA-256 (FIPS 180-4) of a byte range. */
+void rp_sha256(const void *data, size_t size, uint8_t digest[32]);
 
 /* Asynchronous frames: submit enqueues a whole frame on the scene's stream
  * and returns immediately; poll reports completion; wait blocks and copies

@@ -189,6 +189,21 @@ _sig("rp_pbrt_get_info", C.c_int, [C.c_void_p, C.POINTER(PbrtInfo)])
 _sig("rp_pbrt_destroy", None, [C.c_void_p])
 _sig("rp_write_exr", C.c_int, [C.c_char_p, u32, u32, C.POINTER(f32), f64, u32, C.POINTER(C.c_char_p),
                                C.POINTER(C.c_char_p), u32])
+class FilmInfo(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("samples", u64), ("seed", u64), ("randomization", u32),
+                ("segments", u32), ("scene_identity", C.c_uint8 * 32)]
+
+
+_sig("rp_film_create", C.c_int, [u32, u32, C.POINTER(C.c_void_p)])
+_sig("rp_film_accumulate", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FrameConfig), C.c_void_p,
+                                     C.POINTER(FrameStats)])
+_sig("rp_film_resolve", C.c_int, [C.c_void_p, C.c_void_p])
+_sig("rp_film_get_info", C.c_int, [C.c_void_p, C.POINTER(FilmInfo)])
+_sig("rp_checkpoint_write", C.c_int, [C.c_void_p, C.c_char_p])
+_sig("rp_checkpoint_read", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)])
+_sig("rp_film_destroy", None, [C.c_void_p])
+_sig("rp_scene_identity", C.c_int, [C.c_void_p, C.POINTER(C.c_uint8)])
+_sig("rp_sha256", None, [C.c_void_p, C.c_size_t, C.POINTER(C.c_uint8)])
 _sig("rp_frame_submit", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.POINTER(C.c_void_p)])
 _sig("rp_frame_poll", C.c_int, [C.c_void_p, C.POINTER(u32)])
 _sig("rp_frame_wait", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FrameStats)])
@@ -201,6 +216,8 @@ EXPORTED = [
     "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
     "rp_pbrt_load", "rp_pbrt_load_string", "rp_pbrt_desc", "rp_pbrt_get_info", "rp_pbrt_destroy",
     "rp_write_exr", "rp_frame_submit", "rp_frame_poll", "rp_frame_wait", "rp_frame_release",
+    "rp_film_create", "rp_film_accumulate", "rp_film_resolve", "rp_film_get_info", "rp_checkpoint_write",
+    "rp_checkpoint_read", "rp_film_destroy", "rp_scene_identity", "rp_sha256",
 ]
 
 
@@ -236,6 +253,7 @@ class Synthetic:
         _check("rp_synthetic_create", _lib.rp_synthetic_create(kind, seed, C.byref(o), C.byref(h)))
         self._h = h
         self.desc = _lib.rp_synthetic_desc(h)
+        self.desc._owner = self  # the pointer keeps the storage it points into alive
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -356,6 +374,11 @@ class Scene:
                                                   occ.ctypes.data_as(C.c_void_p), C.byref(sec)))
         return occ, sec.value
 
+    def identity(self):
+        d = (C.c_uint8 * 32)()
+        _check("rp_scene_identity", _lib.rp_scene_identity(self._h, d))
+        return bytes(d)
+
     def submit(self, cfg):
         """Asynchronous frame (rp_frame_submit); returns a Frame, or raises with code BUSY
         when two frames are already retained."""
@@ -391,6 +414,58 @@ class Frame:
         self.release()
 
 
+class Film:
+    """Running film with checkpoint/resume (rp_film_*, rp_checkpoint_*)."""
+
+    def __init__(self, width=None, height=None, handle=None):
+        if handle is None:
+            h = C.c_void_p()
+            _check("rp_film_create", _lib.rp_film_create(width, height, C.byref(h)))
+            handle = h
+        self._h = handle
+
+    @classmethod
+    def read(cls, path):
+        h = C.c_void_p()
+        _check("rp_checkpoint_read", _lib.rp_checkpoint_read(str(path).encode(), C.byref(h)))
+        return cls(handle=h)
+
+    def write(self, path):
+        _check("rp_checkpoint_write", _lib.rp_checkpoint_write(self._h, str(path).encode()))
+
+    def info(self):
+        i = FilmInfo()
+        _check("rp_film_get_info", _lib.rp_film_get_info(self._h, C.byref(i)))
+        out = {f: getattr(i, f) for f, _ in FilmInfo._fields_}
+        out["scene_identity"] = bytes(i.scene_identity)
+        return out
+
+    def accumulate(self, scene, cfg, film=True):
+        i = self.info()
+        out = np.zeros((i["height"], i["width"], 3), np.float32) if film else None
+        st = FrameStats()
+        ptr = out.ctypes.data_as(C.c_void_p) if film else None
+        _check("rp_film_accumulate", _lib.rp_film_accumulate(scene.handle, self._h, C.byref(cfg), ptr, C.byref(st)))
+        return out, {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
+    def resolve(self):
+        i = self.info()
+        out = np.zeros((i["height"], i["width"], 3), np.float32)
+        _check("rp_film_resolve", _lib.rp_film_resolve(self._h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.rp_film_destroy(self._h)
+            self._h = None
+
+
+def sha256(data: bytes) -> bytes:
+    d = (C.c_uint8 * 32)()
+    _lib.rp_sha256(data, len(data), d)
+    return bytes(d)
+
+
 class PbrtScene:
     """PBRT-v4 scene-file subset translated to a scene description (rp_pbrt_load)."""
 
@@ -402,6 +477,7 @@ class PbrtScene:
             _check("rp_pbrt_load", _lib.rp_pbrt_load(str(path).encode(), C.byref(h)))
         self._h = h
         self.desc = _lib.rp_pbrt_desc(h)
+        self.desc._owner = self  # the pointer keeps the storage it points into alive
         i = PbrtInfo()
         _check("rp_pbrt_get_info", _lib.rp_pbrt_get_info(h, C.byref(i)))
         self.info = {f: getattr(i, f) for f, _ in PbrtInfo._fields_}

@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(BS) k_pool_shade(PoolView V, uint32_t p, uint3
             shadow = sr.active;
             cont = cont && path_alive(BSc.S, ps);
             retire = !cont;
-            store_state(V.B, slot, ps);
+            store_state<false>(V.B, slot, ps);
             if (shadow) {
                 V.B.sh_o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
                 V.B.sh_d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);

@@ -7,6 +7,7 @@
 #include "../../include/rp_io.h"
 #include "../../include/rp_synthetic.h"
 
+#include <algorithm>
 #include <cerrno>
 #include <chrono>
 #include <cstdio>
@@ -47,17 +48,18 @@ void usage() {
     std::puts("usage:\n"
               "  rypath --render-pbrt SCENE.pbrt --output IMAGE.exr [--samples N] [--seed S]\n"
               "         [--mode megakernel|wavefront|sorted] [--hero-wavelengths uniform|visible]\n"
-              "         [--exposure E] [--half] [--timings]\n"
+              "         [--exposure E] [--half] [--timings] [--batch-size N]\n"
+              "         [--checkpoint PATH] [--resume PATH] [--checkpoint-interval SECONDS (default: 60)]\n"
               "  rypath --check-pbrt SCENE.pbrt\n"
               "  rypath --render-demo --output IMAGE.exr [--width W] [--height H] [--samples N] [--seed S]\n"
               "  rypath --device-info | --version");
 }
 
 struct Options {
-    std::string scene, output;
-    uint64_t samples = 0, seed = 0, width = 256, height = 256;
+    std::string scene, output, checkpoint, resume;
+    uint64_t samples = 0, seed = 0, width = 256, height = 256, batch = 0;
     bool have_seed = false, half = false, timings = false;
-    double exposure = 1;
+    double exposure = 1, interval = 60;
     uint32_t mode = RP_MODE_WAVEFRONT_SORTED;
     int hero = -1;
 };
@@ -76,7 +78,16 @@ int parse_options(int argc, char **argv, int first, Options &o) {
         if (i + 1 >= argc || std::strncmp(argv[i + 1], "--", 2) == 0) return fail(k + " requires a value");
         const char *v = argv[++i];
         if (k == "--output") o.output = v;
-        else if (k == "--samples") {
+        else if (k == "--checkpoint") o.checkpoint = v;
+        else if (k == "--resume") o.resume = v;
+        else if (k == "--checkpoint-interval") {
+            char *end = nullptr;
+            o.interval = std::strtod(v, &end);
+            if (*end || !(o.interval >= 0)) return fail("--checkpoint-interval must be a nonnegative number of seconds");
+        } else if (k == "--batch-size") {
+            if (!parse_u64(v, o.batch) || o.batch == 0 || o.batch > 0xffffffffull)
+                return fail("--batch-size must be a positive 32-bit unsigned integer");
+        } else if (k == "--samples") {
             if (!parse_u64(v, o.samples) || o.samples == 0 || o.samples > 0xffffffffull)
                 return fail("--samples must be a positive 32-bit unsigned integer");
         } else if (k == "--seed") {
@@ -127,30 +138,80 @@ int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t s
     if (st != RP_SUCCESS) return status_exit(st, "scene build");
     rp_build_info info{};
     rp_scene_info(scene, &info);
-    rp_frame_config cfg{};
-    cfg.mode = o.mode;
-    cfg.spp = spp;
-    cfg.seed = seed;
-    cfg.randomization = randomization;
-    std::vector<float> xyz(size_t(desc.camera.width) * desc.camera.height * 3);
-    rp_frame_stats stats{};
-    st = rp_render_frame(scene, &cfg, xyz.data(), &stats);
+    // The film accumulates batches of samples; a resumed or checkpointed
+    // render equals the uninterrupted one bit for bit.
+    rp_film *film = nullptr;
+    if (!o.resume.empty()) st = rp_checkpoint_read(o.resume.c_str(), &film);
+    else st = rp_film_create(desc.camera.width, desc.camera.height, &film);
+    if (st != RP_SUCCESS) {
+        rp_scene_destroy(scene);
+        return status_exit(st, o.resume.empty() ? "film" : "resume");
+    }
+    rp_film_info fi{};
+    rp_film_get_info(film, &fi);
+    if (fi.samples > spp) {
+        rp_film_destroy(film);
+        rp_scene_destroy(scene);
+        return fail("the checkpoint holds more samples than requested");
+    }
+    const uint64_t batch = o.batch ? o.batch : (o.checkpoint.empty() ? spp : std::max<uint64_t>(1, spp / 16));
+    rp_frame_stats total{};
+    auto last_write = std::chrono::steady_clock::now();
+    while (fi.samples < spp) {
+        rp_frame_config cfg{};
+        cfg.mode = o.mode;
+        cfg.spp = uint32_t(std::min<uint64_t>(batch, spp - fi.samples));
+        cfg.sample_begin = fi.samples;
+        cfg.seed = seed;
+        cfg.randomization = randomization;
+        rp_frame_stats stats{};
+        st = rp_film_accumulate(scene, film, &cfg, nullptr, &stats);
+        if (st != RP_SUCCESS) {
+            rp_film_destroy(film);
+            rp_scene_destroy(scene);
+            return status_exit(st, "render");
+        }
+        total.seconds += stats.seconds;
+        total.paths += stats.paths;
+        total.extension_rays += stats.extension_rays;
+        total.shadow_rays += stats.shadow_rays;
+        total.failed_paths += stats.failed_paths;
+        rp_film_get_info(film, &fi);
+        const bool last = fi.samples == spp;
+        const double since = std::chrono::duration<double>(std::chrono::steady_clock::now() - last_write).count();
+        if (!o.checkpoint.empty() && (last || since >= o.interval)) {
+            if ((st = rp_checkpoint_write(film, o.checkpoint.c_str())) != RP_SUCCESS) {
+                rp_film_destroy(film);
+                rp_scene_destroy(scene);
+                return status_exit(st, "checkpoint");
+            }
+            last_write = std::chrono::steady_clock::now();
+        }
+    }
     rp_scene_destroy(scene);
-    if (st != RP_SUCCESS) return status_exit(st, "render");
-    if (stats.failed_paths) std::fprintf(stderr, "rypath: %llu paths failed\n", (unsigned long long)stats.failed_paths);
-    std::string spp_s = std::to_string(spp), seed_s = std::to_string(seed);
-    const char *keys[] = {"rypath.source", "rypath.samples", "rypath.seed"};
-    const char *vals[] = {source.c_str(), spp_s.c_str(), seed_s.c_str()};
+    std::vector<float> xyz(size_t(desc.camera.width) * desc.camera.height * 3);
+    st = rp_film_resolve(film, xyz.data());
+    rp_film_destroy(film);
+    if (st != RP_SUCCESS) return status_exit(st, "resolve");
+    if (total.failed_paths) std::fprintf(stderr, "rypath: %llu paths failed\n", (unsigned long long)total.failed_paths);
+    std::string spp_s = std::to_string(spp), seed_s = std::to_string(seed), ident;
+    for (uint8_t b : fi.scene_identity) {
+        char h[3];
+        std::snprintf(h, sizeof h, "%02x", b);
+        ident += h;
+    }
+    const char *keys[] = {"rypath.source", "rypath.samples", "rypath.seed", "rypath.checkpointIdentity"};
+    const char *vals[] = {source.c_str(), spp_s.c_str(), seed_s.c_str(), ident.c_str()};
     st = rp_write_exr(o.output.c_str(), desc.camera.width, desc.camera.height, xyz.data(), o.exposure, o.half ? 1 : 0,
-                      keys, vals, 3);
+                      keys, vals, 4);
     if (st != RP_SUCCESS) return status_exit(st, "write");
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (o.timings)
         std::printf("bvh_build_s=%.4f frame_s=%.4f paths=%llu mpaths_per_s=%.2f extension_rays=%llu "
                     "shadow_rays=%llu wall_s=%.3f\n",
-                    info.build_seconds, stats.seconds, (unsigned long long)stats.paths,
-                    stats.paths / stats.seconds * 1e-6, (unsigned long long)stats.extension_rays,
-                    (unsigned long long)stats.shadow_rays, wall);
+                    info.build_seconds, total.seconds, (unsigned long long)total.paths,
+                    total.seconds > 0 ? total.paths / total.seconds * 1e-6 : 0.0,
+                    (unsigned long long)total.extension_rays, (unsigned long long)total.shadow_rays, wall);
     return 0;
 }
 

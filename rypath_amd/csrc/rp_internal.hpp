@@ -108,4 +108,30 @@ struct HostScene {
 };
 std::string validate_desc(const rp_scene_desc &d);
 
+// SHA-256 over a canonical little-endian serialisation of every field of
+// the description that changes rendered values.
+std::array<uint8_t, 32> scene_identity(const rp_scene_desc &d);
+
 } // namespace rp
+
+struct rp_film;
+namespace rp {
+// Consistency of a running film (dimensions, provenance, finite sums).
+std::string validate_film(const rp_film &f);
+} // namespace rp
+
+// Running raw-XYZ film (include/rp_io.h): the compensated per-pixel sums of
+// the samples [0, samples), bound to one scene, seed and randomization.
+struct rp_film {
+    uint32_t width = 0, height = 0;
+    uint64_t samples = 0;
+    uint64_t seed = 0;
+    uint32_t randomization = 0;
+    std::array<uint8_t, 32> identity{}; // scene identity (valid when samples > 0)
+    std::vector<double> planes;         // 6 * width * height: sum X, corr X, sum Y, corr Y, sum Z, corr Z
+    struct Segment {
+        uint64_t begin, end;
+        std::string device;
+    };
+    std::vector<Segment> segments; // provenance: sample ranges and the device that rendered them
+};

@@ -95,6 +95,9 @@ struct rp_scene {
     // asynchronous frame worker.
     std::mutex exec;
     std::shared_ptr<rp::FrameSession> frames;
+    std::array<uint8_t, 32> identity{}; // scene_identity(desc), computed on first use
+    bool has_identity = false;
+    std::string device_name;            // provenance of film segments
     ~rp_scene();
 };
 
@@ -224,12 +227,12 @@ __global__ void k_film_zero(uint64_t n, double *film) {
     if (i < n) film[i] = 0.0;
 }
 
-__global__ void k_film_finish(uint32_t npx, uint32_t spp, const double *film, float *out) {
+__global__ void k_film_finish(uint32_t npx, uint64_t samples, const double *film, float *out) {
     uint32_t pixel = blockIdx.x * blockDim.x + threadIdx.x;
     if (pixel >= npx) return;
     for (int c = 0; c < 3; ++c)
         out[3 * size_t(pixel) + c] =
-            float((film[(2 * c) * size_t(npx) + pixel] + film[(2 * c + 1) * size_t(npx) + pixel]) / double(spp));
+            float((film[(2 * c) * size_t(npx) + pixel] + film[(2 * c + 1) * size_t(npx) + pixel]) / double(samples));
 }
 
 __global__ void __launch_bounds__(kBlock) k_trace_closest(rpd::Scene S, const rp_ray *rays, uint64_t n, rp_hit *hits) {
@@ -370,8 +373,18 @@ rpd::PathParams path_params(uint64_t seed, uint32_t randomization, uint32_t emer
     return P;
 }
 
+// Continuation of a running film: the compensated sums (6 planes: sum and
+// correction per XYZ channel) after `samples` samples per pixel.  Adding the
+// next samples in order continues the same Neumaier recurrence, so a resumed
+// render equals an uninterrupted one bit for bit.
+struct FilmCarry {
+    std::vector<double> *planes; // in: state before the frame; out: state after
+    uint64_t samples;            // samples per pixel already in `planes`
+};
+
 // Renders the frame into the scene's film buffer; returns seconds (device events).
-double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_device, rp_frame_stats *stats) {
+double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_device, rp_frame_stats *stats,
+                         FilmCarry *carry = nullptr) {
     const rp_camera &cam = s->host.desc.camera;
     const uint32_t npx = cam.width * cam.height;
     const uint32_t spp = cfg.spp;
@@ -383,8 +396,14 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     const uint64_t max_chunk_paths = uint64_t(1) << 26;
     uint32_t chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(spp, max_chunk_paths / npx)));
     s->film.alloc(6 * size_t(npx));
-    hipLaunchKernelGGL(k_film_zero, dim3(uint32_t((6 * uint64_t(npx) + 255) / 256)), dim3(256), 0, s->stream,
-                       6 * uint64_t(npx), s->film.p);
+    if (carry && carry->samples) {
+        if (carry->planes->size() != 6 * size_t(npx)) throw std::runtime_error("film size does not match the scene");
+        HIP_CHECK(hipMemcpyAsync(s->film.p, carry->planes->data(), 6 * size_t(npx) * sizeof(double),
+                                 hipMemcpyHostToDevice, s->stream));
+    } else {
+        hipLaunchKernelGGL(k_film_zero, dim3(uint32_t((6 * uint64_t(npx) + 255) / 256)), dim3(256), 0, s->stream,
+                           6 * uint64_t(npx), s->film.p);
+    }
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, s->counters.p, 8u);
     hipEvent_t e0, e1;
     HIP_CHECK(hipEventCreate(&e0));
@@ -407,8 +426,18 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
                            s->film.p);
     }
-    hipLaunchKernelGGL(k_film_finish, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, spp, s->film.p, xyz_device);
+    const uint64_t total = (carry ? carry->samples : 0) + spp;
+    if (xyz_device)
+        hipLaunchKernelGGL(k_film_finish, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, total, s->film.p,
+                           xyz_device);
     HIP_CHECK(hipEventRecord(e1, s->stream));
+    if (carry) {
+        carry->planes->resize(6 * size_t(npx));
+        HIP_CHECK(hipMemcpyAsync(carry->planes->data(), s->film.p, 6 * size_t(npx) * sizeof(double),
+                                 hipMemcpyDeviceToHost, s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        carry->samples = total;
+    }
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
@@ -654,6 +683,77 @@ int rp_render_frame(rp_scene *s, const rp_frame_config *cfg, float *xyz_host, rp
         return RP_SUCCESS;
     } catch (const std::exception &e) {
         return fail("rp_render_frame", e);
+    }
+}
+
+int rp_scene_identity(rp_scene *s, uint8_t identity[32]) {
+    if (!s || !identity) {
+        set_error("rp_scene_identity: null argument");
+        return RP_INVALID;
+    }
+    if (!s->has_identity) {
+        s->identity = scene_identity(s->host.desc);
+        s->has_identity = true;
+    }
+    std::memcpy(identity, s->identity.data(), 32);
+    return RP_SUCCESS;
+}
+
+int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, float *xyz_host, rp_frame_stats *stats) {
+    if (!s || !f || !cfg) {
+        set_error("rp_film_accumulate: null argument");
+        return RP_INVALID;
+    }
+    const rp_camera &cam = s->host.desc.camera;
+    if (f->width != cam.width || f->height != cam.height) {
+        set_error("rp_film_accumulate: film size differs from the scene's camera");
+        return RP_INVALID;
+    }
+    if (cfg->sample_begin != f->samples) {
+        set_error("rp_film_accumulate: sample_begin must equal the film's sample count " + std::to_string(f->samples));
+        return RP_INVALID;
+    }
+    uint8_t id[32];
+    rp_scene_identity(s, id);
+    if (f->samples) {
+        if (cfg->seed != f->seed || cfg->randomization != f->randomization) {
+            set_error("rp_film_accumulate: seed and randomization must match the film's");
+            return RP_INVALID;
+        }
+        if (std::memcmp(id, f->identity.data(), 32) != 0) {
+            set_error("rp_film_accumulate: the film belongs to another scene (identity mismatch)");
+            return RP_INVALID;
+        }
+    }
+    try {
+        std::lock_guard<std::mutex> lock(s->exec);
+        HIP_CHECK(hipSetDevice(s->device));
+        if (s->device_name.empty()) {
+            hipDeviceProp_t prop{};
+            HIP_CHECK(hipGetDeviceProperties(&prop, s->device));
+            s->device_name = std::string(prop.name) + " " + prop.gcnArchName;
+        }
+        const uint32_t npx = cam.width * cam.height;
+        DevBuf<float> out;
+        if (xyz_host) out.alloc(3 * size_t(npx));
+        std::vector<double> planes = f->planes;
+        FilmCarry carry{&planes, f->samples};
+        render_frame_impl(s, *cfg, xyz_host ? out.p : nullptr, stats, &carry);
+        if (xyz_host) HIP_CHECK(hipMemcpy(xyz_host, out.p, 3 * size_t(npx) * sizeof(float), hipMemcpyDeviceToHost));
+        // commit only after the whole frame succeeded (all-or-nothing, as FilmBatch::commit)
+        const uint64_t begin = f->samples;
+        f->planes.swap(planes);
+        f->samples = carry.samples;
+        f->seed = cfg->seed;
+        f->randomization = cfg->randomization;
+        std::memcpy(f->identity.data(), id, 32);
+        if (!f->segments.empty() && f->segments.back().device == s->device_name && f->segments.back().end == begin)
+            f->segments.back().end = f->samples;
+        else
+            f->segments.push_back({begin, f->samples, s->device_name});
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_film_accumulate", e);
     }
 }
 

@@ -79,6 +79,9 @@ RP_DEV void load_state(const WfState &B, uint32_t i, PathState &ps) {
     ps.flags = u.z;
     ps.status = u.w;
 }
+// kIds: also write the identity word (pixel, sample, wavelength word), which
+// never changes after generation; the shading kernels skip it.
+template <bool kIds = true>
 RP_DEV void store_state(const WfState &B, uint32_t i, const PathState &ps) {
     B.o_t[i] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.tmax);
     B.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
@@ -89,7 +92,7 @@ RP_DEV void store_state(const WfState &B, uint32_t i, const PathState &ps) {
     B.eta[i] = make_float4(ps.eta_scale[0], ps.eta_scale[1], ps.eta_scale[2], ps.eta_scale[3]);
     B.L[i] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
     B.prev_p[i] = make_float4(ps.prev_p.x, ps.prev_p.y, ps.prev_p.z, __uint_as_float(ps.primary));
-    B.id[i] = make_uint4(ps.pixel, uint32_t(ps.sample), uint32_t(ps.sample >> 32), ps.w2);
+    if (kIds) B.id[i] = make_uint4(ps.pixel, uint32_t(ps.sample), uint32_t(ps.sample >> 32), ps.w2);
     B.meta[i] = make_uint4(uint32_t(ps.index), uint32_t(ps.index >> 32), ps.count, ps.crossings);
     B.meta2[i] = make_uint4(ps.medium, ps.mdraw, ps.flags, ps.status);
 }
@@ -242,7 +245,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shade(Scene S, PathParams P, Wf
         cont = path_event(S, P, ps, h, sr, nh);
         shadow = sr.active;
         cont = cont && path_alive(S, ps);
-        store_state(B, slot, ps);
+        store_state<false>(B, slot, ps);
         if (shadow) {
             B.sh_o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
             B.sh_d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);

@@ -98,3 +98,53 @@ def test_cli_render_pbrt(gpu_available, tmp_path):
     assert (e["width"], e["height"]) == (256, 256)
     assert {t for _, t in e["channels"]} == {1}
     assert np.isfinite(rgb).all() and rgb.mean() > 0
+
+
+def test_checkpoint_resume_equals_uninterrupted(rp, gpu_available, tmp_path):
+    """Accumulate 8 + 8 samples, checkpoint, read back, accumulate 16 more:
+    the film equals one 32-sample frame bit for bit (the Neumaier state is
+    the whole accumulator), and the running means equal shorter frames."""
+    s = rp.Synthetic(rp.SYNTH_VOLUME, seed=3, width=40, height=30, volume_subdivision=3)
+    scene = rp.Scene(s.desc)
+    ref32, _ = scene.render_frame(rp.frame_config(32, mode=rp.MODE_WAVEFRONT, seed=4))
+    ref16, _ = scene.render_frame(rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=4))
+    film = rp.Film(40, 30)
+    for begin in (0, 8):
+        running, _ = film.accumulate(scene, rp.frame_config(8, mode=rp.MODE_WAVEFRONT, sample_begin=begin, seed=4))
+    assert np.array_equal(running, ref16) and np.array_equal(film.resolve(), ref16)
+    path = tmp_path / "film.ckpt"
+    film.write(path)
+    resumed = rp.Film.read(path)
+    info = resumed.info()
+    assert info["samples"] == 16 and info["scene_identity"] == scene.identity() and info["segments"] == 1
+    out, _ = resumed.accumulate(scene, rp.frame_config(16, mode=rp.MODE_MEGAKERNEL, sample_begin=16, seed=4))
+    assert np.array_equal(out, ref32) and np.array_equal(resumed.resolve(), ref32)
+    print("[checkpoint] resumed film equals the uninterrupted 32-sample frame", flush=True)
+    # continuation rules
+    with pytest.raises(rp.RypathError):
+        resumed.accumulate(scene, rp.frame_config(4, sample_begin=16, seed=4))  # not at the film's end
+    with pytest.raises(rp.RypathError):
+        resumed.accumulate(scene, rp.frame_config(4, sample_begin=32, seed=5))  # another seed
+    other = rp.Scene(rp.Synthetic(rp.SYNTH_VOLUME, seed=4, width=40, height=30, volume_subdivision=3).desc)
+    with pytest.raises(rp.RypathError) as e:
+        resumed.accumulate(other, rp.frame_config(4, sample_begin=32, seed=4))
+    assert "another scene" in str(e.value)
+
+
+def test_cli_checkpoint_and_resume(gpu_available, tmp_path):
+    ck = tmp_path / "run.ckpt"
+    full = tmp_path / "full.exr"
+    assert run_cli("--render-demo", "--output", full, "--width", 32, "--height", 32, "--samples", 24).returncode == 0
+    part = tmp_path / "part.exr"
+    r = run_cli("--render-demo", "--output", part, "--width", 32, "--height", 32, "--samples", 8,
+                "--checkpoint", ck, "--batch-size", 4)
+    assert r.returncode == 0 and ck.exists()
+    resumed = tmp_path / "resumed.exr"
+    r = run_cli("--render-demo", "--output", resumed, "--width", 32, "--height", 32, "--samples", 24,
+                "--resume", ck, "--checkpoint", ck)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    a, _ = exr_util.read_rgb(full)
+    b, e = exr_util.read_rgb(resumed)
+    assert np.array_equal(a, b)
+    assert len(e["strings"]["rypath.checkpointIdentity"]) == 64
