@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
 
 RP_DEV uint32_t pool_key(const PoolView &V, uint32_t slot, uint32_t bits) {
     const Scene &S = V.scenes[V.sid[slot]].S;
-    return material_key(S, V.B, slot, bits);
+    return material_key(S, V.B.hit_id, slot, bits);
 }
 __global__ void k_pool_sort_hist(PoolView V, uint32_t p, uint32_t bits, uint32_t *hist) {
     __shared__ uint32_t h[256];

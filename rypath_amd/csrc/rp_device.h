@@ -588,24 +588,25 @@ struct Lane {
     bool smooth;
 };
 RP_DEV Lane prepare_lane(const Scene &S, const rp_material &m, float lambda) {
+    const uint32_t kind = m.kind;
     Lane l;
-    l.kind = m.kind;
+    l.kind = kind;
     l.r = 0;
     l.eta = 1;
     l.k = 0;
     l.ax = l.ay = 0;
     l.smooth = false;
-    if (m.kind == RP_MATERIAL_DIFFUSE) {
+    if (kind == RP_MATERIAL_DIFFUSE) {
         l.r = spectrum_eval(m.reflectance, lambda, S.spectra);
         return l;
     }
     l.eta = spectrum_eval(m.eta, lambda, S.spectra);
-    if (m.kind == RP_MATERIAL_CONDUCTOR) l.k = spectrum_eval(m.k, lambda, S.spectra);
+    if (kind == RP_MATERIAL_CONDUCTOR) l.k = spectrum_eval(m.k, lambda, S.spectra);
     float u = m.roughness_u, v = m.roughness_v;
     l.ax = m.remap_roughness ? __builtin_sqrtf(u) : u;
     l.ay = m.remap_roughness ? __builtin_sqrtf(v) : v;
-    if (m.kind == RP_MATERIAL_DIELECTRIC && l.eta == 0.0f) l.eta = 1.0f;
-    l.smooth = omax(l.ax, l.ay) < 1e-3f || (m.kind == RP_MATERIAL_DIELECTRIC && l.eta == 1.0f);
+    if (kind == RP_MATERIAL_DIELECTRIC && l.eta == 0.0f) l.eta = 1.0f;
+    l.smooth = omax(l.ax, l.ay) < 1e-3f || (kind == RP_MATERIAL_DIELECTRIC && l.eta == 1.0f);
     if (!l.smooth) {
         l.ax = omax(l.ax, 1e-4f);
         l.ay = omax(l.ay, 1e-4f);
@@ -1000,12 +1001,15 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, flo
 
 // Visibility with transmittance through null interfaces and media.
 // With `mlen` the length travelled inside media is accumulated (adjoint).
+// kInterfaces = false compiles only the any-hit branch (scenes without null
+// interfaces), which keeps the shadow kernel's register footprint small.
+template <bool kInterfaces = true>
 RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uint32_t medium, const Wavelengths &wl,
                           float tr[4], uint32_t boundary_limit, float *mlen = nullptr,
                           uint32_t mlen_medium = RP_NONE) {
     for (int k = 0; k < 4; ++k) tr[k] = 1.0f;
     if (mlen) *mlen = 0.0f;
-    if (!S.has_interfaces) {
+    if (!kInterfaces || !S.has_interfaces) {
         Hit h;
         if (traverse<true>(S, st, o, d, tmax, h)) return false;
         if (medium != RP_NONE) {

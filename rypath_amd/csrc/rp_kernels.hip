@@ -90,7 +90,7 @@ struct rp_scene {
     DevBuf<rp_hit> hits;
     DevBuf<uint8_t> occluded;
     hipStream_t stream = nullptr;
-    rpd::WavefrontBuffers wf;
+    rpd::FrameBuffers wf;
     // Serialises device work on this scene between API calls and the
     // asynchronous frame worker.
     std::mutex exec;

@@ -1,7 +1,9 @@
 // Wavefront form of the path tracer: generate -> extend -> (sort) -> shade ->
 // shadow -> accumulate, with path state in HBM (structure of arrays, 16-byte
-// fields) and queues compacted with wave ballots.  The per-event arithmetic is
-// rp_path.h's, so a wavefront frame equals the megakernel frame bit for bit.
+// fields).  The frame path compacts the state itself every bounce (two state
+// sets, wave-ballot appends); the batch pool keeps slots in place and
+// compacts index queues.  The per-event arithmetic is rp_path.h's, so a
+// wavefront frame equals the megakernel frame bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -118,19 +120,39 @@ RP_DEV void wave_count(unsigned long long *counter, uint32_t v) {
 
 constexpr int kWfBlock = 128;
 
+// ------------------------------------------------------------ frame wavefront
+// The frame path keeps the live paths' state physically compacted: bounce k
+// reads state set C[0, qn) and the shading kernel appends the surviving paths
+// to set N (wave-aggregated positions), so every state, hit and shadow access
+// is a dense, coalesced stream.  A path that ends leaves its radiance and
+// (status, wavelength word) at its lane (pixel + npx * sample offset); a
+// shadow ray knows where its contribution goes: the survivor's new position,
+// or (high bit set) the lane of a path that ended at this event.
+struct FrameWf {
+    float4 *hit_t;
+    uint2 *hit_id;
+    float4 *sh_o, *sh_d, *sh_val;
+    uint4 *sh_meta; // (medium, no-MIS flag, wavelength word, destination)
+    float4 *L_end;  // per lane
+    uint2 *end;     // per lane: (status, wavelength word)
+    uint32_t *perm; // material-sorted order of [0, qn)
+    uint32_t *qcount; // [0] survivors, [1] shadow rays
+    uint32_t *hist;   // [256] histogram, [256] cursors
+};
+constexpr uint32_t kEnded = 0x80000000u;
+
 __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint32_t npx, uint64_t sample_begin,
-                                                       uint64_t n, WfState B) {
+                                                       uint64_t n, WfState C) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PathState ps;
     uint32_t w0, w1;
     path_begin(S, P, uint32_t(i % npx), sample_begin + i / npx, ps, w0, w1);
-    store_state(B, uint32_t(i), ps);
-    B.qa[i] = uint32_t(i);
+    store_state(C, uint32_t(i), ps);
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState B, const uint32_t *q, uint32_t qn,
+__global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, FrameWf F, uint32_t qn,
                                                           unsigned long long *counters) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
@@ -142,13 +164,12 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState B, cons
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t rays = 0, nodes = 0, tris = 0;
     if (i < qn) {
-        uint32_t slot = q[i];
-        float4 a = B.o_t[slot], b = B.d[slot];
+        float4 a = C.o_t[i], b = C.d[i];
         Hit h;
         TraversalCounts tc = {0, 0};
         traverse<false, kCount>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h, &tc);
-        B.hit_t[slot] = make_float4(h.t, h.b0, h.b1, h.b2);
-        B.hit_id[slot] = make_uint2(h.prim, h.id);
+        F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
+        F.hit_id[i] = make_uint2(h.prim, h.id);
         rays = 1;
         if (kCount) {
             nodes = tc.nodes;
@@ -162,8 +183,8 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState B, cons
     }
 }
 
-RP_DEV uint32_t material_key(const Scene &S, const WfState &B, uint32_t slot, uint32_t bits) {
-    uint32_t prim = B.hit_id[slot].x;
+RP_DEV uint32_t material_key(const Scene &S, const uint2 *hit_id, uint32_t slot, uint32_t bits) {
+    uint32_t prim = hit_id[slot].x;
     if (prim == kEmpty) return 0;
     uint32_t obj = __float_as_uint(S.tris[3 * prim + 1].w);
     uint32_t m = S.objects[obj].material;
@@ -174,14 +195,16 @@ RP_DEV uint32_t material_key(const Scene &S, const WfState &B, uint32_t slot, ui
     return key < K ? key : K - 1;
 }
 
-__global__ void __launch_bounds__(256) k_sort_hist(Scene S, WfState B, const uint32_t *q, uint32_t qn,
+// Counting sort of queue entries by material key; `q` = nullptr sorts the
+// positions [0, qn) themselves.
+__global__ void __launch_bounds__(256) k_sort_hist(Scene S, const uint2 *hit_id, const uint32_t *q, uint32_t qn,
                                                      uint32_t bits, uint32_t *hist) {
     __shared__ uint32_t h[256];
     uint32_t K = 1u << bits;
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
     __syncthreads();
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < qn) atomicAdd(&h[material_key(S, B, q[i], bits)], 1u);
+    if (i < qn) atomicAdd(&h[material_key(S, hit_id, q ? q[i] : i, bits)], 1u);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
         if (h[k]) atomicAdd(&hist[k], h[k]);
@@ -195,7 +218,7 @@ __global__ void k_sort_scan(uint32_t bits, uint32_t *hist) {
         acc += c;
     }
 }
-__global__ void __launch_bounds__(256) k_sort_scatter(Scene S, WfState B, const uint32_t *q, uint32_t qn,
+__global__ void __launch_bounds__(256) k_sort_scatter(Scene S, const uint2 *hit_id, const uint32_t *q, uint32_t qn,
                                                         uint32_t bits, uint32_t *hist, uint32_t *out) {
     __shared__ uint32_t h[256], base[256];
     uint32_t K = 1u << bits;
@@ -204,8 +227,8 @@ __global__ void __launch_bounds__(256) k_sort_scatter(Scene S, WfState B, const 
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t key = 0, rank = 0, slot = 0;
     if (i < qn) {
-        slot = q[i];
-        key = material_key(S, B, slot, bits);
+        slot = q ? q[i] : i;
+        key = material_key(S, hit_id, slot, bits);
         rank = atomicAdd(&h[key], 1u);
     }
     __syncthreads();
@@ -221,19 +244,31 @@ __global__ void k_wf_reset(uint32_t *qcount, uint32_t *hist, uint32_t bits) {
         if (k < K || (k >= 256 && k < 256 + K)) hist[k] = 0;
 }
 
-__global__ void __launch_bounds__(kWfBlock) k_wf_shade(Scene S, PathParams P, WfState B, const uint32_t *q,
-                                                         uint32_t qn, uint32_t *qnext) {
-    uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+// Minimum waves per SIMD requested for the shading kernel (register budget
+// 512 / waves); 0 leaves the choice to the compiler.
+#ifndef RP_SHADE_WAVES
+#define RP_SHADE_WAVES 0
+#endif
+#if RP_SHADE_WAVES > 0
+#define RP_SHADE_ATTR __attribute__((amdgpu_waves_per_eu(RP_SHADE_WAVES)))
+#else
+#define RP_SHADE_ATTR
+#endif
+
+// One path event for the path at C[j] when `valid`: survivors are appended
+// to N and shadow rays to the shadow stream (every lane of the wave must
+// call this together: the appends are wave-aggregated).
+RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, const WfState &N, const FrameWf &F,
+                      bool valid, uint32_t j, uint32_t npx, uint64_t sample_begin) {
     bool cont = false, shadow = false;
-    uint32_t slot = 0;
+    PathState ps;
     ShadowReq sr;
     sr.active = false;
-    if (i < qn) {
-        slot = q[i];
-        PathState ps;
-        load_state(B, slot, ps);
-        float4 ht = B.hit_t[slot];
-        uint2 hi = B.hit_id[slot];
+    uint32_t lane = 0;
+    if (valid) {
+        load_state(C, j, ps);
+        float4 ht = F.hit_t[j];
+        uint2 hi = F.hit_id[j];
         Hit h;
         h.prim = hi.x;
         h.id = hi.y;
@@ -245,21 +280,37 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shade(Scene S, PathParams P, Wf
         cont = path_event(S, P, ps, h, sr, nh);
         shadow = sr.active;
         cont = cont && path_alive(S, ps);
-        store_state<false>(B, slot, ps);
-        if (shadow) {
-            B.sh_o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
-            B.sh_d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
-            B.sh_val[slot] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
-            B.sh_meta[slot] = make_uint2(sr.medium, sr.delta_or_nomis ? 1u : 0u);
-        }
+        lane = uint32_t(ps.sample - sample_begin) * npx + ps.pixel;
     }
-    uint32_t pos = wave_append(cont, &B.qcount[0]);
-    if (cont) qnext[pos] = slot;
-    uint32_t spos = wave_append(shadow, &B.qcount[1]);
-    if (shadow) B.qs[spos] = slot;
+    const uint32_t pos = wave_append(cont, &F.qcount[0]);
+    if (cont) {
+        store_state(N, pos, ps);
+    } else if (valid) {
+        F.L_end[lane] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+        F.end[lane] = make_uint2(ps.status, ps.w2);
+    }
+    const uint32_t spos = wave_append(shadow, &F.qcount[1]);
+    if (shadow) {
+        F.sh_o[spos] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
+        F.sh_d[spos] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
+        F.sh_val[spos] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
+        F.sh_meta[spos] = make_uint4(sr.medium, sr.delta_or_nomis ? 1u : 0u, ps.w2, cont ? pos : (kEnded | lane));
+    }
 }
 
-__global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, WfState B, uint32_t sn,
+// Any material: reads C[i], or C[perm[i]] in material-sorted order.
+__global__ void __launch_bounds__(kWfBlock) RP_SHADE_ATTR k_wf_shade(Scene S, PathParams P, WfState C, WfState N,
+                                                                     FrameWf F, uint32_t qn, bool sorted, uint32_t npx,
+                                                                     uint64_t sample_begin) {
+    const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    const bool valid = i < qn;
+    shade_one(S, P, C, N, F, valid, valid ? (sorted ? F.perm[i] : i) : 0u, npx, sample_begin);
+}
+
+// Shadow rays of one bounce; adds visible contributions to the survivor's
+// radiance in N or to an ended path's lane.
+template <bool kInterfaces>
+__global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, WfState N, FrameWf F, uint32_t sn,
                                                           unsigned long long *counters) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
@@ -271,9 +322,8 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t rays = 0;
     if (i < sn) {
-        uint32_t slot = B.qs[i];
-        float4 o = B.sh_o[slot], d = B.sh_d[slot], v = B.sh_val[slot];
-        uint2 meta = B.sh_meta[slot];
+        float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
+        uint4 meta = F.sh_meta[i];
         ShadowReq sr;
         sr.o = {o.x, o.y, o.z};
         sr.tmax = o.w;
@@ -282,30 +332,31 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
         sr.val[0] = v.x, sr.val[1] = v.y, sr.val[2] = v.z, sr.val[3] = v.w;
         sr.medium = meta.x;
         sr.delta_or_nomis = meta.y != 0;
-        uint32_t w2 = B.id[slot].w;
-        Wavelengths wl = path_wavelengths(S, w2);
+        Wavelengths wl = path_wavelengths(S, meta.z);
         float tr[4];
-        bool vis = transmittance(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
+        bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
         if (vis) {
-            float4 L = B.L[slot];
+            float4 *dst = (meta.w & kEnded) ? &F.L_end[meta.w & ~kEnded] : &N.L[meta.w];
+            float4 L = *dst;
             PathState ps;
             ps.L[0] = L.x, ps.L[1] = L.y, ps.L[2] = L.z, ps.L[3] = L.w;
             shadow_apply(ps, sr, vis, tr);
-            B.L[slot] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+            *dst = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
         }
         rays = 1;
     }
     wave_count(&counters[1], rays);
 }
 
-__global__ void __launch_bounds__(256) k_wf_finish(Scene S, WfState B, uint64_t n, float *xyz,
+__global__ void __launch_bounds__(256) k_wf_finish(Scene S, FrameWf F, uint64_t n, float *xyz,
                                                      unsigned long long *counters) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     uint32_t failed = 0;
     if (i < n) {
-        float4 L = B.L[i];
-        uint32_t status = B.meta2[i].w;
-        Wavelengths wl = path_wavelengths(S, B.id[i].w);
+        float4 L = F.L_end[i];
+        uint2 e = F.end[i];
+        uint32_t status = e.x;
+        Wavelengths wl = path_wavelengths(S, e.y);
         float Lv[4] = {L.x, L.y, L.z, L.w};
         float v[3] = {0, 0, 0};
         if (status == RP_PATH_SUCCESS) sensor_xyz(S, wl, Lv, v);
@@ -353,61 +404,116 @@ inline void WavefrontBuffers::ensure(uint64_t n) {
     capacity = n;
 }
 
+// Owner of the frame wavefront's buffers: two compacted state sets, the
+// per-position hit and shadow streams and the per-lane results.
+struct FrameBuffers {
+    WfState state[2]{}; // only the path-state fields are used
+    FrameWf f{};
+    uint64_t capacity = 0;
+    uint32_t *host_count = nullptr; // pinned
+    FrameBuffers() = default;
+    FrameBuffers(const FrameBuffers &) = delete;
+    FrameBuffers &operator=(const FrameBuffers &) = delete;
+    ~FrameBuffers() { release(); }
+    void release() {
+        for (WfState &s : state) {
+            void *ptrs[] = {s.o_t, s.d, s.beta, s.density, s.prev_density, s.prev_pdf, s.eta, s.L, s.prev_p,
+                            s.id, s.meta, s.meta2};
+            for (void *p : ptrs)
+                if (p) (void)hipFree(p);
+            s = WfState{};
+        }
+        void *ptrs[] = {f.hit_t, f.hit_id, f.sh_o, f.sh_d, f.sh_val, f.sh_meta, f.L_end, f.end, f.perm,
+                        f.qcount, f.hist};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        f = FrameWf{};
+        if (host_count) (void)hipHostFree(host_count);
+        host_count = nullptr;
+        capacity = 0;
+    }
+    void ensure(uint64_t n);
+};
+
+inline void FrameBuffers::ensure(uint64_t n) {
+    if (n <= capacity) return;
+    release();
+    for (WfState &s : state) {
+        wf_alloc(s.o_t, n), wf_alloc(s.d, n), wf_alloc(s.beta, n), wf_alloc(s.density, n);
+        wf_alloc(s.prev_density, n), wf_alloc(s.prev_pdf, n), wf_alloc(s.eta, n), wf_alloc(s.L, n);
+        wf_alloc(s.prev_p, n), wf_alloc(s.id, n), wf_alloc(s.meta, n), wf_alloc(s.meta2, n);
+    }
+    wf_alloc(f.hit_t, n), wf_alloc(f.hit_id, n);
+    wf_alloc(f.sh_o, n), wf_alloc(f.sh_d, n), wf_alloc(f.sh_val, n), wf_alloc(f.sh_meta, n);
+    wf_alloc(f.L_end, n), wf_alloc(f.end, n), wf_alloc(f.perm, n);
+    wf_alloc(f.qcount, 4);
+    wf_alloc(f.hist, 512);
+    WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 4 * sizeof(uint32_t), hipHostMallocDefault));
+    capacity = n;
+}
+
 // Runs n paths (path i: pixel i % npx, sample sample_begin + i / npx) to
 // completion and writes planar per-path XYZ.  Returns device seconds spent in
 // the closest-hit (extend) kernels.
-inline double wavefront_render(const Scene &S, const PathParams &P, WavefrontBuffers &WB, const rp_frame_config &cfg,
+inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers &FB, const rp_frame_config &cfg,
                                uint32_t npx, uint64_t sample_begin, uint64_t n, float *xyz,
                                unsigned long long *counters, hipStream_t stream, uint64_t &launches) {
-    if (n > 0xffffffffull) throw std::runtime_error("wavefront chunk too large");
-    WB.ensure(n);
-    const WfState &B = WB.v;
+    if (n > (uint64_t(1) << 31)) throw std::runtime_error("wavefront chunk too large");
+    FB.ensure(n);
+    const FrameWf &F = FB.f;
     const bool sorted = cfg.mode == RP_MODE_WAVEFRONT_SORTED;
     uint32_t bits = cfg.sort_bits ? cfg.sort_bits : 2;
     if (bits < 1 || bits > 8) throw std::runtime_error("sort_bits must be in [1, 8]");
-    hipLaunchKernelGGL(k_wf_generate, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n, B);
-    uint32_t *q = B.qa, *qnext = B.qb;
+    int cur = 0;
+    hipLaunchKernelGGL(k_wf_generate, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n,
+                       FB.state[cur]);
     uint32_t qn = uint32_t(n);
     hipEvent_t e0, e1;
     WF_CHECK(hipEventCreate(&e0));
     WF_CHECK(hipEventCreate(&e1));
     double trace = 0;
     while (qn > 0) {
+        const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
+        const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
         WF_CHECK(hipEventRecord(e0, stream));
         if (cfg.flags & RP_FRAME_COUNT_TRAFFIC)
-            hipLaunchKernelGGL(k_wf_extend<true>, dim3((qn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, B,
-                               q, qn, counters);
+            hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
         else
-            hipLaunchKernelGGL(k_wf_extend<false>, dim3((qn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, B,
-                               q, qn, counters);
+            hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
         WF_CHECK(hipEventRecord(e1, stream));
         ++launches;
-        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, B.qcount, B.hist, bits);
-        const uint32_t *shade_q = q;
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, bits);
         if (sorted) {
             dim3 g((qn + 255) / 256);
-            hipLaunchKernelGGL(k_sort_hist, g, dim3(256), 0, stream, S, B, q, qn, bits, B.hist);
-            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(64), 0, stream, bits, B.hist);
-            hipLaunchKernelGGL(k_sort_scatter, g, dim3(256), 0, stream, S, B, q, qn, bits, B.hist, B.qsort);
-            shade_q = B.qsort;
+            hipLaunchKernelGGL(k_sort_hist, g, dim3(256), 0, stream, S, F.hit_id, nullptr, qn, bits, F.hist);
+            hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(64), 0, stream, bits, F.hist);
+            hipLaunchKernelGGL(k_sort_scatter, g, dim3(256), 0, stream, S, F.hit_id, nullptr, qn, bits, F.hist, F.perm);
         }
-        hipLaunchKernelGGL(k_wf_shade, dim3((qn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, P, B, shade_q,
-                           qn, qnext);
-        WF_CHECK(hipMemcpyAsync(WB.host_count, B.qcount, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx, sample_begin);
+        WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         WF_CHECK(hipStreamSynchronize(stream));
         float ms = 0;
         WF_CHECK(hipEventElapsedTime(&ms, e0, e1));
         trace += ms * 1e-3;
-        uint32_t next = WB.host_count[0], sn = WB.host_count[1];
-        if (sn)
-            hipLaunchKernelGGL(k_wf_shadow, dim3((sn + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, S, P, B, sn,
-                               counters);
-        std::swap(q, qnext);
+        const uint32_t next = FB.host_count[0], sn = FB.host_count[1];
+        if (next > qn || sn > qn) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw std::runtime_error("wavefront queue counts out of range");
+        }
+        if (sn) {
+            const dim3 sg((sn + kWfBlock - 1) / kWfBlock);
+            if (S.has_interfaces)
+                hipLaunchKernelGGL(k_wf_shadow<true>, sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+            else
+                hipLaunchKernelGGL(k_wf_shadow<false>, sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+        }
+        cur = 1 - cur;
         qn = next;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    hipLaunchKernelGGL(k_wf_finish, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, B, n, xyz, counters);
+    hipLaunchKernelGGL(k_wf_finish, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, F, n, xyz, counters);
     WF_CHECK(hipGetLastError());
     return trace;
 }
