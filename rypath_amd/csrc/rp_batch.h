@@ -4,9 +4,11 @@
 // rendering that scene alone:
 //  * persistent: one launch of a persistent megakernel; each wave takes 64
 //    consecutive global path ids from a device counter until the pool is empty;
-//  * wavefront: fixed-capacity slot pool; refill -> extend -> (sort) -> shade ->
-//    shadow -> retire with device-side queue counts, grid-stride kernels and no
-//    host synchronisation inside the bounce loop (replayable as one HIP graph).
+//  * wavefront: a fixed-capacity pool of compacted path state (two state sets,
+//    as the frame wavefront): refill -> extend -> (sort) -> shade -> shadow ->
+//    retire with device-side counts, grid-stride kernels and no host
+//    synchronisation inside the bounce loop (replayable as one HIP graph).
+//    New paths from any scene are appended behind the survivors each bounce.
 #pragma once
 #include "rp_wavefront.h"
 
@@ -24,9 +26,8 @@ struct BatchScene {
 };
 
 struct PoolCounters {
-    uint32_t qn[2];     // active queue sizes (ping-pong)
-    uint32_t sn, rn;    // shadow / retire queue sizes
-    uint32_t free_head, free_tail; // ring of free slots (monotonic counters)
+    uint32_t qn[2];     // live paths in state set 0 / 1
+    uint32_t sn, en;    // shadow rays / ended paths of the current bounce
     uint64_t next;      // next global path id to issue
     uint64_t total;
     uint64_t done;      // retired paths
@@ -92,31 +93,29 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
     }
 }
 
+
 // ------------------------------------------------------ wavefront pool
 struct PoolView {
-    WfState B;
+    WfState S[2];        // two compacted state sets (path-state fields only)
+    uint32_t *sid[2];    // scene of each position, per state set
+    FrameWf F;           // hits, shadow stream, sort order, histogram
+    uint32_t *sh_sid;    // scene of each shadow ray
+    uint4 *e_id;         // ended paths: (pixel, sample lo, sample hi, wavelength word)
     const BatchScene *scenes;
     uint32_t nscenes;
     uint32_t capacity;
-    uint32_t *sid;       // scene id per slot
-    uint32_t *q[2];      // active queues
-    uint32_t *qsorted;
-    uint32_t *free_ring; // capacity entries
-    uint32_t *retire;
     PoolCounters *c;
 };
+// Ended paths reuse the frame buffers' per-lane arrays: F.L_end (radiance)
+// and F.end (status, scene).
 
 #define RP_GRID_STRIDE(n)                                                                                     \
     for (uint32_t base_ = blockIdx.x * blockDim.x; base_ < (n); base_ += gridDim.x * blockDim.x)
 
 __global__ void k_pool_init(PoolView V, uint64_t total) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < V.capacity) V.free_ring[i] = i;
-    if (i == 0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
         V.c->qn[0] = V.c->qn[1] = 0;
-        V.c->sn = V.c->rn = 0;
-        V.c->free_head = 0;
-        V.c->free_tail = V.capacity;
+        V.c->sn = V.c->en = 0;
         V.c->next = 0;
         V.c->total = total;
         V.c->done = 0;
@@ -124,48 +123,48 @@ __global__ void k_pool_init(PoolView V, uint64_t total) {
     }
 }
 
-// New paths into free slots, appended after the current input queue.
-__global__ void k_pool_refill(PoolView V, uint32_t p) {
-    const uint32_t avail = V.c->free_tail - V.c->free_head;
+RP_DEV uint32_t pool_new_paths(const PoolView &V, uint32_t p) {
+    const uint32_t room = V.capacity - V.c->qn[p];
     const uint64_t remaining = V.c->total - V.c->next;
-    const uint32_t n = uint32_t(remaining < avail ? remaining : avail);
-    const uint32_t qbase = V.c->qn[p], head = V.c->free_head;
+    return uint32_t(remaining < room ? remaining : room);
+}
+
+// New paths (in global id order, any scene) appended behind the survivors.
+__global__ void k_pool_refill(PoolView V, uint32_t p) {
+    const uint32_t n = pool_new_paths(V, p);
+    const uint32_t live = V.c->qn[p];
     const uint64_t next = V.c->next;
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
         if (i < n) {
-            uint32_t slot = V.free_ring[(head + i) % V.capacity];
-            uint64_t g = next + i;
-            uint32_t s = find_scene(V.scenes, V.nscenes, g);
+            const uint64_t g = next + i;
+            const uint32_t s = find_scene(V.scenes, V.nscenes, g);
             const BatchScene &BS = V.scenes[s];
-            uint64_t local = g - BS.path_begin;
+            const uint64_t local = g - BS.path_begin;
             PathState ps;
             uint32_t w0, w1;
             path_begin(BS.S, BS.P, uint32_t(local % BS.npx), BS.sample_begin + local / BS.npx, ps, w0, w1);
-            store_state(V.B, slot, ps);
-            V.sid[slot] = s;
-            V.q[p][qbase + i] = slot;
+            store_state(V.S[p], live + i, ps);
+            V.sid[p][live + i] = s;
         }
     }
 }
-// Single thread: publishes what k_pool_refill issued and clears the output-side counters.
-__global__ void k_pool_commit(PoolView V, uint32_t p, uint32_t *hist, uint32_t bits) {
+// Single block: publishes what k_pool_refill issued and clears the
+// output-side counters and the sort histogram.
+__global__ void k_pool_commit(PoolView V, uint32_t p, uint32_t bits) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const uint32_t avail = V.c->free_tail - V.c->free_head;
-        const uint64_t remaining = V.c->total - V.c->next;
-        const uint32_t n = uint32_t(remaining < avail ? remaining : avail);
-        V.c->free_head += n;
+        const uint32_t n = pool_new_paths(V, p);
         V.c->next += n;
         V.c->qn[p] += n;
         V.c->qn[1 - p] = 0;
         V.c->sn = 0;
-        V.c->rn = 0;
+        V.c->en = 0;
     }
     if (bits) {
         uint32_t K = 1u << bits;
         for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
-            hist[k] = 0;
-            hist[256 + k] = 0;
+            V.F.hist[k] = 0;
+            V.F.hist[256 + k] = 0;
         }
     }
 }
@@ -184,72 +183,76 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
         if (i < n) {
-            uint32_t slot = V.q[p][i];
-            const Scene &S = V.scenes[V.sid[slot]].S;
-            float4 a = V.B.o_t[slot], b = V.B.d[slot];
+            const Scene &S = V.scenes[V.sid[p][i]].S;
+            float4 a = V.S[p].o_t[i], b = V.S[p].d[i];
             Hit h;
             traverse<false>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h);
-            V.B.hit_t[slot] = make_float4(h.t, h.b0, h.b1, h.b2);
-            V.B.hit_id[slot] = make_uint2(h.prim, h.id);
+            V.F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
+            V.F.hit_id[i] = make_uint2(h.prim, h.id);
             ++rays;
         }
     }
     wave_count(&V.c->rays[0], rays);
 }
 
-RP_DEV uint32_t pool_key(const PoolView &V, uint32_t slot, uint32_t bits) {
-    const Scene &S = V.scenes[V.sid[slot]].S;
-    return material_key(S, V.B.hit_id, slot, bits);
+RP_DEV uint32_t pool_key(const PoolView &V, uint32_t p, uint32_t i, uint32_t bits) {
+    return material_key(V.scenes[V.sid[p][i]].S, V.F.hit_id, i, bits);
 }
-__global__ void k_pool_sort_hist(PoolView V, uint32_t p, uint32_t bits, uint32_t *hist) {
+__global__ void k_pool_sort_hist(PoolView V, uint32_t p, uint32_t bits) {
     __shared__ uint32_t h[256];
     const uint32_t K = 1u << bits, n = V.c->qn[p];
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
     __syncthreads();
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
-        if (i < n) atomicAdd(&h[pool_key(V, V.q[p][i], bits)], 1u);
+        if (i < n) atomicAdd(&h[pool_key(V, p, i, bits)], 1u);
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
-        if (h[k]) atomicAdd(&hist[k], h[k]);
+        if (h[k]) atomicAdd(&V.F.hist[k], h[k]);
 }
-// One block per grid: each tile of blockDim entries is ranked in LDS.
-__global__ void k_pool_sort_scatter(PoolView V, uint32_t p, uint32_t bits, uint32_t *hist) {
+// Each tile of blockDim entries is ranked in LDS.
+__global__ void k_pool_sort_scatter(PoolView V, uint32_t p, uint32_t bits) {
     __shared__ uint32_t h[256], base[256];
     const uint32_t K = 1u << bits, n = V.c->qn[p];
     RP_GRID_STRIDE(n) {
         for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
         __syncthreads();
-        uint32_t i = base_ + threadIdx.x, key = 0, rank = 0, slot = 0;
+        uint32_t i = base_ + threadIdx.x, key = 0, rank = 0;
         if (i < n) {
-            slot = V.q[p][i];
-            key = pool_key(V, slot, bits);
+            key = pool_key(V, p, i, bits);
             rank = atomicAdd(&h[key], 1u);
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) base[k] = h[k] ? atomicAdd(&hist[256 + k], h[k]) : 0;
+        for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) base[k] = h[k] ? atomicAdd(&V.F.hist[256 + k], h[k]) : 0;
         __syncthreads();
-        if (i < n) V.qsorted[base[key] + rank] = slot;
+        if (i < n) V.F.perm[base[key] + rank] = i;
         __syncthreads();
     }
 }
 
+// One path event per live path of set p: survivors are appended to set
+// 1 - p, ended paths to the ended list, shadow rays to the shadow stream with
+// their destination (survivor position, or kEnded | ended index).
 template <int BS>
 __global__ void __launch_bounds__(BS) k_pool_shade(PoolView V, uint32_t p, uint32_t sorted) {
     const uint32_t n = V.c->qn[p];
-    const uint32_t *q = sorted ? V.qsorted : V.q[p];
+    const WfState &C = V.S[p], &N = V.S[1 - p];
     RP_GRID_STRIDE(n) {
-        uint32_t i = base_ + threadIdx.x;
-        bool cont = false, shadow = false, retire = false;
-        uint32_t slot = 0;
-        if (i < n) {
-            slot = q[i];
-            const BatchScene &BSc = V.scenes[V.sid[slot]];
-            PathState ps;
-            load_state(V.B, slot, ps);
-            float4 ht = V.B.hit_t[slot];
-            uint2 hi = V.B.hit_id[slot];
+        const uint32_t i = base_ + threadIdx.x;
+        const bool valid = i < n;
+        bool cont = false, shadow = false;
+        uint32_t s = 0;
+        PathState ps;
+        ShadowReq sr;
+        sr.active = false;
+        if (valid) {
+            const uint32_t j = sorted ? V.F.perm[i] : i;
+            s = V.sid[p][j];
+            const BatchScene &BSc = V.scenes[s];
+            load_state(C, j, ps);
+            float4 ht = V.F.hit_t[j];
+            uint2 hi = V.F.hit_id[j];
             Hit h;
             h.prim = hi.x;
             h.id = hi.y;
@@ -257,31 +260,38 @@ __global__ void __launch_bounds__(BS) k_pool_shade(PoolView V, uint32_t p, uint3
             h.b0 = ht.y;
             h.b1 = ht.z;
             h.b2 = ht.w;
-            ShadowReq sr;
             NoHook nh;
             cont = path_event(BSc.S, BSc.P, ps, h, sr, nh);
             shadow = sr.active;
             cont = cont && path_alive(BSc.S, ps);
-            retire = !cont;
-            store_state<false>(V.B, slot, ps);
-            if (shadow) {
-                V.B.sh_o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
-                V.B.sh_d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
-                V.B.sh_val[slot] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
-                V.B.sh_meta[slot] = make_uint2(sr.medium, sr.delta_or_nomis ? 1u : 0u);
-            }
         }
-        uint32_t pos = wave_append(cont, &V.c->qn[1 - p]);
-        if (cont) V.q[1 - p][pos] = slot;
-        uint32_t spos = wave_append(shadow, &V.c->sn);
-        if (shadow) V.B.qs[spos] = slot;
-        uint32_t rpos = wave_append(retire, &V.c->rn);
-        if (retire) V.retire[rpos] = slot;
+        const bool ended = valid && !cont;
+        const uint32_t pos = wave_append(cont, &V.c->qn[1 - p]);
+        if (cont) {
+            store_state(N, pos, ps);
+            V.sid[1 - p][pos] = s;
+        }
+        const uint32_t e = wave_append(ended, &V.c->en);
+        if (ended) {
+            V.F.L_end[e] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+            V.F.end[e] = make_uint2(ps.status, s);
+            V.e_id[e] = make_uint4(ps.pixel, uint32_t(ps.sample), uint32_t(ps.sample >> 32), ps.w2);
+        }
+        const uint32_t spos = wave_append(shadow, &V.c->sn);
+        if (shadow) {
+            V.F.sh_o[spos] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
+            V.F.sh_d[spos] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
+            V.F.sh_val[spos] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
+            V.F.sh_meta[spos] = make_uint4(sr.medium, sr.delta_or_nomis ? 1u : 0u, ps.w2, cont ? pos : (kEnded | e));
+            V.sh_sid[spos] = s;
+        }
     }
 }
 
-template <int BS>
-__global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V) {
+// kInterfaces = false when no scene of the batch has null interfaces (the
+// visibility test then needs no interface walk, and far fewer registers).
+template <int BS, bool kInterfaces>
+__global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V, uint32_t p) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
     Stack st;
@@ -290,14 +300,14 @@ __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V) {
     st.stride = BS;
     st.sp = 0;
     const uint32_t n = V.c->sn;
+    const WfState &N = V.S[1 - p];
     uint32_t rays = 0;
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
         if (i < n) {
-            uint32_t slot = V.B.qs[i];
-            const BatchScene &BSc = V.scenes[V.sid[slot]];
-            float4 o = V.B.sh_o[slot], d = V.B.sh_d[slot], v = V.B.sh_val[slot];
-            uint2 meta = V.B.sh_meta[slot];
+            const BatchScene &BSc = V.scenes[V.sh_sid[i]];
+            float4 o = V.F.sh_o[i], d = V.F.sh_d[i], v = V.F.sh_val[i];
+            uint4 meta = V.F.sh_meta[i];
             ShadowReq sr;
             sr.o = {o.x, o.y, o.z};
             sr.tmax = o.w;
@@ -306,15 +316,16 @@ __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V) {
             sr.val[0] = v.x, sr.val[1] = v.y, sr.val[2] = v.z, sr.val[3] = v.w;
             sr.medium = meta.x;
             sr.delta_or_nomis = meta.y != 0;
-            Wavelengths wl = path_wavelengths(BSc.S, V.B.id[slot].w);
+            Wavelengths wl = path_wavelengths(BSc.S, meta.z);
             float tr[4];
-            bool vis = transmittance(BSc.S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, BSc.P.boundary);
+            bool vis = transmittance<kInterfaces>(BSc.S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, BSc.P.boundary);
             if (vis) {
-                float4 L = V.B.L[slot];
+                float4 *dst = (meta.w & kEnded) ? &V.F.L_end[meta.w & ~kEnded] : &N.L[meta.w];
+                float4 L = *dst;
                 PathState ps;
                 ps.L[0] = L.x, ps.L[1] = L.y, ps.L[2] = L.z, ps.L[3] = L.w;
                 shadow_apply(ps, sr, vis, tr);
-                V.B.L[slot] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+                *dst = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
             }
             ++rays;
         }
@@ -322,33 +333,29 @@ __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V) {
     wave_count(&V.c->rays[1], rays);
 }
 
-// Finished paths: sensor XYZ into the scene's per-path planes, slot back to the free ring.
+// Ended paths: sensor XYZ into their scene's per-path planes.
 __global__ void k_pool_retire(PoolView V) {
-    const uint32_t n = V.c->rn;
+    const uint32_t n = V.c->en;
     uint32_t failed = 0, retired = 0;
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
-        bool ok = i < n;
-        uint32_t slot = ok ? V.retire[i] : 0;
-        if (ok) {
-            const BatchScene &BSc = V.scenes[V.sid[slot]];
-            float4 L = V.B.L[slot];
-            uint4 id = V.B.id[slot];
-            uint32_t status = V.B.meta2[slot].w;
+        if (i < n) {
+            const uint2 em = V.F.end[i];
+            const BatchScene &BSc = V.scenes[em.y];
+            const float4 L = V.F.L_end[i];
+            const uint4 id = V.e_id[i];
             Wavelengths wl = path_wavelengths(BSc.S, id.w);
             float Lv[4] = {L.x, L.y, L.z, L.w};
             float v[3] = {0, 0, 0};
-            if (status == RP_PATH_SUCCESS) sensor_xyz(BSc.S, wl, Lv, v);
+            if (em.x == RP_PATH_SUCCESS) sensor_xyz(BSc.S, wl, Lv, v);
             else failed += 1;
-            uint64_t sample = uint64_t(id.y) | (uint64_t(id.z) << 32);
-            uint64_t local = (sample - BSc.sample_begin) * BSc.npx + id.x;
+            const uint64_t sample = uint64_t(id.y) | (uint64_t(id.z) << 32);
+            const uint64_t local = (sample - BSc.sample_begin) * BSc.npx + id.x;
             BSc.xyz[local] = v[0];
             BSc.xyz[BSc.paths + local] = v[1];
             BSc.xyz[2 * BSc.paths + local] = v[2];
             retired += 1;
         }
-        uint32_t pos = wave_append(ok, &V.c->free_tail);
-        if (ok) V.free_ring[pos % V.capacity] = slot;
     }
     unsigned long long r = retired;
     for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off);

@@ -90,7 +90,6 @@ struct rp_scene {
     DevBuf<rp_hit> hits;
     DevBuf<uint8_t> occluded;
     hipStream_t stream = nullptr;
-    rpd::FrameBuffers wf;
     // Serialises device work on this scene between API calls and the
     // asynchronous frame worker.
     std::mutex exec;
@@ -373,6 +372,25 @@ rpd::PathParams path_params(uint64_t seed, uint32_t randomization, uint32_t emer
     return P;
 }
 
+// Wavefront buffers shared by every scene on a device (≈ 500 B per path in
+// flight, up to 2^26 paths per chunk): one arena per device instead of one
+// per scene keeps many resident scenes from multiplying that footprint.
+// Frames of different scenes on one device take turns on it.  The arenas
+// live until the process ends.
+struct FrameArena {
+    std::mutex m;
+    rpd::FrameBuffers fb;
+};
+FrameArena &frame_arena(int device) {
+    static std::mutex guard;
+    static std::vector<FrameArena *> *arenas = new std::vector<FrameArena *>();
+    std::lock_guard<std::mutex> g(guard);
+    if (device < 0) throw std::runtime_error("invalid device");
+    if (size_t(device) >= arenas->size()) arenas->resize(size_t(device) + 1, nullptr);
+    if (!(*arenas)[size_t(device)]) (*arenas)[size_t(device)] = new FrameArena();
+    return *(*arenas)[size_t(device)];
+}
+
 // Continuation of a running film: the compensated sums (6 planes: sum and
 // correction per XYZ channel) after `samples` samples per pixel.  Adding the
 // next samples in order continues the same Neumaier recurrence, so a resumed
@@ -411,6 +429,14 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     HIP_CHECK(hipEventRecord(e0, s->stream));
     double trace_seconds = 0;
     uint64_t launches = 0;
+    // the device's wavefront arena is held until this frame's last kernel has
+    // finished (the event synchronisation below happens before the unlock)
+    std::unique_lock<std::mutex> arena_lock;
+    FrameArena *arena = nullptr;
+    if (cfg.mode != RP_MODE_MEGAKERNEL) {
+        arena = &frame_arena(s->device);
+        arena_lock = std::unique_lock<std::mutex>(arena->m);
+    }
     for (uint32_t s0 = 0; s0 < spp; s0 += chunk) {
         uint32_t ns = std::min(chunk, spp - s0);
         uint64_t n = uint64_t(ns) * npx;
@@ -419,8 +445,8 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
             hipLaunchKernelGGL(k_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream, s->dev, P,
                                npx, cfg.sample_begin + s0, n, s->path_xyz.p, s->counters.p);
         } else {
-            trace_seconds += rpd::wavefront_render(s->dev, P, s->wf, cfg, npx, cfg.sample_begin + s0, n, s->path_xyz.p,
-                                                   s->counters.p, s->stream, launches);
+            trace_seconds += rpd::wavefront_render(s->dev, P, arena->fb, cfg, npx, cfg.sample_begin + s0, n,
+                                                   s->path_xyz.p, s->counters.p, s->stream, launches);
         }
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
@@ -909,8 +935,10 @@ struct rp_batch {
     DevBuf<float> film_out;
     DevBuf<unsigned long long> work;
     // wavefront pool
-    rpd::WavefrontBuffers wf;
-    DevBuf<uint32_t> sid, q0, q1, qsorted, ring, retire;
+    rpd::FrameBuffers wf;
+    DevBuf<uint32_t> sid0, sid1, sh_sid;
+    DevBuf<uint4> e_id;
+    bool any_interfaces = false; // some scene of the batch has null interfaces
     DevBuf<rpd::PoolCounters> counters;
     rpd::PoolCounters *host_counters = nullptr;
     uint32_t capacity = 0;
@@ -938,16 +966,16 @@ template <int BS> void launch_persistent(rp_batch *b, dim3 grid) {
 
 rpd::PoolView pool_view(rp_batch *b) {
     rpd::PoolView V{};
-    V.B = b->wf.v;
+    V.S[0] = b->wf.state[0];
+    V.S[1] = b->wf.state[1];
+    V.sid[0] = b->sid0.p;
+    V.sid[1] = b->sid1.p;
+    V.F = b->wf.f;
+    V.sh_sid = b->sh_sid.p;
+    V.e_id = b->e_id.p;
     V.scenes = b->scenes.p;
     V.nscenes = uint32_t(b->host_scenes.size());
     V.capacity = b->capacity;
-    V.sid = b->sid.p;
-    V.q[0] = b->q0.p;
-    V.q[1] = b->q1.p;
-    V.qsorted = b->qsorted.p;
-    V.free_ring = b->ring.p;
-    V.retire = b->retire.p;
     V.c = b->counters.p;
     return V;
 }
@@ -955,15 +983,16 @@ rpd::PoolView pool_view(rp_batch *b) {
 template <int BS> void pool_iteration(rp_batch *b, const rpd::PoolView &V, uint32_t p, uint32_t bits, dim3 grid) {
     hipStream_t st = b->stream;
     hipLaunchKernelGGL(rpd::k_pool_refill, grid, dim3(256), 0, st, V, p);
-    hipLaunchKernelGGL(rpd::k_pool_commit, dim3(1), dim3(256), 0, st, V, p, b->wf.v.hist, bits);
+    hipLaunchKernelGGL(rpd::k_pool_commit, dim3(1), dim3(256), 0, st, V, p, bits);
     hipLaunchKernelGGL(rpd::k_pool_extend<BS>, grid, dim3(BS), 0, st, V, p);
     if (bits) {
-        hipLaunchKernelGGL(rpd::k_pool_sort_hist, grid, dim3(256), 0, st, V, p, bits, b->wf.v.hist);
-        hipLaunchKernelGGL(rpd::k_sort_scan, dim3(1), dim3(64), 0, st, bits, b->wf.v.hist);
-        hipLaunchKernelGGL(rpd::k_pool_sort_scatter, grid, dim3(256), 0, st, V, p, bits, b->wf.v.hist);
+        hipLaunchKernelGGL(rpd::k_pool_sort_hist, grid, dim3(256), 0, st, V, p, bits);
+        hipLaunchKernelGGL(rpd::k_sort_scan, dim3(1), dim3(64), 0, st, bits, V.F.hist);
+        hipLaunchKernelGGL(rpd::k_pool_sort_scatter, grid, dim3(256), 0, st, V, p, bits);
     }
     hipLaunchKernelGGL(rpd::k_pool_shade<BS>, grid, dim3(BS), 0, st, V, p, bits ? 1u : 0u);
-    hipLaunchKernelGGL(rpd::k_pool_shadow<BS>, grid, dim3(BS), 0, st, V);
+    if (b->any_interfaces) hipLaunchKernelGGL((rpd::k_pool_shadow<BS, true>), grid, dim3(BS), 0, st, V, p);
+    else hipLaunchKernelGGL((rpd::k_pool_shadow<BS, false>), grid, dim3(BS), 0, st, V, p);
     hipLaunchKernelGGL(rpd::k_pool_retire, grid, dim3(256), 0, st, V);
 }
 
@@ -984,8 +1013,7 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
     if (bits > 8) throw std::runtime_error("sort_bits must be in [0, 8]");
     if (cap != b->capacity) {
         b->wf.ensure(cap);
-        b->sid.alloc(cap), b->q0.alloc(cap), b->q1.alloc(cap), b->qsorted.alloc(cap), b->ring.alloc(cap),
-            b->retire.alloc(cap);
+        b->sid0.alloc(cap), b->sid1.alloc(cap), b->sh_sid.alloc(cap), b->e_id.alloc(cap);
         b->counters.alloc(1);
         b->capacity = cap;
         if (b->graph) {
@@ -998,7 +1026,7 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
                                 hipHostMallocDefault));
     rpd::PoolView V = pool_view(b);
     const dim3 grid(uint32_t(cu_count(b->device)) * 8);
-    hipLaunchKernelGGL(rpd::k_pool_init, dim3((cap + 255) / 256), dim3(256), 0, b->stream, V, b->total);
+    hipLaunchKernelGGL(rpd::k_pool_init, dim3(1), dim3(64), 0, b->stream, V, b->total);
     const int pairs = 4; // 8 bounce iterations per submission
     auto submit = [&] {
         if (bs == 64) pool_block<64>(b, V, bits, grid, pairs);
@@ -1149,6 +1177,7 @@ int rp_batch_create(const rp_batch_item *items, uint32_t count, rp_batch **out) 
             b->total += bs.paths;
             b->offsets.push_back(floats);
             floats += 3 * bs.paths;
+            b->any_interfaces = b->any_interfaces || bs.S.has_interfaces;
             b->host_scenes.push_back(bs);
         }
         HIP_CHECK(hipSetDevice(b->device));

@@ -1,12 +1,13 @@
 // Wavefront form of the path tracer: generate -> extend -> (sort) -> shade ->
 // shadow -> accumulate, with path state in HBM (structure of arrays, 16-byte
-// fields).  The frame path compacts the state itself every bounce (two state
-// sets, wave-ballot appends); the batch pool keeps slots in place and
-// compacts index queues.  The per-event arithmetic is rp_path.h's, so a
-// wavefront frame equals the megakernel frame bit for bit.
+// fields) compacted every bounce: two state sets, survivors appended at
+// wave-ballot positions (the frame path here and the batch pool in
+// rp_batch.h).  The per-event arithmetic is rp_path.h's, so a wavefront frame
+// equals the megakernel frame bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -15,34 +16,10 @@
 
 namespace rpd {
 
-// Plain view of the device buffers (passed by value to kernels).
+// One set of path state (structure of arrays; passed by value to kernels).
 struct WfState {
-    // path state
     float4 *o_t, *d, *beta, *density, *prev_density, *prev_pdf, *eta, *L, *prev_p;
     uint4 *id, *meta, *meta2;
-    // hits
-    float4 *hit_t;
-    uint2 *hit_id;
-    // shadow requests
-    float4 *sh_o, *sh_d, *sh_val;
-    uint2 *sh_meta; // (medium, no-MIS flag)
-    // queues
-    uint32_t *qa, *qb, *qs, *qsort;
-    uint32_t *qcount; // [0] next, [1] shadow
-    uint32_t *hist;   // [256] histogram, [256] cursors
-};
-
-// Owner of the wavefront buffers (host side).
-struct WavefrontBuffers {
-    WfState v{};
-    uint64_t capacity = 0;
-    uint32_t *host_count = nullptr; // pinned
-    WavefrontBuffers() = default;
-    WavefrontBuffers(const WavefrontBuffers &) = delete;
-    WavefrontBuffers &operator=(const WavefrontBuffers &) = delete;
-    void release();
-    void ensure(uint64_t n);
-    ~WavefrontBuffers() { release(); }
 };
 
 // ------------------------------------------------------------ state I/O
@@ -375,34 +352,7 @@ __global__ void __launch_bounds__(256) k_wf_finish(Scene S, FrameWf F, uint64_t 
             throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(err_));                     \
     } while (0)
 
-inline void WavefrontBuffers::release() {
-    void *ptrs[] = {v.o_t, v.d, v.beta, v.density, v.prev_density, v.prev_pdf, v.eta, v.L, v.prev_p, v.id,
-                    v.meta, v.meta2, v.hit_t, v.hit_id, v.sh_o, v.sh_d, v.sh_val, v.sh_meta, v.qa, v.qb,
-                    v.qs, v.qsort, v.qcount, v.hist};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
-    if (host_count) (void)hipHostFree(host_count);
-    v = WfState{};
-    capacity = 0;
-    host_count = nullptr;
-}
-
 template <typename T> inline void wf_alloc(T *&p, uint64_t n) { WF_CHECK(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T))); }
-
-inline void WavefrontBuffers::ensure(uint64_t n) {
-    if (n <= capacity) return;
-    release();
-    wf_alloc(v.o_t, n), wf_alloc(v.d, n), wf_alloc(v.beta, n), wf_alloc(v.density, n), wf_alloc(v.prev_density, n);
-    wf_alloc(v.prev_pdf, n), wf_alloc(v.eta, n), wf_alloc(v.L, n), wf_alloc(v.prev_p, n);
-    wf_alloc(v.id, n), wf_alloc(v.meta, n), wf_alloc(v.meta2, n);
-    wf_alloc(v.hit_t, n), wf_alloc(v.hit_id, n);
-    wf_alloc(v.sh_o, n), wf_alloc(v.sh_d, n), wf_alloc(v.sh_val, n), wf_alloc(v.sh_meta, n);
-    wf_alloc(v.qa, n), wf_alloc(v.qb, n), wf_alloc(v.qs, n), wf_alloc(v.qsort, n);
-    wf_alloc(v.qcount, 4);
-    wf_alloc(v.hist, 512);
-    WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 4 * sizeof(uint32_t), hipHostMallocDefault));
-    capacity = n;
-}
 
 // Owner of the frame wavefront's buffers: two compacted state sets, the
 // per-position hit and shadow streams and the per-lane results.
@@ -476,10 +426,11 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
         WF_CHECK(hipEventRecord(e0, stream));
-        if (cfg.flags & RP_FRAME_COUNT_TRAFFIC)
+        if (cfg.flags & RP_FRAME_COUNT_TRAFFIC) {
             hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
-        else
+        } else {
             hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
+        }
         WF_CHECK(hipEventRecord(e1, stream));
         ++launches;
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, bits);
