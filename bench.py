@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="mega,wave,sorted,pool")
-    ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume", "batch"])
+    ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume", "batch", "inverse"])
+    ap.add_argument("--fd-trials", type=int, default=4, help="inverse: independent sample sets of the gradient check")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: frames only (no roofline replay, no CPU leg), then a 1 GiB device "
@@ -99,6 +100,108 @@ def cpu_baseline(synth, width, height, budget):
     return {"value": paths / dt / 1e6, "unit": "M paths/s", "cores": threads, "kind": "port",
             "sample": f"{rows} full-width rows x 1 spp ({paths} paths) of the same frame; oracle BVH build "
                       f"{build:.1f}s excluded"}
+
+
+def inverse_workload(args):
+    """Inverse rendering step on the 256x256 Cornell box with a fog box and 58
+    parameters (48 wall reflectance bins, 2 GGX roughnesses, 8 fog sigma_s
+    bins): a 16 spp forward render, the L2 loss against a 1024 spp target
+    rendered from parameters perturbed by N(0, 0.1) in logit/log space, and a
+    16 spp path-replay adjoint.  Then the gradient check: directional
+    derivatives per parameter group against central differences with step
+    1e-3 and 4096 spp common random numbers, over independent sample sets."""
+    import numpy as np
+    import torch
+
+    import rypath_amd as rp
+
+    W = H = 256
+    spp, fd_spp, fd_h = 16, 4096, 1e-3
+    synth = rp.Synthetic(rp.SYNTH_INVERSE, seed=2, width=W, height=H)
+    scene = rp.Scene(synth.desc)
+    params, values = rp.inverse_params(synth)
+    values = values.astype(np.float64)
+    rng = np.random.default_rng(7)
+    z = rng.normal(0, 0.1, len(values))
+    target_values = values.copy()
+    refl = slice(0, 48)
+    logit = np.log(values[refl] / (1 - values[refl]))
+    target_values[refl] = 1 / (1 + np.exp(-(logit + z[refl])))
+    target_values[48:] = values[48:] * np.exp(z[48:])
+    img = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
+    target = torch.empty_like(img)
+    scene.set_params(params, target_values.astype(np.float32))
+    scene.render_frame_device(rp.frame_config(1024, mode=rp.MODE_WAVEFRONT, seed=999), target.data_ptr())
+    scene.set_params(params, values.astype(np.float32))
+    torch.cuda.synchronize()
+
+    def loss_and_grad(cfg, want_grad=True):
+        scene.render_frame_device(cfg, img.data_ptr())
+        diff = img.double() - target.double()
+        loss = float((diff * diff).mean())
+        if not want_grad:
+            return loss, None
+        dl = (2.0 * diff / diff.numel()).float().cpu().numpy()
+        g, _ = scene.backward(cfg, params, dl)
+        return loss, g.astype(np.float64)
+
+    for k in range(args.warmup):
+        loss_and_grad(rp.frame_config(spp, mode=rp.MODE_WAVEFRONT, seed=10 + k))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(args.steps):
+        loss, g = loss_and_grad(rp.frame_config(spp, mode=rp.MODE_WAVEFRONT, seed=100 + k))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    print(f"[bench] inverse step: {dt * 1e3:.2f} ms ({1 / dt:.1f} gradients/s), loss {loss:.4e}",
+          file=sys.stderr, flush=True)
+
+    groups = {"walls": slice(0, 48), "roughness": slice(48, 50), "sigma_s": slice(50, 58)}
+    dirs = {}
+    for i, (name, sl) in enumerate(groups.items()):
+        v = np.zeros(len(values))
+        v[sl] = np.random.default_rng(40 + i).normal(0, 1, sl.stop - sl.start) * np.abs(values[sl])
+        dirs[name] = v
+    fd = {n: [] for n in groups}
+    ad = {n: [] for n in groups}
+    t = time.perf_counter()
+    for trial in range(args.fd_trials):
+        cfg = rp.frame_config(fd_spp, mode=rp.MODE_WAVEFRONT, seed=5000 + trial)
+        _, g = loss_and_grad(cfg)
+        for name, v in dirs.items():
+            ad[name].append(float(g @ v))
+            ls = []
+            for sgn in (+1, -1):
+                scene.set_params(params, (values + sgn * fd_h * v).astype(np.float32))
+                ls.append(loss_and_grad(cfg, want_grad=False)[0])
+            scene.set_params(params, values.astype(np.float32))
+            fd[name].append((ls[0] - ls[1]) / (2 * fd_h))
+        print(f"[bench] gradient check trial {trial}: " +
+              ", ".join(f"{n} adjoint {ad[n][-1]:.4e} fd {fd[n][-1]:.4e}" for n in groups), file=sys.stderr, flush=True)
+    check_s = time.perf_counter() - t
+    check = {}
+    for name in groups:
+        a = np.array(ad[name])
+        f = np.array(fd[name])
+        n = len(a)
+        se = lambda x: float(x.std(ddof=1) / np.sqrt(n)) if n > 1 else None
+        check[name] = {"adjoint": float(a.mean()), "adjoint_se": se(a), "central_difference": float(f.mean()),
+                       "central_difference_se": se(f),
+                       "rel_err": float(abs(f.mean() - a.mean()) / abs(a.mean()))}
+    worst = max(c["rel_err"] for c in check.values())
+    paths = 2 * W * H * spp
+    line = {
+        "metric": "adjoint gradients/s (256x256, 16 spp forward + 16 spp adjoint, 58 params)",
+        "value": round(1 / dt, 2), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "inverse-cornell-fog-256-16spp", "width": W, "height": H, "spp": spp,
+                   "params": len(params), "loss": "mean squared XYZ error vs 1024 spp target"},
+        "M_paths_s": round(paths / dt / 1e6, 2),
+        "gradient_check": {"step": fd_h, "spp": fd_spp, "trials": args.fd_trials, "groups": check,
+                           "max_rel_err": round(worst, 5), "seconds": round(check_s, 1)},
+    }
+    print(json.dumps(line), flush=True)
 
 
 def batch_workload(args):
@@ -182,6 +285,8 @@ def main():
     torch.cuda.init()
     if args.workload == "batch":
         return batch_workload(args)
+    if args.workload == "inverse":
+        return inverse_workload(args)
     kind, w, h, spp, opts, label = WORKLOADS[args.workload]
     if args.workload != "terrain":
         args.width, args.height, args.spp = w, h, spp

@@ -297,6 +297,9 @@ typedef struct rp_batch_stats {
     uint64_t extension_rays, shadow_rays;
     uint64_t failed_paths;
     uint32_t graph_launches, pad;
+    /* wavefront pool: path events and shadow rays of waves that held more
+     * than one scene, handled by the per-lane-scene kernels */
+    uint64_t mixed_scene_lanes;
 } rp_batch_stats;
 
 int rp_batch_create(const rp_batch_item *items, uint32_t count, rp_batch **out);

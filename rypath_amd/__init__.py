@@ -130,7 +130,7 @@ class BatchConfig(C.Structure):
 
 class BatchStats(C.Structure):
     _fields_ = [("seconds", f64), ("paths", u64), ("extension_rays", u64), ("shadow_rays", u64),
-                ("failed_paths", u64), ("graph_launches", u32), ("pad", u32)]
+                ("failed_paths", u64), ("graph_launches", u32), ("pad", u32), ("mixed_scene_lanes", u64)]
 
 
 class Param(C.Structure):

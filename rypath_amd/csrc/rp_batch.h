@@ -25,13 +25,19 @@ struct BatchScene {
     uint32_t pad;
 };
 
+// The three queues the shading kernel appends to (survivors, ended paths,
+// shadow rays) each have their own 256-B line: atomics on one line are
+// serialised (kCounterStride).
 struct PoolCounters {
     uint32_t qn[2];     // live paths in state set 0 / 1
-    uint32_t sn, en;    // shadow rays / ended paths of the current bounce
+    uint32_t ov, ovs;   // lanes of mixed-scene waves handed over: path events / shadow rays
     uint64_t next;      // next global path id to issue
     uint64_t total;
     uint64_t done;      // retired paths
     unsigned long long rays[3]; // extension, shadow, failed
+    unsigned long long mixed;   // path events and shadow rays handled per lane
+    alignas(256) uint32_t sn;   // shadow rays of the current bounce
+    alignas(256) uint32_t en;   // ended paths of the current bounce
 };
 
 RP_DEV uint32_t find_scene(const BatchScene *scenes, uint32_t n, uint64_t g) {
@@ -87,9 +93,9 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
         failed += __shfl_down(failed, off);
     }
     if (lane == 0) {
-        atomicAdd(&work[1], ext);
-        atomicAdd(&work[2], sh);
-        atomicAdd(&work[3], failed);
+        atomicAdd(&work[kCounterStride], ext);
+        atomicAdd(&work[2 * kCounterStride], sh);
+        if (failed) atomicAdd(&work[3 * kCounterStride], failed);
     }
 }
 
@@ -101,6 +107,7 @@ struct PoolView {
     FrameWf F;           // hits, shadow stream, sort order, histogram
     uint32_t *sh_sid;    // scene of each shadow ray
     uint4 *e_id;         // ended paths: (pixel, sample lo, sample hi, wavelength word)
+    uint32_t *ov;        // positions handed from scene-uniform to per-lane kernels
     const BatchScene *scenes;
     uint32_t nscenes;
     uint32_t capacity;
@@ -112,6 +119,34 @@ struct PoolView {
 #define RP_GRID_STRIDE(n)                                                                                     \
     for (uint32_t base_ = blockIdx.x * blockDim.x; base_ < (n); base_ += gridDim.x * blockDim.x)
 
+// Runs f(scene) once per distinct scene among the wave's active lanes
+// (a waterfall over readfirstlane).  The scene index is wave-uniform inside
+// f and the table is read through the constant address space, so the scene's
+// fields load through the scalar unit into SGPRs exactly as a kernel
+// argument's do; a per-lane scene would hold every node, triangle and
+// material base address in VGPRs (190 instead of 160 in the shade kernel:
+// 2 waves per SIMD instead of 3).  The loop is wave-uniform (a lane never
+// leaves it early, so the compiler cannot move f past the exit where u is
+// per-lane again), and lane membership is taken from a ballot rather than
+// from `s == u`, which the compiler would use to substitute the per-lane s
+// for u inside f.  Refill appends paths in scene order, so a wave nearly
+// always holds a single scene and f runs once.
+typedef const __attribute__((address_space(4))) BatchScene ConstBatchScene;
+template <class Fn>
+RP_DEV void for_scene(const BatchScene *scenes, uint32_t s, Fn &&f) {
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(true);
+    while (pending) {
+        const uint32_t u = __builtin_amdgcn_readlane(s, __builtin_ctzll(pending));
+        const uint64_t mine = __ballot(s == u);
+        if ((mine >> lane) & 1) {
+            ConstBatchScene *t = (ConstBatchScene *)scenes;
+            f((const BatchScene &)t[u]);
+        }
+        pending &= ~mine;
+    }
+}
+
 __global__ void k_pool_init(PoolView V, uint64_t total) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         V.c->qn[0] = V.c->qn[1] = 0;
@@ -120,6 +155,7 @@ __global__ void k_pool_init(PoolView V, uint64_t total) {
         V.c->total = total;
         V.c->done = 0;
         V.c->rays[0] = V.c->rays[1] = V.c->rays[2] = 0;
+        V.c->mixed = 0;
     }
 }
 
@@ -149,16 +185,19 @@ __global__ void k_pool_refill(PoolView V, uint32_t p) {
         }
     }
 }
-// Single block: publishes what k_pool_refill issued and clears the
-// output-side counters and the sort histogram.
+// Single block: publishes what k_pool_refill issued, counts this
+// iteration's extension rays (one per live path) and clears the output-side
+// counters and the sort histogram.
 __global__ void k_pool_commit(PoolView V, uint32_t p, uint32_t bits) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         const uint32_t n = pool_new_paths(V, p);
         V.c->next += n;
         V.c->qn[p] += n;
+        V.c->rays[0] += V.c->qn[p];
         V.c->qn[1 - p] = 0;
         V.c->sn = 0;
         V.c->en = 0;
+        V.c->ov = V.c->ovs = 0;
     }
     if (bits) {
         uint32_t K = 1u << bits;
@@ -179,20 +218,18 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
     st.stride = BS;
     st.sp = 0;
     const uint32_t n = V.c->qn[p];
-    uint32_t rays = 0;
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
         if (i < n) {
-            const Scene &S = V.scenes[V.sid[p][i]].S;
             float4 a = V.S[p].o_t[i], b = V.S[p].d[i];
             Hit h;
-            traverse<false>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h);
+            for_scene(V.scenes, V.sid[p][i], [&](const BatchScene &B) {
+                traverse<false>(B.S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h);
+            });
             V.F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
             V.F.hit_id[i] = make_uint2(h.prim, h.id);
-            ++rays;
         }
     }
-    wave_count(&V.c->rays[0], rays);
 }
 
 RP_DEV uint32_t pool_key(const PoolView &V, uint32_t p, uint32_t i, uint32_t bits) {
@@ -231,112 +268,202 @@ __global__ void k_pool_sort_scatter(PoolView V, uint32_t p, uint32_t bits) {
     }
 }
 
-// One path event per live path of set p: survivors are appended to set
-// 1 - p, ended paths to the ended list, shadow rays to the shadow stream with
-// their destination (survivor position, or kEnded | ended index).
+// Scene-uniform waves.  The shading and shadow kernels run one scene per
+// wave with the scene's fields in SGPRs (as for_scene does, but without a
+// loop: the path state held across a waterfall loop costs the shade kernel
+// about 40 VGPRs).  A wave whose valid lanes hold more than one scene keeps
+// the lanes of its first valid lane's scene and hands the others, through
+// the `ov` list, to a per-lane kernel (k_pool_*_mixed) launched right after
+// it.  Such waves are rare: refill issues paths in scene order, so a wave
+// mixes scenes only where that order changes.
+RP_DEV uint32_t wave_scene(bool valid, uint32_t s, bool &mine) {
+    const uint64_t vmask = __ballot(valid);
+    const uint32_t u = __builtin_amdgcn_readlane(s, vmask ? __builtin_ctzll(vmask) : 0);
+    mine = (__ballot(valid && s == u) >> __lane_id()) & 1;
+    return u;
+}
+RP_DEV const BatchScene &uniform_scene(const BatchScene *scenes, uint32_t u) {
+    return (const BatchScene &)((ConstBatchScene *)scenes)[u];
+}
+
+// One path event for the path at S[p][j] of scene s (B = its record) when
+// `valid`: survivors are appended to set 1 - p, ended paths to the ended
+// list, shadow rays to the shadow stream with their destination (survivor
+// position, or kEnded | ended index).  Every thread of the block calls this
+// together (the appends are block-aggregated).
+template <int BS>
+RP_DEV void pool_shade_path(const PoolView &V, uint32_t p, bool valid, uint32_t j, uint32_t s, const BatchScene &B,
+                            AppendLds<BS, 3> &sh) {
+    const WfState &C = V.S[p], &N = V.S[1 - p];
+    bool cont = false, shadow = false;
+    PathState ps;
+    ShadowReq sr;
+    sr.active = false;
+    if (valid) {
+        load_state(C, j, ps);
+        float4 ht = V.F.hit_t[j];
+        uint2 hi = V.F.hit_id[j];
+        Hit h;
+        h.prim = hi.x;
+        h.id = hi.y;
+        h.t = ht.x;
+        h.b0 = ht.y;
+        h.b1 = ht.z;
+        h.b2 = ht.w;
+        NoHook nh;
+        cont = path_event(B.S, B.P, ps, h, sr, nh);
+        shadow = sr.active;
+        cont = cont && path_alive(B.S, ps);
+    }
+    const bool ended = valid && !cont;
+    const bool pred[3] = {cont, ended, shadow};
+    uint32_t *const queue[3] = {&V.c->qn[1 - p], &V.c->en, &V.c->sn};
+    uint32_t at[3];
+    block_append(sh, pred, queue, at);
+    const uint32_t pos = at[0], e = at[1], spos = at[2];
+    if (cont) {
+        store_state(N, pos, ps);
+        V.sid[1 - p][pos] = s;
+    }
+    if (ended) {
+        V.F.L_end[e] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+        V.F.end[e] = make_uint2(ps.status, s);
+        V.e_id[e] = make_uint4(ps.pixel, uint32_t(ps.sample), uint32_t(ps.sample >> 32), ps.w2);
+    }
+    if (shadow) {
+        V.F.sh_o[spos] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
+        V.F.sh_d[spos] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
+        V.F.sh_val[spos] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
+        V.F.sh_meta[spos] = make_uint4(sr.medium, sr.delta_or_nomis ? 1u : 0u, ps.w2, cont ? pos : (kEnded | e));
+        V.sh_sid[spos] = s;
+    }
+}
+
+// One thread per live path of set p (the grid covers the capacity; blocks
+// past the live count return at once).
 template <int BS>
 __global__ void __launch_bounds__(BS) k_pool_shade(PoolView V, uint32_t p, uint32_t sorted) {
+    __shared__ AppendLds<BS, 3> sh;
     const uint32_t n = V.c->qn[p];
-    const WfState &C = V.S[p], &N = V.S[1 - p];
+    if (blockIdx.x * BS >= n) return;
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t j = 0, s = 0;
+    if (valid) {
+        j = sorted ? V.F.perm[i] : i;
+        s = V.sid[p][j];
+    }
+    bool mine;
+    const uint32_t u = wave_scene(valid, s, mine);
+    const bool other = valid && !mine;
+    const uint32_t o = wave_append(other, &V.c->ov);
+    if (other) V.ov[o] = j;
+    pool_shade_path<BS>(V, p, mine, j, u, uniform_scene(V.scenes, u), sh);
+}
+
+// The path events k_pool_shade handed over, one scene per lane.
+__global__ void __launch_bounds__(256) k_pool_shade_mixed(PoolView V, uint32_t p) {
+    __shared__ AppendLds<256, 3> sh;
+    const uint32_t n = V.c->ov;
+    uint32_t mixed = 0;
     RP_GRID_STRIDE(n) {
         const uint32_t i = base_ + threadIdx.x;
         const bool valid = i < n;
-        bool cont = false, shadow = false;
-        uint32_t s = 0;
+        const uint32_t j = valid ? V.ov[i] : 0u;
+        const uint32_t s = valid ? V.sid[p][j] : 0u;
+        pool_shade_path<256>(V, p, valid, j, s, V.scenes[s], sh);
+        mixed += valid ? 1u : 0u;
+    }
+    wave_count(&V.c->mixed, mixed);
+}
+
+// Shadow ray i of the current bounce (scene record B): adds a visible
+// contribution to the survivor's radiance in set 1 - p or to the ended
+// path's entry.
+template <bool kInterfaces>
+RP_DEV void pool_shadow_ray(const PoolView &V, uint32_t p, Stack &st, uint32_t i, const BatchScene &B) {
+    const WfState &N = V.S[1 - p];
+    float4 o = V.F.sh_o[i], d = V.F.sh_d[i], v = V.F.sh_val[i];
+    uint4 meta = V.F.sh_meta[i];
+    ShadowReq sr;
+    sr.o = {o.x, o.y, o.z};
+    sr.tmax = o.w;
+    sr.d = {d.x, d.y, d.z};
+    sr.w = d.w;
+    sr.val[0] = v.x, sr.val[1] = v.y, sr.val[2] = v.z, sr.val[3] = v.w;
+    sr.medium = meta.x;
+    sr.delta_or_nomis = meta.y != 0;
+    Wavelengths wl = path_wavelengths(B.S, meta.z);
+    float tr[4];
+    bool vis = transmittance<kInterfaces>(B.S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, B.P.boundary);
+    if (vis) {
+        float4 *dst = (meta.w & kEnded) ? &V.F.L_end[meta.w & ~kEnded] : &N.L[meta.w];
+        float4 L = *dst;
         PathState ps;
-        ShadowReq sr;
-        sr.active = false;
-        if (valid) {
-            const uint32_t j = sorted ? V.F.perm[i] : i;
-            s = V.sid[p][j];
-            const BatchScene &BSc = V.scenes[s];
-            load_state(C, j, ps);
-            float4 ht = V.F.hit_t[j];
-            uint2 hi = V.F.hit_id[j];
-            Hit h;
-            h.prim = hi.x;
-            h.id = hi.y;
-            h.t = ht.x;
-            h.b0 = ht.y;
-            h.b1 = ht.z;
-            h.b2 = ht.w;
-            NoHook nh;
-            cont = path_event(BSc.S, BSc.P, ps, h, sr, nh);
-            shadow = sr.active;
-            cont = cont && path_alive(BSc.S, ps);
-        }
-        const bool ended = valid && !cont;
-        const uint32_t pos = wave_append(cont, &V.c->qn[1 - p]);
-        if (cont) {
-            store_state(N, pos, ps);
-            V.sid[1 - p][pos] = s;
-        }
-        const uint32_t e = wave_append(ended, &V.c->en);
-        if (ended) {
-            V.F.L_end[e] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
-            V.F.end[e] = make_uint2(ps.status, s);
-            V.e_id[e] = make_uint4(ps.pixel, uint32_t(ps.sample), uint32_t(ps.sample >> 32), ps.w2);
-        }
-        const uint32_t spos = wave_append(shadow, &V.c->sn);
-        if (shadow) {
-            V.F.sh_o[spos] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
-            V.F.sh_d[spos] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
-            V.F.sh_val[spos] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
-            V.F.sh_meta[spos] = make_uint4(sr.medium, sr.delta_or_nomis ? 1u : 0u, ps.w2, cont ? pos : (kEnded | e));
-            V.sh_sid[spos] = s;
-        }
+        ps.L[0] = L.x, ps.L[1] = L.y, ps.L[2] = L.z, ps.L[3] = L.w;
+        shadow_apply(ps, sr, vis, tr);
+        *dst = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
     }
 }
 
 // kInterfaces = false when no scene of the batch has null interfaces (the
 // visibility test then needs no interface walk, and far fewer registers).
+// One thread per shadow ray; the grid covers the capacity.
 template <int BS, bool kInterfaces>
 __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V, uint32_t p) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
+    const uint32_t n = V.c->sn;
+    if (blockIdx.x * BS >= n) return;
     Stack st;
     st.lds_node = s_node + threadIdx.x;
     st.lds_t = s_t + threadIdx.x;
     st.stride = BS;
     st.sp = 0;
-    const uint32_t n = V.c->sn;
-    const WfState &N = V.S[1 - p];
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t s = valid ? V.sh_sid[i] : 0u;
+    bool mine;
+    const uint32_t u = wave_scene(valid, s, mine);
+    const bool other = valid && !mine;
+    const uint32_t o = wave_append(other, &V.c->ovs);
+    if (other) V.ov[o] = i;
+    if (mine) pool_shadow_ray<kInterfaces>(V, p, st, i, uniform_scene(V.scenes, u));
+}
+
+// The shadow rays k_pool_shadow handed over, one scene per lane.
+template <bool kInterfaces>
+__global__ void __launch_bounds__(256) k_pool_shadow_mixed(PoolView V, uint32_t p) {
+    __shared__ uint32_t s_node[kShortStack * 256];
+    __shared__ float s_t[kShortStack * 256];
+    Stack st;
+    st.lds_node = s_node + threadIdx.x;
+    st.lds_t = s_t + threadIdx.x;
+    st.stride = 256;
+    st.sp = 0;
+    const uint32_t n = V.c->ovs;
     uint32_t rays = 0;
     RP_GRID_STRIDE(n) {
-        uint32_t i = base_ + threadIdx.x;
+        const uint32_t i = base_ + threadIdx.x;
         if (i < n) {
-            const BatchScene &BSc = V.scenes[V.sh_sid[i]];
-            float4 o = V.F.sh_o[i], d = V.F.sh_d[i], v = V.F.sh_val[i];
-            uint4 meta = V.F.sh_meta[i];
-            ShadowReq sr;
-            sr.o = {o.x, o.y, o.z};
-            sr.tmax = o.w;
-            sr.d = {d.x, d.y, d.z};
-            sr.w = d.w;
-            sr.val[0] = v.x, sr.val[1] = v.y, sr.val[2] = v.z, sr.val[3] = v.w;
-            sr.medium = meta.x;
-            sr.delta_or_nomis = meta.y != 0;
-            Wavelengths wl = path_wavelengths(BSc.S, meta.z);
-            float tr[4];
-            bool vis = transmittance<kInterfaces>(BSc.S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, BSc.P.boundary);
-            if (vis) {
-                float4 *dst = (meta.w & kEnded) ? &V.F.L_end[meta.w & ~kEnded] : &N.L[meta.w];
-                float4 L = *dst;
-                PathState ps;
-                ps.L[0] = L.x, ps.L[1] = L.y, ps.L[2] = L.z, ps.L[3] = L.w;
-                shadow_apply(ps, sr, vis, tr);
-                *dst = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
-            }
+            const uint32_t j = V.ov[i];
+            pool_shadow_ray<kInterfaces>(V, p, st, j, V.scenes[V.sh_sid[j]]);
             ++rays;
         }
     }
-    wave_count(&V.c->rays[1], rays);
+    wave_count(&V.c->mixed, rays);
 }
 
-// Ended paths: sensor XYZ into their scene's per-path planes.
+// Ended paths: sensor XYZ into their scene's per-path planes.  All en ended
+// paths retire here, and the bounce's shadow rays are final: one thread
+// adds both counts.
 __global__ void k_pool_retire(PoolView V) {
     const uint32_t n = V.c->en;
-    uint32_t failed = 0, retired = 0;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        V.c->done += n;
+        V.c->rays[1] += V.c->sn;
+    }
+    uint32_t failed = 0;
     RP_GRID_STRIDE(n) {
         uint32_t i = base_ + threadIdx.x;
         if (i < n) {
@@ -354,12 +481,8 @@ __global__ void k_pool_retire(PoolView V) {
             BSc.xyz[local] = v[0];
             BSc.xyz[BSc.paths + local] = v[1];
             BSc.xyz[2 * BSc.paths + local] = v[2];
-            retired += 1;
         }
     }
-    unsigned long long r = retired;
-    for (int off = 32; off > 0; off >>= 1) r += __shfl_down(r, off);
-    if ((threadIdx.x & 63) == 0 && r) atomicAdd(reinterpret_cast<unsigned long long *>(&V.c->done), r);
     wave_count(&V.c->rays[2], failed);
 }
 

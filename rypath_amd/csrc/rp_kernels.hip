@@ -113,6 +113,7 @@ template <int B> struct LdsStack {
     }
 };
 
+using rpd::kCounterStride;
 __device__ __forceinline__ void count_rays(unsigned long long *counters, uint32_t ext, uint32_t sh, uint32_t failed) {
     // wave-level reduction, one atomic per wave per counter
     unsigned long long e = ext, s = sh, f = failed;
@@ -122,9 +123,9 @@ __device__ __forceinline__ void count_rays(unsigned long long *counters, uint32_
         f += __shfl_down(f, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&counters[0], e);
-        atomicAdd(&counters[1], s);
-        atomicAdd(&counters[2], f);
+        if (e) atomicAdd(&counters[0], e);
+        if (s) atomicAdd(&counters[kCounterStride], s);
+        if (f) atomicAdd(&counters[2 * kCounterStride], f);
     }
 }
 
@@ -217,8 +218,16 @@ __global__ void k_film_add(uint32_t npx, uint32_t samples, uint64_t n, const flo
     }
 }
 
+// Statistics counters live one per 256-B line (counter k at
+// c[k * kCounterStride]): atomics on one line are serialised.
 __global__ void k_zero_counters(unsigned long long *c, uint32_t n) {
-    if (threadIdx.x < n) c[threadIdx.x] = 0;
+    if (threadIdx.x < n) c[threadIdx.x * kCounterStride] = 0;
+}
+// Copies counters [0, n) of a strided counter block to the host.
+void read_counters(const unsigned long long *dev, uint32_t n, unsigned long long *out) {
+    std::vector<unsigned long long> tmp(size_t(n) * kCounterStride);
+    HIP_CHECK(hipMemcpy(tmp.data(), dev, tmp.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (uint32_t k = 0; k < n; ++k) out[k] = tmp[size_t(k) * kCounterStride];
 }
 
 __global__ void k_film_zero(uint64_t n, double *film) {
@@ -323,7 +332,7 @@ void upload_scene(rp_scene *s) {
     s->sobol_dir.upload(dirT);
     s->sobol_intervals.upload(sd.intervals);
     s->sobol_inverse.upload(sd.inverse);
-    s->counters.alloc(8);
+    s->counters.alloc(8 * kCounterStride);
 
     rpd::Scene &S = s->dev;
     S.tris = s->tris.p;
@@ -428,7 +437,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, s->stream));
     double trace_seconds = 0;
-    uint64_t launches = 0;
+    uint64_t launches = 0, wf_rays[2] = {0, 0};
     // the device's wavefront arena is held until this frame's last kernel has
     // finished (the event synchronisation below happens before the unlock)
     std::unique_lock<std::mutex> arena_lock;
@@ -446,7 +455,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
                                npx, cfg.sample_begin + s0, n, s->path_xyz.p, s->counters.p);
         } else {
             trace_seconds += rpd::wavefront_render(s->dev, P, arena->fb, cfg, npx, cfg.sample_begin + s0, n,
-                                                   s->path_xyz.p, s->counters.p, s->stream, launches);
+                                                   s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays);
         }
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
@@ -472,11 +481,11 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     HIP_CHECK(hipGetLastError());
     if (stats) {
         unsigned long long c[8];
-        HIP_CHECK(hipMemcpy(c, s->counters.p, sizeof c, hipMemcpyDeviceToHost));
+        read_counters(s->counters.p, 8, c);
         stats->seconds = ms * 1e-3;
         stats->paths = uint64_t(npx) * spp;
-        stats->extension_rays = c[0];
-        stats->shadow_rays = c[1];
+        stats->extension_rays = c[0] + wf_rays[0];
+        stats->shadow_rays = c[1] + wf_rays[1];
         stats->failed_paths = c[2];
         stats->trace_seconds = trace_seconds;
         stats->extend_launches = launches;
@@ -936,7 +945,7 @@ struct rp_batch {
     DevBuf<unsigned long long> work;
     // wavefront pool
     rpd::FrameBuffers wf;
-    DevBuf<uint32_t> sid0, sid1, sh_sid;
+    DevBuf<uint32_t> sid0, sid1, sh_sid, ov;
     DevBuf<uint4> e_id;
     bool any_interfaces = false; // some scene of the batch has null interfaces
     DevBuf<rpd::PoolCounters> counters;
@@ -973,6 +982,7 @@ rpd::PoolView pool_view(rp_batch *b) {
     V.F = b->wf.f;
     V.sh_sid = b->sh_sid.p;
     V.e_id = b->e_id.p;
+    V.ov = b->ov.p;
     V.scenes = b->scenes.p;
     V.nscenes = uint32_t(b->host_scenes.size());
     V.capacity = b->capacity;
@@ -980,8 +990,12 @@ rpd::PoolView pool_view(rp_batch *b) {
     return V;
 }
 
-template <int BS> void pool_iteration(rp_batch *b, const rpd::PoolView &V, uint32_t p, uint32_t bits, dim3 grid) {
+// `grid` is a fixed grid-stride grid; the scene-uniform shade and shadow
+// kernels take one thread per entry and a grid that covers the capacity.
+template <int BS>
+void pool_iteration(rp_batch *b, const rpd::PoolView &V, uint32_t p, uint32_t bits, dim3 grid) {
     hipStream_t st = b->stream;
+    const dim3 cover((V.capacity + BS - 1) / BS);
     hipLaunchKernelGGL(rpd::k_pool_refill, grid, dim3(256), 0, st, V, p);
     hipLaunchKernelGGL(rpd::k_pool_commit, dim3(1), dim3(256), 0, st, V, p, bits);
     hipLaunchKernelGGL(rpd::k_pool_extend<BS>, grid, dim3(BS), 0, st, V, p);
@@ -990,9 +1004,15 @@ template <int BS> void pool_iteration(rp_batch *b, const rpd::PoolView &V, uint3
         hipLaunchKernelGGL(rpd::k_sort_scan, dim3(1), dim3(64), 0, st, bits, V.F.hist);
         hipLaunchKernelGGL(rpd::k_pool_sort_scatter, grid, dim3(256), 0, st, V, p, bits);
     }
-    hipLaunchKernelGGL(rpd::k_pool_shade<BS>, grid, dim3(BS), 0, st, V, p, bits ? 1u : 0u);
-    if (b->any_interfaces) hipLaunchKernelGGL((rpd::k_pool_shadow<BS, true>), grid, dim3(BS), 0, st, V, p);
-    else hipLaunchKernelGGL((rpd::k_pool_shadow<BS, false>), grid, dim3(BS), 0, st, V, p);
+    hipLaunchKernelGGL(rpd::k_pool_shade<BS>, cover, dim3(BS), 0, st, V, p, bits ? 1u : 0u);
+    hipLaunchKernelGGL(rpd::k_pool_shade_mixed, grid, dim3(256), 0, st, V, p);
+    if (b->any_interfaces) {
+        hipLaunchKernelGGL((rpd::k_pool_shadow<BS, true>), cover, dim3(BS), 0, st, V, p);
+        hipLaunchKernelGGL(rpd::k_pool_shadow_mixed<true>, grid, dim3(256), 0, st, V, p);
+    } else {
+        hipLaunchKernelGGL((rpd::k_pool_shadow<BS, false>), cover, dim3(BS), 0, st, V, p);
+        hipLaunchKernelGGL(rpd::k_pool_shadow_mixed<false>, grid, dim3(256), 0, st, V, p);
+    }
     hipLaunchKernelGGL(rpd::k_pool_retire, grid, dim3(256), 0, st, V);
 }
 
@@ -1013,7 +1033,7 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
     if (bits > 8) throw std::runtime_error("sort_bits must be in [0, 8]");
     if (cap != b->capacity) {
         b->wf.ensure(cap);
-        b->sid0.alloc(cap), b->sid1.alloc(cap), b->sh_sid.alloc(cap), b->e_id.alloc(cap);
+        b->sid0.alloc(cap), b->sid1.alloc(cap), b->sh_sid.alloc(cap), b->e_id.alloc(cap), b->ov.alloc(cap);
         b->counters.alloc(1);
         b->capacity = cap;
         if (b->graph) {
@@ -1079,9 +1099,9 @@ double batch_render_impl(rp_batch *b, const rp_batch_config &cfg, float **xyz_ho
     HIP_CHECK(hipEventCreate(&e1));
     HIP_CHECK(hipEventRecord(e0, b->stream));
     uint32_t launches = 0;
-    unsigned long long rays[3] = {0, 0, 0};
+    unsigned long long rays[3] = {0, 0, 0}, mixed = 0;
     if (cfg.scheduler == RP_SCHED_PERSISTENT) {
-        b->work.alloc(4);
+        b->work.alloc(4 * kCounterStride);
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, b->stream, b->work.p, 4u);
         const uint32_t bs = cfg.group_size ? cfg.group_size : 128;
         const dim3 grid(uint32_t(cu_count(b->device)) * (bs == 64 ? 16 : bs == 128 ? 8 : 4));
@@ -1118,10 +1138,11 @@ double batch_render_impl(rp_batch *b, const rp_batch_config &cfg, float **xyz_ho
     (void)hipEventDestroy(e1);
     if (cfg.scheduler == RP_SCHED_PERSISTENT) {
         unsigned long long w[4];
-        HIP_CHECK(hipMemcpy(w, b->work.p, sizeof w, hipMemcpyDeviceToHost));
+        read_counters(b->work.p, 4, w);
         rays[0] = w[1], rays[1] = w[2], rays[2] = w[3];
     } else {
         rays[0] = b->host_counters->rays[0], rays[1] = b->host_counters->rays[1], rays[2] = b->host_counters->rays[2];
+        mixed = b->host_counters->mixed;
     }
     if (stats) {
         stats->seconds = ms * 1e-3;
@@ -1131,6 +1152,7 @@ double batch_render_impl(rp_batch *b, const rp_batch_config &cfg, float **xyz_ho
         stats->failed_paths = rays[2];
         stats->graph_launches = launches;
         stats->pad = 0;
+        stats->mixed_scene_lanes = mixed;
     }
     return ms * 1e-3;
 }
@@ -1219,25 +1241,34 @@ int rp_autotune(rp_batch *b, rp_batch_config *best, double *best_seconds) {
         const uint32_t randomization = best->randomization;
         double best_t = 1e300;
         rp_batch_config best_cfg{};
+        // Each candidate renders twice and the second render is timed: the
+        // first one allocates the pool for a new capacity and captures its graph.
         auto trial = [&](rp_batch_config c) {
             c.randomization = randomization;
-            double t = batch_render_impl(b, c, nullptr, nullptr);
+            (void)batch_render_impl(b, c, nullptr, nullptr);
+            const double t = batch_render_impl(b, c, nullptr, nullptr);
             if (t < best_t) {
                 best_t = t;
                 best_cfg = c;
             }
+            return t;
         };
         for (uint32_t g : {64u, 128u, 256u}) trial({0, g, 0, 0, 0, RP_SCHED_PERSISTENT});
-        const uint32_t g0 = 128;
-        for (uint32_t cap : {1u << 22, 1u << 24, 1u << 25}) trial({cap, g0, 0, 1, 0, RP_SCHED_WAVEFRONT});
-        rp_batch_config wf = best_cfg.scheduler == RP_SCHED_WAVEFRONT ? best_cfg : rp_batch_config{1u << 24, g0, 0, 1, 0, RP_SCHED_WAVEFRONT};
-        for (uint32_t g : {64u, 256u}) {
-            rp_batch_config c = wf;
-            c.group_size = g;
-            trial(c);
-        }
+        // wavefront pool: capacity x workgroup size at sort width 0, then the
+        // sort width on the fastest of those
+        rp_batch_config wf{};
+        double wf_t = 1e300;
+        for (uint32_t cap : {1u << 22, 1u << 24, 1u << 25})
+            for (uint32_t g : {128u, 256u}) {
+                const rp_batch_config c{cap, g, 0, 1, 0, RP_SCHED_WAVEFRONT};
+                const double t = trial(c);
+                if (t < wf_t) {
+                    wf_t = t;
+                    wf = c;
+                }
+            }
         for (uint32_t bits : {2u, 4u}) {
-            rp_batch_config c = best_cfg.scheduler == RP_SCHED_WAVEFRONT ? best_cfg : wf;
+            rp_batch_config c = wf;
             c.sort_bits = bits;
             trial(c);
         }
@@ -1411,7 +1442,7 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
         HIP_CHECK(hipMemcpy(grad, g.p, count * sizeof(float), hipMemcpyDeviceToHost));
         if (stats) {
             unsigned long long c[8];
-            HIP_CHECK(hipMemcpy(c, s->counters.p, sizeof c, hipMemcpyDeviceToHost));
+            read_counters(s->counters.p, 8, c);
             std::memset(stats, 0, sizeof *stats);
             stats->seconds = ms * 1e-3;
             stats->paths = total;

@@ -95,12 +95,64 @@ RP_DEV void wave_count(unsigned long long *counter, uint32_t v) {
     if ((threadIdx.x & 63) == 0 && e) atomicAdd(counter, e);
 }
 
+// Atomics on one L2 cache line are serialised: measured on MI355X, one
+// returning atomicAdd per wave on a single counter costs ~11 ns per wave
+// (518,400 waves: 5.9 ms; three counters on one line: 17.6 ms; three
+// counters on separate 256-B lines: 6.0 ms; one atomic per 256-lane block:
+// 1.5 ms).  Hence: every statistics counter has its own 256-B line (counter
+// k at counters[k * kCounterStride]), queue counters that are appended to
+// in the same kernel sit on separate lines, and the appends are aggregated
+// per block.
+constexpr int kCounterStride = 32; // unsigned long long entries per line
+constexpr int kQueueStride = 64;   // uint32_t entries per line
+
+// LDS of block_append: per-wave counts (then per-wave offsets) and the
+// block's base position per queue.
+template <int BS, int K> struct AppendLds {
+    uint32_t cnt[K][BS / 64];
+    uint32_t base[K];
+};
+
+// Block-aggregated append to K queues: pos[k] is this thread's position in
+// queue k when pred[k] holds (positions of one block are contiguous, in
+// lane order).  One atomic per queue and block.  Every thread of the block
+// must call it (it synchronises the block twice).
+template <int BS, int K>
+RP_DEV void block_append(AppendLds<BS, K> &sh, const bool (&pred)[K], uint32_t *const (&queue)[K],
+                         uint32_t (&pos)[K]) {
+    static_assert(BS % 64 == 0 && K <= 64, "block_append shape");
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    uint64_t m[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        m[k] = __ballot(pred[k]);
+        if (lane == 0) sh.cnt[k][wave] = uint32_t(__builtin_popcountll(m[k]));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (threadIdx.x == uint32_t(k)) {
+            uint32_t acc = 0;
+            for (int w = 0; w < BS / 64; ++w) {
+                const uint32_t c = sh.cnt[k][w];
+                sh.cnt[k][w] = acc;
+                acc += c;
+            }
+            sh.base[k] = acc ? atomicAdd(queue[k], acc) : 0u;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) pos[k] = sh.base[k] + sh.cnt[k][wave] + uint32_t(__builtin_popcountll(m[k] & below));
+}
+
 constexpr int kWfBlock = 128;
 
 // ------------------------------------------------------------ frame wavefront
 // The frame path keeps the live paths' state physically compacted: bounce k
 // reads state set C[0, qn) and the shading kernel appends the surviving paths
-// to set N (wave-aggregated positions), so every state, hit and shadow access
+// to set N (block-aggregated positions), so every state, hit and shadow access
 // is a dense, coalesced stream.  A path that ends leaves its radiance and
 // (status, wavelength word) at its lane (pixel + npx * sample offset); a
 // shadow ray knows where its contribution goes: the survivor's new position,
@@ -113,7 +165,7 @@ struct FrameWf {
     float4 *L_end;  // per lane
     uint2 *end;     // per lane: (status, wavelength word)
     uint32_t *perm; // material-sorted order of [0, qn)
-    uint32_t *qcount; // [0] survivors, [1] shadow rays
+    uint32_t *qcount; // [0] survivors, [kQueueStride] shadow rays (separate lines)
     uint32_t *hist;   // [256] histogram, [256] cursors
 };
 constexpr uint32_t kEnded = 0x80000000u;
@@ -139,7 +191,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
     st.stride = kWfBlock;
     st.sp = 0;
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
-    uint32_t rays = 0, nodes = 0, tris = 0;
+    uint32_t nodes = 0, tris = 0;
     if (i < qn) {
         float4 a = C.o_t[i], b = C.d[i];
         Hit h;
@@ -147,16 +199,16 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
         traverse<false, kCount>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h, &tc);
         F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
         F.hit_id[i] = make_uint2(h.prim, h.id);
-        rays = 1;
         if (kCount) {
             nodes = tc.nodes;
             tris = tc.tris;
         }
     }
-    wave_count(&counters[0], rays);
+    // ray counts are the queue counts (summed on the host); only the
+    // instrumented replay tallies node fetches and triangle tests
     if (kCount) {
-        wave_count(&counters[3], nodes);
-        wave_count(&counters[4], tris);
+        wave_count(&counters[3 * kCounterStride], nodes);
+        wave_count(&counters[4 * kCounterStride], tris);
     }
 }
 
@@ -215,7 +267,7 @@ __global__ void __launch_bounds__(256) k_sort_scatter(Scene S, const uint2 *hit_
 }
 __global__ void k_wf_reset(uint32_t *qcount, uint32_t *hist, uint32_t bits) {
     uint32_t t = threadIdx.x;
-    if (t < 4) qcount[t] = 0;
+    if (t < 2) qcount[t * kQueueStride] = 0;
     uint32_t K = 1u << bits;
     for (uint32_t k = t; k < 512; k += blockDim.x)
         if (k < K || (k >= 256 && k < 256 + K)) hist[k] = 0;
@@ -233,10 +285,10 @@ __global__ void k_wf_reset(uint32_t *qcount, uint32_t *hist, uint32_t bits) {
 #endif
 
 // One path event for the path at C[j] when `valid`: survivors are appended
-// to N and shadow rays to the shadow stream (every lane of the wave must
-// call this together: the appends are wave-aggregated).
+// to N and shadow rays to the shadow stream (every thread of the block must
+// call this together: the appends are block-aggregated).
 RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, const WfState &N, const FrameWf &F,
-                      bool valid, uint32_t j, uint32_t npx, uint64_t sample_begin) {
+                      bool valid, uint32_t j, uint32_t npx, uint64_t sample_begin, AppendLds<kWfBlock, 2> &sh) {
     bool cont = false, shadow = false;
     PathState ps;
     ShadowReq sr;
@@ -259,14 +311,17 @@ RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, con
         cont = cont && path_alive(S, ps);
         lane = uint32_t(ps.sample - sample_begin) * npx + ps.pixel;
     }
-    const uint32_t pos = wave_append(cont, &F.qcount[0]);
+    const bool pred[2] = {cont, shadow};
+    uint32_t *const queue[2] = {&F.qcount[0], &F.qcount[kQueueStride]};
+    uint32_t at[2];
+    block_append(sh, pred, queue, at);
+    const uint32_t pos = at[0], spos = at[1];
     if (cont) {
         store_state(N, pos, ps);
     } else if (valid) {
         F.L_end[lane] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
         F.end[lane] = make_uint2(ps.status, ps.w2);
     }
-    const uint32_t spos = wave_append(shadow, &F.qcount[1]);
     if (shadow) {
         F.sh_o[spos] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
         F.sh_d[spos] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
@@ -279,9 +334,10 @@ RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, con
 __global__ void __launch_bounds__(kWfBlock) RP_SHADE_ATTR k_wf_shade(Scene S, PathParams P, WfState C, WfState N,
                                                                      FrameWf F, uint32_t qn, bool sorted, uint32_t npx,
                                                                      uint64_t sample_begin) {
+    __shared__ AppendLds<kWfBlock, 2> sh;
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     const bool valid = i < qn;
-    shade_one(S, P, C, N, F, valid, valid ? (sorted ? F.perm[i] : i) : 0u, npx, sample_begin);
+    shade_one(S, P, C, N, F, valid, valid ? (sorted ? F.perm[i] : i) : 0u, npx, sample_begin, sh);
 }
 
 // Shadow rays of one bounce; adds visible contributions to the survivor's
@@ -297,7 +353,6 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
     st.stride = kWfBlock;
     st.sp = 0;
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
-    uint32_t rays = 0;
     if (i < sn) {
         float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
         uint4 meta = F.sh_meta[i];
@@ -320,9 +375,8 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
             shadow_apply(ps, sr, vis, tr);
             *dst = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
         }
-        rays = 1;
     }
-    wave_count(&counters[1], rays);
+    // shadow ray counts are the queue counts (summed on the host)
 }
 
 __global__ void __launch_bounds__(256) k_wf_finish(Scene S, FrameWf F, uint64_t n, float *xyz,
@@ -342,7 +396,7 @@ __global__ void __launch_bounds__(256) k_wf_finish(Scene S, FrameWf F, uint64_t 
         xyz[n + i] = v[1];
         xyz[2 * n + i] = v[2];
     }
-    wave_count(&counters[2], failed);
+    wave_count(&counters[2 * kCounterStride], failed);
 }
 
 #define WF_CHECK(expr)                                                                                        \
@@ -396,18 +450,21 @@ inline void FrameBuffers::ensure(uint64_t n) {
     wf_alloc(f.hit_t, n), wf_alloc(f.hit_id, n);
     wf_alloc(f.sh_o, n), wf_alloc(f.sh_d, n), wf_alloc(f.sh_val, n), wf_alloc(f.sh_meta, n);
     wf_alloc(f.L_end, n), wf_alloc(f.end, n), wf_alloc(f.perm, n);
-    wf_alloc(f.qcount, 4);
+    wf_alloc(f.qcount, 2 * kQueueStride);
     wf_alloc(f.hist, 512);
-    WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 4 * sizeof(uint32_t), hipHostMallocDefault));
+    WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 2 * kQueueStride * sizeof(uint32_t),
+                           hipHostMallocDefault));
     capacity = n;
 }
 
 // Runs n paths (path i: pixel i % npx, sample sample_begin + i / npx) to
 // completion and writes planar per-path XYZ.  Returns device seconds spent in
-// the closest-hit (extend) kernels.
+// the closest-hit (extend) kernels; adds the extension and shadow ray counts
+// (the queue counts of every bounce) to rays[0] and rays[1].
 inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers &FB, const rp_frame_config &cfg,
                                uint32_t npx, uint64_t sample_begin, uint64_t n, float *xyz,
-                               unsigned long long *counters, hipStream_t stream, uint64_t &launches) {
+                               unsigned long long *counters, hipStream_t stream, uint64_t &launches,
+                               uint64_t rays[2]) {
     if (n > (uint64_t(1) << 31)) throw std::runtime_error("wavefront chunk too large");
     FB.ensure(n);
     const FrameWf &F = FB.f;
@@ -441,12 +498,15 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
             hipLaunchKernelGGL(k_sort_scatter, g, dim3(256), 0, stream, S, F.hit_id, nullptr, qn, bits, F.hist, F.perm);
         }
         hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx, sample_begin);
-        WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, (kQueueStride + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                stream));
         WF_CHECK(hipStreamSynchronize(stream));
         float ms = 0;
         WF_CHECK(hipEventElapsedTime(&ms, e0, e1));
         trace += ms * 1e-3;
-        const uint32_t next = FB.host_count[0], sn = FB.host_count[1];
+        const uint32_t next = FB.host_count[0], sn = FB.host_count[kQueueStride];
+        rays[0] += qn;
+        rays[1] += sn;
         if (next > qn || sn > qn) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
