@@ -102,6 +102,18 @@ def cpu_baseline(synth, width, height, budget):
                       f"{build:.1f}s excluded"}
 
 
+def inverse_line(args, dt, W, H, spp, nparams):
+    return {
+        "metric": f"adjoint gradients/s ({W}x{H}, {spp} spp forward + {spp} spp adjoint, {nparams} params)",
+        "value": round(1 / dt, 2), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "inverse-cornell-fog-256-16spp", "width": W, "height": H, "spp": spp,
+                   "params": nparams, "loss": "mean squared XYZ error vs 1024 spp target"},
+        "M_paths_s": round(2 * W * H * spp / dt / 1e6, 2),
+    }
+
+
 def inverse_workload(args):
     """Inverse rendering step on the 256x256 Cornell box with a fog box and 58
     parameters (48 wall reflectance bins, 2 GGX roughnesses, 8 fog sigma_s
@@ -156,24 +168,49 @@ def inverse_workload(args):
     print(f"[bench] inverse step: {dt * 1e3:.2f} ms ({1 / dt:.1f} gradients/s), loss {loss:.4e}",
           file=sys.stderr, flush=True)
 
+    # Gradient check.  Roulette decisions flip with the parameters and make
+    # central differences at a 1e-3 step far noisier than 2 %; the check runs
+    # forward and adjoint with roulette off (RP_FRAME_NO_ROULETTE), which
+    # changes no derivative term.  The loss is linearised at an independent
+    # pilot render (fixed weights w = dLoss/dXYZ of the pilot): with weights
+    # taken from the same samples as the derivative, the two estimators would
+    # differ by their different covariance with the image noise, not by the
+    # derivative.  Each group is probed along the pilot adjoint gradient's own
+    # direction, so the directional derivative is as large as the group allows.
     groups = {"walls": slice(0, 48), "roughness": slice(48, 50), "sigma_s": slice(50, 58)}
+    if args.fd_trials < 2:
+        print(json.dumps(dict(inverse_line(args, dt, W, H, spp, len(params)), gradient_check=None)), flush=True)
+        return
+    flags = rp.FRAME_NO_ROULETTE
+    pilot = rp.frame_config(fd_spp, mode=rp.MODE_WAVEFRONT, seed=4999, flags=flags)
+    scene.render_frame_device(pilot, img.data_ptr())
+    wts = (2.0 * (img.double() - target.double()) / img.numel())
+    wts_np = wts.float().cpu().numpy()
+    g0, _ = scene.backward(pilot, params, wts_np)
+    g0 = g0.astype(np.float64)
+
+    def functional(cfg):
+        scene.render_frame_device(cfg, img.data_ptr())
+        return float((img.double() * wts).sum())
+
     dirs = {}
-    for i, (name, sl) in enumerate(groups.items()):
+    for name, sl in groups.items():
         v = np.zeros(len(values))
-        v[sl] = np.random.default_rng(40 + i).normal(0, 1, sl.stop - sl.start) * np.abs(values[sl])
-        dirs[name] = v
+        v[sl] = g0[sl] * np.abs(values[sl])
+        dirs[name] = v / np.linalg.norm(v)
     fd = {n: [] for n in groups}
     ad = {n: [] for n in groups}
     t = time.perf_counter()
     for trial in range(args.fd_trials):
-        cfg = rp.frame_config(fd_spp, mode=rp.MODE_WAVEFRONT, seed=5000 + trial)
-        _, g = loss_and_grad(cfg)
+        cfg = rp.frame_config(fd_spp, mode=rp.MODE_WAVEFRONT, seed=5000 + trial, flags=flags)
+        g, _ = scene.backward(cfg, params, wts_np)
+        g = g.astype(np.float64)
         for name, v in dirs.items():
             ad[name].append(float(g @ v))
             ls = []
             for sgn in (+1, -1):
                 scene.set_params(params, (values + sgn * fd_h * v).astype(np.float32))
-                ls.append(loss_and_grad(cfg, want_grad=False)[0])
+                ls.append(functional(cfg))
             scene.set_params(params, values.astype(np.float32))
             fd[name].append((ls[0] - ls[1]) / (2 * fd_h))
         print(f"[bench] gradient check trial {trial}: " +
@@ -187,20 +224,15 @@ def inverse_workload(args):
         se = lambda x: float(x.std(ddof=1) / np.sqrt(n)) if n > 1 else None
         check[name] = {"adjoint": float(a.mean()), "adjoint_se": se(a), "central_difference": float(f.mean()),
                        "central_difference_se": se(f),
-                       "rel_err": float(abs(f.mean() - a.mean()) / abs(a.mean()))}
+                       "rel_err": float(abs(f.mean() - a.mean()) / abs(a.mean())),
+                       # same sample set per trial: the paired difference has the smaller spread
+                       "paired_rel_err_se": (se(f - a) / abs(a.mean())) if n > 1 else None}
     worst = max(c["rel_err"] for c in check.values())
-    paths = 2 * W * H * spp
-    line = {
-        "metric": "adjoint gradients/s (256x256, 16 spp forward + 16 spp adjoint, 58 params)",
-        "value": round(1 / dt, 2), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "inverse-cornell-fog-256-16spp", "width": W, "height": H, "spp": spp,
-                   "params": len(params), "loss": "mean squared XYZ error vs 1024 spp target"},
-        "M_paths_s": round(paths / dt / 1e6, 2),
-        "gradient_check": {"step": fd_h, "spp": fd_spp, "trials": args.fd_trials, "groups": check,
-                           "max_rel_err": round(worst, 5), "seconds": round(check_s, 1)},
-    }
+    line = dict(inverse_line(args, dt, W, H, spp, len(params)),
+                gradient_check={"step": fd_h, "spp": fd_spp, "trials": args.fd_trials, "roulette": False,
+                                "direction": "adjoint gradient per group (independent pilot)",
+                                "functional": "loss linearised at an independent pilot render", "groups": check,
+                                "max_rel_err": round(worst, 5), "seconds": round(check_s, 1)})
     print(json.dumps(line), flush=True)
 
 

@@ -207,11 +207,16 @@ typedef struct rp_frame_config {
     uint32_t capacity;         /* wavefront paths in flight, 0 = auto */
     uint32_t group_size;       /* threads per workgroup, 0 = auto */
     uint32_t sort_bits;        /* material sort-key width, 0 = auto */
-    uint32_t flags;            /* RP_FRAME_COUNT_TRAFFIC: replay-count BVH node/triangle fetches */
+    uint32_t flags;            /* RP_FRAME_* bits below */
     uint32_t pad;
 } rp_frame_config;
 
-#define RP_FRAME_COUNT_TRAFFIC 1u
+#define RP_FRAME_COUNT_TRAFFIC 1u /* replay-count BVH node/triangle fetches of the closest-hit queries */
+/* Throughput roulette off (forward and rp_backward): every path runs to
+ * max_depth or escape.  For derivative checks by finite differences, where
+ * the roulette decisions that flip with the parameters dominate the noise;
+ * rendering keeps the reference's roulette (the default). */
+#define RP_FRAME_NO_ROULETTE 2u
 
 typedef struct rp_frame_stats {
     double seconds;                 /* device time of the frame (events) */

@@ -283,6 +283,7 @@ def request(seed=1, pixel_begin=0, pixel_count=1, sample_begin=0, sample_count=1
 
 
 FRAME_COUNT_TRAFFIC = 1
+FRAME_NO_ROULETTE = 2
 
 
 def frame_config(spp, mode=MODE_WAVEFRONT_SORTED, sample_begin=0, seed=1, randomization=RANDOMIZE_FASTOWEN,

@@ -1,7 +1,9 @@
 // Path-replay adjoint (Vicini, Speierer, Jakob 2021) for the hero-wavelength
 // path tracer.  Each path is traced twice from its Sobol sequence: the primal
-// pass gives the radiance L_k of every wavelength lane, the replay walks the
-// same events with an AdjointHook and accumulates, for every factor a_v(theta)
+// pass (the frame wavefront, which leaves every path's radiance and status
+// in its per-lane arrays) gives the radiance L_k of every wavelength lane,
+// the replay walks the same events with an AdjointHook and accumulates, for
+// every factor a_v(theta)
 // applied to the throughput, dLoss/dL_k * (radiance carried after v) *
 // d log a_v / d theta.  Sampling decisions and MIS weights are detached (their
 // densities are constants of the estimator), so nothing is stored per path.
@@ -58,7 +60,12 @@ RP_DEV float conductor_dlogf_dr(const Lane &l, const rp_material &m, F3 wo, F3 w
 struct AdjointHook {
     static constexpr bool active = true;
     const GradTables *G;
-    float *grad;   // LDS accumulator (kMaxParams)
+    // LDS accumulator: parameter p at grad[p * stride].  With stride = block
+    // size and grad = this thread's column, every thread owns its sums (no
+    // atomics, bank-conflict free, a fixed summation order); with stride 0
+    // the block shares one row through LDS atomics (many parameters).
+    float *grad;
+    uint32_t stride;
     float a[4];    // dLoss/dL_k
     float L[4];    // primal radiance
     float S[4];    // radiance accumulated so far in the replay
@@ -76,7 +83,9 @@ struct AdjointHook {
         }
     }
     RP_DEV void accumulate(int32_t idx, float v) {
-        if (idx >= 0 && v != 0.0f && __builtin_isfinite(v)) atomicAdd(&grad[idx], v);
+        if (!(idx >= 0 && v != 0.0f && __builtin_isfinite(v))) return;
+        if (stride) grad[uint32_t(idx) * stride] += v;
+        else atomicAdd(&grad[idx], v);
     }
     RP_DEV void add(const float *v) {
         for (int k = 0; k < 4; ++k) S[k] = S[k] + v[k];
@@ -132,86 +141,125 @@ struct AdjointHook {
     }
 };
 
-// Primal + replay for n paths (path i: pixel i % npx, sample sample_begin + i / npx).
-template <int BS>
+// Replay of one path whose primal radiance is L (status `status`); its
+// gradient terms go to hk's accumulator.
+RP_DEV void adjoint_path(const Scene &S, const PathParams &P, const GradTables &G, Stack &st, uint32_t pixel,
+                         uint64_t sample, uint32_t spp, const float *dl_dxyz, float4 L, uint32_t status,
+                         AdjointHook &hk) {
+    if (status != RP_PATH_SUCCESS) return;
+    PathState ps;
+    uint32_t w0, w1;
+    path_begin(S, P, pixel, sample, ps, w0, w1);
+    const Wavelengths wl = path_wavelengths(S, ps.w2);
+    const float dx = dl_dxyz[3 * size_t(pixel)], dy = dl_dxyz[3 * size_t(pixel) + 1],
+                dz = dl_dxyz[3 * size_t(pixel) + 2];
+    const float Lp[4] = {L.x, L.y, L.z, L.w};
+    for (int k = 0; k < 4; ++k) {
+        hk.a[k] = 0.0f;
+        if (wl.pdf[k] > 0) {
+            int i = int(floorf(wl.lambda[k] + 0.5f)) - 360;
+            i = i < 0 ? 0 : (i > 470 ? 470 : i);
+            float4 c = S.cie[i];
+            hk.a[k] = ((dx * c.x + dy * c.y + dz * c.z) / wl.pdf[k]) / (4.0f * float(spp));
+        }
+        hk.L[k] = Lp[k];
+        hk.S[k] = 0.0f;
+    }
+    while (path_alive(S, ps)) {
+        Hit h;
+        traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
+        hk.reset_pending();
+        ShadowReq sr;
+        bool cont = path_event(S, P, ps, h, sr, hk);
+        if (sr.active) {
+            float tr[4], mlen = 0.0f;
+            bool vis = transmittance(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary, &mlen,
+                                     G.sigma_medium);
+            if (vis) {
+                float c[4];
+                for (int k = 0; k < 4; ++k) c[k] = sr.val[k] * tr[k];
+                if (!sr.delta_or_nomis)
+                    for (int k = 0; k < 4; ++k) c[k] = c[k] * sr.w;
+                hk.add(c);
+                for (int k = 0; k < 4; ++k) {
+                    hk.accumulate(hk.nee_idx[k], hk.a[k] * c[k] * hk.nee_dlog[k]);
+                    if (G.sigma_medium != RP_NONE && mlen > 0) {
+                        int32_t idx = G.sigma_base[G.sigma_medium] +
+                                      int32_t(lambda_bin(wl.lambda[k], G.sigma_bins[G.sigma_medium]));
+                        hk.accumulate(idx, hk.a[k] * c[k] * (-mlen));
+                    }
+                }
+            }
+        }
+        for (int k = 0; k < 4; ++k) hk.accumulate(hk.cont_idx[k], hk.a[k] * (hk.L[k] - hk.S[k]) * hk.cont_dlog[k]);
+        if (!cont) break;
+    }
+}
+
+// Parameters up to which every thread keeps its own LDS column of sums
+// (kPerThread): nparams * block * 4 B of dynamic LDS.
+constexpr uint32_t kPerThreadParams = 96;
+
+// Replay for paths [0, n) (path i: pixel i % npx, sample sample_begin +
+// i / npx; primal radiance L_end[i], status end[i].x), grid-stride over a
+// fixed grid so that block b always takes the same paths in the same order.
+// Block b adds its sums to row b of `partial` ([gridDim.x][nparams]);
+// k_grad_reduce folds the rows.  No global atomics, and with kPerThread no
+// atomics at all: the gradient is the same bit for bit on every run.
+template <int BS, bool kPerThread>
 __global__ void __launch_bounds__(BS) k_adjoint(Scene S, PathParams P, GradTables G, uint32_t npx, uint64_t sample_begin,
-                                                 uint32_t spp, uint64_t n, const float *dl_dxyz, float *grad_out,
-                                                 unsigned long long *counters) {
+                                                 uint32_t spp, uint64_t n, const float *dl_dxyz, const float4 *L_end,
+                                                 const uint2 *end, float *partial) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
-    __shared__ float sgrad[kMaxParams];
-    for (uint32_t p = threadIdx.x; p < kMaxParams; p += BS) sgrad[p] = 0.0f;
-    __syncthreads();
+    extern __shared__ float sgrad[]; // kPerThread: [nparams][BS]; else [nparams]
+    const uint32_t np = G.nparams;
+    if (kPerThread) {
+        for (uint32_t p = 0; p < np; ++p) sgrad[p * BS + threadIdx.x] = 0.0f;
+    } else {
+        for (uint32_t p = threadIdx.x; p < np; p += BS) sgrad[p] = 0.0f;
+        __syncthreads();
+    }
     Stack st;
     st.lds_node = s_node + threadIdx.x;
     st.lds_t = s_t + threadIdx.x;
     st.stride = BS;
     st.sp = 0;
-    uint64_t lane = uint64_t(blockIdx.x) * BS + threadIdx.x;
-    uint32_t rays = 0;
-    if (lane < n) {
-        const uint32_t pixel = uint32_t(lane % npx);
-        const uint64_t sample = sample_begin + lane / npx;
-        PathResult r;
-        trace_path(S, P, st, pixel, sample, r);
-        rays = r.ext_rays + r.shadow_rays;
-        if (r.status == RP_PATH_SUCCESS) {
-            AdjointHook hk;
-            hk.G = &G;
-            hk.grad = sgrad;
-            const float dx = dl_dxyz[3 * size_t(pixel)], dy = dl_dxyz[3 * size_t(pixel) + 1],
-                        dz = dl_dxyz[3 * size_t(pixel) + 2];
-            for (int k = 0; k < 4; ++k) {
-                hk.a[k] = 0.0f;
-                if (r.wl.pdf[k] > 0) {
-                    int i = int(floorf(r.wl.lambda[k] + 0.5f)) - 360;
-                    i = i < 0 ? 0 : (i > 470 ? 470 : i);
-                    float4 c = S.cie[i];
-                    hk.a[k] = ((dx * c.x + dy * c.y + dz * c.z) / r.wl.pdf[k]) / (4.0f * float(spp));
-                }
-                hk.L[k] = r.L[k];
-                hk.S[k] = 0.0f;
-            }
-            PathState ps;
-            uint32_t w0, w1;
-            path_begin(S, P, pixel, sample, ps, w0, w1);
-            const Wavelengths wl = path_wavelengths(S, ps.w2);
-            while (path_alive(S, ps)) {
-                Hit h;
-                traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
-                hk.reset_pending();
-                ShadowReq sr;
-                bool cont = path_event(S, P, ps, h, sr, hk);
-                if (sr.active) {
-                    float tr[4], mlen = 0.0f;
-                    bool vis = transmittance(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary, &mlen,
-                                             G.sigma_medium);
-                    if (vis) {
-                        float c[4];
-                        for (int k = 0; k < 4; ++k) c[k] = sr.val[k] * tr[k];
-                        if (!sr.delta_or_nomis)
-                            for (int k = 0; k < 4; ++k) c[k] = c[k] * sr.w;
-                        hk.add(c);
-                        for (int k = 0; k < 4; ++k) {
-                            hk.accumulate(hk.nee_idx[k], hk.a[k] * c[k] * hk.nee_dlog[k]);
-                            if (G.sigma_medium != RP_NONE && mlen > 0) {
-                                int32_t idx = G.sigma_base[G.sigma_medium] +
-                                              int32_t(lambda_bin(wl.lambda[k], G.sigma_bins[G.sigma_medium]));
-                                hk.accumulate(idx, hk.a[k] * c[k] * (-mlen));
-                            }
-                        }
-                    }
-                }
-                for (int k = 0; k < 4; ++k)
-                    hk.accumulate(hk.cont_idx[k], hk.a[k] * (hk.L[k] - hk.S[k]) * hk.cont_dlog[k]);
-                if (!cont) break;
-            }
-        }
-    }
+    AdjointHook hk;
+    hk.G = &G;
+    hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;
+    hk.stride = kPerThread ? BS : 0u;
+    for (uint64_t lane = uint64_t(blockIdx.x) * BS + threadIdx.x; lane < n; lane += uint64_t(gridDim.x) * BS)
+        adjoint_path(S, P, G, st, uint32_t(lane % npx), sample_begin + lane / npx, spp, dl_dxyz, L_end[lane],
+                     end[lane].x, hk);
     __syncthreads();
-    for (uint32_t p = threadIdx.x; p < G.nparams; p += BS)
-        if (sgrad[p] != 0.0f) atomicAdd(&grad_out[p], sgrad[p]);
-    wave_count(&counters[0], rays);
+    if (kPerThread) {
+        // fold the columns: a fixed tree over the block's threads
+        for (uint32_t h = BS / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h)
+                for (uint32_t p = 0; p < np; ++p) sgrad[p * BS + threadIdx.x] += sgrad[p * BS + threadIdx.x + h];
+            __syncthreads();
+        }
+        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] += sgrad[p * BS];
+    } else {
+        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] += sgrad[p];
+    }
+}
+
+// grad[p] = sum over the rows of partial[rows][np], one block per parameter,
+// a fixed summation order.
+__global__ void __launch_bounds__(256) k_grad_reduce(const float *partial, uint32_t rows, uint32_t np, float *grad) {
+    __shared__ float s[256];
+    const uint32_t p = blockIdx.x;
+    float acc = 0.0f;
+    for (uint32_t r = threadIdx.x; r < rows; r += 256) acc += partial[size_t(r) * np + p];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[p] = s[0];
 }
 
 } // namespace rpd

@@ -217,18 +217,17 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
     st.lds_t = s_t + threadIdx.x;
     st.stride = BS;
     st.sp = 0;
-    const uint32_t n = V.c->qn[p];
-    RP_GRID_STRIDE(n) {
-        uint32_t i = base_ + threadIdx.x;
-        if (i < n) {
-            float4 a = V.S[p].o_t[i], b = V.S[p].d[i];
-            Hit h;
-            for_scene(V.scenes, V.sid[p][i], [&](const BatchScene &B) {
-                traverse<false>(B.S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h);
-            });
-            V.F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
-            V.F.hit_id[i] = make_uint2(h.prim, h.id);
-        }
+    const uint32_t n = V.c->qn[p], nb = (n + BS - 1) / BS;
+    if (blockIdx.x >= nb) return;
+    const uint32_t i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) {
+        float4 a = V.S[p].o_t[i], b = V.S[p].d[i];
+        Hit h;
+        for_scene(V.scenes, V.sid[p][i], [&](const BatchScene &B) {
+            traverse<false>(B.S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h);
+        });
+        V.F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
+        V.F.hit_id[i] = make_uint2(h.prim, h.id);
     }
 }
 
@@ -344,8 +343,8 @@ RP_DEV void pool_shade_path(const PoolView &V, uint32_t p, bool valid, uint32_t 
 template <int BS>
 __global__ void __launch_bounds__(BS) k_pool_shade(PoolView V, uint32_t p, uint32_t sorted) {
     __shared__ AppendLds<BS, 3> sh;
-    const uint32_t n = V.c->qn[p];
-    if (blockIdx.x * BS >= n) return;
+    const uint32_t n = V.c->qn[p], nb = (n + BS - 1) / BS;
+    if (blockIdx.x >= nb) return;
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
     const bool valid = i < n;
     uint32_t j = 0, s = 0;
@@ -413,8 +412,8 @@ template <int BS, bool kInterfaces>
 __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V, uint32_t p) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
-    const uint32_t n = V.c->sn;
-    if (blockIdx.x * BS >= n) return;
+    const uint32_t n = V.c->sn, nb = (n + BS - 1) / BS;
+    if (blockIdx.x >= nb) return;
     Stack st;
     st.lds_node = s_node + threadIdx.x;
     st.lds_t = s_t + threadIdx.x;

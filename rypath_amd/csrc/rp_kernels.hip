@@ -372,12 +372,14 @@ void upload_scene(rp_scene *s) {
                            s->sobol_dir.n * 4 + s->lights.n * sizeof(rpd::LightEntry);
 }
 
-rpd::PathParams path_params(uint64_t seed, uint32_t randomization, uint32_t emergency, uint32_t boundary) {
+rpd::PathParams path_params(uint64_t seed, uint32_t randomization, uint32_t emergency, uint32_t boundary,
+                            uint32_t frame_flags = 0) {
     rpd::PathParams P;
     P.seed = seed;
     P.mode = randomization;
     P.emergency = emergency ? emergency : 1024u;
     P.boundary = boundary ? boundary : 4096u;
+    P.roulette = (frame_flags & RP_FRAME_NO_ROULETTE) ? 0u : 1u;
     return P;
 }
 
@@ -419,7 +421,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     if (cfg.mode > RP_MODE_WAVEFRONT_SORTED) throw std::runtime_error("unknown mode");
     if (cfg.randomization > RP_RANDOMIZE_OWEN) throw std::runtime_error("unknown randomization");
     HIP_CHECK(hipSetDevice(s->device));
-    rpd::PathParams P = path_params(cfg.seed, cfg.randomization, 0, 0);
+    rpd::PathParams P = path_params(cfg.seed, cfg.randomization, 0, 0, cfg.flags);
     const uint64_t max_chunk_paths = uint64_t(1) << 26;
     uint32_t chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(spp, max_chunk_paths / npx)));
     s->film.alloc(6 * size_t(npx));
@@ -990,15 +992,16 @@ rpd::PoolView pool_view(rp_batch *b) {
     return V;
 }
 
-// `grid` is a fixed grid-stride grid; the scene-uniform shade and shadow
-// kernels take one thread per entry and a grid that covers the capacity.
+// `grid` is a fixed grid-stride grid; the extend kernel and the scene-uniform
+// shade and shadow kernels take one thread per entry and a grid that covers
+// the capacity (blocks past the live count return at once).
 template <int BS>
 void pool_iteration(rp_batch *b, const rpd::PoolView &V, uint32_t p, uint32_t bits, dim3 grid) {
     hipStream_t st = b->stream;
     const dim3 cover((V.capacity + BS - 1) / BS);
     hipLaunchKernelGGL(rpd::k_pool_refill, grid, dim3(256), 0, st, V, p);
     hipLaunchKernelGGL(rpd::k_pool_commit, dim3(1), dim3(256), 0, st, V, p, bits);
-    hipLaunchKernelGGL(rpd::k_pool_extend<BS>, grid, dim3(BS), 0, st, V, p);
+    hipLaunchKernelGGL(rpd::k_pool_extend<BS>, cover, dim3(BS), 0, st, V, p);
     if (bits) {
         hipLaunchKernelGGL(rpd::k_pool_sort_hist, grid, dim3(256), 0, st, V, p, bits);
         hipLaunchKernelGGL(rpd::k_sort_scan, dim3(1), dim3(64), 0, st, bits, V.F.hist);
@@ -1413,12 +1416,24 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
         rpd::GradTables G = grad_tables(s, params, count, gb);
         const rp_camera &cam = s->host.desc.camera;
         const uint32_t npx = cam.width * cam.height;
-        DevBuf<float> dl, g;
+        DevBuf<float> dl, g, partial;
         dl.upload(dl_dxyz, 3 * size_t(npx));
         g.alloc(std::max<uint32_t>(count, 1));
-        std::vector<float> zeros(std::max<uint32_t>(count, 1), 0.0f);
-        HIP_CHECK(hipMemcpy(g.p, zeros.data(), zeros.size() * sizeof(float), hipMemcpyHostToDevice));
-        rpd::PathParams P = path_params(cfg->seed, cfg->randomization, 0, 0);
+        HIP_CHECK(hipMemsetAsync(g.p, 0, std::max<uint32_t>(count, 1) * sizeof(float), s->stream));
+        // a fixed grid (block b always takes the same paths) with one row of
+        // partial sums per block, folded by k_grad_reduce.  One path per
+        // thread up to 65,536 blocks: with fewer, longer-running blocks the
+        // slowest paths of each block set the pace (2,048 blocks of 4 paths
+        // per thread measured 7 % slower on the 256x256 16 spp adjoint).
+        constexpr int kAdjBlock = 128;
+        const uint64_t total_paths = uint64_t(npx) * cfg->spp;
+        const uint32_t rows = uint32_t(
+            std::max<uint64_t>(1, std::min<uint64_t>(65536, (total_paths + kAdjBlock - 1) / kAdjBlock)));
+        partial.alloc(size_t(rows) * std::max<uint32_t>(count, 1));
+        HIP_CHECK(hipMemsetAsync(partial.p, 0, size_t(rows) * std::max<uint32_t>(count, 1) * sizeof(float), s->stream));
+        const bool per_thread = count <= rpd::kPerThreadParams;
+        const size_t lds = per_thread ? size_t(count) * kAdjBlock * sizeof(float) : size_t(count) * sizeof(float);
+        rpd::PathParams P = path_params(cfg->seed, cfg->randomization, 0, 0, cfg->flags);
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, s->counters.p, 8u);
         hipEvent_t e0, e1;
         HIP_CHECK(hipEventCreate(&e0));
@@ -1426,13 +1441,34 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
         HIP_CHECK(hipEventRecord(e0, s->stream));
         const uint64_t total = uint64_t(npx) * cfg->spp;
         const uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t(1) << 26) / npx); // whole samples per launch
+        // primal: the frame wavefront over the same paths, which leaves each
+        // path's radiance and status in the device arena's per-lane arrays;
+        // the arena is held until the replay of the last chunk has finished
+        // (the event synchronisation below happens before the unlock)
+        FrameArena &arena = frame_arena(s->device);
+        std::unique_lock<std::mutex> arena_lock(arena.m);
+        rp_frame_config wcfg = *cfg;
+        wcfg.mode = RP_MODE_WAVEFRONT;
+        uint64_t launches = 0, wf_rays[2] = {0, 0};
         for (uint64_t s0 = 0; s0 < cfg->spp; s0 += per_chunk) {
             uint64_t ns = std::min<uint64_t>(per_chunk, cfg->spp - s0);
             uint64_t n = ns * npx;
-            hipLaunchKernelGGL(rpd::k_adjoint<128>, dim3(uint32_t((n + 127) / 128)), dim3(128), 0, s->stream, s->dev, P,
-                               G, npx, cfg->sample_begin + s0, cfg->spp, n, dl.p, g.p, s->counters.p);
+            s->path_xyz.alloc(3 * n);
+            (void)rpd::wavefront_render(s->dev, P, arena.fb, wcfg, npx, cfg->sample_begin + s0, n, s->path_xyz.p,
+                                        s->counters.p, s->stream, launches, wf_rays);
+            const float4 *L_end = arena.fb.f.L_end;
+            const uint2 *end = arena.fb.f.end;
+            if (per_thread)
+                hipLaunchKernelGGL((rpd::k_adjoint<kAdjBlock, true>), dim3(rows), dim3(kAdjBlock), lds, s->stream,
+                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl.p, L_end, end,
+                                   partial.p);
+            else
+                hipLaunchKernelGGL((rpd::k_adjoint<kAdjBlock, false>), dim3(rows), dim3(kAdjBlock), lds, s->stream,
+                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl.p, L_end, end,
+                                   partial.p);
             HIP_CHECK(hipGetLastError());
         }
+        if (count) hipLaunchKernelGGL(rpd::k_grad_reduce, dim3(count), dim3(256), 0, s->stream, partial.p, rows, count, g.p);
         HIP_CHECK(hipEventRecord(e1, s->stream));
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
@@ -1446,7 +1482,9 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
             std::memset(stats, 0, sizeof *stats);
             stats->seconds = ms * 1e-3;
             stats->paths = total;
-            stats->extension_rays = c[0];
+            stats->extension_rays = wf_rays[0];
+            stats->shadow_rays = wf_rays[1];
+            stats->failed_paths = c[2];
         }
         return RP_SUCCESS;
     } catch (const std::exception &e) {

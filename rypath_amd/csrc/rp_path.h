@@ -15,6 +15,7 @@ struct PathParams {
     uint64_t seed;
     uint32_t mode; // Sobol randomization
     uint32_t emergency, boundary;
+    uint32_t roulette; // 1: throughput roulette (the reference's transport); 0: off
 };
 
 constexpr uint32_t kFlagPrevDelta = 1u, kFlagRoulette = 2u;
@@ -89,7 +90,7 @@ RP_DEV void path_begin(const Scene &S, const PathParams &P, uint32_t pixel, uint
     ps.crossings = 0;
     ps.medium = S.camera.medium;
     ps.mdraw = 0;
-    ps.flags = kFlagRoulette;
+    ps.flags = P.roulette ? kFlagRoulette : 0u;
     ps.status = RP_PATH_SUCCESS;
     ps.primary = RP_NONE;
 }

@@ -32,6 +32,64 @@ def test_adjoint_matches_oracle_forward_mode(rp, orc, inverse):
     assert rel < 1e-3
 
 
+def test_adjoint_no_roulette_matches_oracle_forward_mode(rp, orc, inverse):
+    """RP_FRAME_NO_ROULETTE reaches the adjoint replay: the gradient equals the
+    oracle's forward-mode derivative with the oracle's roulette off."""
+    import ctypes as C
+
+    synth, scene = inverse
+    params, values = rp.inverse_params(synth)
+    dl = np.random.default_rng(6).normal(0, 1, (32, 32, 3)).astype(np.float32)
+    cfg = rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, seed=5, flags=rp.FRAME_NO_ROULETTE)
+    g, _ = scene.backward(cfg, params, dl)
+    g_rr, _ = scene.backward(rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, seed=5), params, dl)
+    orc.L.or_set_roulette.argtypes = [C.c_int]
+    orc.L.or_set_roulette(0)
+    try:
+        ref = oracle_lib.oracle_gradient(orc, orc.scene(synth.desc), params, dl, spp=4, seed=5)
+    finally:
+        orc.L.or_set_roulette(1)
+    rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+    print(f"[adjoint] no roulette: relative L2 difference vs oracle {rel:.2e}", flush=True)
+    assert rel < 1e-3
+    assert not np.array_equal(g, g_rr)
+
+
+def test_reflectance_derivative_within_2_percent_of_central_differences(rp, inverse):
+    """With roulette off the film is a polynomial in the wall reflectance
+    bins (no sampling decision depends on them), so central differences with
+    common random numbers have no discontinuity noise: the adjoint's
+    directional derivative of an L2 loss along its own wall-bin direction is
+    within 2 % of central differences with step 1e-3."""
+    synth, scene = inverse
+    params, values = rp.inverse_params(synth)
+    v = np.zeros(len(values))
+    rng = np.random.default_rng(11)
+    target = rng.uniform(0.5, 2.0, (32, 32, 3)).astype(np.float64)
+    cfg = rp.frame_config(256, mode=rp.MODE_WAVEFRONT, seed=21, flags=rp.FRAME_NO_ROULETTE)
+    img, _ = scene.render_frame(cfg)
+    scale = float(np.mean(img))
+    tgt = target * scale
+
+    def loss(x):
+        scene.set_params(params, x.astype(np.float32))
+        im, _ = scene.render_frame(cfg)
+        return float(((im.astype(np.float64) - tgt) ** 2).mean())
+
+    scene.set_params(params, values)
+    dl = (2.0 * (img.astype(np.float64) - tgt) / img.size).astype(np.float32)
+    g, _ = scene.backward(cfg, params, dl)
+    v[:48] = g[:48] * np.abs(values[:48])
+    v /= np.linalg.norm(v)
+    h = 1e-3
+    fd = (loss(values + h * v) - loss(values - h * v)) / (2 * h)
+    scene.set_params(params, values)
+    ad = float(g.astype(np.float64) @ v)
+    print(f"[fd] walls, roulette off: adjoint {ad:.6e} central differences {fd:.6e} "
+          f"rel {abs(fd - ad) / abs(ad):.2e}", flush=True)
+    assert abs(fd - ad) <= 0.02 * abs(ad)
+
+
 @pytest.mark.parametrize("group,h", [("walls", 0.02), ("roughness", 0.02), ("sigma_s", 0.1)])
 def test_adjoint_vs_central_differences(rp, inverse, group, h):
     """Directional derivative along a random direction in one parameter

@@ -93,6 +93,34 @@ def test_frame_vs_oracle(rp, orc, cornell):
     assert abs(img.mean() / ref.mean() - 1) < 5e-3
 
 
+def test_frame_no_roulette_vs_oracle(rp, orc, cornell):
+    """RP_FRAME_NO_ROULETTE: every frame mode agrees bit for bit, the film
+    matches the oracle with its roulette switched off, and the flag takes
+    effect (more extension rays than with roulette)."""
+    import ctypes as C
+
+    synth, scene = cornell
+    films, rays = [], []
+    for mode in (rp.MODE_MEGAKERNEL, rp.MODE_WAVEFRONT):
+        img, st = scene.render_frame(rp.frame_config(8, mode=mode, seed=2, flags=rp.FRAME_NO_ROULETTE))
+        films.append(img)
+        rays.append(st["extension_rays"])
+    assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))
+    assert rays[0] == rays[1]
+    _, st_rr = scene.render_frame(rp.frame_config(8, mode=rp.MODE_WAVEFRONT, seed=2))
+    print(f"[no-roulette] extension rays {rays[0]} (with roulette {st_rr['extension_rays']})", flush=True)
+    assert rays[0] > st_rr["extension_rays"]
+    orc.L.or_set_roulette.argtypes = [C.c_int]
+    orc.L.or_set_roulette(0)
+    try:
+        ref, _ = orc.scene(synth.desc).render_frame(8, seed=2)
+    finally:
+        orc.L.or_set_roulette(1)
+    frac = agreement(films[0].reshape(-1, 3), ref.reshape(-1, 3))
+    print(f"[no-roulette] per-pixel agreement with the oracle {frac:.6f}", flush=True)
+    assert frac >= AGREE_FRACTION
+
+
 def test_frame_sample_ranges_compose(rp, cornell):
     # two 4-spp frames at sample offsets 0 and 4 average to the 8-spp frame
     _, scene = cornell
