@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume", "batch", "inverse"])
     ap.add_argument("--fd-trials", type=int, default=4, help="inverse: independent sample sets of the gradient check")
     ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--pool-config", default=None,
+                    help="pool mode: 'capacity,group_size,sort_bits' instead of running the autotuner")
     ap.add_argument("--profile", action="store_true",
                     help="profiling run: frames only (no roofline replay, no CPU leg), then a 1 GiB device "
                          "copy used to calibrate FETCH_SIZE / WRITE_SIZE")
@@ -333,7 +335,12 @@ def main():
     for name in args.modes.split(","):
         if name == "pool":
             batch = rp.Batch([(scene, args.spp, 1)])
-            bcfg, _ = batch.autotune()
+            if args.pool_config:
+                cap, gs, bits = (int(x) for x in args.pool_config.split(","))
+                bcfg = rp.Batch.config(scheduler=rp.SCHED_WAVEFRONT, capacity=cap, group_size=gs, sort_bits=bits,
+                                       use_graph=1)
+            else:
+                bcfg, _ = batch.autotune()
             for _ in range(args.warmup):
                 batch.render(bcfg, films=False)
             torch.cuda.synchronize()
@@ -383,8 +390,18 @@ def main():
         achieved = alg / st["trace_seconds"] / 1e9
         traffic = pmc_traffic() if (args.workload == "terrain" and (args.width, args.height, args.spp) ==
                                     (1920, 1080, 16) and frame_mode == "wave") else None
+        # measured read peaks on this device (rp_probe_read_bandwidth): an
+        # 8 GiB buffer streams from HBM; 24 MiB read 400 times stays in the
+        # XCDs' L2 (3 MiB per XCD, above the 32 KiB L1 share of each CU)
+        hbm_meas = rp.probe_read_bandwidth(8 << 30, 2)
+        l2_meas = rp.probe_read_bandwidth(24 << 20, 400)
+        print(f"[bench] measured read bandwidth: HBM {hbm_meas:.0f} GB/s, L2 {l2_meas:.0f} GB/s",
+              file=sys.stderr, flush=True)
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "measured_read_peak_hbm": round(hbm_meas, 1), "frac_of_measured_hbm": round(achieved / hbm_meas, 4),
+                    "measured_read_peak_l2": round(l2_meas, 1), "frac_of_measured_l2": round(achieved / l2_meas, 4),
+                    "traffic": traffic,
                     "traffic_source": PMC_SUMMARY.relative_to(ROOT).as_posix() if traffic else None,
                     "algorithmic_bytes_per_launch": round(alg / max(st["extend_launches"], 1)),
                     "kernel": f"k_wf_extend (closest hit, {frame_mode} frame)",

@@ -274,6 +274,13 @@ int rp_trace_closest(rp_scene *scene, const rp_ray *rays, uint64_t count, rp_hit
 int rp_trace_any(rp_scene *scene, const rp_ray *rays, uint64_t count, uint8_t *occluded,
                  double *seconds);
 
+/* Device read bandwidth (no reference counterpart; used to state roofline
+ * fractions against measured peaks): streams a `bytes` buffer `passes` times
+ * with 16-byte loads over all CUs and returns GB/s (1e9 B/s).  A buffer far
+ * above the L2 and Infinity Cache measures HBM; a few MiB read many times
+ * measures L2 (every XCD caches the eighth of the buffer its CUs read). */
+int rp_probe_read_bandwidth(int device, uint64_t bytes, uint32_t passes, double *gbytes_per_second);
+
 /* ---- batched work pool (uneven scenes, one persistent launch) ----------- */
 typedef struct rp_batch_item {
     rp_scene *scene;

@@ -165,6 +165,7 @@ _sig("rp_render_frame", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p
 _sig("rp_render_frame_device", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
 _sig("rp_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
 _sig("rp_trace_any", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
+_sig("rp_probe_read_bandwidth", C.c_int, [C.c_int, u64, u32, C.POINTER(f64)])
 _sig("rp_batch_create", C.c_int, [C.POINTER(BatchItem), u32, C.POINTER(C.c_void_p)])
 _sig("rp_batch_render", C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(C.c_void_p), C.POINTER(BatchStats)])
 _sig("rp_batch_destroy", None, [C.c_void_p])
@@ -212,6 +213,7 @@ _sig("rp_frame_release", None, [C.c_void_p])
 EXPORTED = [
     "rp_api_version", "rp_device_count", "rp_last_error", "rp_scene_create", "rp_scene_info", "rp_scene_destroy",
     "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_trace_closest", "rp_trace_any",
+    "rp_probe_read_bandwidth",
     "rp_batch_create", "rp_batch_render", "rp_batch_destroy", "rp_autotune", "rp_set_params", "rp_backward",
     "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
     "rp_pbrt_load", "rp_pbrt_load_string", "rp_pbrt_desc", "rp_pbrt_get_info", "rp_pbrt_destroy",
@@ -499,6 +501,14 @@ class PbrtScene:
         if getattr(self, "_h", None):
             _lib.rp_pbrt_destroy(self._h)
             self._h = None
+
+
+def probe_read_bandwidth(bytes_, passes, device=0):
+    """Device read bandwidth in GB/s over a `bytes_` buffer read `passes`
+    times (rp_probe_read_bandwidth)."""
+    out = f64()
+    _check("rp_probe_read_bandwidth", _lib.rp_probe_read_bandwidth(device, bytes_, passes, C.byref(out)))
+    return out.value
 
 
 def write_exr(path, xyz, exposure=1.0, half=False, metadata=None):

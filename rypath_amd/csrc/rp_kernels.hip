@@ -931,6 +931,73 @@ int rp_trace_any(rp_scene *s, const rp_ray *rays, uint64_t count, uint8_t *occlu
 
 } // extern "C"
 
+namespace {
+// Read-bandwidth probe: grid-stride float4 loads over the buffer, four in
+// flight per iteration, `passes` times; the sum keeps the loads alive.
+__global__ void __launch_bounds__(256) k_probe_read(const float4 *buf, uint64_t n4, uint32_t passes, float *out) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float acc = 0.0f;
+    for (uint32_t p = 0; p < passes; ++p) {
+        uint64_t i = t;
+        for (; i + 3 * stride < n4; i += 4 * stride) {
+            const float4 a = buf[i], b = buf[i + stride], c = buf[i + 2 * stride], d = buf[i + 3 * stride];
+            acc += ((a.x + a.y) + (a.z + a.w)) + ((b.x + b.y) + (b.z + b.w)) + ((c.x + c.y) + (c.z + c.w)) +
+                   ((d.x + d.y) + (d.z + d.w));
+        }
+        for (; i < n4; i += stride) {
+            const float4 a = buf[i];
+            acc += (a.x + a.y) + (a.z + a.w);
+        }
+    }
+    out[t] = acc;
+}
+} // namespace
+
+extern "C" {
+
+int rp_probe_read_bandwidth(int device, uint64_t bytes, uint32_t passes, double *gbytes_per_second) {
+    if (!gbytes_per_second || bytes < 4096 || bytes % 16 || passes == 0) {
+        set_error("rp_probe_read_bandwidth: need bytes >= 4096, a multiple of 16, passes >= 1 and an output");
+        return RP_INVALID;
+    }
+    try {
+        HIP_CHECK(hipSetDevice(device));
+        hipStream_t st = nullptr;
+        HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        std::unique_ptr<ihipStream_t, void (*)(hipStream_t)> stream_guard(st, [](hipStream_t x) {
+            (void)hipStreamDestroy(x);
+        });
+        DevBuf<float4> buf;
+        buf.alloc(bytes / 16);
+        HIP_CHECK(hipMemsetAsync(buf.p, 0, bytes, st));
+        int cus = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        const dim3 grid(uint32_t(cus > 0 ? cus : 256) * 16), block(256);
+        DevBuf<float> out;
+        out.alloc(size_t(grid.x) * block.x);
+        hipLaunchKernelGGL(k_probe_read, grid, block, 0, st, buf.p, bytes / 16, 1u, out.p); // warm-up
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        HIP_CHECK(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(k_probe_read, grid, block, 0, st, buf.p, bytes / 16, passes, out.p);
+        HIP_CHECK(hipEventRecord(e1, st));
+        HIP_CHECK(hipEventSynchronize(e1));
+        HIP_CHECK(hipGetLastError());
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *gbytes_per_second = double(bytes) * passes / (double(ms) * 1e-3) / 1e9;
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_probe_read_bandwidth", e);
+    }
+}
+
+} // extern "C"
+
 // ============================================================== batches
 struct rp_batch {
     std::vector<rp_batch_item> items;
