@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--modes", default="mega,wave,sorted,pool")
     ap.add_argument("--workload", default="terrain", choices=["terrain", "cornell", "volume", "batch", "inverse"])
     ap.add_argument("--fd-trials", type=int, default=4, help="inverse: independent sample sets of the gradient check")
+    ap.add_argument("--fd-rough-trials", type=int, default=0,
+                    help="inverse: extra sample sets for roughness at --fd-rough-step (its 1e-3 quotient is noisy)")
+    ap.add_argument("--fd-rough-step", type=float, default=0.01)
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--pool-config", default=None,
                     help="pool mode: 'capacity,group_size,sort_bits' instead of running the autotuner")
@@ -217,6 +220,20 @@ def inverse_workload(args):
             fd[name].append((ls[0] - ls[1]) / (2 * fd_h))
         print(f"[bench] gradient check trial {trial}: " +
               ", ".join(f"{n} adjoint {ad[n][-1]:.4e} fd {fd[n][-1]:.4e}" for n in groups), file=sys.stderr, flush=True)
+    # Roughness: VNDF sampling moves directions with the roughness, and at a
+    # 1e-3 step the difference quotient's discontinuity noise (variance ~
+    # 1 / (spp * step)) is far above 2 %.  A larger step on more sample sets,
+    # compared with the adjoint mean of the trials above.
+    rough_fd = []
+    for trial in range(args.fd_rough_trials):
+        cfg = rp.frame_config(fd_spp, mode=rp.MODE_WAVEFRONT, seed=7000 + trial, flags=flags)
+        v = dirs["roughness"]
+        ls = []
+        for sgn in (+1, -1):
+            scene.set_params(params, (values + sgn * args.fd_rough_step * v).astype(np.float32))
+            ls.append(functional(cfg))
+        scene.set_params(params, values.astype(np.float32))
+        rough_fd.append((ls[0] - ls[1]) / (2 * args.fd_rough_step))
     check_s = time.perf_counter() - t
     check = {}
     for name in groups:
@@ -229,7 +246,14 @@ def inverse_workload(args):
                        "rel_err": float(abs(f.mean() - a.mean()) / abs(a.mean())),
                        # same sample set per trial: the paired difference has the smaller spread
                        "paired_rel_err_se": (se(f - a) / abs(a.mean())) if n > 1 else None}
-    worst = max(c["rel_err"] for c in check.values())
+    if rough_fd:
+        f = np.array(rough_fd)
+        a = float(np.mean(ad["roughness"]))
+        check["roughness_step_%g" % args.fd_rough_step] = {
+            "step": args.fd_rough_step, "trials": len(f), "adjoint": a, "central_difference": float(f.mean()),
+            "central_difference_se": float(f.std(ddof=1) / np.sqrt(len(f))) if len(f) > 1 else None,
+            "rel_err": float(abs(f.mean() - a) / abs(a))}
+    worst = max(c["rel_err"] for n, c in check.items() if n in groups)
     line = dict(inverse_line(args, dt, W, H, spp, len(params)),
                 gradient_check={"step": fd_h, "spp": fd_spp, "trials": args.fd_trials, "roulette": False,
                                 "direction": "adjoint gradient per group (independent pilot)",

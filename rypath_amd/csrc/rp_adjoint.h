@@ -221,10 +221,7 @@ __global__ void __launch_bounds__(BS) k_adjoint(Scene S, PathParams P, GradTable
         __syncthreads();
     }
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = BS;
-    st.sp = 0;
+    st.init(s_node, s_t, BS);
     AdjointHook hk;
     hk.G = &G;
     hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;

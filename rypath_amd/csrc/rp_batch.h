@@ -57,10 +57,7 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = BS;
-    st.sp = 0;
+    st.init(s_node, s_t, BS);
     const uint32_t lane = threadIdx.x & 63;
     unsigned long long ext = 0, sh = 0, failed = 0;
     for (;;) {
@@ -213,10 +210,7 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = BS;
-    st.sp = 0;
+    st.init(s_node, s_t, BS);
     const uint32_t n = V.c->qn[p], nb = (n + BS - 1) / BS;
     if (blockIdx.x >= nb) return;
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
@@ -415,10 +409,7 @@ __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V, uint32_t p) {
     const uint32_t n = V.c->sn, nb = (n + BS - 1) / BS;
     if (blockIdx.x >= nb) return;
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = BS;
-    st.sp = 0;
+    st.init(s_node, s_t, BS);
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
     const bool valid = i < n;
     const uint32_t s = valid ? V.sh_sid[i] : 0u;
@@ -436,10 +427,7 @@ __global__ void __launch_bounds__(256) k_pool_shadow_mixed(PoolView V, uint32_t 
     __shared__ uint32_t s_node[kShortStack * 256];
     __shared__ float s_t[kShortStack * 256];
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = 256;
-    st.sp = 0;
+    st.init(s_node, s_t, 256);
     const uint32_t n = V.c->ovs;
     uint32_t rays = 0;
     RP_GRID_STRIDE(n) {

@@ -377,13 +377,26 @@ constexpr int kMaxStack = 64;                      // the BVH builder's bound is
 constexpr int kSpillStack = kMaxStack - kShortStack; // per-lane overflow (scratch)
 static_assert(kShortStack >= 1 && kShortStack <= kMaxStack, "short stack size");
 
+// The short stack is addressed through the LDS address space: through a
+// generic pointer the compiler emits flat loads and stores, which wait on
+// (and are waited for with) every outstanding global load of the lane.
+typedef __attribute__((address_space(3))) uint32_t LdsU32;
+typedef __attribute__((address_space(3))) float LdsF32;
+
 struct Stack {
-    uint32_t *lds_node; // [kShortStack][block]
-    float *lds_t;
-    int stride;         // block size
+    LdsU32 *lds_node; // this lane's column of [kShortStack][block]
+    LdsF32 *lds_t;
+    int stride;       // block size
     int sp;
     uint32_t spill_node[kSpillStack];
     float spill_t[kSpillStack];
+    // node, t: the block's __shared__ arrays of kShortStack * block entries.
+    RP_DEV void init(uint32_t *node, float *t, int block) {
+        lds_node = (LdsU32 *)(node + threadIdx.x);
+        lds_t = (LdsF32 *)(t + threadIdx.x);
+        stride = block;
+        sp = 0;
+    }
     RP_DEV void push(uint32_t n, float t) {
         if (sp < kShortStack) {
             lds_node[sp * stride] = n;
@@ -424,11 +437,14 @@ RP_DEV uint32_t test_children(const uint4 *nodes, uint32_t node, F3 o, F3 inv, f
     child[3] = q3.y;
     const float grow = 1.0f + 2.0f * gamma_n(3);
     uint32_t mask = 0;
+    // A bound is origin + q * scale with an 8-bit q and a power-of-two scale:
+    // the product is exact, so one fused multiply-add rounds exactly like the
+    // separate multiply and add (the arithmetic the BVH builder verifies).
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        float lx = ox + byte_of(q1.x, i) * sx, hx = ox + byte_of(q1.w, i) * sx;
-        float ly = oy + byte_of(q1.y, i) * sy, hy = oy + byte_of(q2.x, i) * sy;
-        float lz = oz + byte_of(q1.z, i) * sz, hz = oz + byte_of(q2.y, i) * sz;
+        float lx = __builtin_fmaf(byte_of(q1.x, i), sx, ox), hx = __builtin_fmaf(byte_of(q1.w, i), sx, ox);
+        float ly = __builtin_fmaf(byte_of(q1.y, i), sy, oy), hy = __builtin_fmaf(byte_of(q2.x, i), sy, oy);
+        float lz = __builtin_fmaf(byte_of(q1.z, i), sz, oz), hz = __builtin_fmaf(byte_of(q2.y, i), sz, oz);
         float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
         float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
         float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;

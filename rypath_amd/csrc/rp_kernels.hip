@@ -106,10 +106,7 @@ constexpr int kBlock = 128;
 
 template <int B> struct LdsStack {
     static __device__ __forceinline__ void init(rpd::Stack &st, uint32_t *n, float *t) {
-        st.lds_node = n + threadIdx.x;
-        st.lds_t = t + threadIdx.x;
-        st.stride = B;
-        st.sp = 0;
+        st.init(n, t, B);
     }
 };
 

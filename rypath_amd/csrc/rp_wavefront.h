@@ -192,10 +192,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = kWfBlock;
-    st.sp = 0;
+    st.init(s_node, s_t, kWfBlock);
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t nodes = 0, tris = 0;
     if (i < qn) {
@@ -354,10 +351,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
     Stack st;
-    st.lds_node = s_node + threadIdx.x;
-    st.lds_t = s_t + threadIdx.x;
-    st.stride = kWfBlock;
-    st.sp = 0;
+    st.init(s_node, s_t, kWfBlock);
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     if (i < sn) {
         float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
