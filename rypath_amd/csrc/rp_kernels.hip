@@ -65,6 +65,8 @@ using namespace rp;
 
 struct rp_scene {
     HostScene host;
+    uint64_t serial = 0;         // unique per scene in the process
+    uint64_t params_version = 0; // bumped by rp_set_params
     Bvh4 bvh;
     rp_build_info info{};
     int device = 0;
@@ -385,9 +387,40 @@ rpd::PathParams path_params(uint64_t seed, uint32_t randomization, uint32_t emer
 // per scene keeps many resident scenes from multiplying that footprint.
 // Frames of different scenes on one device take turns on it.  The arenas
 // live until the process ends.
+// The arena's per-lane radiance and status arrays hold the primal of the
+// last wavefront render on the device; `primal` says which one, so that a
+// path-replay adjoint right after a wavefront forward of the same frame
+// (same scene, parameters and frame configuration) can skip its own primal.
+struct PrimalTag {
+    bool valid = false;
+    uint64_t scene = 0, params_version = 0, seed = 0, sample_begin = 0, paths = 0;
+    uint32_t randomization = 0, spp = 0, roulette_off = 0, npx = 0;
+    uint64_t rays[2] = {0, 0};
+    bool same(const rp_scene *s, const rp_frame_config &c, uint32_t px) const {
+        return valid && scene == s->serial && params_version == s->params_version && seed == c.seed &&
+               sample_begin == c.sample_begin && spp == c.spp && randomization == c.randomization &&
+               roulette_off == ((c.flags & RP_FRAME_NO_ROULETTE) ? 1u : 0u) && npx == px &&
+               paths == uint64_t(px) * c.spp;
+    }
+    void set(const rp_scene *s, const rp_frame_config &c, uint32_t px, const uint64_t r[2]) {
+        valid = true;
+        scene = s->serial;
+        params_version = s->params_version;
+        seed = c.seed;
+        sample_begin = c.sample_begin;
+        spp = c.spp;
+        randomization = c.randomization;
+        roulette_off = (c.flags & RP_FRAME_NO_ROULETTE) ? 1u : 0u;
+        npx = px;
+        paths = uint64_t(px) * c.spp;
+        rays[0] = r[0];
+        rays[1] = r[1];
+    }
+};
 struct FrameArena {
     std::mutex m;
     rpd::FrameBuffers fb;
+    PrimalTag primal;
 };
 FrameArena &frame_arena(int device) {
     static std::mutex guard;
@@ -453,6 +486,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
             hipLaunchKernelGGL(k_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream, s->dev, P,
                                npx, cfg.sample_begin + s0, n, s->path_xyz.p, s->counters.p);
         } else {
+            arena->primal.valid = false;
             trace_seconds += rpd::wavefront_render(s->dev, P, arena->fb, cfg, npx, cfg.sample_begin + s0, n,
                                                    s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays);
         }
@@ -460,6 +494,8 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
                            s->film.p);
     }
+    // one chunk: the arena now holds this frame's primal for an adjoint
+    if (arena && chunk >= spp) arena->primal.set(s, cfg, npx, wf_rays);
     const uint64_t total = (carry ? carry->samples : 0) + spp;
     if (xyz_device)
         hipLaunchKernelGGL(k_film_finish, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, total, s->film.p,
@@ -618,6 +654,8 @@ int rp_scene_create(const rp_scene_desc *desc, const rp_build_options *options, 
     if (options) o = *options;
     try {
         auto s = std::make_unique<rp_scene>();
+        static std::atomic<uint64_t> serials{0};
+        s->serial = ++serials;
         s->device = int(o.device);
         s->host.copy_from(*desc);
         s->bvh = build_bvh4(s->host.desc.positions, s->host.desc.indices, s->host.desc.triangle_count, o.leaf_size,
@@ -1450,6 +1488,7 @@ int rp_set_params(rp_scene *s, const rp_param *params, uint32_t count, const flo
     }
     try {
         std::lock_guard<std::mutex> lock(s->exec);
+        ++s->params_version;
         for (uint32_t i = 0; i < count; ++i) set_param(s, params[i], values[i]);
         HIP_CHECK(hipSetDevice(s->device));
         s->spectra.upload(s->host.spectra);
@@ -1509,17 +1548,30 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
         // path's radiance and status in the device arena's per-lane arrays;
         // the arena is held until the replay of the last chunk has finished
         // (the event synchronisation below happens before the unlock)
+        // A wavefront frame of the same scene, parameters and configuration
+        // rendered just before (one chunk) left exactly this primal in the
+        // arena: then only the replay runs.
         FrameArena &arena = frame_arena(s->device);
         std::unique_lock<std::mutex> arena_lock(arena.m);
         rp_frame_config wcfg = *cfg;
         wcfg.mode = RP_MODE_WAVEFRONT;
         uint64_t launches = 0, wf_rays[2] = {0, 0};
+        const bool one_chunk = per_chunk >= cfg->spp;
+        const bool reuse = one_chunk && arena.primal.same(s, *cfg, npx);
+        if (reuse) {
+            wf_rays[0] = arena.primal.rays[0];
+            wf_rays[1] = arena.primal.rays[1];
+        }
         for (uint64_t s0 = 0; s0 < cfg->spp; s0 += per_chunk) {
             uint64_t ns = std::min<uint64_t>(per_chunk, cfg->spp - s0);
             uint64_t n = ns * npx;
-            s->path_xyz.alloc(3 * n);
-            (void)rpd::wavefront_render(s->dev, P, arena.fb, wcfg, npx, cfg->sample_begin + s0, n, s->path_xyz.p,
-                                        s->counters.p, s->stream, launches, wf_rays);
+            if (!reuse) {
+                arena.primal.valid = false;
+                s->path_xyz.alloc(3 * n);
+                (void)rpd::wavefront_render(s->dev, P, arena.fb, wcfg, npx, cfg->sample_begin + s0, n,
+                                            s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays);
+                if (one_chunk) arena.primal.set(s, *cfg, npx, wf_rays);
+            }
             const float4 *L_end = arena.fb.f.L_end;
             const uint2 *end = arena.fb.f.end;
             if (per_thread)
@@ -1549,6 +1601,7 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
             stats->extension_rays = wf_rays[0];
             stats->shadow_rays = wf_rays[1];
             stats->failed_paths = c[2];
+            stats->extend_launches = launches; // 0: the primal of the preceding wavefront frame was reused
         }
         return RP_SUCCESS;
     } catch (const std::exception &e) {

@@ -55,6 +55,37 @@ def test_adjoint_no_roulette_matches_oracle_forward_mode(rp, orc, inverse):
     assert not np.array_equal(g, g_rr)
 
 
+def test_backward_reuses_a_matching_wavefront_primal(rp, inverse):
+    """rp_backward right after a wavefront frame of the same scene, parameters
+    and configuration reuses that frame's primal (extend_launches 0) and
+    returns the same gradient bit for bit; another frame in between, or new
+    parameter values, make it trace its own primal."""
+    synth, scene = inverse
+    params, values = rp.inverse_params(synth)
+    dl = np.random.default_rng(8).normal(0, 1, (32, 32, 3)).astype(np.float32)
+    cfg = rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=31)
+    other = rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=32)
+    scene.render_frame(other)
+    fresh, st = scene.backward(cfg, params, dl)
+    assert st["extend_launches"] > 0
+    scene.render_frame(cfg)
+    reused, st = scene.backward(cfg, params, dl)
+    assert st["extend_launches"] == 0
+    assert np.array_equal(fresh.view(np.uint32), reused.view(np.uint32))
+    again, st = scene.backward(cfg, params, dl)  # its own primal is still in the arena
+    assert st["extend_launches"] == 0 and np.array_equal(fresh.view(np.uint32), again.view(np.uint32))
+    scene.render_frame(rp.frame_config(16, mode=rp.MODE_MEGAKERNEL, seed=33))  # leaves the arena alone
+    _, st = scene.backward(cfg, params, dl)
+    assert st["extend_launches"] == 0
+    scene.render_frame(cfg)
+    scene.set_params(params, values)  # same values, new parameter version
+    _, st = scene.backward(cfg, params, dl)
+    assert st["extend_launches"] > 0
+    scene.render_frame(other)
+    _, st = scene.backward(cfg, params, dl)
+    assert st["extend_launches"] > 0
+
+
 def test_reflectance_derivative_within_2_percent_of_central_differences(rp, inverse):
     """With roulette off the film is a polynomial in the wall reflectance
     bins (no sampling decision depends on them), so central differences with
