@@ -119,58 +119,87 @@ def inverse_line(args, dt, W, H, spp, nparams):
     }
 
 
+class InverseScene:
+    """The inverse-rendering workload: the 256x256 Cornell box with a fog box
+    and 58 parameters (48 wall reflectance bins, 2 GGX roughnesses, 8 fog
+    sigma_s bins), the L2 loss against a 1024 spp target rendered from
+    parameters perturbed by N(0, 0.1) in logit/log space."""
+
+    W = H = 256
+    spp = 16
+
+    def __init__(self):
+        import numpy as np
+        import torch
+
+        import rypath_amd as rp
+
+        self.rp = rp
+        synth = rp.Synthetic(rp.SYNTH_INVERSE, seed=2, width=self.W, height=self.H)
+        self.synth = synth
+        self.scene = rp.Scene(synth.desc)
+        params, values = rp.inverse_params(synth)
+        self.params, self.values = params, values.astype(np.float64)
+        rng = np.random.default_rng(7)
+        z = rng.normal(0, 0.1, len(values))
+        target_values = self.values.copy()
+        refl = slice(0, 48)
+        logit = np.log(self.values[refl] / (1 - self.values[refl]))
+        target_values[refl] = 1 / (1 + np.exp(-(logit + z[refl])))
+        target_values[48:] = self.values[48:] * np.exp(z[48:])
+        self.img = torch.empty((self.H, self.W, 3), dtype=torch.float32, device="cuda")
+        self.target = torch.empty_like(self.img)
+        self.scene.set_params(params, target_values.astype(np.float32))
+        self.scene.render_frame_device(rp.frame_config(1024, mode=rp.MODE_WAVEFRONT, seed=999),
+                                       self.target.data_ptr())
+        self.scene.set_params(params, self.values.astype(np.float32))
+        torch.cuda.synchronize()
+
+    def loss_and_grad(self, cfg, want_grad=True):
+        import numpy as np
+
+        self.scene.render_frame_device(cfg, self.img.data_ptr())
+        diff = self.img.double() - self.target.double()
+        loss = float((diff * diff).mean())
+        if not want_grad:
+            return loss, None
+        dl = (2.0 * diff / diff.numel()).float().contiguous()
+        g, _ = self.scene.backward_device(cfg, self.params, dl.data_ptr())
+        return loss, g.astype(np.float64)
+
+    def step_seconds(self, steps, warmup):
+        """Mean wall time of one step: 16 spp forward (frame wavefront), the loss
+        and its image gradient (PyTorch), 16 spp path-replay adjoint."""
+        import torch
+
+        rp = self.rp
+        for k in range(warmup):
+            self.loss_and_grad(rp.frame_config(self.spp, mode=rp.MODE_WAVEFRONT, seed=10 + k))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(steps):
+            self.loss, _ = self.loss_and_grad(rp.frame_config(self.spp, mode=rp.MODE_WAVEFRONT, seed=100 + k))
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+
 def inverse_workload(args):
-    """Inverse rendering step on the 256x256 Cornell box with a fog box and 58
-    parameters (48 wall reflectance bins, 2 GGX roughnesses, 8 fog sigma_s
-    bins): a 16 spp forward render, the L2 loss against a 1024 spp target
-    rendered from parameters perturbed by N(0, 0.1) in logit/log space, and a
-    16 spp path-replay adjoint.  Then the gradient check: directional
-    derivatives per parameter group against central differences with step
-    1e-3 and 4096 spp common random numbers, over independent sample sets."""
+    """Inverse rendering step (InverseScene), then the gradient check:
+    directional derivatives per parameter group against central differences
+    with step 1e-3 and 4096 spp common random numbers, over independent
+    sample sets."""
     import numpy as np
     import torch
 
     import rypath_amd as rp
 
-    W = H = 256
-    spp, fd_spp, fd_h = 16, 4096, 1e-3
-    synth = rp.Synthetic(rp.SYNTH_INVERSE, seed=2, width=W, height=H)
-    scene = rp.Scene(synth.desc)
-    params, values = rp.inverse_params(synth)
-    values = values.astype(np.float64)
-    rng = np.random.default_rng(7)
-    z = rng.normal(0, 0.1, len(values))
-    target_values = values.copy()
-    refl = slice(0, 48)
-    logit = np.log(values[refl] / (1 - values[refl]))
-    target_values[refl] = 1 / (1 + np.exp(-(logit + z[refl])))
-    target_values[48:] = values[48:] * np.exp(z[48:])
-    img = torch.empty((H, W, 3), dtype=torch.float32, device="cuda")
-    target = torch.empty_like(img)
-    scene.set_params(params, target_values.astype(np.float32))
-    scene.render_frame_device(rp.frame_config(1024, mode=rp.MODE_WAVEFRONT, seed=999), target.data_ptr())
-    scene.set_params(params, values.astype(np.float32))
-    torch.cuda.synchronize()
-
-    def loss_and_grad(cfg, want_grad=True):
-        scene.render_frame_device(cfg, img.data_ptr())
-        diff = img.double() - target.double()
-        loss = float((diff * diff).mean())
-        if not want_grad:
-            return loss, None
-        dl = (2.0 * diff / diff.numel()).float().cpu().numpy()
-        g, _ = scene.backward(cfg, params, dl)
-        return loss, g.astype(np.float64)
-
-    for k in range(args.warmup):
-        loss_and_grad(rp.frame_config(spp, mode=rp.MODE_WAVEFRONT, seed=10 + k))
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for k in range(args.steps):
-        loss, g = loss_and_grad(rp.frame_config(spp, mode=rp.MODE_WAVEFRONT, seed=100 + k))
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t) / args.steps
-    print(f"[bench] inverse step: {dt * 1e3:.2f} ms ({1 / dt:.1f} gradients/s), loss {loss:.4e}",
+    inv = InverseScene()
+    W, H, spp = inv.W, inv.H, inv.spp
+    fd_spp, fd_h = 4096, 1e-3
+    scene, params, values, img, target = inv.scene, inv.params, inv.values, inv.img, inv.target
+    loss_and_grad = inv.loss_and_grad
+    dt = inv.step_seconds(args.steps, args.warmup)
+    print(f"[bench] inverse step: {dt * 1e3:.2f} ms ({1 / dt:.1f} gradients/s), loss {inv.loss:.4e}",
           file=sys.stderr, flush=True)
 
     # Gradient check.  Roulette decisions flip with the parameters and make
@@ -433,6 +462,18 @@ def main():
                     "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),
                     "tris_per_ray": round(cnt["triangle_tests"] / cnt["extension_rays"], 2),
                     "avg_launch_ms": round(st["trace_seconds"] / max(st["extend_launches"], 1) * 1e3, 3)}
+    # the second headline figure: inverse-rendering gradients per second
+    adjoint = None
+    if args.workload == "terrain":
+        inv = InverseScene()
+        dt_inv = inv.step_seconds(10, 2)
+        adjoint = dict(inverse_line(args, dt_inv, inv.W, inv.H, inv.spp, len(inv.params)), steps=10, warmup=2,
+                       gradient_check="bench.py --workload inverse --fd-trials 6 --fd-rough-trials 32")
+        for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
+            adjoint.pop(k, None)
+        print(f"[bench] inverse step: {dt_inv * 1e3:.2f} ms ({1 / dt_inv:.1f} gradients/s)", file=sys.stderr,
+              flush=True)
+        del inv
     cpu = None if args.no_cpu else cpu_baseline(synth, args.width, args.height, args.cpu_seconds)
     st = r["stats"]
     line = {
@@ -449,7 +490,7 @@ def main():
         "batched_vs_plain": (round(results["pool"]["mpaths"] / results["mega"]["mpaths"], 3)
                              if "pool" in results and "mega" in results else None),
         "bvh_build_s": round(info["build_seconds"], 2), "upload_s": round(info["upload_seconds"], 2),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
 

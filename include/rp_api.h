@@ -346,6 +346,11 @@ int rp_set_params(rp_scene *scene, const rp_param *params, uint32_t count, const
  * is reused and stats->extend_launches is 0. */
 int rp_backward(rp_scene *scene, const rp_frame_config *config, const rp_param *params,
                 uint32_t count, const float *dl_dxyz, float *grad, rp_frame_stats *stats);
+/* Same, with dL/dXYZ read in place from device memory on the scene's device
+ * (a PyTorch tensor's data_ptr()); the call synchronises the device first, so
+ * work that produced it on any stream is complete.  grad is host memory. */
+int rp_backward_device(rp_scene *scene, const rp_frame_config *config, const rp_param *params,
+                       uint32_t count, const float *dl_dxyz_device, float *grad, rp_frame_stats *stats);
 
 #ifdef __cplusplus
 }

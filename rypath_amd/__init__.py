@@ -173,6 +173,8 @@ _sig("rp_autotune", C.c_int, [C.c_void_p, C.POINTER(BatchConfig), C.POINTER(f64)
 _sig("rp_set_params", C.c_int, [C.c_void_p, C.POINTER(Param), u32, C.POINTER(f32)])
 _sig("rp_backward", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.POINTER(Param), u32, C.POINTER(f32),
                               C.POINTER(f32), C.POINTER(FrameStats)])
+_sig("rp_backward_device", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.POINTER(Param), u32, C.c_void_p,
+                                     C.POINTER(f32), C.POINTER(FrameStats)])
 _sig("rp_synthetic_create", C.c_int, [u32, u64, C.POINTER(SyntheticOptions), C.POINTER(C.c_void_p)])
 _sig("rp_synthetic_desc", C.POINTER(SceneDesc), [C.c_void_p])
 _sig("rp_synthetic_destroy", None, [C.c_void_p])
@@ -215,6 +217,7 @@ EXPORTED = [
     "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_trace_closest", "rp_trace_any",
     "rp_probe_read_bandwidth",
     "rp_batch_create", "rp_batch_render", "rp_batch_destroy", "rp_autotune", "rp_set_params", "rp_backward",
+    "rp_backward_device",
     "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
     "rp_pbrt_load", "rp_pbrt_load_string", "rp_pbrt_desc", "rp_pbrt_get_info", "rp_pbrt_destroy",
     "rp_write_exr", "rp_frame_submit", "rp_frame_poll", "rp_frame_wait", "rp_frame_release",
@@ -359,6 +362,16 @@ class Scene:
         st = FrameStats()
         _check("rp_backward", _lib.rp_backward(self._h, C.byref(cfg), arr, len(arr), dl.ctypes.data_as(C.POINTER(f32)),
                                                 g.ctypes.data_as(C.POINTER(f32)), C.byref(st)))
+        return g, {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
+    def backward_device(self, cfg, params, dl_ptr):
+        """rp_backward with dL/dXYZ (H*W*3 float32, contiguous) in device memory
+        at `dl_ptr`, e.g. a CUDA tensor's data_ptr()."""
+        arr = params_array(params)
+        g = np.zeros(len(arr), np.float32)
+        st = FrameStats()
+        _check("rp_backward_device", _lib.rp_backward_device(self._h, C.byref(cfg), arr, len(arr), C.c_void_p(dl_ptr),
+                                                              g.ctypes.data_as(C.POINTER(f32)), C.byref(st)))
         return g, {f: getattr(st, f) for f, _ in FrameStats._fields_}
 
     def trace_closest(self, rays):

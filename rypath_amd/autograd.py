@@ -2,8 +2,9 @@
 
 RenderXYZ.apply(values, scene, params, cfg) renders the raw XYZ film (H, W, 3)
 on the GPU with the parameter values written into the scene (rp_set_params +
-rp_render_frame_device); its backward calls rp_backward with the incoming
-dLoss/dXYZ image and returns dLoss/dvalues.
+rp_render_frame_device); its backward calls rp_backward_device with the
+incoming dLoss/dXYZ image (read in place on the device) and returns
+dLoss/dvalues.
 """
 import numpy as np
 import torch
@@ -24,8 +25,8 @@ class RenderXYZ(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_film):
-        dl = grad_film.detach().float().contiguous().cpu().numpy()
-        g, _ = ctx.scene.backward(ctx.cfg, ctx.params, dl)
+        dl = grad_film.detach().float().contiguous()
+        g, _ = ctx.scene.backward_device(ctx.cfg, ctx.params, dl.data_ptr())
         return torch.from_numpy(g).to(ctx.device), None, None, None
 
 

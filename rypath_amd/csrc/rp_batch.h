@@ -50,6 +50,34 @@ RP_DEV uint32_t find_scene(const BatchScene *scenes, uint32_t n, uint64_t g) {
     return lo;
 }
 
+// Runs f(scene) once per distinct scene among the wave's active lanes
+// (a waterfall over readfirstlane).  The scene index is wave-uniform inside
+// f and the table is read through the constant address space, so the scene's
+// fields load through the scalar unit into SGPRs exactly as a kernel
+// argument's do; a per-lane scene would hold every node, triangle and
+// material base address in VGPRs (190 instead of 160 in the shade kernel:
+// 2 waves per SIMD instead of 3).  The loop is wave-uniform (a lane never
+// leaves it early, so the compiler cannot move f past the exit where u is
+// per-lane again), and lane membership is taken from a ballot rather than
+// from `s == u`, which the compiler would use to substitute the per-lane s
+// for u inside f.  Refill appends paths in scene order, so a wave nearly
+// always holds a single scene and f runs once.
+typedef const __attribute__((address_space(4))) BatchScene ConstBatchScene;
+template <class Fn>
+RP_DEV void for_scene(const BatchScene *scenes, uint32_t s, Fn &&f) {
+    const uint32_t lane = __lane_id();
+    uint64_t pending = __ballot(true);
+    while (pending) {
+        const uint32_t u = __builtin_amdgcn_readlane(s, __builtin_ctzll(pending));
+        const uint64_t mine = __ballot(s == u);
+        if ((mine >> lane) & 1) {
+            ConstBatchScene *t = (ConstBatchScene *)scenes;
+            f((const BatchScene &)t[u]);
+        }
+        pending &= ~mine;
+    }
+}
+
 // ---------------------------------------------------- persistent megakernel
 template <int BS>
 __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scenes, uint32_t nscenes,
@@ -67,21 +95,22 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
         if (base >= total) break;
         uint64_t g = base + lane;
         if (g < total) {
-            uint32_t s = find_scene(scenes, nscenes, g);
-            const BatchScene &B = scenes[s];
-            uint64_t local = g - B.path_begin;
-            uint32_t pixel = uint32_t(local % B.npx);
-            uint64_t sample = B.sample_begin + local / B.npx;
-            PathResult r;
-            trace_path(B.S, B.P, st, pixel, sample, r);
-            float v[3] = {0, 0, 0};
-            if (r.status == RP_PATH_SUCCESS) sensor_xyz(B.S, r.wl, r.L, v);
-            else failed += 1;
-            B.xyz[local] = v[0];
-            B.xyz[B.paths + local] = v[1];
-            B.xyz[2 * B.paths + local] = v[2];
-            ext += r.ext_rays;
-            sh += r.shadow_rays;
+            // a wave's 64 consecutive ids span one scene, or two at a boundary
+            for_scene(scenes, find_scene(scenes, nscenes, g), [&](const BatchScene &B) {
+                const uint64_t local = g - B.path_begin;
+                const uint32_t pixel = uint32_t(local % B.npx);
+                const uint64_t sample = B.sample_begin + local / B.npx;
+                PathResult r;
+                trace_path(B.S, B.P, st, pixel, sample, r);
+                float v[3] = {0, 0, 0};
+                if (r.status == RP_PATH_SUCCESS) sensor_xyz(B.S, r.wl, r.L, v);
+                else failed += 1;
+                B.xyz[local] = v[0];
+                B.xyz[B.paths + local] = v[1];
+                B.xyz[2 * B.paths + local] = v[2];
+                ext += r.ext_rays;
+                sh += r.shadow_rays;
+            });
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -116,33 +145,6 @@ struct PoolView {
 #define RP_GRID_STRIDE(n)                                                                                     \
     for (uint32_t base_ = blockIdx.x * blockDim.x; base_ < (n); base_ += gridDim.x * blockDim.x)
 
-// Runs f(scene) once per distinct scene among the wave's active lanes
-// (a waterfall over readfirstlane).  The scene index is wave-uniform inside
-// f and the table is read through the constant address space, so the scene's
-// fields load through the scalar unit into SGPRs exactly as a kernel
-// argument's do; a per-lane scene would hold every node, triangle and
-// material base address in VGPRs (190 instead of 160 in the shade kernel:
-// 2 waves per SIMD instead of 3).  The loop is wave-uniform (a lane never
-// leaves it early, so the compiler cannot move f past the exit where u is
-// per-lane again), and lane membership is taken from a ballot rather than
-// from `s == u`, which the compiler would use to substitute the per-lane s
-// for u inside f.  Refill appends paths in scene order, so a wave nearly
-// always holds a single scene and f runs once.
-typedef const __attribute__((address_space(4))) BatchScene ConstBatchScene;
-template <class Fn>
-RP_DEV void for_scene(const BatchScene *scenes, uint32_t s, Fn &&f) {
-    const uint32_t lane = __lane_id();
-    uint64_t pending = __ballot(true);
-    while (pending) {
-        const uint32_t u = __builtin_amdgcn_readlane(s, __builtin_ctzll(pending));
-        const uint64_t mine = __ballot(s == u);
-        if ((mine >> lane) & 1) {
-            ConstBatchScene *t = (ConstBatchScene *)scenes;
-            f((const BatchScene &)t[u]);
-        }
-        pending &= ~mine;
-    }
-}
 
 __global__ void k_pool_init(PoolView V, uint64_t total) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {

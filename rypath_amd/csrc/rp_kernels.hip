@@ -1501,26 +1501,33 @@ int rp_set_params(rp_scene *s, const rp_param *params, uint32_t count, const flo
     }
 }
 
-int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params, uint32_t count,
-                const float *dl_dxyz, float *grad, rp_frame_stats *stats) {
+} // extern "C"
+
+namespace {
+// dl_on_device: dl_dxyz is device memory on the scene's device (read in
+// place, after a device synchronisation); else host memory (uploaded).
+int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
+                  uint32_t count, const float *dl_dxyz, bool dl_on_device, float *grad, rp_frame_stats *stats) {
     if (!s || !cfg || (count && !params) || !dl_dxyz || !grad) {
-        set_error("rp_backward: null argument");
+        set_error(std::string(name) + ": null argument");
         return RP_INVALID;
     }
     if (count > rpd::kMaxParams) {
-        set_error("rp_backward: at most 256 parameters");
+        set_error(std::string(name) + ": at most 256 parameters");
         return RP_INVALID;
     }
     try {
         std::lock_guard<std::mutex> lock(s->exec);
         HIP_CHECK(hipSetDevice(s->device));
+        if (dl_on_device) HIP_CHECK(hipDeviceSynchronize());
         if (cfg->spp == 0) throw std::runtime_error("spp must be positive");
         GradBuffers gb;
         rpd::GradTables G = grad_tables(s, params, count, gb);
         const rp_camera &cam = s->host.desc.camera;
         const uint32_t npx = cam.width * cam.height;
         DevBuf<float> dl, g, partial;
-        dl.upload(dl_dxyz, 3 * size_t(npx));
+        if (!dl_on_device) dl.upload(dl_dxyz, 3 * size_t(npx));
+        const float *dl_dev = dl_on_device ? dl_dxyz : dl.p;
         g.alloc(std::max<uint32_t>(count, 1));
         HIP_CHECK(hipMemsetAsync(g.p, 0, std::max<uint32_t>(count, 1) * sizeof(float), s->stream));
         // a fixed grid (block b always takes the same paths) with one row of
@@ -1576,11 +1583,11 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
             const uint2 *end = arena.fb.f.end;
             if (per_thread)
                 hipLaunchKernelGGL((rpd::k_adjoint<kAdjBlock, true>), dim3(rows), dim3(kAdjBlock), lds, s->stream,
-                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl.p, L_end, end,
+                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl_dev, L_end, end,
                                    partial.p);
             else
                 hipLaunchKernelGGL((rpd::k_adjoint<kAdjBlock, false>), dim3(rows), dim3(kAdjBlock), lds, s->stream,
-                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl.p, L_end, end,
+                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl_dev, L_end, end,
                                    partial.p);
             HIP_CHECK(hipGetLastError());
         }
@@ -1605,8 +1612,20 @@ int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params,
         }
         return RP_SUCCESS;
     } catch (const std::exception &e) {
-        return fail("rp_backward", e);
+        return fail(name, e);
     }
+}
+} // namespace
+
+extern "C" {
+
+int rp_backward(rp_scene *s, const rp_frame_config *cfg, const rp_param *params, uint32_t count,
+                const float *dl_dxyz, float *grad, rp_frame_stats *stats) {
+    return backward_impl("rp_backward", s, cfg, params, count, dl_dxyz, false, grad, stats);
+}
+int rp_backward_device(rp_scene *s, const rp_frame_config *cfg, const rp_param *params, uint32_t count,
+                       const float *dl_dxyz_device, float *grad, rp_frame_stats *stats) {
+    return backward_impl("rp_backward_device", s, cfg, params, count, dl_dxyz_device, true, grad, stats);
 }
 
 } // extern "C"

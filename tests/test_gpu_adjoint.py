@@ -154,6 +154,21 @@ def test_adjoint_vs_central_differences(rp, inverse, group, h):
     assert abs(fm - am) <= 4 * np.hypot(fse, ase) + 0.03 * abs(am)
 
 
+def test_backward_device_equals_host(rp, inverse):
+    """rp_backward_device reads dL/dXYZ in place from device memory and
+    returns the gradient rp_backward returns for the same image, bit for bit."""
+    import torch
+
+    synth, scene = inverse
+    params, _ = rp.inverse_params(synth)
+    dl = np.random.default_rng(12).normal(0, 1, (32, 32, 3)).astype(np.float32)
+    cfg = rp.frame_config(8, mode=rp.MODE_WAVEFRONT, seed=41)
+    g_host, _ = scene.backward(cfg, params, dl)
+    t = torch.from_numpy(dl).cuda()
+    g_dev, _ = scene.backward_device(cfg, params, t.data_ptr())
+    assert np.array_equal(g_host.view(np.uint32), g_dev.view(np.uint32))
+
+
 def test_autograd_harness(rp, inverse):
     import torch
 

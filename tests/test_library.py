@@ -101,3 +101,23 @@ def test_terrain_config_sizes(rp):
     assert d.triangle_count == 1023 * 1023 * 2 + 64 * 5120 + 32
     kinds = [d.materials[i].kind for i in range(d.material_count)]
     assert set(kinds) <= {0, 1, 2}
+
+
+def test_probe_and_device_backward_reject_invalid_arguments(rp):
+    """Argument checks that run before any device call."""
+    with pytest.raises(rp.RypathError) as e:
+        rp.probe_read_bandwidth(100, 1)  # below 4096 bytes
+    assert e.value.code == rp.INVALID
+    with pytest.raises(rp.RypathError) as e:
+        rp.probe_read_bandwidth(4096 + 8, 1)  # not a multiple of 16
+    assert e.value.code == rp.INVALID
+    with pytest.raises(rp.RypathError) as e:
+        rp.probe_read_bandwidth(1 << 20, 0)  # no pass
+    assert e.value.code == rp.INVALID
+    lib = rp.library()
+    g = (C.c_float * 4)()
+    cfg = rp.frame_config(1)
+    assert lib.rp_backward_device(None, C.byref(cfg), None, 0, C.c_void_p(1), g, None) == rp.INVALID
+    buf = C.create_string_buffer(256)
+    lib.rp_last_error(buf, len(buf))
+    assert b"rp_backward_device" in buf.value
