@@ -6,9 +6,11 @@
 //    consecutive global path ids from a device counter until the pool is empty;
 //  * wavefront: a fixed-capacity pool of compacted path state (two state sets,
 //    as the frame wavefront): refill -> extend -> (sort) -> shade -> shadow ->
-//    retire with device-side counts, grid-stride kernels and no host
-//    synchronisation inside the bounce loop (replayable as one HIP graph).
-//    New paths from any scene are appended behind the survivors each bounce.
+//    retire with device-side counts and no host synchronisation inside the
+//    bounce loop (replayable as one HIP graph).  New paths from any scene are
+//    appended behind the survivors each bounce; the heavy kernels run one
+//    scene per wave (scene record in SGPRs), with per-lane kernels for the
+//    rare waves that hold several scenes.
 #pragma once
 #include "rp_wavefront.h"
 
@@ -201,8 +203,8 @@ __global__ void k_pool_commit(PoolView V, uint32_t p, uint32_t bits) {
     if (bits) {
         uint32_t K = 1u << bits;
         for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) {
-            V.F.hist[k] = 0;
-            V.F.hist[256 + k] = 0;
+            V.F.hist[k * kQueueStride] = 0;
+            V.F.hist[(256 + k) * kQueueStride] = 0;
         }
     }
 }
@@ -241,7 +243,7 @@ __global__ void k_pool_sort_hist(PoolView V, uint32_t p, uint32_t bits) {
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
-        if (h[k]) atomicAdd(&V.F.hist[k], h[k]);
+        if (h[k]) atomicAdd(&V.F.hist[k * kQueueStride], h[k]);
 }
 // Each tile of blockDim entries is ranked in LDS.
 __global__ void k_pool_sort_scatter(PoolView V, uint32_t p, uint32_t bits) {
@@ -256,7 +258,8 @@ __global__ void k_pool_sort_scatter(PoolView V, uint32_t p, uint32_t bits) {
             rank = atomicAdd(&h[key], 1u);
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) base[k] = h[k] ? atomicAdd(&V.F.hist[256 + k], h[k]) : 0;
+        for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+            base[k] = h[k] ? atomicAdd(&V.F.hist[(256 + k) * kQueueStride], h[k]) : 0;
         __syncthreads();
         if (i < n) V.F.perm[base[key] + rank] = i;
         __syncthreads();

@@ -1,8 +1,8 @@
 // Wavefront form of the path tracer: generate -> extend -> (sort) -> shade ->
 // shadow -> accumulate, with path state in HBM (structure of arrays, 16-byte
 // fields) compacted every bounce: two state sets, survivors appended at
-// wave-ballot positions (the frame path here and the batch pool in
-// rp_batch.h).  The per-event arithmetic is rp_path.h's, so a wavefront frame
+// block-aggregated ballot positions (the frame path here and the batch pool
+// in rp_batch.h).  The per-event arithmetic is rp_path.h's, so a wavefront frame
 // equals the megakernel frame bit for bit.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -76,7 +76,8 @@ RP_DEV void store_state(const WfState &B, uint32_t i, const PathState &ps) {
     B.meta2[i] = make_uint4(ps.medium, ps.mdraw, ps.flags, ps.status);
 }
 
-// Wave-aggregated queue append: one atomic per wave.
+// Wave-aggregated queue append: one atomic per wave that has a lane to
+// append (for rare appends; the hot queues use block_append below).
 RP_DEV uint32_t wave_append(bool pred, uint32_t *counter) {
     uint64_t mask = __ballot(pred);
     if (mask == 0) return 0;
@@ -172,7 +173,7 @@ struct FrameWf {
     uint2 *end;     // per lane: (status, wavelength word)
     uint32_t *perm; // material-sorted order of [0, qn)
     uint32_t *qcount; // [0] survivors, [kQueueStride] shadow rays (separate lines)
-    uint32_t *hist;   // [256] histogram, [256] cursors
+    uint32_t *hist;   // 256 bucket counts, then 256 cursors; entry k at hist[k * kQueueStride]
 };
 constexpr uint32_t kEnded = 0x80000000u;
 
@@ -239,14 +240,14 @@ __global__ void __launch_bounds__(256) k_sort_hist(Scene S, const uint2 *hit_id,
     if (i < qn) atomicAdd(&h[material_key(S, hit_id, q ? q[i] : i, bits)], 1u);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
-        if (h[k]) atomicAdd(&hist[k], h[k]);
+        if (h[k]) atomicAdd(&hist[k * kQueueStride], h[k]);
 }
 __global__ void k_sort_scan(uint32_t bits, uint32_t *hist) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     uint32_t K = 1u << bits, acc = 0;
     for (uint32_t k = 0; k < K; ++k) {
-        uint32_t c = hist[k];
-        hist[256 + k] = acc; // cursor
+        uint32_t c = hist[k * kQueueStride];
+        hist[(256 + k) * kQueueStride] = acc; // cursor
         acc += c;
     }
 }
@@ -264,7 +265,8 @@ __global__ void __launch_bounds__(256) k_sort_scatter(Scene S, const uint2 *hit_
         rank = atomicAdd(&h[key], 1u);
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) base[k] = h[k] ? atomicAdd(&hist[256 + k], h[k]) : 0;
+    for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+        base[k] = h[k] ? atomicAdd(&hist[(256 + k) * kQueueStride], h[k]) : 0;
     __syncthreads();
     if (i < qn) out[base[key] + rank] = slot;
 }
@@ -272,8 +274,10 @@ __global__ void k_wf_reset(uint32_t *qcount, uint32_t *hist, uint32_t bits) {
     uint32_t t = threadIdx.x;
     if (t < 2) qcount[t * kQueueStride] = 0;
     uint32_t K = 1u << bits;
-    for (uint32_t k = t; k < 512; k += blockDim.x)
-        if (k < K || (k >= 256 && k < 256 + K)) hist[k] = 0;
+    for (uint32_t k = t; k < K; k += blockDim.x) {
+        hist[k * kQueueStride] = 0;
+        hist[(256 + k) * kQueueStride] = 0;
+    }
 }
 
 // Minimum waves per SIMD requested for the shading kernel (register budget
@@ -451,7 +455,7 @@ inline void FrameBuffers::ensure(uint64_t n) {
     wf_alloc(f.sh_o, n), wf_alloc(f.sh_d, n), wf_alloc(f.sh_val, n), wf_alloc(f.sh_meta, n);
     wf_alloc(f.L_end, n), wf_alloc(f.end, n), wf_alloc(f.perm, n);
     wf_alloc(f.qcount, 2 * kQueueStride);
-    wf_alloc(f.hist, 512);
+    wf_alloc(f.hist, 512 * kQueueStride); // every bucket counter on its own line
     WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 2 * kQueueStride * sizeof(uint32_t),
                            hipHostMallocDefault));
     capacity = n;
