@@ -11,11 +11,15 @@
 // Parameters: binned diffuse reflectance tables, GGX conductor roughness
 // (isotropic, PBRT remapping alpha = sqrt(r)) and binned medium sigma_s.
 #pragma once
-#include "rp_path.h"
+#include "rp_wavefront.h"
 
 namespace rpd {
 
 constexpr uint32_t kMaxParams = 256;
+// Parameters up to which every thread of the shading step keeps its own LDS
+// column of sums (np * 128 * 4 B of dynamic LDS); above, a shared row with
+// LDS atomics.
+constexpr uint32_t kPerThreadParams = 96;
 
 struct GradTables {
     const int32_t *refl_base;   // per material: first parameter index, -1 = none
@@ -141,110 +145,8 @@ struct AdjointHook {
     }
 };
 
-// Replay of one path whose primal radiance is L (status `status`); its
-// gradient terms go to hk's accumulator.
-RP_DEV void adjoint_path(const Scene &S, const PathParams &P, const GradTables &G, Stack &st, uint32_t pixel,
-                         uint64_t sample, uint32_t spp, const float *dl_dxyz, float4 L, uint32_t status,
-                         AdjointHook &hk) {
-    if (status != RP_PATH_SUCCESS) return;
-    PathState ps;
-    uint32_t w0, w1;
-    path_begin(S, P, pixel, sample, ps, w0, w1);
-    const Wavelengths wl = path_wavelengths(S, ps.w2);
-    const float dx = dl_dxyz[3 * size_t(pixel)], dy = dl_dxyz[3 * size_t(pixel) + 1],
-                dz = dl_dxyz[3 * size_t(pixel) + 2];
-    const float Lp[4] = {L.x, L.y, L.z, L.w};
-    for (int k = 0; k < 4; ++k) {
-        hk.a[k] = 0.0f;
-        if (wl.pdf[k] > 0) {
-            int i = int(floorf(wl.lambda[k] + 0.5f)) - 360;
-            i = i < 0 ? 0 : (i > 470 ? 470 : i);
-            float4 c = S.cie[i];
-            hk.a[k] = ((dx * c.x + dy * c.y + dz * c.z) / wl.pdf[k]) / (4.0f * float(spp));
-        }
-        hk.L[k] = Lp[k];
-        hk.S[k] = 0.0f;
-    }
-    while (path_alive(S, ps)) {
-        Hit h;
-        traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
-        hk.reset_pending();
-        ShadowReq sr;
-        bool cont = path_event(S, P, ps, h, sr, hk);
-        if (sr.active) {
-            float tr[4], mlen = 0.0f;
-            bool vis = transmittance(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary, &mlen,
-                                     G.sigma_medium);
-            if (vis) {
-                float c[4];
-                for (int k = 0; k < 4; ++k) c[k] = sr.val[k] * tr[k];
-                if (!sr.delta_or_nomis)
-                    for (int k = 0; k < 4; ++k) c[k] = c[k] * sr.w;
-                hk.add(c);
-                for (int k = 0; k < 4; ++k) {
-                    hk.accumulate(hk.nee_idx[k], hk.a[k] * c[k] * hk.nee_dlog[k]);
-                    if (G.sigma_medium != RP_NONE && mlen > 0) {
-                        int32_t idx = G.sigma_base[G.sigma_medium] +
-                                      int32_t(lambda_bin(wl.lambda[k], G.sigma_bins[G.sigma_medium]));
-                        hk.accumulate(idx, hk.a[k] * c[k] * (-mlen));
-                    }
-                }
-            }
-        }
-        for (int k = 0; k < 4; ++k) hk.accumulate(hk.cont_idx[k], hk.a[k] * (hk.L[k] - hk.S[k]) * hk.cont_dlog[k]);
-        if (!cont) break;
-    }
-}
-
-// Parameters up to which every thread keeps its own LDS column of sums
-// (kPerThread): nparams * block * 4 B of dynamic LDS.
-constexpr uint32_t kPerThreadParams = 96;
-
-// Replay for paths [0, n) (path i: pixel i % npx, sample sample_begin +
-// i / npx; primal radiance L_end[i], status end[i].x), grid-stride over a
-// fixed grid so that block b always takes the same paths in the same order.
-// Block b adds its sums to row b of `partial` ([gridDim.x][nparams]);
-// k_grad_reduce folds the rows.  No global atomics, and with kPerThread no
-// atomics at all: the gradient is the same bit for bit on every run.
-template <int BS, bool kPerThread>
-__global__ void __launch_bounds__(BS) k_adjoint(Scene S, PathParams P, GradTables G, uint32_t npx, uint64_t sample_begin,
-                                                 uint32_t spp, uint64_t n, const float *dl_dxyz, const float4 *L_end,
-                                                 const uint2 *end, float *partial) {
-    __shared__ uint32_t s_node[kShortStack * BS];
-    __shared__ float s_t[kShortStack * BS];
-    extern __shared__ float sgrad[]; // kPerThread: [nparams][BS]; else [nparams]
-    const uint32_t np = G.nparams;
-    if (kPerThread) {
-        for (uint32_t p = 0; p < np; ++p) sgrad[p * BS + threadIdx.x] = 0.0f;
-    } else {
-        for (uint32_t p = threadIdx.x; p < np; p += BS) sgrad[p] = 0.0f;
-        __syncthreads();
-    }
-    Stack st;
-    st.init(s_node, s_t, BS);
-    AdjointHook hk;
-    hk.G = &G;
-    hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;
-    hk.stride = kPerThread ? BS : 0u;
-    for (uint64_t lane = uint64_t(blockIdx.x) * BS + threadIdx.x; lane < n; lane += uint64_t(gridDim.x) * BS)
-        adjoint_path(S, P, G, st, uint32_t(lane % npx), sample_begin + lane / npx, spp, dl_dxyz, L_end[lane],
-                     end[lane].x, hk);
-    __syncthreads();
-    if (kPerThread) {
-        // fold the columns: a fixed tree over the block's threads
-        for (uint32_t h = BS / 2; h > 0; h >>= 1) {
-            if (threadIdx.x < h)
-                for (uint32_t p = 0; p < np; ++p) sgrad[p * BS + threadIdx.x] += sgrad[p * BS + threadIdx.x + h];
-            __syncthreads();
-        }
-        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] += sgrad[p * BS];
-    } else {
-        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] += sgrad[p];
-    }
-}
-
-// grad[p] = sum over the rows of partial[rows][np], one block per parameter,
-// a fixed summation order.
+// grad[p] += sum over the rows of partial[rows][np], one block per
+// parameter, a fixed summation order.
 __global__ void __launch_bounds__(256) k_grad_reduce(const float *partial, uint32_t rows, uint32_t np, float *grad) {
     __shared__ float s[256];
     const uint32_t p = blockIdx.x;
@@ -256,7 +158,306 @@ __global__ void __launch_bounds__(256) k_grad_reduce(const float *partial, uint3
         if (threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
         __syncthreads();
     }
-    if (threadIdx.x == 0) grad[p] = s[0];
+    if (threadIdx.x == 0) grad[p] += s[0];
+}
+
+// ------------------------------------------------------ wavefront replay
+// The replay runs as the frame wavefront does (generate -> extend -> shade
+// -> shadow per bounce, compacted state), with the hook in the shading and
+// shadow kernels:
+//  * the hook's running radiance S is the path state's own L (path_event
+//    adds every contribution to both, the shadow kernel adds NEE to L);
+//  * a vertex's continuation term a*(L - S)*dlog needs S after that vertex's
+//    NEE, so it rides with the survivor (cidx/cdlog, moved with compaction)
+//    and is applied at the start of the next shading step;
+//  * a path that ends has nothing after its last NEE: L - S = 0 there, so
+//    its pending continuation contributes nothing;
+//  * NEE and shadow-medium (sigma_s) terms are added in the shadow kernel
+//    from the weights the shading step sent with the shadow ray.
+// Paths whose primal status is not success contribute nothing, as in the
+// primal film.
+struct AdjWf {
+    int4 *cidx[2];    // pending continuation parameter per lane k, per state set position
+    float4 *cdlog[2]; // its d log / d theta
+    float4 *sh_a;     // per shadow ray: dLoss/dL_k of its path
+    int4 *sh_idx;     // NEE factor parameter per lane k
+    float4 *sh_dlog;  // NEE factor d log / d theta
+    const float4 *Lp; // primal radiance per lane
+    const uint32_t *pstatus; // primal status per lane
+    const float *dl;  // dLoss/dXYZ per pixel
+    uint32_t spp;     // samples per pixel of the whole frame (a_k normalisation)
+};
+
+// a_k = dLoss/dL_k of a sample of `pixel` with wavelengths wl.
+RP_DEV void adjoint_weights(const Scene &S, const AdjWf &A, uint32_t pixel, const Wavelengths &wl, float a[4]) {
+    const float dx = A.dl[3 * size_t(pixel)], dy = A.dl[3 * size_t(pixel) + 1], dz = A.dl[3 * size_t(pixel) + 2];
+    for (int k = 0; k < 4; ++k) {
+        a[k] = 0.0f;
+        if (wl.pdf[k] > 0) {
+            int i = int(floorf(wl.lambda[k] + 0.5f)) - 360;
+            i = i < 0 ? 0 : (i > 470 ? 470 : i);
+            const float4 c = S.cie[i];
+            a[k] = ((dx * c.x + dy * c.y + dz * c.z) / wl.pdf[k]) / (4.0f * float(A.spp));
+        }
+    }
+}
+
+// Folds the block's LDS accumulator into row blockIdx.x of `partial`: with
+// kPerThread the [np][BS] columns by a fixed tree, else the shared row.
+template <int BS, bool kPerThread>
+RP_DEV void fold_block_grad(float *sgrad, uint32_t np, float *partial) {
+    __syncthreads();
+    if (kPerThread) {
+        for (uint32_t h = BS / 2; h > 0; h >>= 1) {
+            if (threadIdx.x < h)
+                for (uint32_t p = 0; p < np; ++p) sgrad[p * BS + threadIdx.x] += sgrad[p * BS + threadIdx.x + h];
+            __syncthreads();
+        }
+        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] = sgrad[p * BS];
+    } else {
+        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] = sgrad[p];
+    }
+}
+
+// One replay event for each path of set C[0, qn) (the frame wavefront's
+// shading step plus the hook).  Dynamic LDS: np * BS floats (kPerThread) or
+// np floats.
+template <bool kPerThread>
+__global__ void __launch_bounds__(kWfBlock) k_adj_shade(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
+                                                         WfState N, FrameWf F, uint32_t p, uint32_t qn, uint32_t npx,
+                                                         uint64_t sample_begin, float *partial) {
+    __shared__ AppendLds<kWfBlock, 2> sh;
+    extern __shared__ float sgrad[];
+    const uint32_t np = G.nparams;
+    if (kPerThread) {
+        for (uint32_t q = 0; q < np; ++q) sgrad[q * kWfBlock + threadIdx.x] = 0.0f;
+    } else {
+        for (uint32_t q = threadIdx.x; q < np; q += kWfBlock) sgrad[q] = 0.0f;
+        __syncthreads();
+    }
+    AdjointHook hk;
+    hk.G = &G;
+    hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;
+    hk.stride = kPerThread ? kWfBlock : 0u;
+    const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    bool cont = false, shadow = false;
+    PathState ps;
+    ShadowReq sr;
+    sr.active = false;
+    if (i < qn) {
+        load_state(C, i, ps);
+        const uint32_t lane = uint32_t(ps.sample - sample_begin) * npx + ps.pixel;
+        if (A.pstatus[lane] == RP_PATH_SUCCESS) {
+            const float4 ht = F.hit_t[i];
+            const uint2 hi = F.hit_id[i];
+            Hit h;
+            h.prim = hi.x;
+            h.id = hi.y;
+            h.t = ht.x;
+            h.b0 = ht.y;
+            h.b1 = ht.z;
+            h.b2 = ht.w;
+            const Wavelengths wl = path_wavelengths(S, ps.w2);
+            adjoint_weights(S, A, ps.pixel, wl, hk.a);
+            const float4 lp = A.Lp[lane];
+            hk.L[0] = lp.x, hk.L[1] = lp.y, hk.L[2] = lp.z, hk.L[3] = lp.w;
+            for (int k = 0; k < 4; ++k) hk.S[k] = ps.L[k];
+            // the previous vertex's continuation, now that its NEE is in L
+            const int4 ci = A.cidx[p][i];
+            const float4 cd = A.cdlog[p][i];
+            const int32_t cik[4] = {ci.x, ci.y, ci.z, ci.w};
+            const float cdk[4] = {cd.x, cd.y, cd.z, cd.w};
+            for (int k = 0; k < 4; ++k) hk.accumulate(cik[k], hk.a[k] * (hk.L[k] - hk.S[k]) * cdk[k]);
+            hk.reset_pending();
+            cont = path_event(S, P, ps, h, sr, hk);
+            shadow = sr.active;
+            cont = cont && path_alive(S, ps);
+        }
+    }
+    const bool pred[2] = {cont, shadow};
+    uint32_t *const queue[2] = {&F.qcount[0], &F.qcount[kQueueStride]};
+    uint32_t at[2];
+    block_append(sh, pred, queue, at);
+    const uint32_t pos = at[0], spos = at[1];
+    if (cont) {
+        store_state(N, pos, ps); // with its identity: compaction moves the path
+        A.cidx[1 - p][pos] = make_int4(hk.cont_idx[0], hk.cont_idx[1], hk.cont_idx[2], hk.cont_idx[3]);
+        A.cdlog[1 - p][pos] = make_float4(hk.cont_dlog[0], hk.cont_dlog[1], hk.cont_dlog[2], hk.cont_dlog[3]);
+    }
+    if (shadow) {
+        F.sh_o[spos] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
+        F.sh_d[spos] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.w);
+        F.sh_val[spos] = make_float4(sr.val[0], sr.val[1], sr.val[2], sr.val[3]);
+        F.sh_meta[spos] = make_uint4(sr.medium, sr.delta_or_nomis ? 1u : 0u, ps.w2, cont ? pos : kEnded);
+        A.sh_a[spos] = make_float4(hk.a[0], hk.a[1], hk.a[2], hk.a[3]);
+        A.sh_idx[spos] = make_int4(hk.nee_idx[0], hk.nee_idx[1], hk.nee_idx[2], hk.nee_idx[3]);
+        A.sh_dlog[spos] = make_float4(hk.nee_dlog[0], hk.nee_dlog[1], hk.nee_dlog[2], hk.nee_dlog[3]);
+    }
+    fold_block_grad<kWfBlock, kPerThread>(sgrad, np, partial);
+}
+
+// Shadow rays of one replay bounce: the visible NEE contribution c goes to
+// the survivor's L (as in the frame wavefront) and the NEE-factor and
+// shadow-medium terms a_k * c_k * d log / d theta to the block's LDS row
+// (shared, LDS atomics: at most eight terms per ray).
+template <bool kInterfaces>
+__global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, GradTables G, AdjWf A, WfState N,
+                                                          FrameWf F, uint32_t sn, float *partial) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    __shared__ float srow[kMaxParams];
+    const uint32_t np = G.nparams;
+    for (uint32_t q = threadIdx.x; q < np; q += kWfBlock) srow[q] = 0.0f;
+    __syncthreads();
+    Stack st;
+    st.init(s_node, s_t, kWfBlock);
+    AdjointHook hk;
+    hk.G = &G;
+    hk.grad = srow;
+    hk.stride = 0;
+    const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    if (i < sn) {
+        const float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
+        const uint4 meta = F.sh_meta[i];
+        ShadowReq sr;
+        sr.o = {o.x, o.y, o.z};
+        sr.tmax = o.w;
+        sr.d = {d.x, d.y, d.z};
+        sr.w = d.w;
+        sr.val[0] = v.x, sr.val[1] = v.y, sr.val[2] = v.z, sr.val[3] = v.w;
+        sr.medium = meta.x;
+        sr.delta_or_nomis = meta.y != 0;
+        const Wavelengths wl = path_wavelengths(S, meta.z);
+        float tr[4], mlen = 0.0f;
+        const bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary, &mlen,
+                                                    G.sigma_medium);
+        if (vis) {
+            float c[4];
+            for (int k = 0; k < 4; ++k) c[k] = sr.val[k] * tr[k];
+            if (!sr.delta_or_nomis)
+                for (int k = 0; k < 4; ++k) c[k] = c[k] * sr.w;
+            if (!(meta.w & kEnded)) {
+                const float4 L = N.L[meta.w];
+                N.L[meta.w] = make_float4(L.x + c[0], L.y + c[1], L.z + c[2], L.w + c[3]);
+            }
+            const float4 a = A.sh_a[i], dg = A.sh_dlog[i];
+            const int4 ix = A.sh_idx[i];
+            const float ak[4] = {a.x, a.y, a.z, a.w}, dk[4] = {dg.x, dg.y, dg.z, dg.w};
+            const int32_t ik[4] = {ix.x, ix.y, ix.z, ix.w};
+            for (int k = 0; k < 4; ++k) {
+                hk.accumulate(ik[k], ak[k] * c[k] * dk[k]);
+                if (G.sigma_medium != RP_NONE && mlen > 0) {
+                    const int32_t idx =
+                        G.sigma_base[G.sigma_medium] + int32_t(lambda_bin(wl.lambda[k], G.sigma_bins[G.sigma_medium]));
+                    hk.accumulate(idx, ak[k] * c[k] * (-mlen));
+                }
+            }
+        }
+    }
+    fold_block_grad<kWfBlock, false>(srow, np, partial);
+}
+
+
+// Scratch of the wavefront replay (one per device, beside the frame arena).
+struct AdjBuffers {
+    int4 *cidx[2] = {nullptr, nullptr};
+    float4 *cdlog[2] = {nullptr, nullptr};
+    float4 *sh_a = nullptr, *sh_dlog = nullptr, *Lp = nullptr;
+    int4 *sh_idx = nullptr;
+    uint32_t *pstatus = nullptr;
+    float *partial = nullptr;
+    uint64_t capacity = 0, partial_floats = 0;
+    AdjBuffers() = default;
+    AdjBuffers(const AdjBuffers &) = delete;
+    AdjBuffers &operator=(const AdjBuffers &) = delete;
+    ~AdjBuffers() { release(); }
+    void release() {
+        void *ptrs[] = {cidx[0], cidx[1], cdlog[0], cdlog[1], sh_a, sh_dlog, Lp, sh_idx, pstatus, partial};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+        cidx[0] = cidx[1] = nullptr;
+        cdlog[0] = cdlog[1] = nullptr;
+        sh_a = sh_dlog = Lp = nullptr;
+        sh_idx = nullptr;
+        pstatus = nullptr;
+        partial = nullptr;
+        capacity = partial_floats = 0;
+    }
+    void ensure(uint64_t n, uint32_t np) {
+        const uint64_t pf = ((n + kWfBlock - 1) / kWfBlock) * std::max<uint32_t>(np, 1);
+        if (n <= capacity && pf <= partial_floats) return;
+        release();
+        for (int s = 0; s < 2; ++s) wf_alloc(cidx[s], n), wf_alloc(cdlog[s], n);
+        wf_alloc(sh_a, n), wf_alloc(sh_dlog, n), wf_alloc(sh_idx, n), wf_alloc(Lp, n), wf_alloc(pstatus, n);
+        wf_alloc(partial, pf);
+        capacity = n;
+        partial_floats = pf;
+    }
+};
+
+__global__ void __launch_bounds__(256) k_adj_prepare(const float4 *L_end, const uint2 *end, uint64_t n, float4 *Lp,
+                                                     uint32_t *pstatus, int4 *cidx) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Lp[i] = L_end[i];
+    pstatus[i] = end[i].x;
+    cidx[i] = make_int4(-1, -1, -1, -1);
+}
+
+// Path replay of paths [0, n) (path i: pixel i % npx, sample sample_begin +
+// i / npx) whose primal radiance and status are in FB's per-lane arrays;
+// adds dLoss/dtheta into grad (device, np floats).
+inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables &G, FrameBuffers &FB,
+                           AdjBuffers &AB, uint32_t npx, uint64_t sample_begin, uint64_t n, uint32_t spp,
+                           const float *dl, float *grad, hipStream_t stream) {
+    if (n > (uint64_t(1) << 31)) throw std::runtime_error("adjoint chunk too large");
+    const uint32_t np = G.nparams;
+    FB.ensure(n);
+    AB.ensure(n, np);
+    const FrameWf &F = FB.f;
+    hipLaunchKernelGGL(k_adj_prepare, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, F.L_end, F.end, n, AB.Lp,
+                       AB.pstatus, AB.cidx[0]);
+    int cur = 0;
+    hipLaunchKernelGGL(k_wf_generate, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n,
+                       FB.state[cur]);
+    AdjWf A;
+    A.cidx[0] = AB.cidx[0], A.cidx[1] = AB.cidx[1];
+    A.cdlog[0] = AB.cdlog[0], A.cdlog[1] = AB.cdlog[1];
+    A.sh_a = AB.sh_a, A.sh_idx = AB.sh_idx, A.sh_dlog = AB.sh_dlog;
+    A.Lp = AB.Lp, A.pstatus = AB.pstatus, A.dl = dl, A.spp = spp;
+    const bool per_thread = np <= kPerThreadParams;
+    const size_t lds = per_thread ? size_t(np) * kWfBlock * sizeof(float) : size_t(np) * sizeof(float);
+    uint32_t qn = uint32_t(n);
+    while (qn > 0) {
+        const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
+        const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
+        hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, nullptr);
+        hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, 0u);
+        if (per_thread)
+            hipLaunchKernelGGL(k_adj_shade<true>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F, uint32_t(cur),
+                               qn, npx, sample_begin, AB.partial);
+        else
+            hipLaunchKernelGGL(k_adj_shade<false>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F,
+                               uint32_t(cur), qn, npx, sample_begin, AB.partial);
+        if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, grid.x, np, grad);
+        WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, (kQueueStride + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                stream));
+        WF_CHECK(hipStreamSynchronize(stream));
+        const uint32_t next = FB.host_count[0], sn = FB.host_count[kQueueStride];
+        if (next > qn || sn > qn) throw std::runtime_error("adjoint queue counts out of range");
+        if (sn) {
+            const dim3 sg((sn + kWfBlock - 1) / kWfBlock);
+            if (S.has_interfaces)
+                hipLaunchKernelGGL(k_adj_shadow<true>, sg, dim3(kWfBlock), 0, stream, S, P, G, A, N, F, sn, AB.partial);
+            else
+                hipLaunchKernelGGL(k_adj_shadow<false>, sg, dim3(kWfBlock), 0, stream, S, P, G, A, N, F, sn,
+                                   AB.partial);
+            if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, sg.x, np, grad);
+        }
+        cur = 1 - cur;
+        qn = next;
+    }
+    WF_CHECK(hipGetLastError());
 }
 
 } // namespace rpd

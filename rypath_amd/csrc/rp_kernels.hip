@@ -421,6 +421,7 @@ struct FrameArena {
     std::mutex m;
     rpd::FrameBuffers fb;
     PrimalTag primal;
+    rpd::AdjBuffers adj; // path-replay scratch
 };
 FrameArena &frame_arena(int device) {
     static std::mutex guard;
@@ -1525,24 +1526,11 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
         rpd::GradTables G = grad_tables(s, params, count, gb);
         const rp_camera &cam = s->host.desc.camera;
         const uint32_t npx = cam.width * cam.height;
-        DevBuf<float> dl, g, partial;
+        DevBuf<float> dl, g;
         if (!dl_on_device) dl.upload(dl_dxyz, 3 * size_t(npx));
         const float *dl_dev = dl_on_device ? dl_dxyz : dl.p;
         g.alloc(std::max<uint32_t>(count, 1));
         HIP_CHECK(hipMemsetAsync(g.p, 0, std::max<uint32_t>(count, 1) * sizeof(float), s->stream));
-        // a fixed grid (block b always takes the same paths) with one row of
-        // partial sums per block, folded by k_grad_reduce.  One path per
-        // thread up to 65,536 blocks: with fewer, longer-running blocks the
-        // slowest paths of each block set the pace (2,048 blocks of 4 paths
-        // per thread measured 7 % slower on the 256x256 16 spp adjoint).
-        constexpr int kAdjBlock = 128;
-        const uint64_t total_paths = uint64_t(npx) * cfg->spp;
-        const uint32_t rows = uint32_t(
-            std::max<uint64_t>(1, std::min<uint64_t>(65536, (total_paths + kAdjBlock - 1) / kAdjBlock)));
-        partial.alloc(size_t(rows) * std::max<uint32_t>(count, 1));
-        HIP_CHECK(hipMemsetAsync(partial.p, 0, size_t(rows) * std::max<uint32_t>(count, 1) * sizeof(float), s->stream));
-        const bool per_thread = count <= rpd::kPerThreadParams;
-        const size_t lds = per_thread ? size_t(count) * kAdjBlock * sizeof(float) : size_t(count) * sizeof(float);
         rpd::PathParams P = path_params(cfg->seed, cfg->randomization, 0, 0, cfg->flags);
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, s->counters.p, 8u);
         hipEvent_t e0, e1;
@@ -1579,19 +1567,9 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
                                             s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays);
                 if (one_chunk) arena.primal.set(s, *cfg, npx, wf_rays);
             }
-            const float4 *L_end = arena.fb.f.L_end;
-            const uint2 *end = arena.fb.f.end;
-            if (per_thread)
-                hipLaunchKernelGGL((rpd::k_adjoint<kAdjBlock, true>), dim3(rows), dim3(kAdjBlock), lds, s->stream,
-                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl_dev, L_end, end,
-                                   partial.p);
-            else
-                hipLaunchKernelGGL((rpd::k_adjoint<kAdjBlock, false>), dim3(rows), dim3(kAdjBlock), lds, s->stream,
-                                   s->dev, P, G, npx, cfg->sample_begin + s0, cfg->spp, n, dl_dev, L_end, end,
-                                   partial.p);
-            HIP_CHECK(hipGetLastError());
+            rpd::adjoint_replay(s->dev, P, G, arena.fb, arena.adj, npx, cfg->sample_begin + s0, n, cfg->spp, dl_dev, g.p,
+                                s->stream);
         }
-        if (count) hipLaunchKernelGGL(rpd::k_grad_reduce, dim3(count), dim3(256), 0, s->stream, partial.p, rows, count, g.p);
         HIP_CHECK(hipEventRecord(e1, s->stream));
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;

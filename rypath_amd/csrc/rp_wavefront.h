@@ -58,8 +58,9 @@ RP_DEV void load_state(const WfState &B, uint32_t i, PathState &ps) {
     ps.flags = u.z;
     ps.status = u.w;
 }
-// kIds: also write the identity word (pixel, sample, wavelength word), which
-// never changes after generation; the shading kernels skip it.
+// kIds: also write the identity word (pixel, sample, wavelength word).  It
+// never changes after generation, so only an in-place update may skip it;
+// a path that moves (compaction) needs it at its new position.
 template <bool kIds = true>
 RP_DEV void store_state(const WfState &B, uint32_t i, const PathState &ps) {
     B.o_t[i] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.tmax);
