@@ -339,7 +339,7 @@ typedef struct rp_param {
 int rp_set_params(rp_scene *scene, const rp_param *params, uint32_t count, const float *values);
 /* Path-replay adjoint: dL/dparams for the frame defined by `config`, given
  * dL/dXYZ per pixel (W*H*3 floats, host).  grad receives `count` floats,
- * the same bit for bit on every run.  The primal (each path's radiance) comes
+ * reproducible to float summation order.  The primal (each path's radiance) comes
  * from a frame-wavefront pass over the same paths; when the last wavefront
  * frame on the device was this scene with the same parameters, seed,
  * randomization, sample range and roulette setting in one chunk, its primal

@@ -222,8 +222,21 @@ RP_DEV void fold_block_grad(float *sgrad, uint32_t np, float *partial) {
 // One replay event for each path of set C[0, qn) (the frame wavefront's
 // shading step plus the hook).  Dynamic LDS: np * BS floats (kPerThread) or
 // np floats.
+// Waves per SIMD requested from the compiler for k_adj_shade (register
+// budget 512 / waves); 0 leaves the choice to the compiler.  Left alone it
+// takes 169 VGPRs (2 waves/SIMD); 3 fits it in 168 without spills and
+// measured 5.24 against 5.43 ms for the 256x256, 16 spp replay.
+#ifndef RP_ADJ_SHADE_WAVES
+#define RP_ADJ_SHADE_WAVES 3
+#endif
+#if RP_ADJ_SHADE_WAVES > 0
+#define RP_ADJ_SHADE_ATTR __attribute__((amdgpu_waves_per_eu(RP_ADJ_SHADE_WAVES)))
+#else
+#define RP_ADJ_SHADE_ATTR
+#endif
+
 template <bool kPerThread>
-__global__ void __launch_bounds__(kWfBlock) k_adj_shade(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
+__global__ void __launch_bounds__(kWfBlock) RP_ADJ_SHADE_ATTR k_adj_shade(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
                                                          WfState N, FrameWf F, uint32_t p, uint32_t qn, uint32_t npx,
                                                          uint64_t sample_begin, float *partial) {
     __shared__ AppendLds<kWfBlock, 2> sh;
