@@ -439,7 +439,12 @@ def main():
         cfg = rp.frame_config(args.spp, mode=modes[frame_mode], seed=1, flags=rp.FRAME_COUNT_TRAFFIC)
         cnt = scene.render_frame_device(cfg, film.data_ptr())
         st = results[frame_mode]["stats"]
-        alg = RAY_BYTES * cnt["extension_rays"] + NODE_BYTES * cnt["node_fetches"] + TRI_BYTES * cnt["triangle_tests"]
+        # the instrumented replay traces every bounce in k_wf_extend; the timed
+        # frame's closest-hit kernels traced extend_rays of its rays (the rest
+        # ran in the latency tail), at the replay's bytes per ray
+        per_ray = (RAY_BYTES * cnt["extension_rays"] + NODE_BYTES * cnt["node_fetches"] +
+                   TRI_BYTES * cnt["triangle_tests"]) / cnt["extension_rays"]
+        alg = per_ray * st["extend_rays"]
         achieved = alg / st["trace_seconds"] / 1e9
         traffic = pmc_traffic() if (args.workload == "terrain" and (args.width, args.height, args.spp) ==
                                     (1920, 1080, 16) and frame_mode == "wave") else None
@@ -458,7 +463,8 @@ def main():
                     "traffic_source": PMC_SUMMARY.relative_to(ROOT).as_posix() if traffic else None,
                     "algorithmic_bytes_per_launch": round(alg / max(st["extend_launches"], 1)),
                     "kernel": f"k_wf_extend (closest hit, {frame_mode} frame)",
-                    "per_ray_bytes": round(alg / cnt["extension_rays"], 1),
+                    "per_ray_bytes": round(per_ray, 1),
+                    "tail_extension_rays": st["extension_rays"] - st["extend_rays"],
                     "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),
                     "tris_per_ray": round(cnt["triangle_tests"] / cnt["extension_rays"], 2),
                     "avg_launch_ms": round(st["trace_seconds"] / max(st["extend_launches"], 1) * 1e3, 3)}

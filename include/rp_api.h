@@ -208,7 +208,9 @@ typedef struct rp_frame_config {
     uint32_t group_size;       /* threads per workgroup, 0 = auto */
     uint32_t sort_bits;        /* material sort-key width, 0 = auto */
     uint32_t flags;            /* RP_FRAME_* bits below */
-    uint32_t pad;
+    uint32_t tail_paths;       /* wavefront: once at most this many paths are live, one kernel
+                                  finishes them, a thread per path; 0 = auto (the lanes the
+                                  tail kernel holds resident: 512 per CU, 131072 on MI355X) */
 } rp_frame_config;
 
 #define RP_FRAME_COUNT_TRAFFIC 1u /* replay-count BVH node/triangle fetches of the closest-hit queries */
@@ -217,6 +219,8 @@ typedef struct rp_frame_config {
  * the roulette decisions that flip with the parameters dominate the noise;
  * rendering keeps the reference's roulette (the default). */
 #define RP_FRAME_NO_ROULETTE 2u
+/* Wavefront: every bounce through the queue kernels (no latency tail). */
+#define RP_FRAME_NO_TAIL 4u
 
 typedef struct rp_frame_stats {
     double seconds;                 /* device time of the frame (events) */
@@ -226,6 +230,8 @@ typedef struct rp_frame_stats {
     double trace_seconds;           /* wavefront: device time in closest-hit kernels */
     uint64_t extend_launches;       /* wavefront: closest-hit kernel launches */
     uint64_t node_fetches, triangle_tests; /* with RP_FRAME_COUNT_TRAFFIC: closest-hit traversal totals */
+    uint64_t extend_rays;           /* wavefront: extension rays traced by the closest-hit kernels
+                                       (the rest of extension_rays by the latency tail) */
 } rp_frame_stats;
 
 /* ---- traversal probe ray/hit records ------------------------------------ */

@@ -96,13 +96,13 @@ class Request(C.Structure):
 
 class FrameConfig(C.Structure):
     _fields_ = [("mode", u32), ("spp", u32), ("sample_begin", u64), ("seed", u64), ("randomization", u32),
-                ("capacity", u32), ("group_size", u32), ("sort_bits", u32), ("flags", u32), ("pad", u32)]
+                ("capacity", u32), ("group_size", u32), ("sort_bits", u32), ("flags", u32), ("tail_paths", u32)]
 
 
 class FrameStats(C.Structure):
     _fields_ = [("seconds", f64), ("paths", u64), ("extension_rays", u64), ("shadow_rays", u64),
                 ("failed_paths", u64), ("trace_seconds", f64), ("extend_launches", u64), ("node_fetches", u64),
-                ("triangle_tests", u64)]
+                ("triangle_tests", u64), ("extend_rays", u64)]
 
 
 class Ray(C.Structure):
@@ -289,12 +289,13 @@ def request(seed=1, pixel_begin=0, pixel_count=1, sample_begin=0, sample_count=1
 
 FRAME_COUNT_TRAFFIC = 1
 FRAME_NO_ROULETTE = 2
+FRAME_NO_TAIL = 4
 
 
 def frame_config(spp, mode=MODE_WAVEFRONT_SORTED, sample_begin=0, seed=1, randomization=RANDOMIZE_FASTOWEN,
-                 capacity=0, group_size=0, sort_bits=0, flags=0):
+                 capacity=0, group_size=0, sort_bits=0, flags=0, tail_paths=0):
     c = FrameConfig()
-    c.flags = flags
+    c.flags, c.tail_paths = flags, tail_paths
     c.mode, c.spp, c.sample_begin, c.seed, c.randomization = mode, spp, sample_begin, seed, randomization
     c.capacity, c.group_size, c.sort_bits = capacity, group_size, sort_bits
     return c

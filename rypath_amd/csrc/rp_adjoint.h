@@ -370,6 +370,82 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, 
     fold_block_grad<kWfBlock, false>(srow, np, partial);
 }
 
+// Latency tail of the replay: one thread finishes each remaining path of
+// C[0, qn), with the events of k_adj_shade and the shadow terms of
+// k_adj_shadow in the same order per path (the pending continuation of a
+// vertex is applied once its NEE is in L).  Gradient sums differ from the
+// bounce kernels' only in float summation order.
+template <bool kPerThread, bool kInterfaces>
+__global__ void __launch_bounds__(kWfBlock) k_adj_tail(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
+                                                        uint32_t p, uint32_t qn, uint32_t npx, uint64_t sample_begin,
+                                                        float *partial) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    extern __shared__ float sgrad[];
+    const uint32_t np = G.nparams;
+    if (kPerThread) {
+        for (uint32_t q = 0; q < np; ++q) sgrad[q * kWfBlock + threadIdx.x] = 0.0f;
+    } else {
+        for (uint32_t q = threadIdx.x; q < np; q += kWfBlock) sgrad[q] = 0.0f;
+        __syncthreads();
+    }
+    Stack st;
+    st.init(s_node, s_t, kWfBlock);
+    AdjointHook hk;
+    hk.G = &G;
+    hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;
+    hk.stride = kPerThread ? kWfBlock : 0u;
+    const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    if (i < qn) {
+        PathState ps;
+        load_state(C, i, ps);
+        const uint32_t lane = uint32_t(ps.sample - sample_begin) * npx + ps.pixel;
+        if (A.pstatus[lane] == RP_PATH_SUCCESS) {
+            const Wavelengths wl = path_wavelengths(S, ps.w2);
+            adjoint_weights(S, A, ps.pixel, wl, hk.a);
+            const float4 lp = A.Lp[lane];
+            hk.L[0] = lp.x, hk.L[1] = lp.y, hk.L[2] = lp.z, hk.L[3] = lp.w;
+            const int4 ci = A.cidx[p][i];
+            const float4 cd = A.cdlog[p][i];
+            hk.cont_idx[0] = ci.x, hk.cont_idx[1] = ci.y, hk.cont_idx[2] = ci.z, hk.cont_idx[3] = ci.w;
+            hk.cont_dlog[0] = cd.x, hk.cont_dlog[1] = cd.y, hk.cont_dlog[2] = cd.z, hk.cont_dlog[3] = cd.w;
+            for (;;) {
+                // the previous vertex's continuation, now that its NEE is in L
+                for (int k = 0; k < 4; ++k) {
+                    hk.S[k] = ps.L[k];
+                    hk.accumulate(hk.cont_idx[k], hk.a[k] * (hk.L[k] - hk.S[k]) * hk.cont_dlog[k]);
+                }
+                Hit h;
+                traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
+                hk.reset_pending();
+                ShadowReq sr;
+                const bool cont = path_event(S, P, ps, h, sr, hk) && path_alive(S, ps);
+                if (sr.active) {
+                    float tr[4], mlen = 0.0f;
+                    const bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr,
+                                                                P.boundary, &mlen, G.sigma_medium);
+                    if (vis) {
+                        float c[4];
+                        for (int k = 0; k < 4; ++k) c[k] = sr.val[k] * tr[k];
+                        if (!sr.delta_or_nomis)
+                            for (int k = 0; k < 4; ++k) c[k] = c[k] * sr.w;
+                        for (int k = 0; k < 4; ++k) {
+                            ps.L[k] = ps.L[k] + c[k];
+                            hk.accumulate(hk.nee_idx[k], hk.a[k] * c[k] * hk.nee_dlog[k]);
+                            if (G.sigma_medium != RP_NONE && mlen > 0) {
+                                const int32_t idx = G.sigma_base[G.sigma_medium] +
+                                                    int32_t(lambda_bin(wl.lambda[k], G.sigma_bins[G.sigma_medium]));
+                                hk.accumulate(idx, hk.a[k] * c[k] * (-mlen));
+                            }
+                        }
+                    }
+                }
+                if (!cont) break;
+            }
+        }
+    }
+    fold_block_grad<kWfBlock, kPerThread>(sgrad, np, partial);
+}
 
 // Scratch of the wavefront replay (one per device, beside the frame arena).
 struct AdjBuffers {
@@ -419,10 +495,11 @@ __global__ void __launch_bounds__(256) k_adj_prepare(const float4 *L_end, const 
 
 // Path replay of paths [0, n) (path i: pixel i % npx, sample sample_begin +
 // i / npx) whose primal radiance and status are in FB's per-lane arrays;
-// adds dLoss/dtheta into grad (device, np floats).
+// adds dLoss/dtheta into grad (device, np floats).  Once at most `tail`
+// paths are live, k_adj_tail finishes them.
 inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables &G, FrameBuffers &FB,
                            AdjBuffers &AB, uint32_t npx, uint64_t sample_begin, uint64_t n, uint32_t spp,
-                           const float *dl, float *grad, hipStream_t stream) {
+                           const float *dl, float *grad, uint32_t tail, hipStream_t stream) {
     if (n > (uint64_t(1) << 31)) throw std::runtime_error("adjoint chunk too large");
     const uint32_t np = G.nparams;
     FB.ensure(n);
@@ -444,6 +521,23 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
     while (qn > 0) {
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
+        if (qn <= tail) {
+            const size_t tl = lds;
+            if (per_thread && S.has_interfaces)
+                hipLaunchKernelGGL((k_adj_tail<true, true>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+            else if (per_thread)
+                hipLaunchKernelGGL((k_adj_tail<true, false>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+            else if (S.has_interfaces)
+                hipLaunchKernelGGL((k_adj_tail<false, true>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+            else
+                hipLaunchKernelGGL((k_adj_tail<false, false>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+            if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, grid.x, np, grad);
+            break;
+        }
         hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, nullptr);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, 0u);
         if (per_thread)

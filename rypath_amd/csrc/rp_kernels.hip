@@ -22,6 +22,8 @@
 #include "rp_batch.h"
 #include "rp_adjoint.h"
 
+static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 80, "C ABI struct sizes");
+
 namespace rp {
 
 #define HIP_CHECK(expr)                                                                                       \
@@ -489,7 +491,8 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         } else {
             arena->primal.valid = false;
             trace_seconds += rpd::wavefront_render(s->dev, P, arena->fb, cfg, npx, cfg.sample_begin + s0, n,
-                                                   s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays);
+                                                   s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays,
+                                                   s0 == 0);
         }
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
@@ -518,10 +521,13 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     if (stats) {
         unsigned long long c[8];
         read_counters(s->counters.p, 8, c);
+        uint64_t tail[2] = {0, 0};
+        if (arena) arena->fb.tail_rays(tail);
         stats->seconds = ms * 1e-3;
         stats->paths = uint64_t(npx) * spp;
-        stats->extension_rays = c[0] + wf_rays[0];
-        stats->shadow_rays = c[1] + wf_rays[1];
+        stats->extension_rays = c[0] + wf_rays[0] + tail[0];
+        stats->shadow_rays = c[1] + wf_rays[1] + tail[1];
+        stats->extend_rays = wf_rays[0];
         stats->failed_paths = c[2];
         stats->trace_seconds = trace_seconds;
         stats->extend_launches = launches;
@@ -1564,11 +1570,11 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
                 arena.primal.valid = false;
                 s->path_xyz.alloc(3 * n);
                 (void)rpd::wavefront_render(s->dev, P, arena.fb, wcfg, npx, cfg->sample_begin + s0, n,
-                                            s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays);
+                                            s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays, s0 == 0);
                 if (one_chunk) arena.primal.set(s, *cfg, npx, wf_rays);
             }
             rpd::adjoint_replay(s->dev, P, G, arena.fb, arena.adj, npx, cfg->sample_begin + s0, n, cfg->spp, dl_dev, g.p,
-                                s->stream);
+                                rpd::tail_limit(*cfg), s->stream);
         }
         HIP_CHECK(hipEventRecord(e1, s->stream));
         HIP_CHECK(hipEventSynchronize(e1));
@@ -1580,11 +1586,14 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
         if (stats) {
             unsigned long long c[8];
             read_counters(s->counters.p, 8, c);
+            uint64_t tail[2];
+            arena.fb.tail_rays(tail); // the primal's (the replay's tail does not count)
             std::memset(stats, 0, sizeof *stats);
             stats->seconds = ms * 1e-3;
             stats->paths = total;
-            stats->extension_rays = wf_rays[0];
-            stats->shadow_rays = wf_rays[1];
+            stats->extension_rays = wf_rays[0] + tail[0];
+            stats->shadow_rays = wf_rays[1] + tail[1];
+            stats->extend_rays = wf_rays[0];
             stats->failed_paths = c[2];
             stats->extend_launches = launches; // 0: the primal of the preceding wavefront frame was reused
         }

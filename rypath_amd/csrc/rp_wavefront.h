@@ -175,6 +175,7 @@ struct FrameWf {
     uint32_t *perm; // material-sorted order of [0, qn)
     uint32_t *qcount; // [0] survivors, [kQueueStride] shadow rays (separate lines)
     uint32_t *hist;   // 256 bucket counts, then 256 cursors; entry k at hist[k * kQueueStride]
+    unsigned long long *tail; // latency tail's extension [0] and shadow [kCounterStride] rays
 };
 constexpr uint32_t kEnded = 0x80000000u;
 
@@ -384,6 +385,71 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
     // shadow ray counts are the queue counts (summed on the host)
 }
 
+// ------------------------------------------------------------ latency tail
+// Once few paths are live, a bounce costs its fixed overheads (four launches
+// and the host round trip for the queue counts, ~50 us) more than its work.
+// Then one thread finishes each remaining path of C[0, qn): closest hit,
+// event, shadow ray, in the order the bounce kernels apply them (the shadow
+// contribution is added after the event, as the shadow kernel adds it), so
+// the radiance is unchanged bit for bit.
+template <bool kInterfaces>
+__global__ void __launch_bounds__(kWfBlock) k_wf_tail(Scene S, PathParams P, WfState C, FrameWf F, uint32_t qn,
+                                                       uint32_t npx, uint64_t sample_begin) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    Stack st;
+    st.init(s_node, s_t, kWfBlock);
+    const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    uint32_t ext = 0, shr = 0;
+    if (i < qn) {
+        PathState ps;
+        load_state(C, i, ps);
+        const Wavelengths wl = path_wavelengths(S, ps.w2);
+        for (;;) {
+            Hit h;
+            traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
+            ++ext;
+            ShadowReq sr;
+            NoHook nh;
+            const bool cont = path_event(S, P, ps, h, sr, nh) && path_alive(S, ps);
+            if (sr.active) {
+                ++shr;
+                float tr[4];
+                const bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
+                shadow_apply(ps, sr, vis, tr);
+            }
+            if (!cont) break;
+        }
+        const uint32_t lane = uint32_t(ps.sample - sample_begin) * npx + ps.pixel;
+        F.L_end[lane] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
+        F.end[lane] = make_uint2(ps.status, ps.w2);
+    }
+    wave_count(&F.tail[0], ext);
+    wave_count(&F.tail[kCounterStride], shr);
+}
+
+__global__ void k_wf_zero_tail(unsigned long long *tail) {
+    if (threadIdx.x < 2) tail[threadIdx.x * kCounterStride] = 0;
+}
+
+// Live paths at or below which the frame finishes in the latency tail.  By
+// default: as many as the tail kernels hold resident at once (2 waves per
+// SIMD at their ~230 VGPRs: CUs * 4 SIMDs * 2 * 64 lanes, 131072 on a
+// 256-CU MI355X); measured on the 256x256 fog Cornell box, 16 spp, the
+// inverse-rendering step took 10.00 / 9.04 / 8.51 / 9.13 ms with no tail /
+// 65536 / 131072 / 262144, and the 512x512 volume frame 24.6 / 20.8 / 19.7 /
+// 20.5 ms.
+inline uint32_t tail_limit(const rp_frame_config &cfg) {
+    // the instrumented replay counts every closest-hit query in k_wf_extend
+    if (cfg.flags & (RP_FRAME_NO_TAIL | RP_FRAME_COUNT_TRAFFIC)) return 0;
+    if (cfg.tail_paths) return cfg.tail_paths;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 65536;
+    return uint32_t(cus) * 4u * 2u * 64u;
+}
+
 __global__ void __launch_bounds__(256) k_wf_finish(Scene S, FrameWf F, uint64_t n, float *xyz,
                                                      unsigned long long *counters) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -433,7 +499,7 @@ struct FrameBuffers {
             s = WfState{};
         }
         void *ptrs[] = {f.hit_t, f.hit_id, f.sh_o, f.sh_d, f.sh_val, f.sh_meta, f.L_end, f.end, f.perm,
-                        f.qcount, f.hist};
+                        f.qcount, f.hist, f.tail};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         f = FrameWf{};
@@ -442,6 +508,16 @@ struct FrameBuffers {
         capacity = 0;
     }
     void ensure(uint64_t n);
+    // The latency tail's extension and shadow rays since the last frame that
+    // began with first_chunk (synchronous).
+    void tail_rays(uint64_t out[2]) const {
+        out[0] = out[1] = 0;
+        if (!f.tail) return;
+        unsigned long long v[kCounterStride + 1];
+        WF_CHECK(hipMemcpy(v, f.tail, sizeof v, hipMemcpyDeviceToHost));
+        out[0] = v[0];
+        out[1] = v[kCounterStride];
+    }
 };
 
 inline void FrameBuffers::ensure(uint64_t n) {
@@ -457,6 +533,7 @@ inline void FrameBuffers::ensure(uint64_t n) {
     wf_alloc(f.L_end, n), wf_alloc(f.end, n), wf_alloc(f.perm, n);
     wf_alloc(f.qcount, 2 * kQueueStride);
     wf_alloc(f.hist, 512 * kQueueStride); // every bucket counter on its own line
+    wf_alloc(f.tail, 2 * kCounterStride);
     WF_CHECK(hipHostMalloc(reinterpret_cast<void **>(&host_count), 2 * kQueueStride * sizeof(uint32_t),
                            hipHostMallocDefault));
     capacity = n;
@@ -465,14 +542,17 @@ inline void FrameBuffers::ensure(uint64_t n) {
 // Runs n paths (path i: pixel i % npx, sample sample_begin + i / npx) to
 // completion and writes planar per-path XYZ.  Returns device seconds spent in
 // the closest-hit (extend) kernels; adds the extension and shadow ray counts
-// (the queue counts of every bounce) to rays[0] and rays[1].
+// of the bounce kernels (their queue counts) to rays[0] and rays[1]; the
+// latency tail's go to the arena's tail counters, zeroed when first_chunk.
 inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers &FB, const rp_frame_config &cfg,
                                uint32_t npx, uint64_t sample_begin, uint64_t n, float *xyz,
                                unsigned long long *counters, hipStream_t stream, uint64_t &launches,
-                               uint64_t rays[2]) {
+                               uint64_t rays[2], bool first_chunk) {
     if (n > (uint64_t(1) << 31)) throw std::runtime_error("wavefront chunk too large");
     FB.ensure(n);
     const FrameWf &F = FB.f;
+    if (first_chunk) hipLaunchKernelGGL(k_wf_zero_tail, dim3(1), dim3(64), 0, stream, F.tail);
+    const uint32_t tail = tail_limit(cfg);
     const bool sorted = cfg.mode == RP_MODE_WAVEFRONT_SORTED;
     uint32_t bits = cfg.sort_bits ? cfg.sort_bits : 2;
     if (bits < 1 || bits > 8) throw std::runtime_error("sort_bits must be in [1, 8]");
@@ -487,6 +567,13 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
     while (qn > 0) {
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
+        if (qn <= tail) {
+            if (S.has_interfaces)
+                hipLaunchKernelGGL(k_wf_tail<true>, grid, dim3(kWfBlock), 0, stream, S, P, C, F, qn, npx, sample_begin);
+            else
+                hipLaunchKernelGGL(k_wf_tail<false>, grid, dim3(kWfBlock), 0, stream, S, P, C, F, qn, npx, sample_begin);
+            break;
+        }
         WF_CHECK(hipEventRecord(e0, stream));
         if (cfg.flags & RP_FRAME_COUNT_TRAFFIC) {
             hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);

@@ -15,21 +15,31 @@ def inverse(rp, gpu_available):
     return s, rp.Scene(s.desc)
 
 
+# replay shapes: every bounce through the queue kernels, the latency tail
+# from the first bounce (4096 paths), and the tail taking over mid-path
+REPLAY_SHAPES = (("bounce kernels", dict(flags=4)), ("tail", dict()), ("tail at 1024 live", dict(tail_paths=1024)))
+
+
 def test_adjoint_matches_oracle_forward_mode(rp, orc, inverse):
     synth, scene = inverse
     params, values = rp.inverse_params(synth)
     rng = np.random.default_rng(0)
     dl = rng.normal(0, 1, (32, 32, 3)).astype(np.float32)
-    cfg = rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, seed=3)
-    g, st = scene.backward(cfg, params, dl)
     ref = oracle_lib.oracle_gradient(orc, orc.scene(synth.desc), params, dl, spp=4, seed=3)
-    rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
-    print(f"[adjoint] relative L2 difference GPU replay vs oracle forward mode: {rel:.2e}", flush=True)
-    for name, sl in (("walls", slice(0, 48)), ("roughness", slice(48, 50)), ("sigma_s", slice(50, 58))):
-        r = np.linalg.norm(g[sl] - ref[sl]) / max(np.linalg.norm(ref[sl]), 1e-30)
-        print(f"[adjoint]   {name}: {r:.2e}", flush=True)
-        assert r < 2e-3
-    assert rel < 1e-3
+    grads = []
+    for label, kw in REPLAY_SHAPES:
+        cfg = rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, seed=3, **kw)
+        g, st = scene.backward(cfg, params, dl)
+        grads.append(g)
+        rel = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        print(f"[adjoint] {label}: relative L2 difference GPU replay vs oracle forward mode: {rel:.2e} "
+              f"(extension rays {st['extension_rays']}, by the bounce kernels {st['extend_rays']})", flush=True)
+        for name, sl in (("walls", slice(0, 48)), ("roughness", slice(48, 50)), ("sigma_s", slice(50, 58))):
+            r = np.linalg.norm(g[sl] - ref[sl]) / max(np.linalg.norm(ref[sl]), 1e-30)
+            print(f"[adjoint]   {name}: {r:.2e}", flush=True)
+            assert r < 2e-3
+        assert rel < 1e-3
+    assert _same_gradient(grads[0], grads[1]) and _same_gradient(grads[0], grads[2])
 
 
 def test_adjoint_no_roulette_matches_oracle_forward_mode(rp, orc, inverse):
@@ -63,8 +73,10 @@ def test_backward_reuses_a_matching_wavefront_primal(rp, inverse):
     synth, scene = inverse
     params, values = rp.inverse_params(synth)
     dl = np.random.default_rng(8).normal(0, 1, (32, 32, 3)).astype(np.float32)
-    cfg = rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=31)
-    other = rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=32)
+    # every bounce through the closest-hit kernels, so that extend_launches
+    # tells whether a primal was traced
+    cfg = rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=31, flags=rp.FRAME_NO_TAIL)
+    other = rp.frame_config(16, mode=rp.MODE_WAVEFRONT, seed=32, flags=rp.FRAME_NO_TAIL)
     scene.render_frame(other)
     fresh, st = scene.backward(cfg, params, dl)
     assert st["extend_launches"] > 0

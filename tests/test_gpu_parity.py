@@ -72,15 +72,32 @@ def test_cornell_closest_hits(rp, orc, cornell):
 
 
 def test_frame_modes_identical(rp, cornell):
+    """Megakernel and wavefront films are bit-identical, with every bounce
+    through the queue kernels (RP_FRAME_NO_TAIL), with the latency tail from
+    the first bounce (32768 paths, below the default 65536) and with the
+    tail taking over mid-frame; the ray counts agree too."""
     _, scene = cornell
-    films = []
-    for mode in (rp.MODE_MEGAKERNEL, rp.MODE_WAVEFRONT, rp.MODE_WAVEFRONT_SORTED):
-        img, st = scene.render_frame(rp.frame_config(8, mode=mode, seed=1))
-        print(f"[modes] mode {mode}: {st}", flush=True)
+    films, counts = [], []
+    shapes = [(rp.MODE_MEGAKERNEL, {})]
+    for mode in (rp.MODE_WAVEFRONT, rp.MODE_WAVEFRONT_SORTED):
+        shapes += [(mode, dict(flags=rp.FRAME_NO_TAIL)), (mode, {}), (mode, dict(tail_paths=16384))]
+    for mode, kw in shapes:
+        img, st = scene.render_frame(rp.frame_config(8, mode=mode, seed=1, **kw))
+        print(f"[modes] mode {mode} {kw}: {st}", flush=True)
         assert st["failed_paths"] == 0
         films.append(img)
-    assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))
-    assert np.array_equal(films[0].view(np.uint32), films[2].view(np.uint32))
+        counts.append((st["extension_rays"], st["shadow_rays"]))
+        if mode == rp.MODE_MEGAKERNEL:
+            assert st["extend_rays"] == 0
+        elif kw.get("flags") == rp.FRAME_NO_TAIL:
+            assert st["extend_rays"] == st["extension_rays"]
+        elif kw.get("tail_paths"):
+            assert 0 < st["extend_rays"] < st["extension_rays"]
+        else:
+            assert st["extend_rays"] == 0 and st["extend_launches"] == 0
+    for f, c in zip(films[1:], counts[1:]):
+        assert np.array_equal(films[0].view(np.uint32), f.view(np.uint32))
+        assert c == counts[0]
 
 
 def test_frame_vs_oracle(rp, orc, cornell):
@@ -157,8 +174,9 @@ def test_terrain_records(rp, orc, terrain):
 def test_terrain_modes_identical(rp, terrain):
     _, scene = terrain
     a, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_MEGAKERNEL))
-    b, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_WAVEFRONT_SORTED, sort_bits=4))
-    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for kw in (dict(flags=rp.FRAME_NO_TAIL), {}, dict(tail_paths=4096)):
+        b, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_WAVEFRONT_SORTED, sort_bits=4, **kw))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_volume_records(rp, orc, gpu_available):
@@ -174,5 +192,6 @@ def test_inverse_scene_records(rp, orc, gpu_available):
     req = rp.request(seed=1, pixel_begin=0, pixel_count=48 * 48, sample_count=4)
     _records_parity(rp, orc, synth, scene, req)
     a, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_MEGAKERNEL))
-    b, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_WAVEFRONT))
-    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for kw in (dict(flags=rp.FRAME_NO_TAIL), {}, dict(tail_paths=1024)):
+        b, _ = scene.render_frame(rp.frame_config(2, mode=rp.MODE_WAVEFRONT, **kw))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
