@@ -508,7 +508,7 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
     hipLaunchKernelGGL(k_adj_prepare, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, F.L_end, F.end, n, AB.Lp,
                        AB.pstatus, AB.cidx[0]);
     int cur = 0;
-    hipLaunchKernelGGL(k_wf_generate, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n,
+    hipLaunchKernelGGL(k_wf_generate<false>, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n,
                        FB.state[cur]);
     AdjWf A;
     A.cidx[0] = AB.cidx[0], A.cidx[1] = AB.cidx[1];
@@ -538,7 +538,7 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
             if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, grid.x, np, grad);
             break;
         }
-        hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, nullptr);
+        hipLaunchKernelGGL((k_wf_extend<false, false>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, nullptr);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, 0u);
         if (per_thread)
             hipLaunchKernelGGL(k_adj_shade<true>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F, uint32_t(cur),

@@ -55,26 +55,12 @@ struct NoHook {
 
 RP_DEV Wavelengths path_wavelengths(const Scene &S, uint32_t w2) { return sample_wavelengths(sobol_uniform(w2), S.hero_mode); }
 
-// Camera sample: Sobol index, camera words, wavelengths, primary ray
-// (shaders/render.metal:238-312 begin_sample, :458-477 primary_ray, box filter).
-RP_DEV void path_begin(const Scene &S, const PathParams &P, uint32_t pixel, uint64_t sample, PathState &ps,
-                       uint32_t &w0, uint32_t &w1) {
-    const uint32_t W = S.camera.width, H = S.camera.height;
-    const uint32_t px = pixel % W, py = pixel / W;
-    uint32_t seed_bits = uint32_t(P.seed);
+// The initial state of path (pixel, sample) apart from its direction, Sobol
+// index and wavelength word (path_begin computes those).
+RP_DEV void path_init(const Scene &S, const PathParams &P, uint32_t pixel, uint64_t sample, PathState &ps) {
+    const rp_camera &cam = S.camera;
     ps.pixel = pixel;
     ps.sample = sample;
-    ps.index = sobol_index(S, px, py, sample);
-    w0 = sobol_word(S, ps.index, 0, RP_RANDOMIZE_NONE, 0);
-    w1 = sobol_word(S, ps.index, 1, RP_RANDOMIZE_NONE, 0);
-    ps.w2 = sobol_word(S, ps.index, 2, P.mode, seed_bits);
-    float ux = camera_uniform(w0, S.m, px), uy = camera_uniform(w1, S.m, py);
-    float fx = (float(px) + ux) / float(W), fy = (float(py) + uy) / float(H);
-    const rp_camera &cam = S.camera;
-    F3 d = {cam.corner[0] + fx * cam.dx[0] + fy * cam.dy[0], cam.corner[1] + fx * cam.dx[1] + fy * cam.dy[1],
-            cam.corner[2] + fx * cam.dx[2] + fy * cam.dy[2]};
-    float scale = omax(fabsf(d.x), omax(fabsf(d.y), fabsf(d.z)));
-    ps.d = normalize(d / scale);
     ps.o = {cam.origin[0], cam.origin[1], cam.origin[2]};
     ps.tmax = __builtin_inff();
     for (int k = 0; k < 4; ++k) {
@@ -93,6 +79,27 @@ RP_DEV void path_begin(const Scene &S, const PathParams &P, uint32_t pixel, uint
     ps.flags = P.roulette ? kFlagRoulette : 0u;
     ps.status = RP_PATH_SUCCESS;
     ps.primary = RP_NONE;
+}
+
+// Camera sample: Sobol index, camera words, wavelengths, primary ray
+// (shaders/render.metal:238-312 begin_sample, :458-477 primary_ray, box filter).
+RP_DEV void path_begin(const Scene &S, const PathParams &P, uint32_t pixel, uint64_t sample, PathState &ps,
+                       uint32_t &w0, uint32_t &w1) {
+    const uint32_t W = S.camera.width, H = S.camera.height;
+    const uint32_t px = pixel % W, py = pixel / W;
+    uint32_t seed_bits = uint32_t(P.seed);
+    path_init(S, P, pixel, sample, ps);
+    ps.index = sobol_index(S, px, py, sample);
+    w0 = sobol_word(S, ps.index, 0, RP_RANDOMIZE_NONE, 0);
+    w1 = sobol_word(S, ps.index, 1, RP_RANDOMIZE_NONE, 0);
+    ps.w2 = sobol_word(S, ps.index, 2, P.mode, seed_bits);
+    float ux = camera_uniform(w0, S.m, px), uy = camera_uniform(w1, S.m, py);
+    float fx = (float(px) + ux) / float(W), fy = (float(py) + uy) / float(H);
+    const rp_camera &cam = S.camera;
+    F3 d = {cam.corner[0] + fx * cam.dx[0] + fy * cam.dy[0], cam.corner[1] + fx * cam.dx[1] + fy * cam.dy[1],
+            cam.corner[2] + fx * cam.dx[2] + fy * cam.dy[2]};
+    float scale = omax(fabsf(d.x), omax(fabsf(d.y), fabsf(d.z)));
+    ps.d = normalize(d / scale);
 }
 
 // True while the path still needs its current ray traced.

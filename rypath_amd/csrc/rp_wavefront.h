@@ -179,6 +179,11 @@ struct FrameWf {
 };
 constexpr uint32_t kEnded = 0x80000000u;
 
+// Camera samples of lanes [0, n).  kCompact: only what path_init cannot
+// rebuild -- direction and wavelength word (C.d), Sobol index (C.meta) --
+// 32 B per path instead of the 192-B state; the first bounce's kernels
+// (kGen) read those and rebuild the rest.
+template <bool kCompact = false>
 __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint32_t npx, uint64_t sample_begin,
                                                        uint64_t n, WfState C) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -186,10 +191,29 @@ __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint
     PathState ps;
     uint32_t w0, w1;
     path_begin(S, P, uint32_t(i % npx), sample_begin + i / npx, ps, w0, w1);
-    store_state(C, uint32_t(i), ps);
+    if (kCompact) {
+        C.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.w2));
+        C.meta[i] = make_uint4(uint32_t(ps.index), uint32_t(ps.index >> 32), 0u, 0u);
+    } else {
+        store_state(C, uint32_t(i), ps);
+    }
 }
 
-template <bool kCount>
+// The state k_wf_generate<true> left for lane j (path j % npx, sample
+// sample_begin + j / npx): equal to path_begin's.
+RP_DEV void load_generated(const Scene &S, const PathParams &P, const WfState &C, uint32_t j, uint32_t npx,
+                           uint64_t sample_begin, PathState &ps) {
+    path_init(S, P, j % npx, sample_begin + j / npx, ps);
+    const float4 d = C.d[j];
+    const uint4 m = C.meta[j];
+    ps.d = {d.x, d.y, d.z};
+    ps.w2 = __float_as_uint(d.w);
+    ps.index = uint64_t(m.x) | (uint64_t(m.y) << 32);
+}
+
+// Closest hit for C[0, qn).  kGen (the first bounce, compact generation):
+// the ray leaves the camera origin with the direction in C.d.
+template <bool kCount, bool kGen = false>
 __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, FrameWf F, uint32_t qn,
                                                           unsigned long long *counters) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
@@ -199,7 +223,9 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t nodes = 0, tris = 0;
     if (i < qn) {
-        float4 a = C.o_t[i], b = C.d[i];
+        float4 a, b = C.d[i];
+        if (kGen) a = make_float4(S.camera.origin[0], S.camera.origin[1], S.camera.origin[2], __builtin_inff());
+        else a = C.o_t[i];
         Hit h;
         TraversalCounts tc = {0, 0};
         traverse<false, kCount>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h, &tc);
@@ -295,7 +321,9 @@ __global__ void k_wf_reset(uint32_t *qcount, uint32_t *hist, uint32_t bits) {
 
 // One path event for the path at C[j] when `valid`: survivors are appended
 // to N and shadow rays to the shadow stream (every thread of the block must
-// call this together: the appends are block-aggregated).
+// call this together: the appends are block-aggregated).  kGen (the first
+// bounce): the path at j is lane j's, compactly generated.
+template <bool kGen>
 RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, const WfState &N, const FrameWf &F,
                       bool valid, uint32_t j, uint32_t npx, uint64_t sample_begin, AppendLds<kWfBlock, 2> &sh) {
     bool cont = false, shadow = false;
@@ -304,7 +332,8 @@ RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, con
     sr.active = false;
     uint32_t lane = 0;
     if (valid) {
-        load_state(C, j, ps);
+        if (kGen) load_generated(S, P, C, j, npx, sample_begin, ps);
+        else load_state(C, j, ps);
         float4 ht = F.hit_t[j];
         uint2 hi = F.hit_id[j];
         Hit h;
@@ -340,13 +369,14 @@ RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, con
 }
 
 // Any material: reads C[i], or C[perm[i]] in material-sorted order.
+template <bool kGen = false>
 __global__ void __launch_bounds__(kWfBlock) RP_SHADE_ATTR k_wf_shade(Scene S, PathParams P, WfState C, WfState N,
                                                                      FrameWf F, uint32_t qn, bool sorted, uint32_t npx,
                                                                      uint64_t sample_begin) {
     __shared__ AppendLds<kWfBlock, 2> sh;
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     const bool valid = i < qn;
-    shade_one(S, P, C, N, F, valid, valid ? (sorted ? F.perm[i] : i) : 0u, npx, sample_begin, sh);
+    shade_one<kGen>(S, P, C, N, F, valid, valid ? (sorted ? F.perm[i] : i) : 0u, npx, sample_begin, sh);
 }
 
 // Shadow rays of one bounce; adds visible contributions to the survivor's
@@ -557,8 +587,15 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
     uint32_t bits = cfg.sort_bits ? cfg.sort_bits : 2;
     if (bits < 1 || bits > 8) throw std::runtime_error("sort_bits must be in [1, 8]");
     int cur = 0;
-    hipLaunchKernelGGL(k_wf_generate, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx, sample_begin, n,
-                       FB.state[cur]);
+    // compact generation unless the frame starts in the tail (which loads
+    // whole states)
+    bool gen = n > tail;
+    if (gen)
+        hipLaunchKernelGGL(k_wf_generate<true>, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx,
+                           sample_begin, n, FB.state[cur]);
+    else
+        hipLaunchKernelGGL(k_wf_generate<false>, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx,
+                           sample_begin, n, FB.state[cur]);
     uint32_t qn = uint32_t(n);
     hipEvent_t e0, e1;
     WF_CHECK(hipEventCreate(&e0));
@@ -575,11 +612,15 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
             break;
         }
         WF_CHECK(hipEventRecord(e0, stream));
-        if (cfg.flags & RP_FRAME_COUNT_TRAFFIC) {
-            hipLaunchKernelGGL(k_wf_extend<true>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
-        } else {
-            hipLaunchKernelGGL(k_wf_extend<false>, grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
-        }
+        const bool count = (cfg.flags & RP_FRAME_COUNT_TRAFFIC) != 0;
+        if (gen && count)
+            hipLaunchKernelGGL((k_wf_extend<true, true>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
+        else if (gen)
+            hipLaunchKernelGGL((k_wf_extend<false, true>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
+        else if (count)
+            hipLaunchKernelGGL((k_wf_extend<true, false>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
+        else
+            hipLaunchKernelGGL((k_wf_extend<false, false>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
         WF_CHECK(hipEventRecord(e1, stream));
         ++launches;
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, bits);
@@ -589,7 +630,13 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
             hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(64), 0, stream, bits, F.hist);
             hipLaunchKernelGGL(k_sort_scatter, g, dim3(256), 0, stream, S, F.hit_id, nullptr, qn, bits, F.hist, F.perm);
         }
-        hipLaunchKernelGGL(k_wf_shade, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx, sample_begin);
+        if (gen)
+            hipLaunchKernelGGL(k_wf_shade<true>, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx,
+                               sample_begin);
+        else
+            hipLaunchKernelGGL(k_wf_shade<false>, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx,
+                               sample_begin);
+        gen = false;
         WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, (kQueueStride + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                 stream));
         WF_CHECK(hipStreamSynchronize(stream));
