@@ -493,7 +493,7 @@ __global__ void __launch_bounds__(256) k_adj_prepare(const float4 *L_end, const 
     cidx[i] = make_int4(-1, -1, -1, -1);
 }
 
-// Path replay of paths [0, n) (path i: pixel i % npx, sample sample_begin +
+// Path replay of paths [0, n) (position i: pixel tile_pixel(i % npx), sample sample_begin +
 // i / npx) whose primal radiance and status are in FB's per-lane arrays;
 // adds dLoss/dtheta into grad (device, np floats).  Once at most `tail`
 // paths are live, k_adj_tail finishes them.

@@ -99,17 +99,17 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
         if (g < total) {
             // a wave's 64 consecutive ids span one scene, or two at a boundary
             for_scene(scenes, find_scene(scenes, nscenes, g), [&](const BatchScene &B) {
-                const uint64_t local = g - B.path_begin;
-                const uint32_t pixel = uint32_t(local % B.npx);
-                const uint64_t sample = B.sample_begin + local / B.npx;
+                const uint64_t local = g - B.path_begin, s = local / B.npx;
+                const uint32_t pixel = tile_pixel(B.S, uint32_t(local % B.npx));
                 PathResult r;
-                trace_path(B.S, B.P, st, pixel, sample, r);
+                trace_path(B.S, B.P, st, pixel, B.sample_begin + s, r);
                 float v[3] = {0, 0, 0};
                 if (r.status == RP_PATH_SUCCESS) sensor_xyz(B.S, r.wl, r.L, v);
                 else failed += 1;
-                B.xyz[local] = v[0];
-                B.xyz[B.paths + local] = v[1];
-                B.xyz[2 * B.paths + local] = v[2];
+                const uint64_t lane = s * B.npx + pixel;
+                B.xyz[lane] = v[0];
+                B.xyz[B.paths + lane] = v[1];
+                B.xyz[2 * B.paths + lane] = v[2];
                 ext += r.ext_rays;
                 sh += r.shadow_rays;
             });
@@ -180,7 +180,8 @@ __global__ void k_pool_refill(PoolView V, uint32_t p) {
             const uint64_t local = g - BS.path_begin;
             PathState ps;
             uint32_t w0, w1;
-            path_begin(BS.S, BS.P, uint32_t(local % BS.npx), BS.sample_begin + local / BS.npx, ps, w0, w1);
+            path_begin(BS.S, BS.P, tile_pixel(BS.S, uint32_t(local % BS.npx)), BS.sample_begin + local / BS.npx, ps,
+                       w0, w1);
             store_state(V.S[p], live + i, ps);
             V.sid[p][live + i] = s;
         }

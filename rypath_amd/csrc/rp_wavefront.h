@@ -179,6 +179,18 @@ struct FrameWf {
 };
 constexpr uint32_t kEnded = 0x80000000u;
 
+// Pixel of position q (< W*H) within one sample of a frame: 8x8 pixel tiles
+// in row-major tile order when both image sizes are multiples of 8, so that
+// a wave's 64 paths start from one tile (coherent camera rays, nearby
+// first hits), else row-major.  Results are stored per (sample, pixel), so
+// the order changes nothing in the film.
+RP_DEV uint32_t tile_pixel(const Scene &S, uint32_t q) {
+    const uint32_t W = S.camera.width, H = S.camera.height;
+    if ((W | H) & 7u) return q;
+    const uint32_t t = q >> 6, l = q & 63u, tw = W >> 3;
+    return ((t / tw) * 8u + (l >> 3)) * W + (t % tw) * 8u + (l & 7u);
+}
+
 // Camera samples of lanes [0, n).  kCompact: only what path_init cannot
 // rebuild -- direction and wavelength word (C.d), Sobol index (C.meta) --
 // 32 B per path instead of the 192-B state; the first bounce's kernels
@@ -190,7 +202,7 @@ __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint
     if (i >= n) return;
     PathState ps;
     uint32_t w0, w1;
-    path_begin(S, P, uint32_t(i % npx), sample_begin + i / npx, ps, w0, w1);
+    path_begin(S, P, tile_pixel(S, uint32_t(i % npx)), sample_begin + i / npx, ps, w0, w1);
     if (kCompact) {
         C.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.w2));
         C.meta[i] = make_uint4(uint32_t(ps.index), uint32_t(ps.index >> 32), 0u, 0u);
@@ -199,11 +211,11 @@ __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint
     }
 }
 
-// The state k_wf_generate<true> left for lane j (path j % npx, sample
-// sample_begin + j / npx): equal to path_begin's.
+// The state k_wf_generate<true> left at position j (pixel
+// tile_pixel(j % npx), sample sample_begin + j / npx): equal to path_begin's.
 RP_DEV void load_generated(const Scene &S, const PathParams &P, const WfState &C, uint32_t j, uint32_t npx,
                            uint64_t sample_begin, PathState &ps) {
-    path_init(S, P, j % npx, sample_begin + j / npx, ps);
+    path_init(S, P, tile_pixel(S, j % npx), sample_begin + j / npx, ps);
     const float4 d = C.d[j];
     const uint4 m = C.meta[j];
     ps.d = {d.x, d.y, d.z};
@@ -569,7 +581,8 @@ inline void FrameBuffers::ensure(uint64_t n) {
     capacity = n;
 }
 
-// Runs n paths (path i: pixel i % npx, sample sample_begin + i / npx) to
+// Runs n paths (position i: pixel tile_pixel(i % npx), sample sample_begin +
+// i / npx; npx = W * H) to
 // completion and writes planar per-path XYZ.  Returns device seconds spent in
 // the closest-hit (extend) kernels; adds the extension and shadow ray counts
 // of the bounce kernels (their queue counts) to rays[0] and rays[1]; the
