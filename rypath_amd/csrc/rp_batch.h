@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
 // ------------------------------------------------------ wavefront pool
 struct PoolView {
     WfState S[2];        // two compacted state sets (path-state fields only)
-    uint32_t *sid[2];    // scene of each position, per state set
+    uint32_t *sid[2];    // scene of each position, per state set (| kSidFresh: new, compact)
     FrameWf F;           // hits, shadow stream, sort order, histogram
     uint32_t *sh_sid;    // scene of each shadow ray
     uint4 *e_id;         // ended paths: (pixel, sample lo, sample hi, wavelength word)
@@ -143,6 +143,11 @@ struct PoolView {
 };
 // Ended paths reuse the frame buffers' per-lane arrays: F.L_end (radiance)
 // and F.end (status, scene).
+// A path refilled this iteration carries this bit in its sid entry: its
+// state is compact (k_pool_refill) and the extend and shading kernels
+// rebuild the rest.  The flag travels with the scene index the kernels read
+// anyway, so no kernel waits on a separate load before its state loads.
+constexpr uint32_t kSidFresh = 0x80000000u;
 
 #define RP_GRID_STRIDE(n)                                                                                     \
     for (uint32_t base_ = blockIdx.x * blockDim.x; base_ < (n); base_ += gridDim.x * blockDim.x)
@@ -166,7 +171,9 @@ RP_DEV uint32_t pool_new_paths(const PoolView &V, uint32_t p) {
     return uint32_t(remaining < room ? remaining : room);
 }
 
-// New paths (in global id order, any scene) appended behind the survivors.
+// New paths (in global id order, any scene) appended behind the survivors,
+// compactly (direction and wavelength word, Sobol index; as
+// k_wf_generate<true>): the extend and shading kernels rebuild the rest.
 __global__ void k_pool_refill(PoolView V, uint32_t p) {
     const uint32_t n = pool_new_paths(V, p);
     const uint32_t live = V.c->qn[p];
@@ -182,8 +189,8 @@ __global__ void k_pool_refill(PoolView V, uint32_t p) {
             uint32_t w0, w1;
             path_begin(BS.S, BS.P, tile_pixel(BS.S, uint32_t(local % BS.npx)), BS.sample_begin + local / BS.npx, ps,
                        w0, w1);
-            store_state(V.S[p], live + i, ps);
-            V.sid[p][live + i] = s;
+            store_compact(V.S[p], live + i, ps, BS.sample_begin);
+            V.sid[p][live + i] = s | kSidFresh;
         }
     }
 }
@@ -220,9 +227,13 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
     if (blockIdx.x >= nb) return;
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
     if (i < n) {
-        float4 a = V.S[p].o_t[i], b = V.S[p].d[i];
+        const uint32_t sw = V.sid[p][i];
+        const bool fresh = (sw & kSidFresh) != 0; // a new path leaves its scene's camera
+        const float4 b = V.S[p].d[i];
+        float4 a = fresh ? make_float4(0.0f, 0.0f, 0.0f, __builtin_inff()) : V.S[p].o_t[i];
         Hit h;
-        for_scene(V.scenes, V.sid[p][i], [&](const BatchScene &B) {
+        for_scene(V.scenes, sw & ~kSidFresh, [&](const BatchScene &B) {
+            if (fresh) a = make_float4(B.S.camera.origin[0], B.S.camera.origin[1], B.S.camera.origin[2], a.w);
             traverse<false>(B.S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h);
         });
         V.F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
@@ -231,7 +242,7 @@ __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
 }
 
 RP_DEV uint32_t pool_key(const PoolView &V, uint32_t p, uint32_t i, uint32_t bits) {
-    return material_key(V.scenes[V.sid[p][i]].S, V.F.hit_id, i, bits);
+    return material_key(V.scenes[V.sid[p][i] & ~kSidFresh].S, V.F.hit_id, i, bits);
 }
 __global__ void k_pool_sort_hist(PoolView V, uint32_t p, uint32_t bits) {
     __shared__ uint32_t h[256];
@@ -291,15 +302,16 @@ RP_DEV const BatchScene &uniform_scene(const BatchScene *scenes, uint32_t u) {
 // position, or kEnded | ended index).  Every thread of the block calls this
 // together (the appends are block-aggregated).
 template <int BS>
-RP_DEV void pool_shade_path(const PoolView &V, uint32_t p, bool valid, uint32_t j, uint32_t s, const BatchScene &B,
-                            AppendLds<BS, 3> &sh) {
+RP_DEV void pool_shade_path(const PoolView &V, uint32_t p, bool valid, uint32_t j, uint32_t s, bool fresh,
+                            const BatchScene &B, AppendLds<BS, 3> &sh) {
     const WfState &C = V.S[p], &N = V.S[1 - p];
     bool cont = false, shadow = false;
     PathState ps;
     ShadowReq sr;
     sr.active = false;
     if (valid) {
-        load_state(C, j, ps);
+        if (fresh) load_compact(B.S, B.P, C, j, B.sample_begin, ps);
+        else load_state(C, j, ps);
         float4 ht = V.F.hit_t[j];
         uint2 hi = V.F.hit_id[j];
         Hit h;
@@ -348,16 +360,19 @@ __global__ void __launch_bounds__(BS) k_pool_shade(PoolView V, uint32_t p, uint3
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
     const bool valid = i < n;
     uint32_t j = 0, s = 0;
+    bool fresh = false;
     if (valid) {
         j = sorted ? V.F.perm[i] : i;
-        s = V.sid[p][j];
+        const uint32_t sw = V.sid[p][j];
+        s = sw & ~kSidFresh;
+        fresh = (sw & kSidFresh) != 0;
     }
     bool mine;
     const uint32_t u = wave_scene(valid, s, mine);
     const bool other = valid && !mine;
     const uint32_t o = wave_append(other, &V.c->ov);
     if (other) V.ov[o] = j;
-    pool_shade_path<BS>(V, p, mine, j, u, uniform_scene(V.scenes, u), sh);
+    pool_shade_path<BS>(V, p, mine, j, u, fresh, uniform_scene(V.scenes, u), sh);
 }
 
 // The path events k_pool_shade handed over, one scene per lane.
@@ -369,8 +384,8 @@ __global__ void __launch_bounds__(256) k_pool_shade_mixed(PoolView V, uint32_t p
         const uint32_t i = base_ + threadIdx.x;
         const bool valid = i < n;
         const uint32_t j = valid ? V.ov[i] : 0u;
-        const uint32_t s = valid ? V.sid[p][j] : 0u;
-        pool_shade_path<256>(V, p, valid, j, s, V.scenes[s], sh);
+        const uint32_t sw = valid ? V.sid[p][j] : 0u, s = sw & ~kSidFresh;
+        pool_shade_path<256>(V, p, valid, j, s, (sw & kSidFresh) != 0, V.scenes[s], sh);
         mixed += valid ? 1u : 0u;
     }
     wave_count(&V.c->mixed, mixed);

@@ -191,6 +191,24 @@ RP_DEV uint32_t tile_pixel(const Scene &S, uint32_t q) {
     return ((t / tw) * 8u + (l >> 3)) * W + (t % tw) * 8u + (l & 7u);
 }
 
+// A new path's state, compactly: what path_init cannot rebuild (direction
+// and wavelength word in C.d; Sobol index, pixel and sample offset from
+// sample_begin in C.meta).
+RP_DEV void store_compact(const WfState &C, uint32_t i, const PathState &ps, uint64_t sample_begin) {
+    C.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.w2));
+    C.meta[i] = make_uint4(uint32_t(ps.index), uint32_t(ps.index >> 32), ps.pixel, uint32_t(ps.sample - sample_begin));
+}
+// The state of the new path stored compactly at C[j]: equal to path_begin's.
+RP_DEV void load_compact(const Scene &S, const PathParams &P, const WfState &C, uint32_t j, uint64_t sample_begin,
+                         PathState &ps) {
+    const float4 d = C.d[j];
+    const uint4 m = C.meta[j];
+    path_init(S, P, m.z, sample_begin + m.w, ps);
+    ps.d = {d.x, d.y, d.z};
+    ps.w2 = __float_as_uint(d.w);
+    ps.index = uint64_t(m.x) | (uint64_t(m.y) << 32);
+}
+
 // Camera samples of lanes [0, n).  kCompact: only what path_init cannot
 // rebuild -- direction and wavelength word (C.d), Sobol index (C.meta) --
 // 32 B per path instead of the 192-B state; the first bounce's kernels
@@ -203,24 +221,8 @@ __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint
     PathState ps;
     uint32_t w0, w1;
     path_begin(S, P, tile_pixel(S, uint32_t(i % npx)), sample_begin + i / npx, ps, w0, w1);
-    if (kCompact) {
-        C.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.w2));
-        C.meta[i] = make_uint4(uint32_t(ps.index), uint32_t(ps.index >> 32), 0u, 0u);
-    } else {
-        store_state(C, uint32_t(i), ps);
-    }
-}
-
-// The state k_wf_generate<true> left at position j (pixel
-// tile_pixel(j % npx), sample sample_begin + j / npx): equal to path_begin's.
-RP_DEV void load_generated(const Scene &S, const PathParams &P, const WfState &C, uint32_t j, uint32_t npx,
-                           uint64_t sample_begin, PathState &ps) {
-    path_init(S, P, tile_pixel(S, j % npx), sample_begin + j / npx, ps);
-    const float4 d = C.d[j];
-    const uint4 m = C.meta[j];
-    ps.d = {d.x, d.y, d.z};
-    ps.w2 = __float_as_uint(d.w);
-    ps.index = uint64_t(m.x) | (uint64_t(m.y) << 32);
+    if (kCompact) store_compact(C, uint32_t(i), ps, sample_begin);
+    else store_state(C, uint32_t(i), ps);
 }
 
 // Closest hit for C[0, qn).  kGen (the first bounce, compact generation):
@@ -344,7 +346,7 @@ RP_DEV void shade_one(const Scene &S, const PathParams &P, const WfState &C, con
     sr.active = false;
     uint32_t lane = 0;
     if (valid) {
-        if (kGen) load_generated(S, P, C, j, npx, sample_begin, ps);
+        if (kGen) load_compact(S, P, C, j, sample_begin, ps);
         else load_state(C, j, ps);
         float4 ht = F.hit_t[j];
         uint2 hi = F.hit_id[j];
