@@ -307,7 +307,13 @@ typedef struct rp_batch_config {
     uint32_t use_graph;   /* nonzero: the wavefront bounce loop is replayed as one captured HIP graph */
     uint32_t randomization;
     uint32_t scheduler;   /* rp_scheduler */
+    uint32_t tail_paths;  /* wavefront pool: once every path is issued and at most this many are
+                             live, one kernel finishes them (a thread per path); 0 = auto
+                             (512 per CU) */
+    uint32_t flags;       /* RP_BATCH_* bits below */
 } rp_batch_config;
+
+#define RP_BATCH_NO_TAIL 1u /* wavefront pool: bounce iterations until the pool is empty */
 
 typedef struct rp_batch_stats {
     double seconds;

@@ -125,7 +125,7 @@ class BatchItem(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("capacity", u32), ("group_size", u32), ("sort_bits", u32), ("use_graph", u32),
-                ("randomization", u32), ("scheduler", u32)]
+                ("randomization", u32), ("scheduler", u32), ("tail_paths", u32), ("flags", u32)]
 
 
 class BatchStats(C.Structure):
@@ -290,6 +290,7 @@ def request(seed=1, pixel_begin=0, pixel_count=1, sample_begin=0, sample_count=1
 FRAME_COUNT_TRAFFIC = 1
 FRAME_NO_ROULETTE = 2
 FRAME_NO_TAIL = 4
+BATCH_NO_TAIL = 1
 
 
 def frame_config(spp, mode=MODE_WAVEFRONT_SORTED, sample_begin=0, seed=1, randomization=RANDOMIZE_FASTOWEN,
@@ -555,10 +556,10 @@ class Batch:
 
     @staticmethod
     def config(scheduler=SCHED_WAVEFRONT, capacity=0, group_size=0, sort_bits=0, use_graph=1,
-               randomization=RANDOMIZE_FASTOWEN):
+               randomization=RANDOMIZE_FASTOWEN, tail_paths=0, flags=0):
         c = BatchConfig()
         c.scheduler, c.capacity, c.group_size, c.sort_bits = scheduler, capacity, group_size, sort_bits
-        c.use_graph, c.randomization = use_graph, randomization
+        c.use_graph, c.randomization, c.tail_paths, c.flags = use_graph, randomization, tail_paths, flags
         return c
 
     def render(self, cfg, films=True):

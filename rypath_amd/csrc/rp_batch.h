@@ -494,4 +494,55 @@ __global__ void k_pool_retire(PoolView V) {
     wave_count(&V.c->rays[2], failed);
 }
 
+// Latency tail of the pool: once every path has been issued and at most a
+// tail's worth are live (state set p, all awaiting their next closest hit),
+// one thread finishes each -- the events of the bounce kernels in the same
+// order, then sensor XYZ into its scene's planes as k_pool_retire writes it.
+// Ray and failure counts go to the pool counters; the host counts the n
+// paths as done.
+template <bool kInterfaces>
+__global__ void __launch_bounds__(kWfBlock) k_pool_tail(PoolView V, uint32_t p, uint32_t n) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    Stack st;
+    st.init(s_node, s_t, kWfBlock);
+    const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    uint32_t ext = 0, shr = 0, failed = 0;
+    if (i < n) {
+        const uint32_t sw = V.sid[p][i];
+        for_scene(V.scenes, sw & ~kSidFresh, [&](const BatchScene &B) {
+            PathState ps;
+            if (sw & kSidFresh) load_compact(B.S, B.P, V.S[p], i, B.sample_begin, ps);
+            else load_state(V.S[p], i, ps);
+            const Wavelengths wl = path_wavelengths(B.S, ps.w2);
+            for (;;) {
+                Hit h;
+                traverse<false>(B.S, st, ps.o, ps.d, ps.tmax, h);
+                ++ext;
+                ShadowReq sr;
+                NoHook nh;
+                const bool cont = path_event(B.S, B.P, ps, h, sr, nh) && path_alive(B.S, ps);
+                if (sr.active) {
+                    ++shr;
+                    float tr[4];
+                    const bool vis =
+                        transmittance<kInterfaces>(B.S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, B.P.boundary);
+                    shadow_apply(ps, sr, vis, tr);
+                }
+                if (!cont) break;
+            }
+            float v[3] = {0, 0, 0};
+            if (ps.status == RP_PATH_SUCCESS) sensor_xyz(B.S, wl, ps.L, v);
+            else failed += 1;
+            const uint64_t lane = (ps.sample - B.sample_begin) * B.npx + ps.pixel;
+            B.xyz[lane] = v[0];
+            B.xyz[B.paths + lane] = v[1];
+            B.xyz[2 * B.paths + lane] = v[2];
+        });
+    }
+    wave_count(&V.c->rays[0], ext);
+    wave_count(&V.c->rays[1], shr);
+    wave_count(&V.c->rays[2], failed);
+}
+
 } // namespace rpd

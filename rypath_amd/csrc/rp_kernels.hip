@@ -22,7 +22,8 @@
 #include "rp_batch.h"
 #include "rp_adjoint.h"
 
-static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 80, "C ABI struct sizes");
+static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 80 && sizeof(rp_batch_config) == 32,
+              "C ABI struct sizes");
 
 namespace rp {
 
@@ -1160,6 +1161,10 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
     const dim3 grid(uint32_t(cu_count(b->device)) * 8);
     hipLaunchKernelGGL(rpd::k_pool_init, dim3(1), dim3(64), 0, b->stream, V, b->total);
     const int pairs = 4; // 8 bounce iterations per submission
+    // live paths at or below which the drained pool finishes in k_pool_tail
+    rp_frame_config tcfg{};
+    tcfg.tail_paths = cfg.tail_paths;
+    const uint32_t tail = (cfg.flags & RP_BATCH_NO_TAIL) ? 0u : rpd::tail_limit(tcfg);
     auto submit = [&] {
         if (bs == 64) pool_block<64>(b, V, bits, grid, pairs);
         else if (bs == 128) pool_block<128>(b, V, bits, grid, pairs);
@@ -1192,6 +1197,21 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
                                  b->stream));
         HIP_CHECK(hipStreamSynchronize(b->stream));
         if (b->host_counters->done >= b->total) break;
+        // every path issued and few live (set 0 after an even number of
+        // iterations): the latency tail finishes them
+        const uint32_t live = b->host_counters->qn[0];
+        if (b->host_counters->next >= b->total && live <= tail) {
+            const dim3 tg((live + rpd::kWfBlock - 1) / rpd::kWfBlock);
+            if (b->any_interfaces)
+                hipLaunchKernelGGL(rpd::k_pool_tail<true>, tg, dim3(rpd::kWfBlock), 0, b->stream, V, 0u, live);
+            else
+                hipLaunchKernelGGL(rpd::k_pool_tail<false>, tg, dim3(rpd::kWfBlock), 0, b->stream, V, 0u, live);
+            HIP_CHECK(hipGetLastError());
+            // the final ray counts, read after the caller's synchronisation
+            HIP_CHECK(hipMemcpyAsync(b->host_counters, b->counters.p, sizeof(rpd::PoolCounters),
+                                     hipMemcpyDeviceToHost, b->stream));
+            break;
+        }
         const double waited = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         if (guard > 200000 || waited > 120.0)
             throw std::runtime_error("wavefront pool did not drain: done " + std::to_string(b->host_counters->done) +
@@ -1350,13 +1370,15 @@ int rp_autotune(rp_batch *b, rp_batch_config *best, double *best_seconds) {
         return RP_INVALID;
     }
     try {
-        const uint32_t randomization = best->randomization;
+        const uint32_t randomization = best->randomization, tail_paths = best->tail_paths, flags = best->flags;
         double best_t = 1e300;
         rp_batch_config best_cfg{};
         // Each candidate renders twice and the second render is timed: the
         // first one allocates the pool for a new capacity and captures its graph.
         auto trial = [&](rp_batch_config c) {
             c.randomization = randomization;
+            c.tail_paths = tail_paths;
+            c.flags = flags;
             (void)batch_render_impl(b, c, nullptr, nullptr);
             const double t = batch_render_impl(b, c, nullptr, nullptr);
             if (t < best_t) {

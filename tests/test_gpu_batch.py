@@ -33,7 +33,10 @@ def _alone(rp, items):
     dict(scheduler=1, capacity=1 << 12, group_size=128, sort_bits=0, use_graph=0),
     dict(scheduler=1, capacity=1 << 12, group_size=256, sort_bits=2, use_graph=1),
     dict(scheduler=1, capacity=1 << 16, group_size=64, sort_bits=4, use_graph=1),
-], ids=["persistent128", "persistent64", "pool-nograph", "pool-graph-sort2", "pool-graph-sort4"])
+    dict(scheduler=1, capacity=1 << 12, group_size=128, use_graph=1, flags=1),
+    dict(scheduler=1, capacity=1 << 12, group_size=128, use_graph=1, tail_paths=256),
+], ids=["persistent128", "persistent64", "pool-nograph", "pool-graph-sort2", "pool-graph-sort4", "pool-no-tail",
+        "pool-tail-256"])
 def test_batch_films_equal_single_frames(rp, uneven, cfg):
     _, items = uneven
     batch = rp.Batch(items)
@@ -65,3 +68,17 @@ def test_autotune_returns_valid_config(rp, uneven):
     films, _ = batch.render(cfg)
     for got, want in zip(films, _alone(rp, items)):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_pool_tail_keeps_ray_counts(rp, uneven):
+    """The pool's latency tail (every path issued, few live) traces the same
+    rays as bounce iterations to the end and as the scenes alone."""
+    _, items = uneven
+    batch = rp.Batch(items)
+    alone = [scene.render_frame(rp.frame_config(spp, mode=rp.MODE_MEGAKERNEL, seed=seed))[1]
+             for scene, spp, seed in items]
+    want = (sum(a["extension_rays"] for a in alone), sum(a["shadow_rays"] for a in alone))
+    for kw in (dict(flags=rp.BATCH_NO_TAIL), dict(), dict(tail_paths=256)):
+        _, st = batch.render(rp.Batch.config(scheduler=1, capacity=1 << 12, group_size=128, **kw))
+        print(f"[pool tail] {kw}: {st}", flush=True)
+        assert (st["extension_rays"], st["shadow_rays"]) == want
