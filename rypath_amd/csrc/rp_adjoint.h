@@ -472,8 +472,11 @@ struct AdjBuffers {
         partial = nullptr;
         capacity = partial_floats = 0;
     }
+    // partial: block rows of the largest launch plus room for the smaller
+    // launches that follow it (up to 3 more of that size, 65536 rows at most)
     void ensure(uint64_t n, uint32_t np) {
-        const uint64_t pf = ((n + kWfBlock - 1) / kWfBlock) * std::max<uint32_t>(np, 1);
+        const uint64_t blocks = (n + kWfBlock - 1) / kWfBlock;
+        const uint64_t pf = (blocks + std::min<uint64_t>(3 * blocks, 65536)) * std::max<uint32_t>(np, 1);
         if (n <= capacity && pf <= partial_floats) return;
         release();
         for (int s = 0; s < 2; ++s) wf_alloc(cidx[s], n), wf_alloc(cdlog[s], n);
@@ -517,36 +520,51 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
     A.Lp = AB.Lp, A.pstatus = AB.pstatus, A.dl = dl, A.spp = spp;
     const bool per_thread = np <= kPerThreadParams;
     const size_t lds = per_thread ? size_t(np) * kWfBlock * sizeof(float) : size_t(np) * sizeof(float);
+    // Block rows of successive launches go to consecutive rows of AB.partial;
+    // one k_grad_reduce folds them into grad when the buffer is full and at
+    // the end (instead of one reduction launch per kernel).
+    const uint64_t row_cap = np ? AB.partial_floats / np : 0;
+    uint64_t rows = 0;
+    auto flush = [&] {
+        if (np && rows) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, uint32_t(rows), np,
+                                           grad);
+        rows = 0;
+    };
+    auto rows_for = [&](uint32_t blocks) -> float * {
+        if (rows + blocks > row_cap) flush();
+        float *out = AB.partial + rows * np;
+        rows += blocks;
+        return out;
+    };
     uint32_t qn = uint32_t(n);
     while (qn > 0) {
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
         if (qn <= tail) {
-            const size_t tl = lds;
+            float *part = rows_for(grid.x);
             if (per_thread && S.has_interfaces)
-                hipLaunchKernelGGL((k_adj_tail<true, true>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+                hipLaunchKernelGGL((k_adj_tail<true, true>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, part);
             else if (per_thread)
-                hipLaunchKernelGGL((k_adj_tail<true, false>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+                hipLaunchKernelGGL((k_adj_tail<true, false>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, part);
             else if (S.has_interfaces)
-                hipLaunchKernelGGL((k_adj_tail<false, true>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
+                hipLaunchKernelGGL((k_adj_tail<false, true>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, part);
             else
-                hipLaunchKernelGGL((k_adj_tail<false, false>), grid, dim3(kWfBlock), tl, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, AB.partial);
-            if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, grid.x, np, grad);
+                hipLaunchKernelGGL((k_adj_tail<false, false>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
+                                   uint32_t(cur), qn, npx, sample_begin, part);
             break;
         }
         hipLaunchKernelGGL((k_wf_extend<false, false>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, nullptr);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, 0u);
+        float *part = rows_for(grid.x);
         if (per_thread)
             hipLaunchKernelGGL(k_adj_shade<true>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F, uint32_t(cur),
-                               qn, npx, sample_begin, AB.partial);
+                               qn, npx, sample_begin, part);
         else
             hipLaunchKernelGGL(k_adj_shade<false>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F,
-                               uint32_t(cur), qn, npx, sample_begin, AB.partial);
-        if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, grid.x, np, grad);
+                               uint32_t(cur), qn, npx, sample_begin, part);
         WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, (kQueueStride + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                 stream));
         WF_CHECK(hipStreamSynchronize(stream));
@@ -554,16 +572,16 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
         if (next > qn || sn > qn) throw std::runtime_error("adjoint queue counts out of range");
         if (sn) {
             const dim3 sg((sn + kWfBlock - 1) / kWfBlock);
+            float *spart = rows_for(sg.x);
             if (S.has_interfaces)
-                hipLaunchKernelGGL(k_adj_shadow<true>, sg, dim3(kWfBlock), 0, stream, S, P, G, A, N, F, sn, AB.partial);
+                hipLaunchKernelGGL(k_adj_shadow<true>, sg, dim3(kWfBlock), 0, stream, S, P, G, A, N, F, sn, spart);
             else
-                hipLaunchKernelGGL(k_adj_shadow<false>, sg, dim3(kWfBlock), 0, stream, S, P, G, A, N, F, sn,
-                                   AB.partial);
-            if (np) hipLaunchKernelGGL(k_grad_reduce, dim3(np), dim3(256), 0, stream, AB.partial, sg.x, np, grad);
+                hipLaunchKernelGGL(k_adj_shadow<false>, sg, dim3(kWfBlock), 0, stream, S, P, G, A, N, F, sn, spart);
         }
         cur = 1 - cur;
         qn = next;
     }
+    flush();
     WF_CHECK(hipGetLastError());
 }
 
