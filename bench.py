@@ -65,15 +65,17 @@ def pmc_traffic():
     """HBM bytes per closest-hit launch of the 1080p terrain wavefront frame,
     from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by
     its copy calibration).  None when the summary is absent."""
+    launches, mib = 0, 0.0
     try:
         for line in PMC_SUMMARY.read_text().splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if cells and cells[0] == "void rpd::k_wf_extend<false>":
-                launches, read_mib, write_mib = int(cells[1]), float(cells[4]), float(cells[5])
-                return round((read_mib + write_mib) * 2 ** 20 / launches)
+            # k_wf_extend<false, kGen>: the first bounce's launch and the others
+            if cells and cells[0].startswith("void rpd::k_wf_extend<false"):
+                launches += int(cells[1])
+                mib += float(cells[4]) + float(cells[5])
     except (OSError, ValueError, IndexError):
         return None
-    return None
+    return round(mib * 2 ** 20 / launches) if launches else None
 
 
 def cpu_baseline(synth, width, height, budget):
