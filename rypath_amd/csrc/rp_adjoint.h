@@ -16,10 +16,6 @@
 namespace rpd {
 
 constexpr uint32_t kMaxParams = 256;
-// Parameters up to which every thread of the shading step keeps its own LDS
-// column of sums (np * 128 * 4 B of dynamic LDS); above, a shared row with
-// LDS atomics.
-constexpr uint32_t kPerThreadParams = 96;
 
 struct GradTables {
     const int32_t *refl_base;   // per material: first parameter index, -1 = none
@@ -64,12 +60,12 @@ RP_DEV float conductor_dlogf_dr(const Lane &l, const rp_material &m, F3 wo, F3 w
 struct AdjointHook {
     static constexpr bool active = true;
     const GradTables *G;
-    // LDS accumulator: parameter p at grad[p * stride].  With stride = block
-    // size and grad = this thread's column, every thread owns its sums (no
-    // atomics, bank-conflict free, a fixed summation order); with stride 0
-    // the block shares one row through LDS atomics (many parameters).
+    // The block's LDS row of sums (np floats), added to with LDS atomics.
+    // Measured against one LDS column per thread (np * 128 floats per block,
+    // a fixed summation order): the columns limited the replay's shading
+    // step to 2.5 waves/SIMD by LDS and took 8.47 ms per inverse step
+    // against 8.17 ms with the shared row.
     float *grad;
-    uint32_t stride;
     float a[4];    // dLoss/dL_k
     float L[4];    // primal radiance
     float S[4];    // radiance accumulated so far in the replay
@@ -88,8 +84,7 @@ struct AdjointHook {
     }
     RP_DEV void accumulate(int32_t idx, float v) {
         if (!(idx >= 0 && v != 0.0f && __builtin_isfinite(v))) return;
-        if (stride) grad[uint32_t(idx) * stride] += v;
-        else atomicAdd(&grad[idx], v);
+        atomicAdd(&grad[idx], v);
     }
     RP_DEV void add(const float *v) {
         for (int k = 0; k < 4; ++k) S[k] = S[k] + v[k];
@@ -202,26 +197,19 @@ RP_DEV void adjoint_weights(const Scene &S, const AdjWf &A, uint32_t pixel, cons
     }
 }
 
-// Folds the block's LDS accumulator into row blockIdx.x of `partial`: with
-// kPerThread the [np][BS] columns by a fixed tree, else the shared row.
-template <int BS, bool kPerThread>
-RP_DEV void fold_block_grad(float *sgrad, uint32_t np, float *partial) {
+// Zeroes the block's LDS row of np sums.
+template <int BS> RP_DEV void clear_block_grad(float *sgrad, uint32_t np) {
+    for (uint32_t q = threadIdx.x; q < np; q += BS) sgrad[q] = 0.0f;
     __syncthreads();
-    if (kPerThread) {
-        for (uint32_t h = BS / 2; h > 0; h >>= 1) {
-            if (threadIdx.x < h)
-                for (uint32_t p = 0; p < np; ++p) sgrad[p * BS + threadIdx.x] += sgrad[p * BS + threadIdx.x + h];
-            __syncthreads();
-        }
-        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] = sgrad[p * BS];
-    } else {
-        for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] = sgrad[p];
-    }
+}
+// Copies the block's LDS row into row blockIdx.x of `partial`.
+template <int BS> RP_DEV void fold_block_grad(const float *sgrad, uint32_t np, float *partial) {
+    __syncthreads();
+    for (uint32_t p = threadIdx.x; p < np; p += BS) partial[size_t(blockIdx.x) * np + p] = sgrad[p];
 }
 
 // One replay event for each path of set C[0, qn) (the frame wavefront's
-// shading step plus the hook).  Dynamic LDS: np * BS floats (kPerThread) or
-// np floats.
+// shading step plus the hook).  Dynamic LDS: np floats.
 // Waves per SIMD requested from the compiler for k_adj_shade (register
 // budget 512 / waves); 0 leaves the choice to the compiler.  Left alone it
 // takes 169 VGPRs (2 waves/SIMD); 3 fits it in 168 without spills and
@@ -235,23 +223,16 @@ RP_DEV void fold_block_grad(float *sgrad, uint32_t np, float *partial) {
 #define RP_ADJ_SHADE_ATTR
 #endif
 
-template <bool kPerThread>
 __global__ void __launch_bounds__(kWfBlock) RP_ADJ_SHADE_ATTR k_adj_shade(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
                                                          WfState N, FrameWf F, uint32_t p, uint32_t qn, uint32_t npx,
                                                          uint64_t sample_begin, float *partial) {
     __shared__ AppendLds<kWfBlock, 2> sh;
     extern __shared__ float sgrad[];
     const uint32_t np = G.nparams;
-    if (kPerThread) {
-        for (uint32_t q = 0; q < np; ++q) sgrad[q * kWfBlock + threadIdx.x] = 0.0f;
-    } else {
-        for (uint32_t q = threadIdx.x; q < np; q += kWfBlock) sgrad[q] = 0.0f;
-        __syncthreads();
-    }
+    clear_block_grad<kWfBlock>(sgrad, np);
     AdjointHook hk;
     hk.G = &G;
-    hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;
-    hk.stride = kPerThread ? kWfBlock : 0u;
+    hk.grad = sgrad;
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     bool cont = false, shadow = false;
     PathState ps;
@@ -306,7 +287,7 @@ __global__ void __launch_bounds__(kWfBlock) RP_ADJ_SHADE_ATTR k_adj_shade(Scene 
         A.sh_idx[spos] = make_int4(hk.nee_idx[0], hk.nee_idx[1], hk.nee_idx[2], hk.nee_idx[3]);
         A.sh_dlog[spos] = make_float4(hk.nee_dlog[0], hk.nee_dlog[1], hk.nee_dlog[2], hk.nee_dlog[3]);
     }
-    fold_block_grad<kWfBlock, kPerThread>(sgrad, np, partial);
+    fold_block_grad<kWfBlock>(sgrad, np, partial);
 }
 
 // Shadow rays of one replay bounce: the visible NEE contribution c goes to
@@ -327,7 +308,6 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, 
     AdjointHook hk;
     hk.G = &G;
     hk.grad = srow;
-    hk.stride = 0;
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     if (i < sn) {
         const float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
@@ -367,7 +347,7 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, 
             }
         }
     }
-    fold_block_grad<kWfBlock, false>(srow, np, partial);
+    fold_block_grad<kWfBlock>(srow, np, partial);
 }
 
 // Latency tail of the replay: one thread finishes each remaining path of
@@ -375,7 +355,7 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, 
 // k_adj_shadow in the same order per path (the pending continuation of a
 // vertex is applied once its NEE is in L).  Gradient sums differ from the
 // bounce kernels' only in float summation order.
-template <bool kPerThread, bool kInterfaces>
+template <bool kInterfaces>
 __global__ void __launch_bounds__(kWfBlock) k_adj_tail(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
                                                         uint32_t p, uint32_t qn, uint32_t npx, uint64_t sample_begin,
                                                         float *partial) {
@@ -383,18 +363,12 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_tail(Scene S, PathParams P, Gr
     __shared__ float s_t[kShortStack * kWfBlock];
     extern __shared__ float sgrad[];
     const uint32_t np = G.nparams;
-    if (kPerThread) {
-        for (uint32_t q = 0; q < np; ++q) sgrad[q * kWfBlock + threadIdx.x] = 0.0f;
-    } else {
-        for (uint32_t q = threadIdx.x; q < np; q += kWfBlock) sgrad[q] = 0.0f;
-        __syncthreads();
-    }
+    clear_block_grad<kWfBlock>(sgrad, np);
     Stack st;
     st.init(s_node, s_t, kWfBlock);
     AdjointHook hk;
     hk.G = &G;
-    hk.grad = kPerThread ? sgrad + threadIdx.x : sgrad;
-    hk.stride = kPerThread ? kWfBlock : 0u;
+    hk.grad = sgrad;
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     if (i < qn) {
         PathState ps;
@@ -444,7 +418,7 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_tail(Scene S, PathParams P, Gr
             }
         }
     }
-    fold_block_grad<kWfBlock, kPerThread>(sgrad, np, partial);
+    fold_block_grad<kWfBlock>(sgrad, np, partial);
 }
 
 // Scratch of the wavefront replay (one per device, beside the frame arena).
@@ -518,8 +492,7 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
     A.cdlog[0] = AB.cdlog[0], A.cdlog[1] = AB.cdlog[1];
     A.sh_a = AB.sh_a, A.sh_idx = AB.sh_idx, A.sh_dlog = AB.sh_dlog;
     A.Lp = AB.Lp, A.pstatus = AB.pstatus, A.dl = dl, A.spp = spp;
-    const bool per_thread = np <= kPerThreadParams;
-    const size_t lds = per_thread ? size_t(np) * kWfBlock * sizeof(float) : size_t(np) * sizeof(float);
+    const size_t lds = size_t(np) * sizeof(float);
     // Block rows of successive launches go to consecutive rows of AB.partial;
     // one k_grad_reduce folds them into grad when the buffer is full and at
     // the end (instead of one reduction launch per kernel).
@@ -542,29 +515,19 @@ inline void adjoint_replay(const Scene &S, const PathParams &P, const GradTables
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
         if (qn <= tail) {
             float *part = rows_for(grid.x);
-            if (per_thread && S.has_interfaces)
-                hipLaunchKernelGGL((k_adj_tail<true, true>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, part);
-            else if (per_thread)
-                hipLaunchKernelGGL((k_adj_tail<true, false>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, part);
-            else if (S.has_interfaces)
-                hipLaunchKernelGGL((k_adj_tail<false, true>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, part);
+            if (S.has_interfaces)
+                hipLaunchKernelGGL(k_adj_tail<true>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, uint32_t(cur),
+                                   qn, npx, sample_begin, part);
             else
-                hipLaunchKernelGGL((k_adj_tail<false, false>), grid, dim3(kWfBlock), lds, stream, S, P, G, A, C,
-                                   uint32_t(cur), qn, npx, sample_begin, part);
+                hipLaunchKernelGGL(k_adj_tail<false>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, uint32_t(cur),
+                                   qn, npx, sample_begin, part);
             break;
         }
         hipLaunchKernelGGL((k_wf_extend<false, false>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, nullptr);
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, 0u);
         float *part = rows_for(grid.x);
-        if (per_thread)
-            hipLaunchKernelGGL(k_adj_shade<true>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F, uint32_t(cur),
-                               qn, npx, sample_begin, part);
-        else
-            hipLaunchKernelGGL(k_adj_shade<false>, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F,
-                               uint32_t(cur), qn, npx, sample_begin, part);
+        hipLaunchKernelGGL(k_adj_shade, grid, dim3(kWfBlock), lds, stream, S, P, G, A, C, N, F, uint32_t(cur), qn, npx,
+                           sample_begin, part);
         WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, (kQueueStride + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
                                 stream));
         WF_CHECK(hipStreamSynchronize(stream));
