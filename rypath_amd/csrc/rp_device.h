@@ -182,11 +182,6 @@ RP_DEV uint32_t scramble(uint32_t v, uint32_t dimension, uint32_t mode, uint32_t
     }
     return v;
 }
-RP_DEV uint32_t sobol_word(const Scene &S, uint64_t index, uint32_t dim, uint32_t mode, uint32_t seed_bits) {
-    uint32_t v = 0;
-    for (uint64_t i = index; i; i &= i - 1) v ^= S.sobol_dir[uint32_t(__builtin_ctzll(i)) * 1024u + dim];
-    return scramble(v, dim, mode, seed_bits);
-}
 // Four consecutive dimensions 4*block..4*block+3 with one 16-byte load per index bit.
 RP_DEV uint4 sobol_block(const Scene &S, uint64_t index, uint32_t block, uint32_t mode, uint32_t seed_bits) {
     uint32_t d = 4 * block;

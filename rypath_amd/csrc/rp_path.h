@@ -90,9 +90,13 @@ RP_DEV void path_begin(const Scene &S, const PathParams &P, uint32_t pixel, uint
     uint32_t seed_bits = uint32_t(P.seed);
     path_init(S, P, pixel, sample, ps);
     ps.index = sobol_index(S, px, py, sample);
-    w0 = sobol_word(S, ps.index, 0, RP_RANDOMIZE_NONE, 0);
-    w1 = sobol_word(S, ps.index, 1, RP_RANDOMIZE_NONE, 0);
-    ps.w2 = sobol_word(S, ps.index, 2, P.mode, seed_bits);
+    // dimensions 0-2 from one 16-byte direction load per index bit (as
+    // sobol_block does); the camera words are unscrambled, the wavelength
+    // word takes the frame's randomization
+    const uint4 w = sobol_block(S, ps.index, 0, RP_RANDOMIZE_NONE, 0);
+    w0 = w.x;
+    w1 = w.y;
+    ps.w2 = scramble(w.z, 2, P.mode, seed_bits);
     float ux = camera_uniform(w0, S.m, px), uy = camera_uniform(w1, S.m, py);
     float fx = (float(px) + ux) / float(W), fy = (float(py) + uy) / float(H);
     const rp_camera &cam = S.camera;
