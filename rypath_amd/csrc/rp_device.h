@@ -182,6 +182,26 @@ RP_DEV uint32_t scramble(uint32_t v, uint32_t dimension, uint32_t mode, uint32_t
     }
     return v;
 }
+// The unscrambled words of dimension blocks a and b (4 dimensions each)
+// from one pass over the index bits: both blocks' direction loads are issued
+// together, early in the event.
+RP_DEV void sobol_raw2(const Scene &S, uint64_t index, uint32_t a, uint32_t b, uint4 &va, uint4 &vb) {
+    va = make_uint4(0, 0, 0, 0);
+    vb = make_uint4(0, 0, 0, 0);
+    const uint4 *dir4 = reinterpret_cast<const uint4 *>(S.sobol_dir);
+    for (uint64_t i = index; i; i &= i - 1) {
+        const uint32_t row = uint32_t(__builtin_ctzll(i)) * 1024u;
+        const uint4 x = dir4[(row + 4 * a) >> 2], y = dir4[(row + 4 * b) >> 2];
+        va.x ^= x.x, va.y ^= x.y, va.z ^= x.z, va.w ^= x.w;
+        vb.x ^= y.x, vb.y ^= y.y, vb.z ^= y.z, vb.w ^= y.w;
+    }
+}
+// Scrambles the raw words of dimension block `block`.
+RP_DEV uint4 sobol_scramble4(uint4 v, uint32_t block, uint32_t mode, uint32_t seed_bits) {
+    const uint32_t d = 4 * block;
+    return make_uint4(scramble(v.x, d, mode, seed_bits), scramble(v.y, d + 1, mode, seed_bits),
+                      scramble(v.z, d + 2, mode, seed_bits), scramble(v.w, d + 3, mode, seed_bits));
+}
 // Four consecutive dimensions 4*block..4*block+3 with one 16-byte load per index bit.
 RP_DEV uint4 sobol_block(const Scene &S, uint64_t index, uint32_t block, uint32_t mode, uint32_t seed_bits) {
     uint32_t d = 4 * block;

@@ -166,7 +166,9 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
             ps.crossings = 0;
             if (ps.count == capacity) return false;
             if (ps.count - 2 >= P.emergency) { ps.status = RP_PATH_EMERGENCY_DEPTH; return false; }
-            uint4 dr = sobol_block(S, ps.index, 4 + ps.count - 2, P.mode, seed_bits);
+            uint4 draw, lraw; // scattering block 4 + depth, light block 133 + depth
+            sobol_raw2(S, ps.index, 4 + ps.count - 2, 133 + ps.count - 2, draw, lraw);
+            const uint4 dr = sobol_scramble4(draw, 4 + ps.count - 2, P.mode, seed_bits);
             float direct_density[4], direct_beta[4];
             for (int k = 0; k < 4; ++k) direct_density[k] = ps.density[k];
             bool direct_allowed = hero_continue(ps.beta, ps.eta_scale, rr_enabled, ps.count - 1, sobol_uniform(dr.w));
@@ -181,7 +183,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
             for (int k = 0; k < 4; ++k) ps.prev_pdf[k] = ppdf;
             ps.flags &= ~kFlagPrevDelta;
             ps.prev_p = mp;
-            uint4 lw = sobol_block(S, ps.index, 133 + ps.count - 2, P.mode, seed_bits);
+            const uint4 lw = sobol_scramble4(lraw, 133 + ps.count - 2, P.mode, seed_bits);
             LightSample ls = sample_light(S, mp, wl, sobol_uniform(lw.x), sobol_uniform(lw.y), sobol_uniform(lw.z));
             if (ls.ok) {
                 float ph = hg(dot(wo, ls.wi), mc.g);
@@ -281,7 +283,9 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
     Lane lanes[4];
     for (int k = 0; k < 4; ++k) lanes[k] = prepare_lane(S, mat, wl.lambda[k]);
     bool connectible = !lanes[0].smooth || lanes[0].kind == RP_MATERIAL_DIFFUSE;
-    uint4 dr = sobol_block(S, ps.index, 4 + ps.count - 2, P.mode, seed_bits);
+    uint4 draw, lraw; // scattering block 4 + depth, light block 133 + depth
+    sobol_raw2(S, ps.index, 4 + ps.count - 2, 133 + ps.count - 2, draw, lraw);
+    const uint4 dr = sobol_scramble4(draw, 4 + ps.count - 2, P.mode, seed_bits);
     float direct_density[4], direct_beta[4];
     for (int k = 0; k < 4; ++k) direct_density[k] = ps.density[k];
     bool direct_allowed = hero_continue(ps.beta, ps.eta_scale, rr_enabled, ps.count - 1, sobol_uniform(dr.w));
@@ -333,7 +337,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
         }
     }
     if (ps.status == RP_PATH_SUCCESS && direct_allowed && connectible) {
-        uint4 lw = sobol_block(S, ps.index, 133 + ps.count - 2, P.mode, seed_bits);
+        const uint4 lw = sobol_scramble4(lraw, 133 + ps.count - 2, P.mode, seed_bits);
         LightSample ls = sample_light(S, sp.p, wl, sobol_uniform(lw.x), sobol_uniform(lw.y), sobol_uniform(lw.z));
         if (ls.ok) {
             F3 wil = fr.local(ls.wi);
