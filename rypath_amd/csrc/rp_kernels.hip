@@ -1200,7 +1200,7 @@ void run_pool(rp_batch *b, const rp_batch_config &cfg, uint32_t &graph_launches)
         // every path issued and few live (set 0 after an even number of
         // iterations): the latency tail finishes them
         const uint32_t live = b->host_counters->qn[0];
-        if (b->host_counters->next >= b->total && live <= tail) {
+        if (b->host_counters->next >= b->total && live > 0 && live <= tail) {
             const dim3 tg((live + rpd::kWfBlock - 1) / rpd::kWfBlock);
             if (b->any_interfaces)
                 hipLaunchKernelGGL(rpd::k_pool_tail<true>, tg, dim3(rpd::kWfBlock), 0, b->stream, V, 0u, live);
