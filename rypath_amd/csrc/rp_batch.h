@@ -515,22 +515,7 @@ __global__ void __launch_bounds__(kWfBlock) k_pool_tail(PoolView V, uint32_t p, 
             if (sw & kSidFresh) load_compact(B.S, B.P, V.S[p], i, B.sample_begin, ps);
             else load_state(V.S[p], i, ps);
             const Wavelengths wl = path_wavelengths(B.S, ps.w2);
-            for (;;) {
-                Hit h;
-                traverse<false>(B.S, st, ps.o, ps.d, ps.tmax, h);
-                ++ext;
-                ShadowReq sr;
-                NoHook nh;
-                const bool cont = path_event(B.S, B.P, ps, h, sr, nh) && path_alive(B.S, ps);
-                if (sr.active) {
-                    ++shr;
-                    float tr[4];
-                    const bool vis =
-                        transmittance<kInterfaces>(B.S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, B.P.boundary);
-                    shadow_apply(ps, sr, vis, tr);
-                }
-                if (!cont) break;
-            }
+            run_path<kInterfaces>(B.S, B.P, st, ps, wl, ext, shr);
             float v[3] = {0, 0, 0};
             if (ps.status == RP_PATH_SUCCESS) sensor_xyz(B.S, wl, ps.L, v);
             else failed += 1;

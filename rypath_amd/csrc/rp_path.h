@@ -387,27 +387,38 @@ struct PathResult {
 };
 
 // Megakernel form: the whole path in registers.
+// Runs the path in ps from its current ray to its end: per bounce the
+// closest hit, the event and its shadow ray, in the order the wavefront
+// kernels apply them (the shadow contribution after the event).  Adds the
+// extension and shadow rays traced to ext and shr.  kInterfaces = false when
+// the scene has no null interfaces.
+template <bool kInterfaces>
+RP_DEV void run_path(const Scene &S, const PathParams &P, Stack &st, PathState &ps, const Wavelengths &wl,
+                     uint32_t &ext, uint32_t &shr) {
+    for (;;) {
+        Hit h;
+        traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
+        ++ext;
+        ShadowReq sr;
+        NoHook nh;
+        const bool cont = path_event(S, P, ps, h, sr, nh) && path_alive(S, ps);
+        if (sr.active) {
+            ++shr;
+            float tr[4];
+            const bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
+            shadow_apply(ps, sr, vis, tr);
+        }
+        if (!cont) break;
+    }
+}
+
 RP_DEV void trace_path(const Scene &S, const PathParams &P, Stack &st, uint32_t pixel, uint64_t sample,
                        PathResult &out) {
     PathState ps;
     path_begin(S, P, pixel, sample, ps, out.w0, out.w1);
     out.ext_rays = out.shadow_rays = 0;
     const Wavelengths wl = path_wavelengths(S, ps.w2);
-    while (path_alive(S, ps)) {
-        Hit h;
-        traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
-        ++out.ext_rays;
-        ShadowReq sr;
-        NoHook nh;
-        bool cont = path_event(S, P, ps, h, sr, nh);
-        if (sr.active) {
-            float tr[4];
-            ++out.shadow_rays;
-            bool vis = transmittance(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
-            shadow_apply(ps, sr, vis, tr);
-        }
-        if (!cont) break;
-    }
+    if (path_alive(S, ps)) run_path<true>(S, P, st, ps, wl, out.ext_rays, out.shadow_rays);
     out.status = ps.status;
     for (int k = 0; k < 4; ++k) out.L[k] = ps.L[k];
     out.wl = wl;

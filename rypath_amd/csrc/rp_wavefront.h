@@ -448,22 +448,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_tail(Scene S, PathParams P, WfS
     if (i < qn) {
         PathState ps;
         load_state(C, i, ps);
-        const Wavelengths wl = path_wavelengths(S, ps.w2);
-        for (;;) {
-            Hit h;
-            traverse<false>(S, st, ps.o, ps.d, ps.tmax, h);
-            ++ext;
-            ShadowReq sr;
-            NoHook nh;
-            const bool cont = path_event(S, P, ps, h, sr, nh) && path_alive(S, ps);
-            if (sr.active) {
-                ++shr;
-                float tr[4];
-                const bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
-                shadow_apply(ps, sr, vis, tr);
-            }
-            if (!cont) break;
-        }
+        run_path<kInterfaces>(S, P, st, ps, path_wavelengths(S, ps.w2), ext, shr);
         const uint32_t lane = uint32_t(ps.sample - sample_begin) * npx + ps.pixel;
         F.L_end[lane] = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
         F.end[lane] = make_uint2(ps.status, ps.w2);
