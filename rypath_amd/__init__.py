@@ -293,7 +293,7 @@ FRAME_NO_TAIL = 4
 BATCH_NO_TAIL = 1
 
 
-def frame_config(spp, mode=MODE_WAVEFRONT_SORTED, sample_begin=0, seed=1, randomization=RANDOMIZE_FASTOWEN,
+def frame_config(spp, mode=MODE_WAVEFRONT, sample_begin=0, seed=1, randomization=RANDOMIZE_FASTOWEN,
                  capacity=0, group_size=0, sort_bits=0, flags=0, tail_paths=0):
     c = FrameConfig()
     c.flags, c.tail_paths = flags, tail_paths
@@ -507,7 +507,7 @@ class PbrtScene:
         c._owner = self  # the description points into this object's storage
         return c
 
-    def frame_config(self, mode=MODE_WAVEFRONT_SORTED, sample_begin=0):
+    def frame_config(self, mode=MODE_WAVEFRONT, sample_begin=0):
         i = self.info
         return frame_config(i["spp"], mode=mode, sample_begin=sample_begin, seed=i["seed"],
                             randomization=i["randomization"])
