@@ -74,6 +74,13 @@ typedef struct rp_film_info {
     uint8_t scene_identity[32]; /* SHA-256 of the scene description (zero until bound) */
 } rp_film_info;
 int rp_film_create(uint32_t width, uint32_t height, rp_film **out);
+/* Whether `film` can continue rendering `scene` with this seed and
+ * randomization: its size must equal the scene camera's, and a film that
+ * holds samples must carry the same seed, randomization and scene identity
+ * (the checks of read_checkpoint's compatibility contract,
+ * src/checkpoint.cpp:300-360).  RP_SUCCESS, or RP_INVALID with the reason in
+ * rp_last_error.  rp_film_accumulate applies the same checks. */
+int rp_film_check(rp_scene *scene, const rp_film *film, uint64_t seed, uint32_t randomization);
 int rp_film_accumulate(rp_scene *scene, rp_film *film, const rp_frame_config *config, float *xyz_host,
                        rp_frame_stats *stats);
 int rp_film_resolve(const rp_film *film, float *xyz_host);

@@ -4,7 +4,10 @@ The HIP library rypath_amd/librypath_amd.so is required: importing this
 package raises if it is missing, and every rendering call runs on the GPU
 through the C ABI (there is no CPU fallback).
 """
+import atexit
 import ctypes as C
+import itertools
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -24,6 +27,36 @@ except ImportError:
     pass
 
 _lib = C.CDLL(str(_LIB_PATH))
+
+# Live native handles, destroyed newest first by an exit hook that runs
+# while the module's globals still exist (a batch before its scenes, a
+# frame before the library goes away).  Objects collected earlier release
+# their handle in __del__ and leave the registry.
+_live = {}
+_seq = itertools.count()
+
+
+def _track(obj, destroy):
+    key = next(_seq)
+    obj._reg = key
+    _live[key] = (weakref.ref(obj), destroy)
+
+
+def _release(obj):
+    h = getattr(obj, "_h", None)
+    entry = _live.pop(getattr(obj, "_reg", None), None)
+    if h and entry is not None:
+        obj._h = None
+        getattr(_lib, entry[1])(h)
+
+
+@atexit.register
+def _teardown():
+    for key in sorted(_live, reverse=True):
+        obj = _live[key][0]()
+        if obj is not None:
+            _release(obj)
+    _live.clear()
 
 NONE = 0xFFFFFFFF
 SUCCESS, NOT_BUILT, NO_DEVICE, FAILURE, INVALID, BUSY = range(6)
@@ -198,6 +231,7 @@ class FilmInfo(C.Structure):
 
 
 _sig("rp_film_create", C.c_int, [u32, u32, C.POINTER(C.c_void_p)])
+_sig("rp_film_check", C.c_int, [C.c_void_p, C.c_void_p, u64, u32])
 _sig("rp_film_accumulate", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FrameConfig), C.c_void_p,
                                      C.POINTER(FrameStats)])
 _sig("rp_film_resolve", C.c_int, [C.c_void_p, C.c_void_p])
@@ -221,7 +255,7 @@ EXPORTED = [
     "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
     "rp_pbrt_load", "rp_pbrt_load_string", "rp_pbrt_desc", "rp_pbrt_get_info", "rp_pbrt_destroy",
     "rp_write_exr", "rp_frame_submit", "rp_frame_poll", "rp_frame_wait", "rp_frame_release",
-    "rp_film_create", "rp_film_accumulate", "rp_film_resolve", "rp_film_get_info", "rp_checkpoint_write",
+    "rp_film_create", "rp_film_check", "rp_film_accumulate", "rp_film_resolve", "rp_film_get_info", "rp_checkpoint_write",
     "rp_checkpoint_read", "rp_film_destroy", "rp_scene_identity", "rp_sha256",
 ]
 
@@ -257,13 +291,13 @@ class Synthetic:
         h = C.c_void_p()
         _check("rp_synthetic_create", _lib.rp_synthetic_create(kind, seed, C.byref(o), C.byref(h)))
         self._h = h
+        _track(self, "rp_synthetic_destroy")
         self.desc = _lib.rp_synthetic_desc(h)
         self.desc._owner = self  # the pointer keeps the storage it points into alive
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.rp_synthetic_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None):  # the exit hook has released every handle before module teardown
+            _release(self)
 
     @property
     def d(self):
@@ -312,17 +346,20 @@ class Scene:
         ptr = desc if isinstance(desc, C._Pointer) else C.pointer(desc)
         _check("rp_scene_create", _lib.rp_scene_create(ptr, C.byref(o), C.byref(h)))
         self._h = h
+        _track(self, "rp_scene_destroy")
         cam = ptr.contents.camera
         self.width, self.height = cam.width, cam.height
+        # the parameter values last written through set_params (their order
+        # and values), which the autograd harness compares against
+        self.param_state = None
 
     @property
     def handle(self):
         return self._h
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.rp_scene_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None):  # the exit hook has released every handle before module teardown
+            _release(self)
 
     def info(self):
         i = BuildInfo()
@@ -353,7 +390,9 @@ class Scene:
         arr = params_array(params)
         vals = np.ascontiguousarray(values, dtype=np.float32)
         assert len(vals) == len(arr)
+        self.param_state = None
         _check("rp_set_params", _lib.rp_set_params(self._h, arr, len(arr), vals.ctypes.data_as(C.POINTER(f32))))
+        self.param_state = (tuple(tuple(p) for p in params), vals.tobytes())
 
     def backward(self, cfg, params, dl_dxyz):
         """Path-replay adjoint: d(sum dl_dxyz * XYZ)/d params for the frame `cfg` (rp_backward)."""
@@ -410,6 +449,7 @@ class Frame:
 
     def __init__(self, handle, width, height):
         self._h, self.width, self.height = handle, width, height
+        _track(self, "rp_frame_release")
 
     def poll(self):
         done = u32()
@@ -424,12 +464,11 @@ class Frame:
         return out, {f: getattr(st, f) for f, _ in FrameStats._fields_}
 
     def release(self):
-        if getattr(self, "_h", None):
-            _lib.rp_frame_release(self._h)
-            self._h = None
+        _release(self)
 
     def __del__(self):
-        self.release()
+        if getattr(self, "_h", None):  # the exit hook has released every handle before module teardown
+            _release(self)
 
 
 class Film:
@@ -441,6 +480,7 @@ class Film:
             _check("rp_film_create", _lib.rp_film_create(width, height, C.byref(h)))
             handle = h
         self._h = handle
+        _track(self, "rp_film_destroy")
 
     @classmethod
     def read(cls, path):
@@ -458,6 +498,10 @@ class Film:
         out["scene_identity"] = bytes(i.scene_identity)
         return out
 
+    def check(self, scene, seed, randomization):
+        """Raises unless this film can continue `scene` with (seed, randomization) (rp_film_check)."""
+        _check("rp_film_check", _lib.rp_film_check(scene.handle, self._h, seed, randomization))
+
     def accumulate(self, scene, cfg, film=True):
         i = self.info()
         out = np.zeros((i["height"], i["width"], 3), np.float32) if film else None
@@ -473,9 +517,8 @@ class Film:
         return out
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.rp_film_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None):  # the exit hook has released every handle before module teardown
+            _release(self)
 
 
 def sha256(data: bytes) -> bytes:
@@ -494,6 +537,7 @@ class PbrtScene:
         else:
             _check("rp_pbrt_load", _lib.rp_pbrt_load(str(path).encode(), C.byref(h)))
         self._h = h
+        _track(self, "rp_pbrt_destroy")
         self.desc = _lib.rp_pbrt_desc(h)
         self.desc._owner = self  # the pointer keeps the storage it points into alive
         i = PbrtInfo()
@@ -513,9 +557,8 @@ class PbrtScene:
                             randomization=i["randomization"])
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.rp_pbrt_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None):  # the exit hook has released every handle before module teardown
+            _release(self)
 
 
 def probe_read_bandwidth(bytes_, passes, device=0):
@@ -548,11 +591,11 @@ class Batch:
         h = C.c_void_p()
         _check("rp_batch_create", _lib.rp_batch_create(arr, len(self.items), C.byref(h)))
         self._h = h
+        _track(self, "rp_batch_destroy")
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.rp_batch_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None):  # the exit hook has released every handle before module teardown
+            _release(self)
 
     @staticmethod
     def config(scheduler=SCHED_WAVEFRONT, capacity=0, group_size=0, sort_bits=0, use_graph=1,

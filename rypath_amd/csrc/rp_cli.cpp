@@ -128,31 +128,48 @@ bool ends_with(const std::string &s, const char *suffix) {
 
 int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t seed, uint32_t randomization,
            const std::string &source) {
+    auto t0 = std::chrono::steady_clock::now();
+    // The film accumulates batches of samples; a resumed or checkpointed
+    // render equals the uninterrupted one bit for bit.  A resumed film must
+    // match the scene's size and the sampler before anything renders
+    // (src/render_job.cpp:125-140); its scene identity is checked once the
+    // scene is built.
+    rp_film *film = nullptr;
+    int st = !o.resume.empty() ? rp_checkpoint_read(o.resume.c_str(), &film)
+                               : rp_film_create(desc.camera.width, desc.camera.height, &film);
+    if (st != RP_SUCCESS) return status_exit(st, o.resume.empty() ? "film" : "resume");
+    rp_film_info fi{};
+    rp_film_get_info(film, &fi);
+    std::string mismatch;
+    if (fi.width != desc.camera.width || fi.height != desc.camera.height)
+        mismatch = "the checkpoint's film is " + std::to_string(fi.width) + "x" + std::to_string(fi.height) +
+                   ", the scene renders " + std::to_string(desc.camera.width) + "x" +
+                   std::to_string(desc.camera.height);
+    else if (fi.samples > spp)
+        mismatch = "the checkpoint holds more samples than requested";
+    else if (fi.samples && (fi.seed != seed || fi.randomization != randomization))
+        mismatch = "the checkpoint was rendered with another seed or randomization";
+    if (!mismatch.empty()) {
+        rp_film_destroy(film);
+        return fail("checkpoint does not match the scene or sampler: " + mismatch);
+    }
     if (rp_device_count() <= 0) {
+        rp_film_destroy(film);
         std::fprintf(stderr, "rypath: no HIP device is available\n");
         return 2;
     }
-    auto t0 = std::chrono::steady_clock::now();
     rp_scene *scene = nullptr;
-    int st = rp_scene_create(&desc, nullptr, &scene);
-    if (st != RP_SUCCESS) return status_exit(st, "scene build");
+    st = rp_scene_create(&desc, nullptr, &scene);
+    if (st != RP_SUCCESS) {
+        rp_film_destroy(film);
+        return status_exit(st, "scene build");
+    }
     rp_build_info info{};
     rp_scene_info(scene, &info);
-    // The film accumulates batches of samples; a resumed or checkpointed
-    // render equals the uninterrupted one bit for bit.
-    rp_film *film = nullptr;
-    if (!o.resume.empty()) st = rp_checkpoint_read(o.resume.c_str(), &film);
-    else st = rp_film_create(desc.camera.width, desc.camera.height, &film);
-    if (st != RP_SUCCESS) {
-        rp_scene_destroy(scene);
-        return status_exit(st, o.resume.empty() ? "film" : "resume");
-    }
-    rp_film_info fi{};
-    rp_film_get_info(film, &fi);
-    if (fi.samples > spp) {
+    if ((st = rp_film_check(scene, film, seed, randomization)) != RP_SUCCESS) {
         rp_film_destroy(film);
         rp_scene_destroy(scene);
-        return fail("the checkpoint holds more samples than requested");
+        return status_exit(st, "resume");
     }
     const uint64_t batch = o.batch ? o.batch : (o.checkpoint.empty() ? spp : std::max<uint64_t>(1, spp / 16));
     rp_frame_stats total{};
@@ -189,7 +206,8 @@ int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t s
         }
     }
     rp_scene_destroy(scene);
-    std::vector<float> xyz(size_t(desc.camera.width) * desc.camera.height * 3);
+    rp_film_get_info(film, &fi);
+    std::vector<float> xyz(size_t(fi.width) * fi.height * 3);
     st = rp_film_resolve(film, xyz.data());
     rp_film_destroy(film);
     if (st != RP_SUCCESS) return status_exit(st, "resolve");
@@ -202,7 +220,7 @@ int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t s
     }
     const char *keys[] = {"rypath.source", "rypath.samples", "rypath.seed", "rypath.checkpointIdentity"};
     const char *vals[] = {source.c_str(), spp_s.c_str(), seed_s.c_str(), ident.c_str()};
-    st = rp_write_exr(o.output.c_str(), desc.camera.width, desc.camera.height, xyz.data(), o.exposure, o.half ? 1 : 0,
+    st = rp_write_exr(o.output.c_str(), fi.width, fi.height, xyz.data(), o.exposure, o.half ? 1 : 0,
                       keys, vals, 4);
     if (st != RP_SUCCESS) return status_exit(st, "write");
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -399,6 +399,7 @@ struct PrimalTag {
     uint64_t scene = 0, params_version = 0, seed = 0, sample_begin = 0, paths = 0;
     uint32_t randomization = 0, spp = 0, roulette_off = 0, npx = 0;
     uint64_t rays[2] = {0, 0};
+    uint64_t failed = 0; // paths whose status is not success
     bool same(const rp_scene *s, const rp_frame_config &c, uint32_t px) const {
         return valid && scene == s->serial && params_version == s->params_version && seed == c.seed &&
                sample_begin == c.sample_begin && spp == c.spp && randomization == c.randomization &&
@@ -418,6 +419,7 @@ struct PrimalTag {
         paths = uint64_t(px) * c.spp;
         rays[0] = r[0];
         rays[1] = r[1];
+        failed = 0;
     }
 };
 struct FrameArena {
@@ -468,9 +470,8 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
                            6 * uint64_t(npx), s->film.p);
     }
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, s->counters.p, 8u);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    rpd::EventPair ev; // destroyed on every exit, exceptions included
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     HIP_CHECK(hipEventRecord(e0, s->stream));
     double trace_seconds = 0;
     uint64_t launches = 0, wf_rays[2] = {0, 0};
@@ -516,12 +517,11 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     HIP_CHECK(hipEventSynchronize(e1));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     HIP_CHECK(hipGetLastError());
+    unsigned long long c[8] = {};
+    if (stats || (arena && arena->primal.valid)) read_counters(s->counters.p, 8, c);
+    if (arena && arena->primal.valid) arena->primal.failed = c[2]; // reported again when an adjoint reuses it
     if (stats) {
-        unsigned long long c[8];
-        read_counters(s->counters.p, 8, c);
         uint64_t tail[2] = {0, 0};
         if (arena) arena->fb.tail_rays(tail);
         stats->seconds = ms * 1e-3;
@@ -536,6 +536,39 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         stats->triangle_tests = c[4];
     }
     return ms * 1e-3;
+}
+
+// SHA-256 identity of the scene's current description; the caller holds
+// s->exec (rp_set_params rewrites the hashed materials, spectra and media
+// under the same lock and drops the cached value).
+std::array<uint8_t, 32> identity_locked(rp_scene *s) {
+    if (!s->has_identity) {
+        s->identity = scene_identity(s->host.desc);
+        s->has_identity = true;
+    }
+    return s->identity;
+}
+
+// Whether film f can continue scene s with (seed, randomization): the
+// resume contract of src/render_job.cpp:125-140 (size, sampler, identity).
+// The caller holds s->exec.
+bool film_compatible(rp_scene *s, const rp_film *f, uint64_t seed, uint32_t randomization, const char *who) {
+    const rp_camera &cam = s->host.desc.camera;
+    if (f->width != cam.width || f->height != cam.height) {
+        set_error(std::string(who) + ": the film is " + std::to_string(f->width) + "x" + std::to_string(f->height) +
+                  ", the scene's camera " + std::to_string(cam.width) + "x" + std::to_string(cam.height));
+        return false;
+    }
+    if (f->samples == 0) return true;
+    if (seed != f->seed || randomization != f->randomization) {
+        set_error(std::string(who) + ": seed and randomization must match the film's");
+        return false;
+    }
+    if (identity_locked(s) != f->identity) {
+        set_error(std::string(who) + ": the film belongs to another scene (identity mismatch)");
+        return false;
+    }
+    return true;
 }
 
 } // namespace
@@ -771,12 +804,19 @@ int rp_scene_identity(rp_scene *s, uint8_t identity[32]) {
         set_error("rp_scene_identity: null argument");
         return RP_INVALID;
     }
-    if (!s->has_identity) {
-        s->identity = scene_identity(s->host.desc);
-        s->has_identity = true;
-    }
-    std::memcpy(identity, s->identity.data(), 32);
+    std::lock_guard<std::mutex> lock(s->exec);
+    const auto id = identity_locked(s);
+    std::memcpy(identity, id.data(), 32);
     return RP_SUCCESS;
+}
+
+int rp_film_check(rp_scene *s, const rp_film *f, uint64_t seed, uint32_t randomization) {
+    if (!s || !f) {
+        set_error("rp_film_check: null argument");
+        return RP_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(s->exec);
+    return film_compatible(s, f, seed, randomization, "rp_film_check") ? RP_SUCCESS : RP_INVALID;
 }
 
 int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, float *xyz_host, rp_frame_stats *stats) {
@@ -785,28 +825,15 @@ int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, floa
         return RP_INVALID;
     }
     const rp_camera &cam = s->host.desc.camera;
-    if (f->width != cam.width || f->height != cam.height) {
-        set_error("rp_film_accumulate: film size differs from the scene's camera");
-        return RP_INVALID;
-    }
-    if (cfg->sample_begin != f->samples) {
-        set_error("rp_film_accumulate: sample_begin must equal the film's sample count " + std::to_string(f->samples));
-        return RP_INVALID;
-    }
-    uint8_t id[32];
-    rp_scene_identity(s, id);
-    if (f->samples) {
-        if (cfg->seed != f->seed || cfg->randomization != f->randomization) {
-            set_error("rp_film_accumulate: seed and randomization must match the film's");
-            return RP_INVALID;
-        }
-        if (std::memcmp(id, f->identity.data(), 32) != 0) {
-            set_error("rp_film_accumulate: the film belongs to another scene (identity mismatch)");
-            return RP_INVALID;
-        }
-    }
     try {
         std::lock_guard<std::mutex> lock(s->exec);
+        if (!film_compatible(s, f, cfg->seed, cfg->randomization, "rp_film_accumulate")) return RP_INVALID;
+        if (cfg->sample_begin != f->samples) {
+            set_error("rp_film_accumulate: sample_begin must equal the film's sample count " +
+                      std::to_string(f->samples));
+            return RP_INVALID;
+        }
+        const std::array<uint8_t, 32> id = identity_locked(s);
         HIP_CHECK(hipSetDevice(s->device));
         if (s->device_name.empty()) {
             hipDeviceProp_t prop{};
@@ -826,7 +853,7 @@ int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, floa
         f->samples = carry.samples;
         f->seed = cfg->seed;
         f->randomization = cfg->randomization;
-        std::memcpy(f->identity.data(), id, 32);
+        f->identity = id;
         if (!f->segments.empty() && f->segments.back().device == s->device_name && f->segments.back().end == begin)
             f->segments.back().end = f->samples;
         else
@@ -937,9 +964,8 @@ static int trace_common(rp_scene *s, const rp_ray *rays, uint64_t count, void *o
         std::lock_guard<std::mutex> lock(s->exec);
         HIP_CHECK(hipSetDevice(s->device));
         s->rays.upload(rays, count);
-        hipEvent_t e0, e1;
-        HIP_CHECK(hipEventCreate(&e0));
-        HIP_CHECK(hipEventCreate(&e1));
+        rpd::EventPair ev; // destroyed on every exit, exceptions included
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         HIP_CHECK(hipEventRecord(e0, s->stream));
         dim3 grid(uint32_t((count + kBlock - 1) / kBlock));
         if (any) {
@@ -954,8 +980,6 @@ static int trace_common(rp_scene *s, const rp_ray *rays, uint64_t count, void *o
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         if (seconds) *seconds = ms * 1e-3;
         if (any) HIP_CHECK(hipMemcpy(out, s->occluded.p, count, hipMemcpyDeviceToHost));
         else HIP_CHECK(hipMemcpy(out, s->hits.p, count * sizeof(rp_hit), hipMemcpyDeviceToHost));
@@ -1020,9 +1044,8 @@ int rp_probe_read_bandwidth(int device, uint64_t bytes, uint32_t passes, double 
         DevBuf<float> out;
         out.alloc(size_t(grid.x) * block.x);
         hipLaunchKernelGGL(k_probe_read, grid, block, 0, st, buf.p, bytes / 16, 1u, out.p); // warm-up
-        hipEvent_t e0, e1;
-        HIP_CHECK(hipEventCreate(&e0));
-        HIP_CHECK(hipEventCreate(&e1));
+        rpd::EventPair ev; // destroyed on every exit, exceptions included
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         HIP_CHECK(hipEventRecord(e0, st));
         hipLaunchKernelGGL(k_probe_read, grid, block, 0, st, buf.p, bytes / 16, passes, out.p);
         HIP_CHECK(hipEventRecord(e1, st));
@@ -1030,8 +1053,6 @@ int rp_probe_read_bandwidth(int device, uint64_t bytes, uint32_t passes, double 
         HIP_CHECK(hipGetLastError());
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         *gbytes_per_second = double(bytes) * passes / (double(ms) * 1e-3) / 1e9;
         return RP_SUCCESS;
     } catch (const std::exception &e) {
@@ -1226,9 +1247,8 @@ double batch_render_impl(rp_batch *b, const rp_batch_config &cfg, float **xyz_ho
     if (cfg.scheduler > RP_SCHED_WAVEFRONT) throw std::runtime_error("unknown scheduler");
     for (auto &bs : b->host_scenes) bs.P.mode = cfg.randomization;
     b->scenes.upload(b->host_scenes);
-    hipEvent_t e0, e1;
-    HIP_CHECK(hipEventCreate(&e0));
-    HIP_CHECK(hipEventCreate(&e1));
+    rpd::EventPair ev; // destroyed on every exit, exceptions included
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     HIP_CHECK(hipEventRecord(e0, b->stream));
     uint32_t launches = 0;
     unsigned long long rays[3] = {0, 0, 0}, mixed = 0;
@@ -1266,8 +1286,6 @@ double batch_render_impl(rp_batch *b, const rp_batch_config &cfg, float **xyz_ho
     HIP_CHECK(hipGetLastError());
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (cfg.scheduler == RP_SCHED_PERSISTENT) {
         unsigned long long w[4];
         read_counters(b->work.p, 4, w);
@@ -1518,6 +1536,7 @@ int rp_set_params(rp_scene *s, const rp_param *params, uint32_t count, const flo
     try {
         std::lock_guard<std::mutex> lock(s->exec);
         ++s->params_version;
+        s->has_identity = false; // the identity hashes the values written below
         for (uint32_t i = 0; i < count; ++i) set_param(s, params[i], values[i]);
         HIP_CHECK(hipSetDevice(s->device));
         s->spectra.upload(s->host.spectra);
@@ -1561,9 +1580,8 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
         HIP_CHECK(hipMemsetAsync(g.p, 0, std::max<uint32_t>(count, 1) * sizeof(float), s->stream));
         rpd::PathParams P = path_params(cfg->seed, cfg->randomization, 0, 0, cfg->flags);
         hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, s->counters.p, 8u);
-        hipEvent_t e0, e1;
-        HIP_CHECK(hipEventCreate(&e0));
-        HIP_CHECK(hipEventCreate(&e1));
+        rpd::EventPair ev; // destroyed on every exit, exceptions included
+        hipEvent_t e0 = ev.a, e1 = ev.b;
         HIP_CHECK(hipEventRecord(e0, s->stream));
         const uint64_t total = uint64_t(npx) * cfg->spp;
         const uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t(1) << 26) / npx); // whole samples per launch
@@ -1602,12 +1620,12 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
         HIP_CHECK(hipEventSynchronize(e1));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         HIP_CHECK(hipMemcpy(grad, g.p, count * sizeof(float), hipMemcpyDeviceToHost));
+        unsigned long long c[8];
+        read_counters(s->counters.p, 8, c);
+        if (reuse) c[2] = arena.primal.failed; // the counters were cleared; the primal's frame counted them
+        else if (one_chunk) arena.primal.failed = c[2];
         if (stats) {
-            unsigned long long c[8];
-            read_counters(s->counters.p, 8, c);
             uint64_t tail[2];
             arena.fb.tail_rays(tail); // the primal's (the replay's tail does not count)
             std::memset(stats, 0, sizeof *stats);

@@ -506,6 +506,24 @@ __global__ void __launch_bounds__(256) k_wf_finish(Scene S, FrameWf F, uint64_t 
             throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(err_));                     \
     } while (0)
 
+// Two timing events owned by a scope: destroyed however the scope exits.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        WF_CHECK(hipEventCreate(&a));
+        if (hipError_t e = hipEventCreate(&b); e != hipSuccess) {
+            (void)hipEventDestroy(a);
+            throw std::runtime_error(std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+};
+
 template <typename T> inline void wf_alloc(T *&p, uint64_t n) { WF_CHECK(hipMalloc(reinterpret_cast<void **>(&p), n * sizeof(T))); }
 
 // Owner of the frame wavefront's buffers: two compacted state sets, the
@@ -597,9 +615,8 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         hipLaunchKernelGGL(k_wf_generate<false>, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx,
                            sample_begin, n, FB.state[cur]);
     uint32_t qn = uint32_t(n);
-    hipEvent_t e0, e1;
-    WF_CHECK(hipEventCreate(&e0));
-    WF_CHECK(hipEventCreate(&e1));
+    EventPair ev; // destroyed on every exit, exceptions included
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     double trace = 0;
     while (qn > 0) {
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
@@ -647,8 +664,6 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         rays[0] += qn;
         rays[1] += sn;
         if (next > qn || sn > qn) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
             throw std::runtime_error("wavefront queue counts out of range");
         }
         if (sn) {
@@ -661,8 +676,6 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         cur = 1 - cur;
         qn = next;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     hipLaunchKernelGGL(k_wf_finish, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, F, n, xyz, counters);
     WF_CHECK(hipGetLastError());
     return trace;

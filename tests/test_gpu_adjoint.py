@@ -204,3 +204,25 @@ def test_autograd_harness(rp, inverse):
     g, _ = scene.backward(cfg, params, dl)
     assert _same_gradient(g, x.grad.numpy())
     assert np.isfinite(x.grad.numpy()).all() and np.abs(x.grad.numpy()).sum() > 0
+
+
+def test_autograd_backward_uses_its_forward_values(rp, inverse):
+    """Two forwards, then the first one's backward: the gradient is taken at
+    the first forward's parameter values (the harness writes them back)."""
+    import torch
+
+    from rypath_amd.autograd import render
+
+    synth, scene = inverse
+    params, values = rp.inverse_params(synth)
+    cfg = rp.frame_config(8, mode=rp.MODE_WAVEFRONT, seed=19)
+    target = torch.zeros((32, 32, 3), device="cuda")
+    x1 = torch.tensor(values, requires_grad=True)
+    img1 = render(x1, scene, params, cfg)
+    x2 = torch.tensor(values * np.float32(0.7), requires_grad=True)
+    render(x2, scene, params, cfg)
+    ((img1 - target) ** 2).mean().backward()
+    dl = (2 * (img1 - target) / img1.numel()).detach().cpu().numpy()
+    scene.set_params(params, values)
+    g, _ = scene.backward(cfg, params, dl)
+    assert _same_gradient(g, x1.grad.numpy())

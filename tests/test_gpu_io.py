@@ -148,3 +148,27 @@ def test_cli_checkpoint_and_resume(gpu_available, tmp_path):
     b, e = exr_util.read_rgb(resumed)
     assert np.array_equal(a, b)
     assert len(e["strings"]["rypath.checkpointIdentity"]) == 64
+
+
+def test_film_identity_follows_parameter_changes(rp, gpu_available):
+    """rp_set_params rewrites values the scene identity hashes: a film
+    accumulated before the change cannot be continued after it, whichever
+    call computed the identity first."""
+    s = rp.Synthetic(rp.SYNTH_INVERSE, seed=2, width=16, height=16)
+    scene = rp.Scene(s.desc)
+    params, values = rp.inverse_params(s)
+    film = rp.Film(16, 16)
+    film.accumulate(scene, rp.frame_config(2, seed=4), film=False)
+    before = scene.identity()
+    film.check(scene, 4, rp.RANDOMIZE_FASTOWEN)
+    scene.set_params(params, values * np.float32(0.9))
+    assert scene.identity() != before
+    with pytest.raises(rp.RypathError) as e:
+        film.check(scene, 4, rp.RANDOMIZE_FASTOWEN)
+    assert "another scene" in str(e.value)
+    with pytest.raises(rp.RypathError):
+        film.accumulate(scene, rp.frame_config(2, sample_begin=2, seed=4), film=False)
+    scene.set_params(params, values)
+    assert scene.identity() == before
+    film.accumulate(scene, rp.frame_config(2, sample_begin=2, seed=4), film=False)
+    assert film.info()["samples"] == 4

@@ -366,3 +366,24 @@ def test_cli_without_device_exits_2(tmp_path):
     r = run_cli("--render-demo", "--output", tmp_path / "x.exr")
     assert r.returncode == 2
     assert not (tmp_path / "x.exr").exists()
+
+
+def test_cli_resume_rejects_a_checkpoint_of_another_film(tmp_path):
+    """A complete checkpoint (as many samples as requested) of another film
+    size or sampler is rejected before anything renders or is written; the
+    demo renders 256x256 by default."""
+    from tests.test_checkpoint import encode, sample_film
+
+    for w, h, samples, seed, rnd, note in [(5, 3, 4, 1, 2, "the checkpoint's film is 5x3"),
+                                           (256, 256, 4, 7, 2, "another seed or randomization"),
+                                           (256, 256, 9, 1, 2, "more samples than requested")]:
+        _, _, _, _, _, ident, _, _ = sample_film()
+        planes = np.ones((6, w * h)) * samples
+        planes[1::2] = 0
+        ckpt = tmp_path / f"c{w}_{seed}_{samples}.ckpt"
+        ckpt.write_bytes(encode(w, h, samples, seed, rnd, ident, [(0, samples, "MI355X gfx950")], planes))
+        out = tmp_path / "x.exr"
+        r = run_cli("--render-demo", "--output", out, "--samples", 4, "--seed", 1, "--resume", ckpt)
+        assert r.returncode == 1, r.stderr
+        assert "checkpoint does not match" in r.stderr and note in r.stderr
+        assert not out.exists()
