@@ -565,6 +565,26 @@ static void dirac_lane(const Lane &l, V3 wo, V3 wi, float uc, float u0, float u1
     }
 }
 
+// ========================================================= planted slips
+// Test switch (or_plant_slip): each value plants one named estimator mistake,
+// so that tests can show that their known-answer gates detect it.  Zero (the
+// default) plants nothing.  The GPU renderer has no such switch.
+enum Slip : int {
+    SLIP_NONE = 0,
+    SLIP_EMISSION_UNWEIGHTED = 1,    // emission reached by BSDF sampling added without its MIS weight
+    SLIP_LIGHT_PMF_DROPPED = 2,      // area-light solid-angle pdf without the light-selection pmf
+    SLIP_ROULETTE_UNCOMPENSATED = 3, // survivors not divided by the survival probability
+    SLIP_PHASE_SIGN = 4,             // phase function of medium NEE evaluated at -cos
+    SLIP_NEE_NO_TRANSMITTANCE = 5,   // medium NEE without the shadow ray's transmittance
+    SLIP_ENV_PDF_HEMISPHERE = 6,     // environment MIS pdf 1/(2 pi) instead of 1/(4 pi)
+    SLIP_SPOT_NO_FALLOFF = 7,        // spot light without its cone falloff
+    SLIP_NEE_NO_COSINE = 8,          // surface NEE without the receiver cosine
+    SLIP_HERO_TRANSMITTANCE = 9,     // every lane's pass-through transmittance at the hero's sigma_t
+    SLIP_CONTINUATION_NO_COSINE = 10 // continuation weight f/pdf without |cos|
+};
+static std::atomic<int> g_slip{SLIP_NONE};
+static bool slip(Slip v) { return g_slip.load(std::memory_order_relaxed) == v; }
+
 // ================================================================ lights
 static float avg_spectrum(const Scene &s, const rp_spectrum &sp) {
     double acc = 0;
@@ -699,7 +719,7 @@ static bool hero_continue(float beta[4], const float eta_scale[4], bool enabled,
     for (int k = 0; k < 4; ++k) {
         surv[k] = beta[k];
         if (std::bit_cast<uint32_t>(beta[k]) == 0) continue;
-        surv[k] = beta[k] / survival;
+        surv[k] = slip(SLIP_ROULETTE_UNCOMPENSATED) ? beta[k] : beta[k] / survival;
         if (!positive_normal(surv[k])) return true;
     }
     if (u < q) return false;
@@ -846,7 +866,8 @@ static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, flo
         const auto &em = s.emissions[s.objects[s.tri_object[L.index]].emission];
         if (!em.two_sided && !(cos_l > 0)) return ls;
         if (cos_l == 0.0f) return ls;
-        ls.pdf = ((L.pmf * L.inv_area) * d2) / std::fabs(cos_l);
+        float pmf = slip(SLIP_LIGHT_PMF_DROPPED) ? 1.0f : L.pmf;
+        ls.pdf = ((pmf * L.inv_area) * d2) / std::fabs(cos_l);
         for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(em.radiance, wl.lambda[k], s.spectra.data());
         ls.ok = ls.pdf > 0 && std::isfinite(ls.pdf);
         return ls;
@@ -863,7 +884,7 @@ static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, flo
         float cos_t = dot(-ls.wi, axis);
         float t = (cos_t - sp.cos_falloff_end) / (sp.cos_falloff_start - sp.cos_falloff_end);
         t = std::min(std::max(t, 0.0f), 1.0f);
-        float fall = t * t * (3.0f - 2.0f * t);
+        float fall = slip(SLIP_SPOT_NO_FALLOFF) ? 1.0f : t * t * (3.0f - 2.0f * t);
         for (int k = 0; k < 4; ++k)
             ls.le[k] = spectrum_eval(sp.intensity, wl.lambda[k], s.spectra.data()) * fall / d2;
         ls.p = pos;
@@ -889,7 +910,7 @@ static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, flo
 
 static float env_pdf(const Scene &s) {
     if (!s.d.has_environment || s.lights.empty()) return 0.0f;
-    return s.lights.back().pmf * kInv4Pi;
+    return s.lights.back().pmf * (slip(SLIP_ENV_PDF_HEMISPHERE) ? 2.0f * kInv4Pi : kInv4Pi);
 }
 
 // Forward-mode derivative of the detached estimator (independent of the
@@ -1133,7 +1154,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 auto lw = draws(c, out.index, 133 + count - 2);
                 LightSample ls = sample_light(s, mp, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]), sobol_uniform(lw[2]));
                 if (ls.ok) {
-                    float ph = hg(dot(wo, ls.wi), mc.g);
+                    float ph = hg(slip(SLIP_PHASE_SIGN) ? -dot(wo, ls.wi) : dot(wo, ls.wi), mc.g);
                     float val[4];
                     bool any = false;
                     for (int k = 0; k < 4; ++k) {
@@ -1155,7 +1176,8 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                         ++out.shadow_rays;
                         float mlen = 0.0f;
                         if (transmittance(c, mp, sd, tm, medium, wl, tr, cr, &mlen, mlen_medium)) {
-                            for (int k = 0; k < 4; ++k) val[k] = val[k] * tr[k];
+                            if (!slip(SLIP_NEE_NO_TRANSMITTANCE))
+                                for (int k = 0; k < 4; ++k) val[k] = val[k] * tr[k];
                             if (!ls.delta) {
                                 float lp[4] = {ls.pdf, ls.pdf, ls.pdf, ls.pdf};
                                 float pp[4] = {ph, ph, ph, ph};
@@ -1178,7 +1200,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 float num[4], pdf[4];
                 for (int k = 0; k < 4; ++k) {
                     float tr = exp_f(-mc.st[k] * seg);
-                    num[k] = tr;
+                    num[k] = slip(SLIP_HERO_TRANSMITTANCE) ? exp_f(-mc.st[0] * seg) : tr;
                     pdf[k] = tr;
                 }
                 if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
@@ -1226,7 +1248,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
             if (em.two_sided || cos_l > 0) {
                 float val[4];
                 for (int k = 0; k < 4; ++k) val[k] = beta[k] * spectrum_eval(em.radiance, wl.lambda[k], s.spectra.data());
-                if (!(count == 2 || prev_delta)) {
+                if (!(count == 2 || prev_delta) && !slip(SLIP_EMISSION_UNWEIGHTED)) {
                     const Light &L = s.lights[s.tri_light[h.tri]];
                     V3 dv = sp.p - prev_p;
                     float d2 = dot(dv, dv);
@@ -1273,7 +1295,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 } else {
                     for (int k = 0; k < 4; ++k) {
                         Eval e = eval_lane(lanes[k], wol, hs.wi);
-                        num[k] = e.f * std::fabs(hs.wi.z);
+                        num[k] = slip(SLIP_CONTINUATION_NO_COSINE) ? e.f : e.f * std::fabs(hs.wi.z);
                         pdf[k] = e.pdf;
                         etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
                     }
@@ -1305,7 +1327,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
             LightSample ls = sample_light(s, sp.p, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]), sobol_uniform(lw[2]));
             if (ls.ok) {
                 V3 wil = fr.local(ls.wi);
-                float cosv = std::fabs(dot(sp.n, ls.wi));
+                float cosv = slip(SLIP_NEE_NO_COSINE) ? 1.0f : std::fabs(dot(sp.n, ls.wi));
                 float val[4], bp[4];
                 bool any = false;
                 for (int k = 0; k < 4; ++k) {
@@ -1374,6 +1396,7 @@ int or_init(const char *sobol_path, const char *cie_path) {
 }
 
 void or_set_roulette(int enabled) { g_roulette.store(enabled != 0); }
+void or_plant_slip(int slip_id) { g_slip.store(slip_id); }
 
 uint64_t or_sobol_index(uint32_t m, uint32_t x, uint32_t y, uint64_t sample) { return sobol_index(m, x, y, sample); }
 uint32_t or_sobol_word(uint64_t index, uint32_t dim, uint32_t mode, uint32_t seed) {

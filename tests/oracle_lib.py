@@ -3,6 +3,7 @@
 Test infrastructure: the oracle is the CPU restatement of the reference
 transport used as the parity checker and as the timed CPU baseline.
 """
+import contextlib
 import ctypes as C
 from pathlib import Path
 
@@ -56,10 +57,32 @@ class Oracle:
             "or_trace_closest": (None, [C.c_void_p, C.c_void_p, u64, C.c_void_p, u32]),
             "or_sample": (C.c_int, [C.c_void_p, C.POINTER(rp.Request), C.c_void_p, u32]),
             "or_render_frame": (C.c_int, [C.c_void_p, u64, u32, u64, u32, C.c_void_p, u32, C.POINTER(u64)]),
+            "or_set_roulette": (None, [C.c_int]),
+            "or_plant_slip": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype, fn.argtypes = res, args
+
+    # --- test switches
+    @contextlib.contextmanager
+    def roulette(self, enabled):
+        """Run the block with the oracle's roulette on or off."""
+        self.L.or_set_roulette(1 if enabled else 0)
+        try:
+            yield
+        finally:
+            self.L.or_set_roulette(1)
+
+    @contextlib.contextmanager
+    def planted_slip(self, slip_id):
+        """Run the block with one named estimator mistake planted in the oracle
+        (the SLIP_* values of or_transport.cpp)."""
+        self.L.or_plant_slip(slip_id)
+        try:
+            yield
+        finally:
+            self.L.or_plant_slip(0)
 
     # --- sampling probes
     def sobol_index(self, m, x, y, s):
