@@ -300,16 +300,10 @@ def batch_workload(args):
 
     import rypath_amd as rp
 
-    rng = np.random.default_rng(2026)
-    synths, items = [], []
     t0 = time.time()
-    for i in range(args.batch_scenes):
-        tris = int(np.exp(rng.uniform(np.log(5e3), np.log(5e5))))
-        res = int(rng.choice([128, 256, 512, 1024]))
-        spp = int(rng.choice([4, 16, 64]))
-        s = rp.Synthetic(rp.SYNTH_BATCH, seed=1000 + i, width=res, height=res, target_triangles=tris)
-        synths.append(s)
-        items.append((rp.Scene(s.desc), spp, 1 + i))
+    mix = rp.batch_mix(args.batch_scenes)
+    synths = [s for s, _, _ in mix]
+    items = [(rp.Scene(s.desc), spp, seed) for s, spp, seed in mix]
     build = time.time() - t0
     total = sum(sc.width * sc.height * spp for sc, spp, _ in items)
     tris = sum(s.d.triangle_count for s in synths)

@@ -312,6 +312,23 @@ class Synthetic:
         return pos, idx
 
 
+def batch_mix(count=64, seed=2026, resolution_divisor=1):
+    """The uneven batch workload: `count` seeded scenes built like the large
+    terrain frame, triangle counts log-uniform in [5k, 500k], square
+    resolutions drawn from {128, 256, 512, 1024} (each divided by
+    `resolution_divisor`), samples per pixel drawn from {4, 16, 64}, depth 8.
+    Returns a list of (Synthetic, spp, sample seed), in draw order."""
+    rng = np.random.default_rng(seed)
+    mix = []
+    for i in range(count):
+        tris = int(np.exp(rng.uniform(np.log(5e3), np.log(5e5))))
+        res = max(1, int(rng.choice([128, 256, 512, 1024])) // resolution_divisor)
+        spp = int(rng.choice([4, 16, 64]))
+        synth = Synthetic(SYNTH_BATCH, seed=1000 + i, width=res, height=res, target_triangles=tris)
+        mix.append((synth, spp, 1 + i))
+    return mix
+
+
 def request(seed=1, pixel_begin=0, pixel_count=1, sample_begin=0, sample_count=1,
             randomization=RANDOMIZE_FASTOWEN, emergency_limit=0, boundary_limit=0):
     r = Request()

@@ -82,3 +82,46 @@ def test_pool_tail_keeps_ray_counts(rp, uneven):
         _, st = batch.render(rp.Batch.config(scheduler=1, capacity=1 << 12, group_size=128, **kw))
         print(f"[pool tail] {kw}: {st}", flush=True)
         assert (st["extension_rays"], st["shadow_rays"]) == want
+
+
+@pytest.fixture(scope="module")
+def mix64(rp, gpu_available):
+    """The 64-scene uneven mix (5k-500k triangles, spp 4/16/64, depth 8) at
+    one eighth of its resolutions (16-128 px squares)."""
+    mix = rp.batch_mix(64, resolution_divisor=8)
+    items = [(rp.Scene(s.desc), spp, seed) for s, spp, seed in mix]
+    return mix, items
+
+
+def test_mix64_pool_films_equal_scenes_alone(rp, mix64):
+    mix, items = mix64
+    batch = rp.Batch(items)
+    cfg, _ = batch.autotune()
+    print(f"[mix64] autotuned {rp.config_dict(cfg)}", flush=True)
+    for c in (cfg, rp.Batch.config(scheduler=rp.SCHED_PERSISTENT, group_size=128)):
+        films, st = batch.render(c)
+        print(f"[mix64] {rp.config_dict(c)}: {st}", flush=True)
+        assert st["failed_paths"] == 0
+        assert st["paths"] == sum(s.width * s.height * spp for s, spp, _ in items)
+        for i, (got, want) in enumerate(zip(films, _alone(rp, items))):
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"scene {i}"
+
+
+def test_mix64_records_match_oracle(rp, orc, mix64):
+    """Records of a subset of the mix (the smallest, the largest and two
+    middle scenes by triangle count) against the oracle."""
+    from tests.parity_util import AGREE_FRACTION, agreement
+
+    mix, items = mix64
+    order = np.argsort([s.d.triangle_count for s, _, _ in mix])
+    for i in (order[0], order[len(order) // 3], order[2 * len(order) // 3], order[-1]):
+        synth, spp, seed = mix[i]
+        scene = items[i][0]
+        n = scene.width * scene.height
+        req = rp.request(seed=seed, pixel_begin=0, pixel_count=min(n, 2048), sample_count=2)
+        print(f"[mix64] scene {i}: {synth.d.triangle_count} triangles", flush=True)
+        gpu = scene.sample(req)
+        cpu = orc.scene(synth.desc).sample(req)
+        assert np.array_equal(gpu["sobol_index"], cpu["sobol_index"])
+        assert np.array_equal(gpu["primary_triangle"], cpu["primary_triangle"])
+        assert agreement(gpu["xyz"], cpu["xyz"]) >= AGREE_FRACTION

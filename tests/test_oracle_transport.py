@@ -111,3 +111,66 @@ def test_furnace_expectation_is_the_neumann_series():
     cie_int = tc.xyz_integral(lambda lam: np.ones_like(lam))
     expected = cie_int * 2.0 * (1 - 0.5 ** 5) / (1 - 0.5)
     assert np.allclose(b_case.expected, expected, rtol=1e-12)
+
+
+def _region_stats(xyz, pix, w, h, region):
+    """Mean and standard error of per-path XYZ per region x region block of pixels."""
+    import numpy as np
+
+    rx, ry = (pix % w) // region, (pix // w) // region
+    key = ry * (w // region) + rx
+    nreg = (w // region) * (h // region)
+    mean, se = np.zeros((nreg, 3)), np.zeros((nreg, 3))
+    for k in range(nreg):
+        v = xyz[key == k]
+        mean[k] = v.mean(0)
+        se[k] = v.std(0, ddof=1) / np.sqrt(len(v))
+    return mean, se
+
+
+def _compare_with_bruteforce(orc, case, spp, brute_ppp, region=8, nsig=5.0):
+    import numpy as np
+
+    from tests import bruteforce
+
+    b = case.builder
+    recs = _oracle_records_for(orc, case, spp)
+    assert (recs["status"] == 0).all()
+    om, ose = _region_stats(tc.path_xyz(recs), recs["pixel"].astype(np.int64), b.width, b.height, region)
+    bx, bpix, deepest = bruteforce.render(b, case.desc, brute_ppp, seed=0x5eed)
+    bm, bse = _region_stats(bx, bpix, b.width, b.height, region)
+    tol = nsig * np.sqrt(ose ** 2 + bse ** 2) + 2e-5 * np.maximum(1.0, np.abs(bm))
+    ok = bool(np.all(np.abs(om - bm) <= tol))
+    return ok, deepest, recs, (f"oracle {om.tolist()} brute force {bm.tolist()} "
+                               f"|d|/se {(np.abs(om - bm) / np.sqrt(ose ** 2 + bse ** 2)).max():.2f}")
+
+
+def _oracle_records_for(orc, case, spp):
+    w, h = case.builder.width, case.builder.height
+    req = rp.request(seed=1, pixel_begin=0, pixel_count=w * h, sample_count=spp)
+    return orc.scene(case.desc).sample(req)
+
+
+def test_oracle_matches_fp64_bsdf_only_estimator(orc):
+    """Multi-bounce transport against the independent float64 BSDF-only
+    estimator (the reference's multibounce gate, docs/rendering.md:184-186:
+    region means within five combined standard errors, a path reaching a
+    fourth hit)."""
+    case = _case_open_box()
+    ok, deepest, recs, msg = _compare_with_bruteforce(orc, case, spp=256, brute_ppp=1024)
+    assert deepest >= 4 and int(recs["vertices"].max()) >= 5
+    assert ok, msg
+
+
+@pytest.mark.parametrize("slip", [6, 10])
+def test_planted_slip_fails_the_bruteforce_gate(orc, slip):
+    case = _case_open_box()
+    with orc.planted_slip(slip):
+        ok, _, _, msg = _compare_with_bruteforce(orc, case, spp=256, brute_ppp=1024)
+    assert not ok, f"slip {slip} ({SLIPS[slip][0]}) passed: {msg}"
+
+
+def _case_open_box():
+    if "open_box" not in _built:
+        _built["open_box"] = tc.open_box()
+    return _built["open_box"]

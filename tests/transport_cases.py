@@ -273,3 +273,26 @@ def pixel_check(records, case, nsig=6.0, rel=2e-3):
     worst = np.argmax((np.abs(mean - exp) / np.maximum(tol, 1e-30)).max(1))
     return (not bad.any(), f"{case.label}: {int(bad.any(1).sum())} of {h * w} pixels off; worst pixel {worst}: "
                            f"mean {mean[worst]} expected {exp[worst]} se {se[worst]}")
+
+
+def open_box(width=16, height=16, max_depth=64):
+    """Diffuse box open at the top (floor y = 0, four walls up to y = 1.2)
+    with a diffuse block inside, lit only by a constant environment and
+    seen from above.  No closed form: the expectation is the independent
+    float64 BSDF-only estimator (tests/bruteforce.py); multi-bounce paths
+    (a fourth hit and beyond) carry a large share of the radiance."""
+    b = SceneBuilder(width, height, max_depth)
+    floor = b.object(b.diffuse(b.linear(0.7, 0.5)))
+    walls = [b.object(b.diffuse(b.linear(*r))) for r in ((0.6, 0.2), (0.15, 0.55), (0.5, 0.5), (0.4, 0.7))]
+    block = b.object(b.diffuse(b.linear(0.8, 0.3)))
+    b.quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1), floor, facing=(0, 1, 0))
+    top = 1.2
+    b.quad((-1, 0, -1), (-1, top, -1), (-1, top, 1), (-1, 0, 1), walls[0], facing=(1, 0, 0))
+    b.quad((1, 0, -1), (1, top, -1), (1, top, 1), (1, 0, 1), walls[1], facing=(-1, 0, 0))
+    b.quad((-1, 0, -1), (1, 0, -1), (1, top, -1), (-1, top, -1), walls[2], facing=(0, 0, 1))
+    b.quad((-1, 0, 1), (1, 0, 1), (1, top, 1), (-1, top, 1), walls[3], facing=(0, 0, -1))
+    b.box((-0.45, 0.0, -0.3), (0.1, 0.55, 0.25), block, inward=False)
+    b.environment(b.linear(1.1, 0.7))
+    b.look_at((0.15, 2.4, 0.35), (0.0, 0.2, 0.0), (0, 0, 1), 48)
+    desc = b.build()
+    return Case(b, desc, None, label="open box under an environment")
