@@ -272,6 +272,20 @@ int rp_render_frame(rp_scene *scene, const rp_frame_config *config, float *xyz_h
 /* Same, with the film written to caller-owned device memory (W*H*3 floats). */
 int rp_render_frame_device(rp_scene *scene, const rp_frame_config *config, float *xyz_device,
                            rp_frame_stats *stats);
+/* Allocation-free frame.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same film as
+ * rp_render_frame_device in RP_MODE_MEGAKERNEL, bit for bit; other modes
+ * return RP_INVALID.  Its counters, film sums and per-path XYZ live in one
+ * static 128 MiB device array per GPU, held until the process ends: the
+ * film must fit in it (60 bytes per pixel plus 2 KiB, so at most 2,236,928
+ * pixels; 1920x1080 fits), else RP_INVALID and nothing runs.  The call makes
+ * no hipMalloc or hipFree.  Every call on a GPU shares that GPU's array and
+ * clears what it needs; this library serialises its calls on one GPU within
+ * the process, and calls that overlapped on one GPU would race.  Blocks
+ * until the film is written; stats->seconds is host wall-clock time. */
+int rp_render_frame_device_noalloc(rp_scene *scene, const rp_frame_config *config, float *xyz_device,
+                                   rp_frame_stats *stats);
 
 /* Closest-hit / any-hit traversal of caller rays (host buffers). any-hit writes
  * 1 (occluded) or 0 per ray into `occluded`. */

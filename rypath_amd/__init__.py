@@ -196,6 +196,8 @@ _sig("rp_scene_destroy", None, [C.c_void_p])
 _sig("rp_sample", C.c_int, [C.c_void_p, C.POINTER(Request), C.c_void_p, u64])
 _sig("rp_render_frame", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
 _sig("rp_render_frame_device", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
+_sig("rp_render_frame_device_noalloc", C.c_int,
+     [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
 _sig("rp_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
 _sig("rp_trace_any", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
 _sig("rp_probe_read_bandwidth", C.c_int, [C.c_int, u64, u32, C.POINTER(f64)])
@@ -248,7 +250,7 @@ _sig("rp_frame_release", None, [C.c_void_p])
 
 EXPORTED = [
     "rp_api_version", "rp_device_count", "rp_last_error", "rp_scene_create", "rp_scene_info", "rp_scene_destroy",
-    "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_trace_closest", "rp_trace_any",
+    "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_render_frame_device_noalloc", "rp_trace_closest", "rp_trace_any",
     "rp_probe_read_bandwidth",
     "rp_batch_create", "rp_batch_render", "rp_batch_destroy", "rp_autotune", "rp_set_params", "rp_backward",
     "rp_backward_device",
@@ -400,6 +402,14 @@ class Scene:
         st = FrameStats()
         _check("rp_render_frame_device", _lib.rp_render_frame_device(self._h, C.byref(cfg), C.c_void_p(device_ptr),
                                                                       C.byref(st)))
+        return {f: getattr(st, f) for f, _ in FrameStats._fields_}
+
+    def render_frame_device_noalloc(self, cfg, device_ptr):
+        """rp_render_frame_device_noalloc: the megakernel frame with every
+        scratch buffer in the library's static device array (no allocation)."""
+        st = FrameStats()
+        _check("rp_render_frame_device_noalloc",
+               _lib.rp_render_frame_device_noalloc(self._h, C.byref(cfg), C.c_void_p(device_ptr), C.byref(st)))
         return {f: getattr(st, f) for f, _ in FrameStats._fields_}
 
     def set_params(self, params, values):

@@ -656,6 +656,48 @@ struct rp_frame {
     std::shared_ptr<rp::FrameJob> job;
 };
 
+// ---------------------------------------------------- allocation-free frames
+// rp_render_frame_device_noalloc keeps the frame's statistics counters, its
+// compensated film planes and the per-path XYZ in `scratch`, a file-scope
+// device array (one copy per GPU, held until the process ends), instead of
+// the scene's device buffers.  In host code `scratch` names a host-side
+// stand-in: kernels get the device address that hipGetSymbolAddress returns
+// on the current device, looked up once per device.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+constexpr size_t kNoAllocCounterBytes = 8 * kCounterStride * sizeof(unsigned long long); // 2 KiB
+constexpr size_t kNoAllocFilmBytesPerPixel = 6 * sizeof(double);
+constexpr size_t kNoAllocXyzBytesPerPath = 3 * sizeof(float);
+constexpr int kNoAllocMaxDevices = 64;
+
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+struct NoAllocSlot {
+    std::mutex m;                // serialises this process's calls on the device
+    unsigned char *ptr = nullptr; // device address of `scratch` on the device
+};
+NoAllocSlot g_noalloc[kNoAllocMaxDevices];
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// The caller holds the slot's lock with `device` current.
+unsigned char *noalloc_scratch(NoAllocSlot &slot) {
+    if (!slot.ptr) {
+        void *p = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned a null scratch address");
+        slot.ptr = static_cast<unsigned char *>(p);
+    }
+    return slot.ptr;
+}
+
+} // namespace
+
 extern "C" {
 
 uint32_t rp_api_version(void) { return RP_API_VERSION; }
@@ -796,6 +838,83 @@ int rp_render_frame(rp_scene *s, const rp_frame_config *cfg, float *xyz_host, rp
         return RP_SUCCESS;
     } catch (const std::exception &e) {
         return fail("rp_render_frame", e);
+    }
+}
+
+int rp_render_frame_device_noalloc(rp_scene *s, const rp_frame_config *cfg, float *xyz_device,
+                                   rp_frame_stats *stats) {
+    static const char *who = "rp_render_frame_device_noalloc";
+    if (!s || !cfg || !xyz_device) {
+        set_error(std::string(who) + ": null argument");
+        return RP_INVALID;
+    }
+    if (cfg->mode != RP_MODE_MEGAKERNEL) {
+        set_error(std::string(who) + ": only RP_MODE_MEGAKERNEL runs without allocation");
+        return RP_INVALID;
+    }
+    if (cfg->spp == 0 || cfg->randomization > RP_RANDOMIZE_OWEN) {
+        set_error(std::string(who) + ": spp must be positive and randomization known");
+        return RP_INVALID;
+    }
+    const rp_camera &cam = s->host.desc.camera;
+    const uint64_t npx = uint64_t(cam.width) * cam.height;
+    const size_t film_off = kNoAllocCounterBytes;
+    const size_t xyz_off = align256(film_off + kNoAllocFilmBytesPerPixel * npx);
+    if (npx == 0 || xyz_off + kNoAllocXyzBytesPerPath * npx > kNoAllocScratchBytes) {
+        set_error(std::string(who) + ": a " + std::to_string(cam.width) + "x" + std::to_string(cam.height) +
+                  " film needs " + std::to_string(xyz_off + kNoAllocXyzBytesPerPath * npx) +
+                  " bytes of scratch; at most " + std::to_string(kNoAllocScratchBytes) +
+                  " (60 bytes per pixel plus 2 KiB)");
+        return RP_INVALID;
+    }
+    if (s->device < 0 || s->device >= kNoAllocMaxDevices) {
+        set_error(std::string(who) + ": device index beyond " + std::to_string(kNoAllocMaxDevices - 1));
+        return RP_INVALID;
+    }
+    try {
+        std::lock_guard<std::mutex> lock(s->exec);
+        NoAllocSlot &slot = g_noalloc[s->device];
+        std::lock_guard<std::mutex> slot_lock(slot.m);
+        HIP_CHECK(hipSetDevice(s->device));
+        unsigned char *base = noalloc_scratch(slot);
+        auto *counters = reinterpret_cast<unsigned long long *>(base);
+        auto *film = reinterpret_cast<double *>(base + film_off);
+        auto *xyz = reinterpret_cast<float *>(base + xyz_off);
+        const auto t0 = std::chrono::steady_clock::now();
+        // samples per chunk: what the per-path XYZ region holds (at least one,
+        // by the size check), at most 2^26 paths
+        const uint64_t room = (kNoAllocScratchBytes - xyz_off) / (kNoAllocXyzBytesPerPath * npx);
+        const uint32_t chunk =
+            uint32_t(std::max<uint64_t>(1, std::min<uint64_t>({cfg->spp, room, (uint64_t(1) << 26) / npx})));
+        rpd::PathParams P = path_params(cfg->seed, cfg->randomization, 0, 0, cfg->flags);
+        // the previous call's contents stay in the array: clear what starts at zero
+        hipLaunchKernelGGL(k_film_zero, dim3(uint32_t((6 * npx + 255) / 256)), dim3(256), 0, s->stream, 6 * npx, film);
+        hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s->stream, counters, 8u);
+        for (uint32_t s0 = 0; s0 < cfg->spp; s0 += chunk) {
+            const uint32_t ns = std::min(chunk, cfg->spp - s0);
+            const uint64_t n = uint64_t(ns) * npx;
+            hipLaunchKernelGGL(k_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream, s->dev, P,
+                               uint32_t(npx), cfg->sample_begin + s0, n, xyz, counters);
+            hipLaunchKernelGGL(k_film_add, dim3(uint32_t((npx + 255) / 256)), dim3(256), 0, s->stream, uint32_t(npx), ns,
+                               n, xyz, film);
+        }
+        hipLaunchKernelGGL(k_film_finish, dim3(uint32_t((npx + 255) / 256)), dim3(256), 0, s->stream, uint32_t(npx),
+                           uint64_t(cfg->spp), film, xyz_device);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (stats) {
+            unsigned long long c[3 * kCounterStride];
+            HIP_CHECK(hipMemcpy(c, counters, sizeof c, hipMemcpyDeviceToHost));
+            *stats = rp_frame_stats{};
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            stats->paths = npx * cfg->spp;
+            stats->extension_rays = c[0];
+            stats->shadow_rays = c[kCounterStride];
+            stats->failed_paths = c[2 * kCounterStride];
+        }
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail(who, e);
     }
 }
 
