@@ -23,6 +23,11 @@ HBM_PEAK_GBS = 8000.0
 RAY_BYTES = 4 + 32 + 24
 NODE_BYTES = 64
 TRI_BYTES = 48
+# Algorithmic bytes per any-hit (shadow) query: the 64-B shadow request
+# (origin/tmax, direction/MIS weight, contribution, metadata) + 64 per node
+# line + 48 per triangle record (the radiance update of a visible ray is not
+# counted).
+SHADOW_RAY_BYTES = 64
 
 
 def parse():
@@ -42,6 +47,7 @@ def parse():
                     help="inverse: extra sample sets for roughness at --fd-rough-step (its 1e-3 quotient is noisy)")
     ap.add_argument("--fd-rough-step", type=float, default=0.01)
     ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true", help="terrain: skip the 64-scene batch key")
     ap.add_argument("--pool-config", default=None,
                     help="pool mode: 'capacity,group_size,sort_bits' instead of running the autotuner")
     ap.add_argument("--profile", action="store_true",
@@ -78,6 +84,32 @@ def pmc_traffic():
     return round(mib * 2 ** 20 / launches) if launches else None
 
 
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_cornell_full(threads):
+    """The whole 256x256, 64 spp, depth-5 Cornell frame (4,194,304 paths) on
+    the oracle, all host threads: M paths/s and seconds."""
+    import rypath_amd as rp
+    from tests import oracle_lib
+
+    orc = oracle_lib.Oracle()
+    synth = rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=256, height=256)
+    oscene = orc.scene(synth.desc)
+    t0 = time.time()
+    oscene.render_frame(64, seed=1, threads=threads)
+    dt = time.time() - t0
+    return {"workload": "cornell-256-64spp-depth5", "M_paths_s": round(256 * 256 * 64 / dt / 1e6, 3),
+            "seconds": round(dt, 2)}
+
+
 def cpu_baseline(synth, width, height, budget):
     """Oracle (scalar C++ restatement) on a bounded band of pixels, all host threads."""
     import rypath_amd as rp
@@ -105,8 +137,10 @@ def cpu_baseline(synth, width, height, budget):
         dt = time.time() - t0
     paths = width * rows
     return {"value": paths / dt / 1e6, "unit": "M paths/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{rows} full-width rows x 1 spp ({paths} paths) of the same frame; oracle BVH build "
-                      f"{build:.1f}s excluded"}
+                      f"{build:.1f}s excluded",
+            "full_frame_cornell": cpu_cornell_full(threads)}
 
 
 def inverse_line(args, dt, W, H, spp, nparams):
@@ -293,64 +327,85 @@ def inverse_workload(args):
     print(json.dumps(line), flush=True)
 
 
-def batch_workload(args):
-    """64 seeded uneven scenes in one pool vs the same scenes rendered one after another."""
-    import numpy as np
+def batch_results(scenes, steps, warmup):
+    """`scenes` seeded uneven scenes (rp.batch_mix) in one pool against the
+    same scenes rendered one after another in the plain (megakernel) and the
+    wavefront form: aggregate rates, and a per-scene table of the sequential
+    frame times."""
     import torch
 
     import rypath_amd as rp
 
     t0 = time.time()
-    mix = rp.batch_mix(args.batch_scenes)
-    synths = [s for s, _, _ in mix]
+    mix = rp.batch_mix(scenes)
     items = [(rp.Scene(s.desc), spp, seed) for s, spp, seed in mix]
     build = time.time() - t0
     total = sum(sc.width * sc.height * spp for sc, spp, _ in items)
-    tris = sum(s.d.triangle_count for s in synths)
+    tris = sum(s.d.triangle_count for s, _, _ in mix)
     films = {sc: torch.empty((sc.height, sc.width, 3), dtype=torch.float32, device="cuda") for sc, _, _ in items}
     out = {}
+    per_scene = [{"scene": i, "width": sc.width, "spp": spp, "triangles": mix[i][0].d.triangle_count,
+                  "paths": sc.width * sc.height * spp} for i, (sc, spp, _) in enumerate(items)]
 
-    def sequential(mode):
-        for _ in range(args.warmup):
+    def sequential(mode, key):
+        for _ in range(warmup):
             for sc, spp, seed in items:
                 sc.render_frame_device(rp.frame_config(spp, mode=mode, seed=seed), films[sc].data_ptr())
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.steps):
-            for sc, spp, seed in items:
+        for _ in range(steps):
+            for i, (sc, spp, seed) in enumerate(items):
+                t1 = time.perf_counter()
                 sc.render_frame_device(rp.frame_config(spp, mode=mode, seed=seed), films[sc].data_ptr())
+                per_scene[i][key] = round((time.perf_counter() - t1) * 1e3, 3)  # synchronous call
         torch.cuda.synchronize()
-        return (time.perf_counter() - t) / args.steps
+        return (time.perf_counter() - t) / steps
 
-    for name, mode in (("sequential_plain", rp.MODE_MEGAKERNEL), ("sequential_wavefront", rp.MODE_WAVEFRONT)):
-        dt = sequential(mode)
+    for name, mode, key in (("sequential_plain", rp.MODE_MEGAKERNEL, "plain_ms"),
+                            ("sequential_wavefront", rp.MODE_WAVEFRONT, "wavefront_ms")):
+        dt = sequential(mode, key)
         out[name] = {"ms": round(dt * 1e3, 2), "M_paths_s": round(total / dt / 1e6, 2)}
-        print(f"[bench] {name}: {dt * 1e3:.1f} ms, {total / dt / 1e6:.1f} M paths/s", file=sys.stderr, flush=True)
+        print(f"[bench] batch {name}: {dt * 1e3:.1f} ms, {total / dt / 1e6:.1f} M paths/s", file=sys.stderr,
+              flush=True)
     batch = rp.Batch(items)
     t = time.time()
     cfg, _ = batch.autotune()
     tune = time.time() - t
     for name, c in (("batched_persistent", rp.Batch.config(scheduler=rp.SCHED_PERSISTENT, group_size=128)),
                     ("batched_autotuned", cfg)):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             batch.render(c, films=False)
         torch.cuda.synchronize()
         t = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             _, st = batch.render(c, films=False)
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t) / args.steps
+        dt = (time.perf_counter() - t) / steps
         out[name] = {"ms": round(dt * 1e3, 2), "M_paths_s": round(total / dt / 1e6, 2), "config": rp.config_dict(c)}
-        print(f"[bench] {name}: {dt * 1e3:.1f} ms, {total / dt / 1e6:.1f} M paths/s", file=sys.stderr, flush=True)
+        print(f"[bench] batch {name}: {dt * 1e3:.1f} ms, {total / dt / 1e6:.1f} M paths/s", file=sys.stderr,
+              flush=True)
     best = max(("batched_persistent", "batched_autotuned"), key=lambda k: out[k]["M_paths_s"])
+    return {"workload": "batch-64-uneven", "scenes": len(items), "paths": total, "triangles": tris, "max_depth": 8,
+            "steps": steps, "warmup": warmup, "best": best, "M_paths_s": out[best]["M_paths_s"],
+            "ms": out[best]["ms"], "modes": out,
+            "batched_vs_sequential_plain": round(out[best]["M_paths_s"] / out["sequential_plain"]["M_paths_s"], 3),
+            "batched_vs_sequential_wavefront": round(out[best]["M_paths_s"] /
+                                                     out["sequential_wavefront"]["M_paths_s"], 3),
+            "bvh_build_s": round(build, 2), "autotune_s": round(tune, 2), "per_scene": per_scene}
+
+
+def batch_workload(args):
+    """64 seeded uneven scenes in one pool vs the same scenes rendered one after another."""
+    b = batch_results(args.batch_scenes, args.steps, args.warmup)
     line = {
-        "metric": "M paths/s (64 uneven scenes, one pool)", "value": out[best]["M_paths_s"], "unit": "M paths/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": out[best]["ms"],
+        "metric": "M paths/s (64 uneven scenes, one pool)", "value": b["M_paths_s"], "unit": "M paths/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batch-64-uneven", "scenes": len(items), "paths": total, "triangles": tris,
+        "config": {"workload": b["workload"], "scenes": b["scenes"], "paths": b["paths"], "triangles": b["triangles"],
                    "max_depth": 8},
-        "modes": out, "batched_vs_sequential_plain": round(out[best]["M_paths_s"] / out["sequential_plain"]["M_paths_s"], 3),
-        "bvh_build_s": round(build, 2), "autotune_s": round(tune, 2),
+        "modes": b["modes"], "batched_vs_sequential_plain": b["batched_vs_sequential_plain"],
+        "batched_vs_sequential_wavefront": b["batched_vs_sequential_wavefront"],
+        "bvh_build_s": b["bvh_build_s"], "autotune_s": b["autotune_s"], "per_scene": b["per_scene"],
     }
     print(json.dumps(line), flush=True)
 
@@ -451,8 +506,32 @@ def main():
         l2_meas = rp.probe_read_bandwidth(24 << 20, 400)
         print(f"[bench] measured read bandwidth: HBM {hbm_meas:.0f} GB/s, L2 {l2_meas:.0f} GB/s",
               file=sys.stderr, flush=True)
+        launches_ = max(st["extend_launches"], 1)
+        avg_launch = st["trace_seconds"] / launches_
+        # any-hit (shadow) kernel: the replay's bytes per shadow ray over the
+        # timed frame's shadow-kernel time
+        sh_rays = max(cnt["shadow_rays"], 1)
+        sh_per_ray = (SHADOW_RAY_BYTES * cnt["shadow_rays"] + NODE_BYTES * cnt["shadow_node_fetches"] +
+                      TRI_BYTES * cnt["shadow_triangle_tests"]) / sh_rays
+        sh_alg = sh_per_ray * st["shadow_kernel_rays"]
+        sh_achieved = sh_alg / st["shadow_seconds"] / 1e9 if st["shadow_seconds"] > 0 else None
+        shadow = {"kernel": "k_wf_shadow (any hit)", "achieved": round(sh_achieved, 1) if sh_achieved else None,
+                  "frac": round(sh_achieved / HBM_PEAK_GBS, 4) if sh_achieved else None,
+                  "per_ray_bytes": round(sh_per_ray, 1),
+                  "nodes_per_ray": round(cnt["shadow_node_fetches"] / sh_rays, 2),
+                  "tris_per_ray": round(cnt["shadow_triangle_tests"] / sh_rays, 2),
+                  "ns_per_ray": round(st["shadow_seconds"] / max(st["shadow_kernel_rays"], 1) * 1e9, 4),
+                  "closest_hit_ns_per_ray": round(st["trace_seconds"] / max(st["extend_rays"], 1) * 1e9, 4),
+                  "avg_launch_ms": round(st["shadow_seconds"] / max(st["shadow_launches"], 1) * 1e3, 3),
+                  "launches": st["shadow_launches"]}
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    # the kernel moves far fewer HBM bytes than it fetches through
+                    # the caches: it is limited by latency and lane divergence,
+                    # not by the HBM roof that `frac` prices it against
+                    "limiter": "latency/divergence",
+                    "frac_by_measured_traffic": (round(traffic / avg_launch / 1e9 / HBM_PEAK_GBS, 4)
+                                                 if traffic else None),
                     "measured_read_peak_hbm": round(hbm_meas, 1), "frac_of_measured_hbm": round(achieved / hbm_meas, 4),
                     "measured_read_peak_l2": round(l2_meas, 1), "frac_of_measured_l2": round(achieved / l2_meas, 4),
                     "traffic": traffic,
@@ -463,7 +542,10 @@ def main():
                     "tail_extension_rays": st["extension_rays"] - st["extend_rays"],
                     "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),
                     "tris_per_ray": round(cnt["triangle_tests"] / cnt["extension_rays"], 2),
-                    "avg_launch_ms": round(st["trace_seconds"] / max(st["extend_launches"], 1) * 1e3, 3)}
+                    "avg_launch_ms": round(avg_launch * 1e3, 3),
+                    "stack_spills_per_ray": round(cnt["stack_spills"] / max(cnt["extension_rays"] +
+                                                                            cnt["shadow_rays"], 1), 5),
+                    "shadow": shadow}
     # the second headline figure: inverse-rendering gradients per second
     adjoint = None
     if args.workload == "terrain":
@@ -476,6 +558,8 @@ def main():
         print(f"[bench] inverse step: {dt_inv * 1e3:.2f} ms ({1 / dt_inv:.1f} gradients/s)", file=sys.stderr,
               flush=True)
         del inv
+    # the uneven 64-scene batch beside the single frame (few steps)
+    batch64 = batch_results(args.batch_scenes, 2, 1) if args.workload == "terrain" and not args.no_batch else None
     cpu = None if args.no_cpu else cpu_baseline(synth, args.width, args.height, args.cpu_seconds)
     st = r["stats"]
     line = {
@@ -492,7 +576,7 @@ def main():
         "batched_vs_plain": (round(results["pool"]["mpaths"] / results["mega"]["mpaths"], 3)
                              if "pool" in results and "mega" in results else None),
         "bvh_build_s": round(info["build_seconds"], 2), "upload_s": round(info["upload_seconds"], 2),
-        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint,
+        "roofline": roofline, "cpu_baseline": cpu, "adjoint": adjoint, "batch64": batch64,
     }
     print(json.dumps(line), flush=True)
 

@@ -232,6 +232,14 @@ typedef struct rp_frame_stats {
     uint64_t node_fetches, triangle_tests; /* with RP_FRAME_COUNT_TRAFFIC: closest-hit traversal totals */
     uint64_t extend_rays;           /* wavefront: extension rays traced by the closest-hit kernels
                                        (the rest of extension_rays by the latency tail) */
+    /* with RP_FRAME_COUNT_TRAFFIC: any-hit (shadow) traversal totals, and the
+     * traversal-stack pushes of both kernels that overflowed the LDS short
+     * stack into per-lane scratch */
+    uint64_t shadow_node_fetches, shadow_triangle_tests, stack_spills;
+    double shadow_seconds;          /* wavefront: device time in the shadow (any-hit) kernels */
+    uint64_t shadow_launches;       /* wavefront: shadow kernel launches */
+    uint64_t shadow_kernel_rays;    /* wavefront: shadow rays traced by the shadow kernels (the rest
+                                       of shadow_rays by the latency tail) */
 } rp_frame_stats;
 
 /* ---- traversal probe ray/hit records ------------------------------------ */

@@ -135,7 +135,9 @@ class FrameConfig(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("seconds", f64), ("paths", u64), ("extension_rays", u64), ("shadow_rays", u64),
                 ("failed_paths", u64), ("trace_seconds", f64), ("extend_launches", u64), ("node_fetches", u64),
-                ("triangle_tests", u64), ("extend_rays", u64)]
+                ("triangle_tests", u64), ("extend_rays", u64), ("shadow_node_fetches", u64),
+                ("shadow_triangle_tests", u64), ("stack_spills", u64), ("shadow_seconds", f64),
+                ("shadow_launches", u64), ("shadow_kernel_rays", u64)]
 
 
 class Ray(C.Structure):

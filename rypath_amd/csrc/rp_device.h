@@ -486,6 +486,7 @@ RP_DEV void load_tri(const Scene &S, uint32_t p, F3 &a, F3 &b, F3 &c, uint32_t &
 
 struct TraversalCounts {
     uint32_t nodes, tris;
+    uint32_t spills; // pushes beyond the LDS short stack (into the per-lane scratch overflow)
 };
 
 // Compare-exchange on (key, id) pairs; equal keys are never exchanged.
@@ -500,7 +501,8 @@ RP_DEV void cswap(float &ka, uint32_t &ia, float &kb, uint32_t &ib) {
 }
 
 // Closest hit (smallest t, ties to the smallest scene triangle id).  With
-// kCount, node fetches and triangle tests are tallied (instrumented replay).
+// kCount, node fetches, triangle tests and stack pushes that overflow into
+// scratch are tallied (instrumented replay).
 template <bool kAny, bool kCount = false>
 RP_DEV bool traverse(const Scene &S, Stack &st, F3 o, F3 d, float tmax, Hit &h, TraversalCounts *tc = nullptr) {
     h.prim = kEmpty;
@@ -552,8 +554,10 @@ RP_DEV bool traverse(const Scene &S, Stack &st, F3 o, F3 d, float tmax, Hit &h, 
                 cswap(k1, i1, k3, i3);
                 cswap(k1, i1, k2, i2);
                 // push the farther hits far-to-near through one push site
-                for (int j = __builtin_popcount(mask) - 1; j >= 1; --j)
+                for (int j = __builtin_popcount(mask) - 1; j >= 1; --j) {
+                    if (kCount && st.sp >= kShortStack) tc->spills += 1;
                     st.push(j == 3 ? i3 : (j == 2 ? i2 : i1), j == 3 ? k3 : (j == 2 ? k2 : k1));
+                }
                 cur = i0;
                 continue;
             }
@@ -1037,15 +1041,16 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, flo
 // With `mlen` the length travelled inside media is accumulated (adjoint).
 // kInterfaces = false compiles only the any-hit branch (scenes without null
 // interfaces), which keeps the shadow kernel's register footprint small.
-template <bool kInterfaces = true>
+// With kCount the traversals' node fetches and triangle tests add to `tc`.
+template <bool kInterfaces = true, bool kCount = false>
 RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uint32_t medium, const Wavelengths &wl,
                           float tr[4], uint32_t boundary_limit, float *mlen = nullptr,
-                          uint32_t mlen_medium = RP_NONE) {
+                          uint32_t mlen_medium = RP_NONE, TraversalCounts *tc = nullptr) {
     for (int k = 0; k < 4; ++k) tr[k] = 1.0f;
     if (mlen) *mlen = 0.0f;
     if (!kInterfaces || !S.has_interfaces) {
         Hit h;
-        if (traverse<true>(S, st, o, d, tmax, h)) return false;
+        if (traverse<true, kCount>(S, st, o, d, tmax, h, tc)) return false;
         if (medium != RP_NONE) {
             MediumCoeff mc = medium_coeff(S, medium, wl);
             float len = length(d);
@@ -1057,7 +1062,7 @@ RP_DEV bool transmittance(const Scene &S, Stack &st, F3 o, F3 d, float tmax, uin
     uint32_t crossings = 0;
     for (;;) {
         Hit h;
-        bool hit = traverse<false>(S, st, o, d, tmax, h);
+        bool hit = traverse<false, kCount>(S, st, o, d, tmax, h, tc);
         float seg = hit ? h.t : tmax;
         if (medium != RP_NONE) {
             MediumCoeff mc = medium_coeff(S, medium, wl);

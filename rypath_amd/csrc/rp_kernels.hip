@@ -22,7 +22,7 @@
 #include "rp_batch.h"
 #include "rp_adjoint.h"
 
-static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 80 && sizeof(rp_batch_config) == 32,
+static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 128 && sizeof(rp_batch_config) == 32,
               "C ABI struct sizes");
 
 namespace rp {
@@ -473,7 +473,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     rpd::EventPair ev; // destroyed on every exit, exceptions included
     hipEvent_t e0 = ev.a, e1 = ev.b;
     HIP_CHECK(hipEventRecord(e0, s->stream));
-    double trace_seconds = 0;
+    double trace_seconds = 0, shadow[2] = {0, 0};
     uint64_t launches = 0, wf_rays[2] = {0, 0};
     // the device's wavefront arena is held until this frame's last kernel has
     // finished (the event synchronisation below happens before the unlock)
@@ -494,7 +494,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
             arena->primal.valid = false;
             trace_seconds += rpd::wavefront_render(s->dev, P, arena->fb, cfg, npx, cfg.sample_begin + s0, n,
                                                    s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays,
-                                                   s0 == 0);
+                                                   s0 == 0, shadow);
         }
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
@@ -534,6 +534,12 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         stats->extend_launches = launches;
         stats->node_fetches = c[3];
         stats->triangle_tests = c[4];
+        stats->shadow_node_fetches = c[5];
+        stats->shadow_triangle_tests = c[6];
+        stats->stack_spills = c[7];
+        stats->shadow_seconds = shadow[0];
+        stats->shadow_launches = uint64_t(shadow[1]);
+        stats->shadow_kernel_rays = wf_rays[1];
     }
     return ms * 1e-3;
 }

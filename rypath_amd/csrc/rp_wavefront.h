@@ -235,26 +235,29 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
     Stack st;
     st.init(s_node, s_t, kWfBlock);
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
-    uint32_t nodes = 0, tris = 0;
+    uint32_t nodes = 0, tris = 0, spills = 0;
     if (i < qn) {
         float4 a, b = C.d[i];
         if (kGen) a = make_float4(S.camera.origin[0], S.camera.origin[1], S.camera.origin[2], __builtin_inff());
         else a = C.o_t[i];
         Hit h;
-        TraversalCounts tc = {0, 0};
+        TraversalCounts tc = {0, 0, 0};
         traverse<false, kCount>(S, st, F3{a.x, a.y, a.z}, F3{b.x, b.y, b.z}, a.w, h, &tc);
         F.hit_t[i] = make_float4(h.t, h.b0, h.b1, h.b2);
         F.hit_id[i] = make_uint2(h.prim, h.id);
         if (kCount) {
             nodes = tc.nodes;
             tris = tc.tris;
+            spills = tc.spills;
         }
     }
     // ray counts are the queue counts (summed on the host); only the
-    // instrumented replay tallies node fetches and triangle tests
+    // instrumented replay tallies node fetches, triangle tests and stack
+    // overflows (counter 7, shared with the shadow kernel)
     if (kCount) {
         wave_count(&counters[3 * kCounterStride], nodes);
         wave_count(&counters[4 * kCounterStride], tris);
+        wave_count(&counters[7 * kCounterStride], spills);
     }
 }
 
@@ -394,8 +397,10 @@ __global__ void __launch_bounds__(kWfBlock) RP_SHADE_ATTR k_wf_shade(Scene S, Pa
 }
 
 // Shadow rays of one bounce; adds visible contributions to the survivor's
-// radiance in N or to an ended path's lane.
-template <bool kInterfaces>
+// radiance in N or to an ended path's lane.  With kCount (instrumented
+// replay) node fetches, triangle tests and stack overflows are tallied in
+// counters 5, 6 and 7.
+template <bool kInterfaces, bool kCount = false>
 __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, WfState N, FrameWf F, uint32_t sn,
                                                           unsigned long long *counters) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
@@ -403,6 +408,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
     Stack st;
     st.init(s_node, s_t, kWfBlock);
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+    TraversalCounts tc = {0, 0, 0};
     if (i < sn) {
         float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
         uint4 meta = F.sh_meta[i];
@@ -416,7 +422,8 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
         sr.delta_or_nomis = meta.y != 0;
         Wavelengths wl = path_wavelengths(S, meta.z);
         float tr[4];
-        bool vis = transmittance<kInterfaces>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary);
+        bool vis = transmittance<kInterfaces, kCount>(S, st, sr.o, sr.d, sr.tmax, sr.medium, wl, tr, P.boundary,
+                                                      nullptr, RP_NONE, &tc);
         if (vis) {
             float4 *dst = (meta.w & kEnded) ? &F.L_end[meta.w & ~kEnded] : &N.L[meta.w];
             float4 L = *dst;
@@ -427,6 +434,11 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
         }
     }
     // shadow ray counts are the queue counts (summed on the host)
+    if (kCount) {
+        wave_count(&counters[5 * kCounterStride], tc.nodes);
+        wave_count(&counters[6 * kCounterStride], tc.tris);
+        wave_count(&counters[7 * kCounterStride], tc.spills);
+    }
 }
 
 // ------------------------------------------------------------ latency tail
@@ -592,10 +604,12 @@ inline void FrameBuffers::ensure(uint64_t n) {
 // the closest-hit (extend) kernels; adds the extension and shadow ray counts
 // of the bounce kernels (their queue counts) to rays[0] and rays[1]; the
 // latency tail's go to the arena's tail counters, zeroed when first_chunk.
+// With `shadow`, the any-hit (shadow) kernels' device seconds and launches
+// are added to shadow[0] and shadow[1].
 inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers &FB, const rp_frame_config &cfg,
                                uint32_t npx, uint64_t sample_begin, uint64_t n, float *xyz,
                                unsigned long long *counters, hipStream_t stream, uint64_t &launches,
-                               uint64_t rays[2], bool first_chunk) {
+                               uint64_t rays[2], bool first_chunk, double *shadow = nullptr) {
     if (n > (uint64_t(1) << 31)) throw std::runtime_error("wavefront chunk too large");
     FB.ensure(n);
     const FrameWf &F = FB.f;
@@ -615,9 +629,12 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         hipLaunchKernelGGL(k_wf_generate<false>, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, P, npx,
                            sample_begin, n, FB.state[cur]);
     uint32_t qn = uint32_t(n);
-    EventPair ev; // destroyed on every exit, exceptions included
+    EventPair ev, ev_sh; // destroyed on every exit, exceptions included
     hipEvent_t e0 = ev.a, e1 = ev.b;
     double trace = 0;
+    bool shadow_pending = false;
+    double shadow_seconds = 0;
+    uint64_t shadow_launches = 0;
     while (qn > 0) {
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
@@ -660,6 +677,11 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         float ms = 0;
         WF_CHECK(hipEventElapsedTime(&ms, e0, e1));
         trace += ms * 1e-3;
+        if (shadow_pending) { // the previous bounce's shadow launch finished before this bounce's kernels
+            WF_CHECK(hipEventElapsedTime(&ms, ev_sh.a, ev_sh.b));
+            shadow_seconds += ms * 1e-3;
+            shadow_pending = false;
+        }
         const uint32_t next = FB.host_count[0], sn = FB.host_count[kQueueStride];
         rays[0] += qn;
         rays[1] += sn;
@@ -668,13 +690,31 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         }
         if (sn) {
             const dim3 sg((sn + kWfBlock - 1) / kWfBlock);
-            if (S.has_interfaces)
-                hipLaunchKernelGGL(k_wf_shadow<true>, sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+            WF_CHECK(hipEventRecord(ev_sh.a, stream));
+            if (S.has_interfaces && count)
+                hipLaunchKernelGGL((k_wf_shadow<true, true>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+            else if (S.has_interfaces)
+                hipLaunchKernelGGL((k_wf_shadow<true, false>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+            else if (count)
+                hipLaunchKernelGGL((k_wf_shadow<false, true>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
             else
-                hipLaunchKernelGGL(k_wf_shadow<false>, sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+                hipLaunchKernelGGL((k_wf_shadow<false, false>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
+            WF_CHECK(hipEventRecord(ev_sh.b, stream));
+            shadow_pending = true;
+            ++shadow_launches;
         }
         cur = 1 - cur;
         qn = next;
+    }
+    if (shadow_pending) {
+        WF_CHECK(hipEventSynchronize(ev_sh.b));
+        float ms = 0;
+        WF_CHECK(hipEventElapsedTime(&ms, ev_sh.a, ev_sh.b));
+        shadow_seconds += ms * 1e-3;
+    }
+    if (shadow) {
+        shadow[0] += shadow_seconds;
+        shadow[1] += double(shadow_launches);
     }
     hipLaunchKernelGGL(k_wf_finish, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, stream, S, F, n, xyz, counters);
     WF_CHECK(hipGetLastError());
