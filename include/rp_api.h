@@ -165,18 +165,33 @@ typedef enum rp_randomization {
     RP_RANDOMIZE_NONE = 0, RP_RANDOMIZE_PERMUTE = 1, RP_RANDOMIZE_FASTOWEN = 2, RP_RANDOMIZE_OWEN = 3
 } rp_randomization;
 
-/* RenderRequest (src/metal_render.hpp:14-32) */
+/* RenderSampler::kind (SamplerKind, src/sampling.hpp:14).  This backend
+ * renders with the Sobol sampler; the other kinds are refused (RP_INVALID). */
+typedef enum rp_sampler_kind {
+    RP_SAMPLER_INDEPENDENT = 0, RP_SAMPLER_SOBOL = 1, RP_SAMPLER_HALTON = 2, RP_SAMPLER_ZSOBOL = 3
+} rp_sampler_kind;
+
+/* RenderRequest (src/metal_render.hpp:14-32), field for field.  72 bytes. */
 typedef struct rp_request {
+    uint32_t width, height;   /* raster size: 0 = the scene camera's, else it must equal the camera's */
     uint64_t seed;            /* sampler seed (PBRT signed 32-bit seed in the low word) */
-    uint32_t randomization;   /* rp_randomization */
+    int32_t material_seed;    /* PBRT's stochastic-material seed: carried for the caller; no material
+                                 of this backend is stochastic, so it does not change the result */
     uint32_t pixel_begin, pixel_count;
     uint32_t sample_count;
     uint64_t sample_begin;
+    uint64_t total_samples;   /* fixed sample budget: 0 = none, else it must contain
+                                 [sample_begin, sample_begin + sample_count) */
+    uint32_t threadgroup_width; /* 0 = the device's choice (128); else 64, 128 or 256 (the
+                                   records do not depend on it) */
     uint32_t emergency_limit; /* 0 = default 1024 */
+    uint32_t sampler_kind;    /* rp_sampler_kind: RP_SAMPLER_SOBOL */
+    uint32_t randomization;   /* rp_randomization (RenderSampler::randomization) */
     uint32_t boundary_limit;  /* 0 = default 4096 */
+    uint32_t pad;
 } rp_request;
 
-/* PacketSample (src/render.hpp:307-316) plus parity diagnostics. 104 bytes. */
+/* PacketSample (src/render.hpp:307-316) plus parity diagnostics. 112 bytes. */
 typedef struct rp_path_record {
     uint32_t pixel, status;
     uint64_t sample;
@@ -188,7 +203,8 @@ typedef struct rp_path_record {
     uint32_t primary_triangle; /* scene triangle id of the camera ray's hit, RP_NONE = miss */
     uint32_t vertices;          /* path vertices including the camera */
     uint32_t camera_words[2];   /* Sobol words of dimensions 0 and 1 */
-    uint32_t pad;
+    float filter_weight;        /* PacketSample::filter_weight: 1 (box pixel filter) */
+    uint32_t splat_begin, splat_count; /* PacketSample splat range: 0, 0 (no light splats) */
 } rp_path_record;
 
 /* ---- frames ------------------------------------------------------------- */

@@ -1567,6 +1567,8 @@ int or_sample(void *sp, const rp_request *rq, rp_path_record *out, uint32_t thre
                 r.vertices = p.vertices;
                 r.camera_words[0] = p.w0;
                 r.camera_words[1] = p.w1;
+                r.filter_weight = 1.0f; // box pixel filter
+                r.splat_begin = r.splat_count = 0;
             }
         }
     };

@@ -67,6 +67,7 @@ MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_WAVEFRONT_SORTED = range(3)
 SYNTH_CORNELL, SYNTH_TERRAIN, SYNTH_BATCH, SYNTH_VOLUME, SYNTH_INVERSE = range(5)
 PARAM_REFLECTANCE_BIN, PARAM_ROUGHNESS, PARAM_SIGMA_S_BIN = range(3)
 SCHED_PERSISTENT, SCHED_WAVEFRONT = range(2)
+SAMPLER_INDEPENDENT, SAMPLER_SOBOL, SAMPLER_HALTON, SAMPLER_ZSOBOL = range(4)
 u32, u64, f32, f64 = C.c_uint32, C.c_uint64, C.c_float, C.c_double
 
 
@@ -123,8 +124,10 @@ class BuildInfo(C.Structure):
 
 
 class Request(C.Structure):
-    _fields_ = [("seed", u64), ("randomization", u32), ("pixel_begin", u32), ("pixel_count", u32),
-                ("sample_count", u32), ("sample_begin", u64), ("emergency_limit", u32), ("boundary_limit", u32)]
+    _fields_ = [("width", u32), ("height", u32), ("seed", u64), ("material_seed", C.c_int32),
+                ("pixel_begin", u32), ("pixel_count", u32), ("sample_count", u32), ("sample_begin", u64),
+                ("total_samples", u64), ("threadgroup_width", u32), ("emergency_limit", u32),
+                ("sampler_kind", u32), ("randomization", u32), ("boundary_limit", u32), ("pad", u32)]
 
 
 class FrameConfig(C.Structure):
@@ -175,9 +178,10 @@ class Param(C.Structure):
 RECORD_DTYPE = np.dtype([
     ("pixel", "<u4"), ("status", "<u4"), ("sample", "<u8"), ("sobol_index", "<u8"),
     ("wavelengths", "<f4", 4), ("wavelength_pdfs", "<f4", 4), ("radiance", "<f4", 4), ("xyz", "<f4", 3),
-    ("primary_triangle", "<u4"), ("vertices", "<u4"), ("camera_words", "<u4", 2), ("pad", "<u4"),
+    ("primary_triangle", "<u4"), ("vertices", "<u4"), ("camera_words", "<u4", 2), ("filter_weight", "<f4"),
+    ("splat_begin", "<u4"), ("splat_count", "<u4"),
 ])
-assert RECORD_DTYPE.itemsize == 104
+assert RECORD_DTYPE.itemsize == 112
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("direction", "<f4", 3), ("pad", "<u4")])
 HIT_DTYPE = np.dtype([("triangle", "<u4"), ("t", "<f4"), ("b1", "<f4"), ("b2", "<f4")])
 
@@ -334,11 +338,15 @@ def batch_mix(count=64, seed=2026, resolution_divisor=1):
 
 
 def request(seed=1, pixel_begin=0, pixel_count=1, sample_begin=0, sample_count=1,
-            randomization=RANDOMIZE_FASTOWEN, emergency_limit=0, boundary_limit=0):
+            randomization=RANDOMIZE_FASTOWEN, emergency_limit=0, boundary_limit=0, width=0, height=0,
+            material_seed=0, total_samples=0, threadgroup_width=0, sampler_kind=None):
+    """An rp_request (RenderRequest); width/height 0 take the scene camera's."""
     r = Request()
-    r.seed, r.randomization, r.pixel_begin, r.pixel_count = seed, randomization, pixel_begin, pixel_count
-    r.sample_begin, r.sample_count = sample_begin, sample_count
-    r.emergency_limit, r.boundary_limit = emergency_limit, boundary_limit
+    r.width, r.height, r.seed, r.material_seed = width, height, seed, material_seed
+    r.randomization, r.pixel_begin, r.pixel_count = randomization, pixel_begin, pixel_count
+    r.sample_begin, r.sample_count, r.total_samples = sample_begin, sample_count, total_samples
+    r.threadgroup_width, r.emergency_limit, r.boundary_limit = threadgroup_width, emergency_limit, boundary_limit
+    r.sampler_kind = SAMPLER_SOBOL if sampler_kind is None else sampler_kind
     return r
 
 

@@ -22,7 +22,8 @@
 #include "rp_batch.h"
 #include "rp_adjoint.h"
 
-static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 128 && sizeof(rp_batch_config) == 32,
+static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 128 && sizeof(rp_batch_config) == 32 &&
+                  sizeof(rp_request) == 72 && sizeof(rp_path_record) == 112,
               "C ABI struct sizes");
 
 namespace rp {
@@ -131,14 +132,17 @@ __device__ __forceinline__ void count_rays(unsigned long long *counters, uint32_
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_records(rpd::Scene S, rpd::PathParams P, uint32_t pixel_begin,
-                                                     uint32_t pixel_count, uint64_t sample_begin, uint64_t n,
-                                                     rp_path_record *out) {
-    __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
-    __shared__ float s_t[rpd::kShortStack * kBlock];
+// Per-path records; B = the request's workgroup width (the records do not
+// depend on it).
+template <int B>
+__global__ void __launch_bounds__(B) k_records(rpd::Scene S, rpd::PathParams P, uint32_t pixel_begin,
+                                               uint32_t pixel_count, uint64_t sample_begin, uint64_t n,
+                                               rp_path_record *out) {
+    __shared__ uint32_t s_node[rpd::kShortStack * B];
+    __shared__ float s_t[rpd::kShortStack * B];
     rpd::Stack st;
-    LdsStack<kBlock>::init(st, s_node, s_t);
-    uint64_t lane = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    LdsStack<B>::init(st, s_node, s_t);
+    uint64_t lane = uint64_t(blockIdx.x) * B + threadIdx.x;
     if (lane >= n) return;
     uint32_t pixel = pixel_begin + uint32_t(lane % pixel_count);
     uint64_t sample = sample_begin + lane / pixel_count;
@@ -163,7 +167,9 @@ __global__ void __launch_bounds__(kBlock) k_records(rpd::Scene S, rpd::PathParam
     rec.vertices = r.vertices;
     rec.camera_words[0] = r.w0;
     rec.camera_words[1] = r.w1;
-    rec.pad = 0;
+    rec.filter_weight = 1.0f; // box pixel filter: every sample has unit weight
+    rec.splat_begin = 0;      // no light-tracing splats on this path
+    rec.splat_count = 0;
     out[lane] = rec;
 }
 
@@ -702,6 +708,33 @@ unsigned char *noalloc_scratch(NoAllocSlot &slot) {
     return slot.ptr;
 }
 
+// RenderRequest validation (src/metal_render.mm:1930-1946, src/sampling.hpp:29-31):
+// raster size, sampler kind and randomization, the pixel and sample ranges
+// (and the fixed sample budget when given), and the workgroup width.
+bool request_valid(const rp_scene *s, const rp_request &rq, const char *who) {
+    const rp_camera &cam = s->host.desc.camera;
+    const uint32_t npx = cam.width * cam.height;
+    auto bad = [&](const std::string &why) {
+        set_error(std::string(who) + ": " + why);
+        return false;
+    };
+    if ((rq.width || rq.height) && (rq.width != cam.width || rq.height != cam.height))
+        return bad("request raster " + std::to_string(rq.width) + "x" + std::to_string(rq.height) +
+                   " differs from the scene camera's " + std::to_string(cam.width) + "x" + std::to_string(cam.height));
+    if (rq.sampler_kind != RP_SAMPLER_SOBOL) return bad("sampler kind not supported by this backend (Sobol only)");
+    if (rq.randomization > RP_RANDOMIZE_OWEN) return bad("unknown randomization");
+    if (rq.pixel_count == 0 || rq.sample_count == 0 || uint64_t(rq.pixel_begin) + rq.pixel_count > npx)
+        return bad("pixel range outside the film or empty request");
+    if (rq.sample_begin + rq.sample_count < rq.sample_begin) return bad("sample range overflows");
+    if (rq.total_samples &&
+        (rq.sample_begin >= rq.total_samples || rq.sample_count > rq.total_samples - rq.sample_begin))
+        return bad("the fixed total sample budget does not contain the sample range");
+    if (rq.threadgroup_width != 0 && rq.threadgroup_width != 64 && rq.threadgroup_width != 128 &&
+        rq.threadgroup_width != 256)
+        return bad("threadgroup width exceeds the kernel's limit (0, 64, 128 or 256)");
+    return true;
+}
+
 } // namespace
 
 extern "C" {
@@ -784,15 +817,7 @@ int rp_sample(rp_scene *s, const rp_request *rq, rp_path_record *out, uint64_t c
         set_error("rp_sample: null argument");
         return RP_INVALID;
     }
-    const uint32_t npx = s->host.desc.camera.width * s->host.desc.camera.height;
-    if (rq->pixel_count == 0 || rq->sample_count == 0 || uint64_t(rq->pixel_begin) + rq->pixel_count > npx) {
-        set_error("rp_sample: pixel range outside the film or empty request");
-        return RP_INVALID;
-    }
-    if (rq->randomization > RP_RANDOMIZE_OWEN) {
-        set_error("rp_sample: unknown randomization");
-        return RP_INVALID;
-    }
+    if (!request_valid(s, *rq, "rp_sample")) return RP_INVALID;
     uint64_t n = uint64_t(rq->pixel_count) * rq->sample_count;
     if (capacity < n) {
         set_error("rp_sample: output capacity too small");
@@ -803,8 +828,17 @@ int rp_sample(rp_scene *s, const rp_request *rq, rp_path_record *out, uint64_t c
         HIP_CHECK(hipSetDevice(s->device));
         s->records.alloc(n);
         rpd::PathParams P = path_params(rq->seed, rq->randomization, rq->emergency_limit, rq->boundary_limit);
-        hipLaunchKernelGGL(k_records, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream, s->dev, P,
-                           rq->pixel_begin, rq->pixel_count, rq->sample_begin, n, s->records.p);
+        const uint32_t B = rq->threadgroup_width ? rq->threadgroup_width : uint32_t(kBlock);
+        const dim3 grid(uint32_t((n + B - 1) / B));
+        if (B == 64)
+            hipLaunchKernelGGL(k_records<64>, grid, dim3(64), 0, s->stream, s->dev, P, rq->pixel_begin,
+                               rq->pixel_count, rq->sample_begin, n, s->records.p);
+        else if (B == 256)
+            hipLaunchKernelGGL(k_records<256>, grid, dim3(256), 0, s->stream, s->dev, P, rq->pixel_begin,
+                               rq->pixel_count, rq->sample_begin, n, s->records.p);
+        else
+            hipLaunchKernelGGL(k_records<128>, grid, dim3(128), 0, s->stream, s->dev, P, rq->pixel_begin,
+                               rq->pixel_count, rq->sample_begin, n, s->records.p);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipMemcpyAsync(out, s->records.p, n * sizeof(rp_path_record), hipMemcpyDeviceToHost, s->stream));
         HIP_CHECK(hipStreamSynchronize(s->stream));

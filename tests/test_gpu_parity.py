@@ -60,6 +60,26 @@ def test_cornell_randomizations(rp, orc, cornell, mode):
     _records_parity(rp, orc, synth, scene, req)
 
 
+def test_request_fields(rp, cornell):
+    """RenderRequest fields: the workgroup width, an explicit raster size and
+    a containing sample budget leave the records unchanged; a raster size
+    other than the camera's, another sampler kind, a budget that does not
+    contain the range and an unsupported width are refused."""
+    _, scene = cornell
+    base = dict(seed=5, pixel_begin=100, pixel_count=3000, sample_begin=2, sample_count=2)
+    want = scene.sample(rp.request(**base))
+    assert (want["filter_weight"] == 1).all() and (want["splat_count"] == 0).all()
+    for kw in (dict(threadgroup_width=64), dict(threadgroup_width=256), dict(width=64, height=64),
+               dict(total_samples=4, material_seed=-3)):
+        got = scene.sample(rp.request(**base, **kw))
+        assert np.array_equal(got, want), kw
+    for kw in (dict(width=32, height=64), dict(sampler_kind=rp.SAMPLER_HALTON), dict(total_samples=3),
+               dict(threadgroup_width=96), dict(pixel_count=4096)):
+        with pytest.raises(rp.RypathError) as e:
+            scene.sample(rp.request(**dict(base, **kw)))
+        assert e.value.code == rp.INVALID, kw
+
+
 def test_cornell_closest_hits(rp, orc, cornell):
     synth, scene = cornell
     rng = np.random.default_rng(5)
