@@ -144,7 +144,19 @@ typedef struct rp_scene_desc {
     rp_camera camera;
     uint32_t max_depth;          /* PBRT maxdepth: scattering events */
     uint32_t hero_wavelengths;   /* 0 = uniform (1/470), 1 = visible (PBRT fit) */
+    uint32_t light_sampler;      /* rp_light_sampler (RenderScene::light_sampler, src/render.hpp:269) */
+    uint32_t pad;
 } rp_scene_desc;
+
+/* LightSamplerKind (src/render.hpp:208): how next-event estimation picks a light.
+ * BVH: PBRT's light bounding-volume hierarchy, importance by position and
+ * normal of the receiving point (the reference's default); POWER: an alias
+ * table proportional to emitted power; UNIFORM: every light alike.  Quad
+ * emitters (two consecutive triangles (a,b,c), (a,c,d) of one emissive object
+ * forming a rectangle) are one light, sampled over their spherical rectangle. */
+typedef enum rp_light_sampler {
+    RP_LIGHT_SAMPLER_BVH = 0, RP_LIGHT_SAMPLER_POWER = 1, RP_LIGHT_SAMPLER_UNIFORM = 2
+} rp_light_sampler;
 
 typedef struct rp_build_options {
     uint32_t leaf_size;       /* max triangles per BVH leaf (1..8), 0 = default 4 */

@@ -7,6 +7,7 @@
 #include "oracle_internal.h"
 
 #include <atomic>
+#include <functional>
 #include <numeric>
 #include <thread>
 
@@ -14,10 +15,23 @@ namespace orc {
 
 // =========================================================== scene + BVH
 struct Light {
-    uint32_t kind; // 0 triangle, 1 spot, 2 environment
-    uint32_t index;
-    float pmf, cdf;
-    float inv_area;
+    uint32_t kind;   // 0 triangle, 1 spot, 2 environment, 3 quad (rectangle of two triangles)
+    uint32_t index;  // triangle id (a quad's first triangle) or spot index
+    uint32_t index2; // a quad's second triangle
+    uint32_t leaf;   // node of the light BVH holding this light (RP_NONE: infinite or powerless)
+    float pmf;       // power sampler: selection probability
+    float threshold; // power sampler: alias threshold
+    uint32_t alias;  // power sampler: alias light
+    float inv_area;  // triangle / quad
+    V3 p00, ex, ey;  // quad: p00 + u ex + v ey, u, v in [0, 1]
+    bool rect;       // quad: passes the binary32 rectangle test (spherical sampling allowed)
+};
+// Light-BVH node (PBRT v4 LightBounds + tree links): bounds, direction cone,
+// emission cutoff, power normalised by the tree's total; flags bit 0 = two-sided.
+struct LightNode {
+    float low[3], high[3], dir[3];
+    float cos_outer, cos_emission, power;
+    uint32_t child, parent, light, flags;
 };
 
 struct Scene {
@@ -38,6 +52,9 @@ struct Scene {
     std::vector<Node> nodes;
     std::vector<uint32_t> order;
     std::vector<Light> lights;
+    std::vector<LightNode> lnodes;   // light BVH (node i + 1 is the first child of an interior node i)
+    uint32_t env_light = RP_NONE;    // the environment's light id
+    float env_pmf = 0;               // its selection probability (context independent)
     std::vector<uint32_t> tri_light; // triangle -> light index or RP_NONE
     float scene_radius = 0;
     uint32_t m = 0;
@@ -572,7 +589,7 @@ static void dirac_lane(const Lane &l, V3 wo, V3 wi, float uc, float u0, float u1
 enum Slip : int {
     SLIP_NONE = 0,
     SLIP_EMISSION_UNWEIGHTED = 1,    // emission reached by BSDF sampling added without its MIS weight
-    SLIP_LIGHT_PMF_DROPPED = 2,      // area-light solid-angle pdf without the light-selection pmf
+    SLIP_LIGHT_PMF_DROPPED = 2,      // emission-hit MIS: area-light pdf without the selection pmf
     SLIP_ROULETTE_UNCOMPENSATED = 3, // survivors not divided by the survival probability
     SLIP_PHASE_SIGN = 4,             // phase function of medium NEE evaluated at -cos
     SLIP_NEE_NO_TRANSMITTANCE = 5,   // medium NEE without the shadow ray's transmittance
@@ -580,60 +597,367 @@ enum Slip : int {
     SLIP_SPOT_NO_FALLOFF = 7,        // spot light without its cone falloff
     SLIP_NEE_NO_COSINE = 8,          // surface NEE without the receiver cosine
     SLIP_HERO_TRANSMITTANCE = 9,     // every lane's pass-through transmittance at the hero's sigma_t
-    SLIP_CONTINUATION_NO_COSINE = 10 // continuation weight f/pdf without |cos|
+    SLIP_CONTINUATION_NO_COSINE = 10, // continuation weight f/pdf without |cos|
+    SLIP_BVH_PMF_NO_NORMAL = 11,     // emission-hit MIS: light-BVH pmf without the receiver normal
+    SLIP_RECT_PDF_UNWARPED = 12      // quad proposal density without its cosine warp
 };
 static std::atomic<int> g_slip{SLIP_NONE};
 static bool slip(Slip v) { return g_slip.load(std::memory_order_relaxed) == v; }
 
 // ================================================================ lights
-static float avg_spectrum(const Scene &s, const rp_spectrum &sp) {
-    double acc = 0;
-    for (int i = 0; i < 471; ++i) acc += double(spectrum_eval(sp, float(360 + i), s.spectra.data()));
-    return float(acc / 471.0);
+// Light tables restated from the reference's preparation (src/metal_render.mm:
+// 540-640 and 1560-1780: powers, bounds, cones) and build_light_sampling
+// (src/light_sampling.cpp:226-292 unite / cost, 382-571 alias table and BVH),
+// which follow PBRT v4's BVHLightSampler and PowerLightSampler.
+struct LightInput {
+    double power = 0, bounds_power = -1;
+    float low[3] = {0, 0, 0}, high[3] = {0, 0, 0}, dir[3] = {0, 0, 1};
+    float cos_outer = -1, cos_emission = 0;
+    bool two_sided = true, infinite = false;
+};
+
+// Power proxy of a spectrum: the visible-wavelength estimate at u = 1/2 of
+// its integral over nanometres (zero: the plain sum over 360..830 nm).
+static double spectrum_power(const Scene &s, const rp_spectrum &sp) {
+    Wavelengths w = sample_wavelengths(0.5f, 1);
+    double v = 0;
+    for (int k = 0; k < 4; ++k)
+        v += double(spectrum_eval(sp, std::round(w.lambda[k]), s.spectra.data())) / double(w.pdf[k]) / 4.0;
+    if (v == 0)
+        for (int i = 0; i < 471; ++i) v += double(spectrum_eval(sp, float(360 + i), s.spectra.data()));
+    return v;
+}
+static double spectrum_max(const Scene &s, const rp_spectrum &sp) {
+    double m = 0;
+    for (int i = 0; i < 471; ++i) m = std::max(m, double(spectrum_eval(sp, float(360 + i), s.spectra.data())));
+    return m;
+}
+
+using D3 = std::array<double, 3>;
+static D3 dsub(D3 a, D3 b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+static double ddot(D3 a, D3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double dlen(D3 a) { return std::sqrt(ddot(a, a)); }
+static D3 dcross(D3 a, D3 b) { return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}; }
+static D3 dvec(V3 v) { return {double(v.x), double(v.y), double(v.z)}; }
+
+// Union of two bounds: boxes, powers, two-sidedness, the smaller emission
+// cosine, and the smallest cone holding both direction cones.
+static LightInput bounds_union(const LightInput &a, const LightInput &b) {
+    if (a.power == 0) return b;
+    if (b.power == 0) return a;
+    LightInput r = a;
+    r.power = a.power + b.power;
+    r.two_sided = a.two_sided || b.two_sided;
+    r.cos_emission = std::min(a.cos_emission, b.cos_emission);
+    for (int i = 0; i < 3; ++i) {
+        r.low[i] = std::min(a.low[i], b.low[i]);
+        r.high[i] = std::max(a.high[i], b.high[i]);
+    }
+    const double pi = 3.14159265358979323846;
+    const double ta = std::acos(std::clamp(double(a.cos_outer), -1.0, 1.0));
+    const double tb = std::acos(std::clamp(double(b.cos_outer), -1.0, 1.0));
+    const D3 wa = {a.dir[0], a.dir[1], a.dir[2]}, wb = {b.dir[0], b.dir[1], b.dir[2]};
+    const double between = std::acos(std::clamp(ddot(wa, wb), -1.0, 1.0));
+    if (std::min(between + tb, pi) <= ta) return r; // b's cone inside a's
+    if (std::min(between + ta, pi) <= tb) {         // a's cone inside b's
+        for (int i = 0; i < 3; ++i) r.dir[i] = b.dir[i];
+        r.cos_outer = b.cos_outer;
+        return r;
+    }
+    const double theta = (ta + tb + between) / 2 + 1e-6;
+    if (theta >= pi) {
+        r.cos_outer = -1;
+        r.dir[0] = r.dir[1] = 0;
+        r.dir[2] = 1;
+        return r;
+    }
+    // rotate wa toward wb by theta - ta
+    D3 t;
+    for (int i = 0; i < 3; ++i) t[i] = wb[i] - ddot(wa, wb) * wa[i];
+    double tl = dlen(t);
+    if (tl < 1e-12) {
+        int axis = std::fabs(wa[0]) < 0.8 ? 0 : 1;
+        D3 e = {0, 0, 0};
+        e[axis] = 1;
+        for (int i = 0; i < 3; ++i) t[i] = e[i] - wa[axis] * wa[i];
+        tl = dlen(t);
+    }
+    const double turn = theta - ta;
+    for (int i = 0; i < 3; ++i) r.dir[i] = float(std::cos(turn) * wa[i] + std::sin(turn) * t[i] / tl);
+    const double dl = std::sqrt(double(r.dir[0]) * r.dir[0] + double(r.dir[1]) * r.dir[1] + double(r.dir[2]) * r.dir[2]);
+    for (int i = 0; i < 3; ++i) r.dir[i] = float(r.dir[i] / dl);
+    r.cos_outer = std::nextafter(float(std::cos(theta)), -INFINITY);
+    return r;
+}
+
+// Split cost of a cluster along `axis` of the parent: power x the cone's
+// solid-angle measure x surface area, with the parent's aspect ratio.
+static double bounds_cost(const LightInput &b, const LightInput &all, int axis) {
+    if (b.power == 0) return 0;
+    const double pi = 3.14159265358979323846;
+    const double o = std::acos(std::clamp(double(b.cos_outer), -1.0, 1.0));
+    const double e = std::acos(std::clamp(double(b.cos_emission), -1.0, 1.0));
+    const double w = std::min(o + e, pi);
+    const double omega = 2 * pi * (1 - b.cos_outer) +
+                         pi / 2 * (2 * w * std::sin(o) - std::cos(o - 2 * w) - 2 * o * std::sin(o) + b.cos_outer);
+    double d[3], ad[3];
+    for (int i = 0; i < 3; ++i) {
+        d[i] = double(b.high[i]) - double(b.low[i]);
+        ad[i] = double(all.high[i]) - double(all.low[i]);
+    }
+    if (ad[axis] <= 0) return INFINITY;
+    const double longest = std::max(ad[0], std::max(ad[1], ad[2]));
+    return b.power * omega * (longest / ad[axis]) * 2 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+
+// The reference's rectangle classification of a patch p = (p00, p10, p01, p11)
+// (src/light_sampling.cpp:294-311): planar, equal corner distances from the centre.
+static bool patch_rectangle(const D3 p[4]) {
+    const D3 n = dcross(dsub(p[1], p[0]), dsub(p[2], p[0]));
+    const double nl = dlen(n), diag = dlen(dsub(p[3], p[0]));
+    bool ok = nl > 0 && diag > 0 && std::fabs(ddot(dsub(p[3], p[0]), n) / (diag * nl)) <= 1e-5;
+    D3 c = {0, 0, 0};
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 3; ++a) c[a] += p[i][a] / 4;
+    const double d0 = ddot(dsub(p[0], c), dsub(p[0], c));
+    const int next[4] = {2, 0, 3, 1};
+    for (int i = 0; i < 4; ++i) {
+        const D3 dl = dsub(p[i], c);
+        ok = ok && dlen(dsub(p[i], p[next[i]])) > 0 && d0 > 0 && std::fabs(ddot(dl, dl) - d0) / d0 <= 1e-4;
+    }
+    return ok;
+}
+// The binary32 affine-rectangle test of the sampler (an exact corner and a right angle).
+static bool rect_f(V3 p00, V3 p10, V3 p01, V3 p11) {
+    const V3 ex = p10 - p00, ey = p01 - p00;
+    const float x2 = dot(ex, ex), y2 = dot(ey, ey);
+    if (!(x2 > 0 && y2 > 0)) return false;
+    const V3 e = p11 - (p00 + ex + ey);
+    return dot(e, e) <= 0x1p-40f * std::max(x2, y2) && std::fabs(dot(ex, ey)) <= 0x1p-20f * std::sqrt(x2 * y2);
 }
 
 static void build_lights(Scene &s) {
-    std::vector<double> power;
+    const double pi = 3.14159265358979323846;
     s.lights.clear();
+    s.lnodes.clear();
+    s.env_light = RP_NONE;
     s.tri_light.assign(s.d.triangle_count, RP_NONE);
+    std::vector<LightInput> in;
+    auto box_of = [](LightInput &b, const V3 *c, int n) {
+        for (int a = 0; a < 3; ++a) {
+            b.low[a] = INFINITY;
+            b.high[a] = -INFINITY;
+        }
+        for (int k = 0; k < n; ++k)
+            for (int a = 0; a < 3; ++a) {
+                b.low[a] = std::min(b.low[a], get(c[k], a));
+                b.high[a] = std::max(b.high[a], get(c[k], a));
+            }
+    };
+    // area lights: emissive triangles in id order; (a,b,c) followed by (a,c,d)
+    // of the same object forming a rectangle are one quad light
     for (uint32_t t = 0; t < s.d.triangle_count; ++t) {
-        const auto &obj = s.objects[s.tri_object[t]];
+        const uint32_t o = s.tri_object[t];
+        const auto &obj = s.objects[o];
         if (obj.emission == RP_NONE) continue;
         const auto &em = s.emissions[obj.emission];
-        V3 a, b, c;
-        s.tri(t, a, b, c);
-        double ex = double(b.x) - a.x, ey = double(b.y) - a.y, ez = double(b.z) - a.z;
-        double fx = double(c.x) - a.x, fy = double(c.y) - a.y, fz = double(c.z) - a.z;
-        double cx = ey * fz - ez * fy, cy = ez * fx - ex * fz, cz = ex * fy - ey * fx;
-        double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        const double mean = spectrum_power(s, em.radiance), bound = spectrum_max(s, em.radiance);
+        const uint32_t *ia = &s.indices[3 * size_t(t)];
+        if (t + 1 < s.d.triangle_count && s.tri_object[t + 1] == o && s.indices[3 * size_t(t) + 3] == ia[0] &&
+            s.indices[3 * size_t(t) + 4] == ia[2]) {
+            const V3 c[4] = {s.vert(ia[0]), s.vert(ia[1]), s.vert(s.indices[3 * size_t(t) + 5]), s.vert(ia[2])};
+            const D3 p[4] = {dvec(c[0]), dvec(c[1]), dvec(c[2]), dvec(c[3])};
+            if (patch_rectangle(p) && rect_f(c[0], c[1], c[2], c[3])) {
+                const double area = dlen(dsub(p[2], p[0])) * dlen(dsub(p[1], p[0]));
+                LightInput b;
+                box_of(b, c, 4);
+                b.power = pi * area * mean * (em.two_sided ? 2.0 : 1.0);
+                b.bounds_power = pi * area * bound;
+                b.two_sided = em.two_sided != 0;
+                // normal cone of the four corner normals
+                const D3 ns[4] = {dcross(dsub(p[1], p[0]), dsub(p[2], p[0])), dcross(dsub(p[1], p[0]), dsub(p[3], p[1])),
+                                  dcross(dsub(p[3], p[2]), dsub(p[2], p[0])), dcross(dsub(p[3], p[2]), dsub(p[3], p[1]))};
+                D3 axis = {0, 0, 0};
+                bool degenerate = false;
+                for (const D3 &n : ns) {
+                    const double l = dlen(n);
+                    if (!(l > 0)) degenerate = true;
+                    else
+                        for (int a = 0; a < 3; ++a) axis[a] += n[a] / l;
+                }
+                const double al = dlen(axis);
+                if (!degenerate && al > 0) {
+                    double cmin = 1;
+                    for (const D3 &n : ns) cmin = std::min(cmin, (ddot(axis, n) / al) / dlen(n));
+                    for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a] / al);
+                    b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
+                }
+                Light L{3u, t, t + 1, RP_NONE, 0, 1, 0, float(1.0 / area), c[0], c[1] - c[0], c[2] - c[0], true};
+                s.tri_light[t] = s.tri_light[t + 1] = uint32_t(s.lights.size());
+                s.lights.push_back(L);
+                in.push_back(b);
+                ++t;
+                continue;
+            }
+        }
+        V3 c[3];
+        s.tri(t, c[0], c[1], c[2]);
+        const D3 n = dcross(dsub(dvec(c[1]), dvec(c[0])), dsub(dvec(c[2]), dvec(c[0])));
+        const double area = 0.5 * dlen(n);
         if (!(area > 0)) continue;
-        double p = double(kPi) * area * double(avg_spectrum(s, em.radiance)) * (em.two_sided ? 2.0 : 1.0);
+        LightInput b;
+        box_of(b, c, 3);
+        b.power = pi * area * mean * (em.two_sided ? 2.0 : 1.0);
+        b.bounds_power = pi * area * bound;
+        b.two_sided = em.two_sided != 0;
+        for (int a = 0; a < 3; ++a) b.dir[a] = float(n[a] / (2 * area));
+        b.cos_outer = std::nextafter(1.0f, 0.0f);
         s.tri_light[t] = uint32_t(s.lights.size());
-        s.lights.push_back({0u, t, 0.0f, 0.0f, float(1.0 / area)});
-        power.push_back(p);
+        s.lights.push_back({0u, t, RP_NONE, RP_NONE, 0, 1, 0, float(1.0 / area), {}, {}, {}, false});
+        in.push_back(b);
     }
     for (uint32_t i = 0; i < s.spots.size(); ++i) {
         const auto &sp = s.spots[i];
-        double p = double(avg_spectrum(s, sp.intensity)) * 2.0 * double(kPi) *
-                   (1.0 - 0.5 * (double(sp.cos_falloff_start) + double(sp.cos_falloff_end)));
-        s.lights.push_back({1u, i, 0.0f, 0.0f, 0.0f});
-        power.push_back(p);
+        LightInput b;
+        for (int a = 0; a < 3; ++a) b.low[a] = b.high[a] = sp.position[a];
+        b.power = 2 * pi * ((1 - double(sp.cos_falloff_start)) + (double(sp.cos_falloff_start) - sp.cos_falloff_end) / 2) *
+                  spectrum_power(s, sp.intensity);
+        b.bounds_power = 4 * pi * spectrum_max(s, sp.intensity);
+        b.two_sided = false;
+        float ce = float(std::cos(std::acos(double(sp.cos_falloff_end)) - std::acos(double(sp.cos_falloff_start))));
+        if (ce == 1) ce = sp.cos_falloff_end != sp.cos_falloff_start ? 0.999f : std::nextafter(1.0f, 0.0f);
+        for (int a = 0; a < 3; ++a) b.dir[a] = sp.direction[a];
+        b.cos_outer = float(std::cos(std::min(pi, std::acos(std::clamp(double(sp.cos_falloff_start), -1.0, 1.0)) + 2e-6)));
+        b.cos_outer = std::max(-1.0f, std::nextafter(b.cos_outer, -INFINITY));
+        b.cos_emission = ce;
+        s.lights.push_back({1u, i, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false});
+        in.push_back(b);
     }
     if (s.d.has_environment) {
-        double r = s.scene_radius;
-        double p = double(kPi) * r * r * 4.0 * double(kPi) * double(avg_spectrum(s, s.d.environment));
-        s.lights.push_back({2u, 0u, 0.0f, 0.0f, 0.0f});
-        power.push_back(p);
+        LightInput b;
+        b.infinite = true;
+        const double r = s.scene_radius;
+        b.power = 4 * pi * pi * r * r * spectrum_power(s, s.d.environment);
+        s.env_light = uint32_t(s.lights.size());
+        s.lights.push_back({2u, 0u, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false});
+        in.push_back(b);
     }
-    double total = 0;
-    for (double p : power) total += p;
-    double cum = 0;
-    for (size_t i = 0; i < s.lights.size(); ++i) {
-        cum += power[i];
-        s.lights[i].pmf = total > 0 ? float(power[i] / total) : 0.0f;
-        s.lights[i].cdf = total > 0 ? float(cum / total) : 0.0f;
+    const size_t n = in.size();
+    if (n == 0) return;
+    // alias table over the powers (Walker / Vose, small and large stacks)
+    double total = 0, bounds_total = 0;
+    std::vector<double> prob(n);
+    std::vector<LightInput> bv = in;
+    std::vector<uint32_t> finite;
+    for (size_t i = 0; i < n; ++i) {
+        bv[i].power = in[i].bounds_power < 0 ? in[i].power : in[i].bounds_power;
+        if (bv[i].power == 0 && in[i].power > 0) bv[i].power = in[i].power;
+        prob[i] = in[i].power == 0 && bv[i].power > 0 ? bv[i].power : in[i].power;
+        total += prob[i];
+        if (!in[i].infinite) bounds_total += bv[i].power;
+        if (!in[i].infinite && bv[i].power > 0) finite.push_back(uint32_t(i));
     }
-    if (!s.lights.empty() && total > 0) s.lights.back().cdf = 1.0f;
+    std::vector<uint32_t> small, large;
+    for (size_t i = 0; i < n; ++i) {
+        const double pr = total > 0 ? prob[i] / total : 1.0 / double(n);
+        s.lights[i].pmf = float(pr);
+        prob[i] = pr * double(n);
+        (prob[i] < 1 ? small : large).push_back(uint32_t(i));
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t a = small.back();
+        small.pop_back();
+        const uint32_t b = large.back();
+        large.pop_back();
+        s.lights[a].threshold = float(prob[a]);
+        s.lights[a].alias = b;
+        prob[b] -= 1 - prob[a];
+        (prob[b] < 1 ? small : large).push_back(b);
+    }
+    for (uint32_t i : small) s.lights[i].threshold = 1, s.lights[i].alias = i;
+    for (uint32_t i : large) s.lights[i].threshold = 1, s.lights[i].alias = i;
+    // the BVH over the finite lights: 12 buckets per axis, splits 1..10 scanned
+    auto bucket_of = [&](uint32_t id, int axis, const double lo[3], const double hi[3]) {
+        const double c = (double(bv[id].low[axis]) + bv[id].high[axis]) / 2;
+        return std::min(11, int(12 * (c - lo[axis]) / (hi[axis] - lo[axis])));
+    };
+    std::function<uint32_t(size_t, size_t, uint32_t, unsigned)> build = [&](size_t b, size_t e, uint32_t parent,
+                                                                              unsigned depth) -> uint32_t {
+        const uint32_t index = uint32_t(s.lnodes.size());
+        s.lnodes.emplace_back();
+        LightInput all;
+        double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t k = b; k < e; ++k) {
+            const LightInput &v = bv[finite[k]];
+            all = bounds_union(all, v);
+            for (int a = 0; a < 3; ++a) {
+                const double c = (double(v.low[a]) + v.high[a]) / 2;
+                clo[a] = std::min(clo[a], c);
+                chi[a] = std::max(chi[a], c);
+            }
+        }
+        LightNode node{};
+        for (int a = 0; a < 3; ++a) {
+            node.low[a] = all.low[a];
+            node.high[a] = all.high[a];
+            node.dir[a] = all.dir[a];
+        }
+        node.cos_outer = all.cos_outer;
+        node.cos_emission = all.cos_emission;
+        node.power = float(all.power / bounds_total);
+        node.parent = parent;
+        node.light = RP_NONE;
+        node.child = RP_NONE;
+        node.flags = all.two_sided ? 1u : 0u;
+        if (e - b == 1) {
+            node.light = finite[b];
+            s.lights[finite[b]].leaf = index;
+        } else {
+            double best = INFINITY;
+            int best_axis = 0, best_split = 0;
+            bool found = false;
+            if (depth < 48)
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (clo[axis] == chi[axis]) continue;
+                    LightInput bucket[12];
+                    for (size_t k = b; k < e; ++k) {
+                        const uint32_t id = finite[k];
+                        const int q = bucket_of(id, axis, clo, chi);
+                        bucket[q] = bounds_union(bucket[q], bv[id]);
+                    }
+                    for (int split = 1; split < 11; ++split) {
+                        LightInput left, right;
+                        for (int k = 0; k <= split; ++k) left = bounds_union(left, bucket[k]);
+                        for (int k = split + 1; k < 12; ++k) right = bounds_union(right, bucket[k]);
+                        const double c = bounds_cost(left, all, axis) + bounds_cost(right, all, axis);
+                        if (c > 0 && c < best) {
+                            best = c;
+                            best_axis = axis;
+                            best_split = split;
+                            found = true;
+                        }
+                    }
+                }
+            size_t mid = b + (e - b) / 2;
+            if (found) {
+                auto it = std::stable_partition(finite.begin() + ptrdiff_t(b), finite.begin() + ptrdiff_t(e),
+                                                [&](uint32_t id) { return bucket_of(id, best_axis, clo, chi) <= best_split; });
+                const size_t cand = size_t(it - finite.begin());
+                if (cand > b && cand < e) mid = cand;
+            }
+            build(b, mid, index, depth + 1);
+            node.child = build(mid, e, index, depth + 1);
+        }
+        s.lnodes[index] = node;
+        return index;
+    };
+    if (!finite.empty()) build(0, finite.size(), RP_NONE, 0);
+    if (s.env_light != RP_NONE) {
+        const uint32_t sampler = s.d.light_sampler;
+        s.env_pmf = sampler == RP_LIGHT_SAMPLER_UNIFORM ? 1.0f / float(n)
+                    : sampler == RP_LIGHT_SAMPLER_POWER ? s.lights[s.env_light].pmf
+                                                        : 1.0f / float(1 + (s.lnodes.empty() ? 0 : 1));
+    }
 }
 
 static void compute_radius(Scene &s) {
@@ -826,48 +1150,292 @@ struct LightSample {
     float dist = 0;
 };
 
-static uint32_t select_light(const Scene &s, float u) {
-    uint32_t lo = 0, hi = uint32_t(s.lights.size());
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) / 2;
-        if (u < s.lights[mid].cdf) hi = mid;
-        else lo = mid + 1;
+// --- light selection (shaders/light_sampling.h:19-127, PBRT BVHLightSampler)
+// Importance of a light-BVH node for a receiving point p with normal n
+// (n = 0: a medium vertex): power x the cosine bound of the direction cone
+// against the bounding sphere, over the squared distance.
+static float light_importance(const LightNode &b, V3 p, V3 n) {
+    const V3 lo = {b.low[0], b.low[1], b.low[2]}, hi = {b.high[0], b.high[1], b.high[2]};
+    const V3 c = (lo + hi) * 0.5f;
+    const V3 delta = p - c;
+    const float r2 = dot(hi - lo, hi - lo) * 0.25f, d2 = dot(delta, delta);
+    const float radius = std::sqrt(r2), dist = std::sqrt(d2);
+    const float cb = d2 <= r2 ? -1.0f : std::sqrt(std::max(0.0f, 1.0f - r2 / d2));
+    const float sb = std::sqrt(std::max(0.0f, 1.0f - cb * cb));
+    const V3 wi = dist > 0 ? delta / dist : V3{0, 0, 1};
+    float cw = std::clamp(dot(V3{b.dir[0], b.dir[1], b.dir[2]}, wi), -1.0f, 1.0f);
+    if (b.flags & 1u) cw = std::fabs(cw);
+    const float sw = std::sqrt(std::max(0.0f, 1.0f - cw * cw));
+    const float co = b.cos_outer, so = std::sqrt(std::max(0.0f, 1.0f - co * co));
+    const float cx = cw > co ? 1.0f : cw * co + sw * so, sx = cw > co ? 0.0f : sw * co - cw * so;
+    const float cp = cx > cb ? 1.0f : cx * cb + sx * sb;
+    if (cp <= b.cos_emission) return 0.0f;
+    float value = b.power * cp / std::max(d2, std::max(radius, FLT_MIN));
+    if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) {
+        const float ci = std::clamp(std::fabs(dot(wi, n)), 0.0f, 1.0f);
+        const float si = std::sqrt(std::max(0.0f, 1.0f - ci * ci));
+        value = value * (ci > cb ? 1.0f : ci * cb + si * sb);
     }
-    return std::min(lo, uint32_t(s.lights.size()) - 1);
+    return std::max(0.0f, value);
 }
 
-static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, float ul, float u0, float u1) {
-    LightSample ls;
-    if (s.lights.empty()) return ls;
-    uint32_t li = select_light(s, ul);
-    const Light &L = s.lights[li];
-    if (!(L.pmf > 0)) return ls;
-    if (L.kind == 0) {
-        V3 a, b, c;
-        s.tri(L.index, a, b, c);
-        float b0, b1;
-        if (u0 < u1) {
-            b0 = u0 * 0.5f;
-            b1 = u1 - b0;
-        } else {
-            b1 = u1 * 0.5f;
-            b0 = u0 - b1;
+struct Selected {
+    uint32_t light = RP_NONE;
+    float pmf = 0.0f;
+};
+static Selected light_select(const Scene &s, V3 p, V3 n, float u) {
+    const uint32_t count = uint32_t(s.lights.size());
+    if (count == 0) return {};
+    u = std::clamp(u, 0.0f, 0x1.fffffep-1f);
+    const uint32_t sampler = s.d.light_sampler;
+    if (sampler == RP_LIGHT_SAMPLER_UNIFORM) return {std::min(uint32_t(u * float(count)), count - 1), 1.0f / float(count)};
+    if (sampler == RP_LIGHT_SAMPLER_POWER) {
+        const float v = u * float(count);
+        const uint32_t i = std::min(uint32_t(v), count - 1);
+        const uint32_t id = v - float(i) < s.lights[i].threshold ? i : s.lights[i].alias;
+        return {id, s.lights[id].pmf};
+    }
+    const uint32_t inf = s.env_light != RP_NONE ? 1u : 0u, tree = s.lnodes.empty() ? 0u : 1u;
+    const float infinite = float(inf) / float(inf + tree);
+    if (u < infinite) return {s.env_light, 1.0f / float(inf + tree)};
+    if (!tree) return {};
+    u = std::min((u - infinite) / (1.0f - infinite), 0x1.fffffep-1f);
+    float pmf = 1.0f - infinite;
+    uint32_t i = 0;
+    for (size_t steps = 0; steps < s.lnodes.size(); ++steps) {
+        const LightNode &node = s.lnodes[i];
+        if (node.light != RP_NONE) {
+            if (light_importance(node, p, n) <= 0.0f) return {};
+            return {node.light, pmf};
         }
-        float b2 = 1.0f - b0 - b1;
-        ls.p = b0 * a + b1 * b + b2 * c;
-        ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
-        ls.n = normalize(cross(a - c, b - c));
-        V3 dv = ls.p - p;
-        float d2 = dot(dv, dv);
+        const float a = light_importance(s.lnodes[i + 1], p, n), b = light_importance(s.lnodes[node.child], p, n);
+        const float sum = a + b;
+        if (!(sum > 0.0f)) return {};
+        const float q = a / sum;
+        if (u < q) {
+            pmf = pmf * q;
+            u = std::min(u / q, 0x1.fffffep-1f);
+            i = i + 1;
+        } else {
+            pmf = pmf * (1.0f - q);
+            u = std::min((u - q) / (1.0f - q), 0x1.fffffep-1f);
+            i = node.child;
+        }
+    }
+    return {};
+}
+static float light_pmf(const Scene &s, V3 p, V3 n, uint32_t id) {
+    const uint32_t count = uint32_t(s.lights.size());
+    if (id >= count) return 0.0f;
+    const uint32_t sampler = s.d.light_sampler;
+    if (sampler == RP_LIGHT_SAMPLER_UNIFORM) return 1.0f / float(count);
+    if (sampler == RP_LIGHT_SAMPLER_POWER) return s.lights[id].pmf;
+    const uint32_t inf = s.env_light != RP_NONE ? 1u : 0u, tree = s.lnodes.empty() ? 0u : 1u;
+    if (id == s.env_light) return 1.0f / float(inf + tree);
+    uint32_t i = s.lights[id].leaf;
+    if (i >= s.lnodes.size()) return 0.0f;
+    if (light_importance(s.lnodes[i], p, n) <= 0.0f) return 0.0f;
+    float pmf = 1.0f - float(inf) / float(inf + tree);
+    for (size_t steps = 0; steps < s.lnodes.size(); ++steps) {
+        const uint32_t parent = s.lnodes[i].parent;
+        if (parent == RP_NONE) return pmf;
+        const float a = light_importance(s.lnodes[parent + 1], p, n);
+        const float b = light_importance(s.lnodes[s.lnodes[parent].child], p, n);
+        const float sum = a + b;
+        if (!(sum > 0.0f)) return 0.0f;
+        pmf = pmf * (i == parent + 1 ? a / sum : b / sum);
+        i = parent;
+    }
+    return 0.0f;
+}
+
+// --- quad emitters: spherical-rectangle proposal (Urena, Fajardo, King 2013,
+// as adapted in src/light_geometry_sampling_shared.h:174-260 and
+// shaders/light_geometry_sampling.h:118-209), warped by the receiver's
+// corner cosines (bilinear, PBRT SampleBilinear) when a normal is given.
+static V3 unit(V3 v) { return v / length(v); }
+static float angle_between(V3 a, V3 b) {
+    return dot(a, b) < 0 ? kPi - 2.0f * asin_f(std::min(1.0f, length(a + b) / 2.0f))
+                         : 2.0f * asin_f(std::min(1.0f, length(a - b) / 2.0f));
+}
+static float spherical_triangle_area(V3 a, V3 b, V3 c) {
+    return std::fabs(2.0f * atan2_f(dot(a, cross3(b, c)), 1.0f + dot(a, b) + dot(a, c) + dot(b, c)));
+}
+static float triangle_solid_angle(V3 p0, V3 p1, V3 p2, V3 r) {
+    const V3 a = p0 - r, b = p1 - r, c = p2 - r;
+    if (!(dot(a, a) > 0) || !(dot(b, b) > 0) || !(dot(c, c) > 0)) return 0.0f;
+    return spherical_triangle_area(unit(a), unit(b), unit(c));
+}
+static float quad_solid_angle(const Light &L, V3 r) {
+    const V3 p10 = L.p00 + L.ex, p01 = L.p00 + L.ey, p11 = L.p00 + L.ex + L.ey;
+    return triangle_solid_angle(L.p00, p10, p11, r) + triangle_solid_angle(L.p00, p11, p01, r);
+}
+struct Bilinear {
+    float a, b, c, d; // weights at (0,0), (1,0), (0,1), (1,1)
+};
+static Bilinear quad_cosines(const Light &L, V3 r, V3 n) {
+    const V3 p10 = L.p00 + L.ex, p01 = L.p00 + L.ey, p11 = L.p00 + L.ex + L.ey;
+    return {std::max(0.01f, std::fabs(dot(unit(L.p00 - r), n))), std::max(0.01f, std::fabs(dot(unit(p10 - r), n))),
+            std::max(0.01f, std::fabs(dot(unit(p01 - r), n))), std::max(0.01f, std::fabs(dot(unit(p11 - r), n)))};
+}
+static Bilinear bilinear_normalized(Bilinear w) {
+    const float m = std::max(std::max(w.a, w.b), std::max(w.c, w.d));
+    return m > 0 ? Bilinear{w.a / m, w.b / m, w.c / m, w.d / m} : Bilinear{1, 1, 1, 1};
+}
+// Inverse CDF of the density proportional to (1 - x) a + x b on [0, 1].
+static float linear_warp(float u, float a, float b) {
+    if (u == 0.0f && a == 0.0f) return 0.0f;
+    return std::min(0x1.fffffep-1f, u * (a + b) / (a + std::sqrt((1.0f - u) * a * a + u * b * b)));
+}
+static void bilinear_warp(float &u0, float &u1, Bilinear w) {
+    w = bilinear_normalized(w);
+    const float y = linear_warp(u1, w.a + w.b, w.c + w.d);
+    const float x = linear_warp(u0, (1.0f - y) * w.a + y * w.c, (1.0f - y) * w.b + y * w.d);
+    u0 = x;
+    u1 = y;
+}
+static float bilinear_density(float x, float y, Bilinear w) {
+    if (x < 0 || x > 1 || y < 0 || y > 1) return 0.0f;
+    w = bilinear_normalized(w);
+    return 4.0f * ((1.0f - x) * (1.0f - y) * w.a + x * (1.0f - y) * w.b + (1.0f - x) * y * w.c + x * y * w.d) /
+           (w.a + w.b + w.c + w.d);
+}
+struct RectFrame {
+    V3 x, y, z;
+    float x0, x1, y0, y1, z0, b0, b1, g2, g3, area;
+};
+static RectFrame rect_frame(const Light &L, V3 r) {
+    RectFrame f;
+    f.x = unit(L.ex);
+    f.y = unit(L.ey);
+    f.z = cross3(f.x, f.y);
+    const V3 delta = L.p00 - r;
+    f.x0 = dot(delta, f.x);
+    f.x1 = f.x0 + length(L.ex);
+    f.y0 = dot(delta, f.y);
+    f.y1 = f.y0 + length(L.ey);
+    f.z0 = dot(delta, f.z);
+    if (f.z0 > 0) {
+        f.z = -f.z;
+        f.z0 = -f.z0;
+    }
+    const V3 v00 = {f.x0, f.y0, f.z0}, v10 = {f.x1, f.y0, f.z0}, v11 = {f.x1, f.y1, f.z0}, v01 = {f.x0, f.y1, f.z0};
+    const V3 n0 = unit(cross3(v00, v10)), n2 = unit(cross3(v11, v01)), n3 = unit(cross3(v01, v00));
+    f.g2 = angle_between(-n2, n3);
+    f.g3 = angle_between(-n3, n0);
+    f.b0 = n0.z;
+    f.b1 = n2.z;
+    f.area = quad_solid_angle(L, r);
+    return f;
+}
+// Square sample -> rectangle coordinates (u along ex, v along ey).
+static void rect_sample(const RectFrame &f, float s0, float s1, float &u, float &v) {
+    const float au = s0 * f.area - f.g2 - f.g3;
+    const float fu = (cos_f(au) * f.b0 - f.b1) / sin_f(au);
+    const float cu = std::min(0x1.fffffep-1f, std::max(-0x1.fffffep-1f, std::copysign(1.0f / std::sqrt(fu * fu + f.b0 * f.b0), fu)));
+    const float xu = std::min(f.x1, std::max(f.x0, -cu * f.z0 / std::sqrt(std::max(0.0f, 1.0f - cu * cu))));
+    const float dd = std::sqrt(xu * xu + f.z0 * f.z0);
+    const float h0 = f.y0 / std::sqrt(dd * dd + f.y0 * f.y0), h1 = f.y1 / std::sqrt(dd * dd + f.y1 * f.y1);
+    const float hv = h0 + s1 * (h1 - h0);
+    const float yv = std::min(f.y1, std::max(f.y0, hv * dd / std::sqrt(std::max(0.0f, 1.0f - hv * hv))));
+    u = std::clamp((xu - f.x0) / (f.x1 - f.x0), 0.0f, 1.0f);
+    v = std::clamp((yv - f.y0) / (f.y1 - f.y0), 0.0f, 1.0f);
+}
+// Rectangle coordinates -> square sample (the inverse of rect_sample).
+static void rect_inverse(const RectFrame &f, float u, float v, float &s0, float &s1) {
+    float xu = f.x0 + u * (f.x1 - f.x0);
+    const float yv = f.y0 + v * (f.y1 - f.y0);
+    if (xu == 0.0f) xu = 1e-10f;
+    const float fusq = std::max(0.0f, 1.0f + f.z0 * f.z0 / (xu * xu) - f.b0 * f.b0);
+    const float fu = std::copysign(std::sqrt(fusq), xu);
+    const float root = std::sqrt(std::max(0.0f, f.b0 * f.b0 - f.b1 * f.b1 + fusq));
+    float au = atan2_f(-f.b1 * fu - std::copysign(f.b0 * root, fu * f.b0), f.b0 * f.b1 - root * std::fabs(fu));
+    if (au > 0) au = au - 2.0f * kPi;
+    s0 = std::clamp((au + f.g2 + f.g3) / f.area, 0.0f, 1.0f);
+    const float dd2 = xu * xu + f.z0 * f.z0;
+    const float h0 = f.y0 / std::sqrt(dd2 + f.y0 * f.y0), h1 = f.y1 / std::sqrt(dd2 + f.y1 * f.y1);
+    const float hv = yv / std::sqrt(dd2 + yv * yv);
+    s1 = std::clamp((hv - h0) / (h1 - h0), 0.0f, 1.0f);
+}
+// Solid-angle density of the quad proposal at rectangle coordinates (u, v)
+// (point q on the quad) seen from r with receiver normal n.
+static float quad_pdf(const Light &L, V3 r, V3 n, float u, float v, V3 q) {
+    const V3 delta = q - r;
+    const float d2 = dot(delta, delta);
+    if (!(d2 > 0)) return 0.0f;
+    const float omega = quad_solid_angle(L, r);
+    if (L.rect && omega > 1e-4f) {
+        const RectFrame f = rect_frame(L, r);
+        float s0, s1;
+        rect_inverse(f, u, v, s0, s1);
+        const bool has_n = (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) && !slip(SLIP_RECT_PDF_UNWARPED);
+        return (has_n ? bilinear_density(s0, s1, quad_cosines(L, r, n)) : 1.0f) / omega;
+    }
+    const V3 nl = unit(cross3(L.ex, L.ey));
+    const float c = std::fabs(dot(nl, delta / std::sqrt(d2)));
+    return c > 0 ? L.inv_area * d2 / c : 0.0f;
+}
+static void quad_coords(const Light &L, V3 q, float &u, float &v) {
+    const V3 d = q - L.p00;
+    u = std::clamp(dot(d, L.ex) / dot(L.ex, L.ex), 0.0f, 1.0f);
+    v = std::clamp(dot(d, L.ey) / dot(L.ey, L.ey), 0.0f, 1.0f);
+}
+
+// A light sample from point p (normal n; zero in media): selection by the
+// scene's light sampler, then the light's own proposal.  pdf includes the
+// selection probability; for delta lights it is that probability alone.
+static LightSample sample_light(const Scene &s, V3 p, V3 n, const Wavelengths &wl, float ul, float u0, float u1) {
+    LightSample ls;
+    const Selected sel = light_select(s, p, n, ul);
+    if (sel.light == RP_NONE || !(sel.pmf > 0)) return ls;
+    const Light &L = s.lights[sel.light];
+    if (L.kind == 0 || L.kind == 3) {
+        float pdf_omega;
+        uint32_t tri;
+        if (L.kind == 0) {
+            V3 a, b, c;
+            s.tri(L.index, a, b, c);
+            float b0, b1;
+            if (u0 < u1) {
+                b0 = u0 * 0.5f;
+                b1 = u1 - b0;
+            } else {
+                b1 = u1 * 0.5f;
+                b0 = u0 - b1;
+            }
+            float b2 = 1.0f - b0 - b1;
+            ls.p = b0 * a + b1 * b + b2 * c;
+            ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
+            ls.n = normalize(cross(a - c, b - c));
+            tri = L.index;
+            const V3 dv = ls.p - p;
+            const float d2 = dot(dv, dv);
+            const float cl = std::fabs(dot(ls.n, dv / std::sqrt(d2)));
+            pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+        } else {
+            float su = std::min(u0, 0x1.fffffep-1f), sv = std::min(u1, 0x1.fffffep-1f), qu = su, qv = sv;
+            const float omega = quad_solid_angle(L, p);
+            if (L.rect && omega > 1e-4f) {
+                if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) bilinear_warp(su, sv, quad_cosines(L, p, n));
+                rect_sample(rect_frame(L, p), su, sv, qu, qv);
+            }
+            const V3 a = qu * L.ex, b = qv * L.ey;
+            ls.p = L.p00 + a + b;
+            ls.err = gamma_n(6) * (vabs(L.p00) + vabs(a) + vabs(b));
+            ls.n = unit(cross3(L.ex, L.ey));
+            tri = L.index;
+            pdf_omega = quad_pdf(L, p, n, qu, qv, ls.p);
+        }
+        const V3 dv = ls.p - p;
+        const float d2 = dot(dv, dv);
         if (!(d2 > 0)) return ls;
         ls.dist = std::sqrt(d2);
         ls.wi = dv / ls.dist;
-        float cos_l = dot(ls.n, -ls.wi);
-        const auto &em = s.emissions[s.objects[s.tri_object[L.index]].emission];
+        const float cos_l = dot(ls.n, -ls.wi);
+        const auto &em = s.emissions[s.objects[s.tri_object[tri]].emission];
         if (!em.two_sided && !(cos_l > 0)) return ls;
         if (cos_l == 0.0f) return ls;
-        float pmf = slip(SLIP_LIGHT_PMF_DROPPED) ? 1.0f : L.pmf;
-        ls.pdf = ((pmf * L.inv_area) * d2) / std::fabs(cos_l);
+        ls.pdf = sel.pmf * pdf_omega;
         for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(em.radiance, wl.lambda[k], s.spectra.data());
         ls.ok = ls.pdf > 0 && std::isfinite(ls.pdf);
         return ls;
@@ -890,7 +1458,7 @@ static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, flo
         ls.p = pos;
         ls.n = {0, 0, 0};
         ls.err = {0, 0, 0};
-        ls.pdf = L.pmf;
+        ls.pdf = sel.pmf;
         ls.delta = true;
         ls.ok = true;
         return ls;
@@ -902,15 +1470,34 @@ static LightSample sample_light(const Scene &s, V3 p, const Wavelengths &wl, flo
     sincos_turns(u1, sn, cs);
     ls.wi = {r * cs, r * sn, z};
     ls.infinite = true;
-    ls.pdf = L.pmf * kInv4Pi;
+    ls.pdf = sel.pmf * kInv4Pi;
     for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(s.d.environment, wl.lambda[k], s.spectra.data());
     ls.ok = true;
     return ls;
 }
 
+// Solid-angle density of light sampling from (p, n) toward the emitter point
+// q of light `id` (an emission hit reached by BSDF sampling): the selection
+// probability times the light's proposal density.
+static float area_light_pdf(const Scene &s, uint32_t id, V3 p, V3 n, V3 q, V3 ng) {
+    const float pmf = slip(SLIP_LIGHT_PMF_DROPPED)   ? 1.0f
+                      : slip(SLIP_BVH_PMF_NO_NORMAL) ? light_pmf(s, p, V3{0, 0, 0}, id)
+                                                     : light_pmf(s, p, n, id);
+    const Light &L = s.lights[id];
+    if (L.kind == 3) {
+        float u, v;
+        quad_coords(L, q, u, v);
+        return pmf * quad_pdf(L, p, n, u, v, q);
+    }
+    const V3 dv = q - p;
+    const float d2 = dot(dv, dv);
+    const float cl = std::fabs(dot(ng, dv / std::sqrt(d2)));
+    return cl > 0 ? pmf * (L.inv_area * d2 / cl) : 0.0f;
+}
+
 static float env_pdf(const Scene &s) {
-    if (!s.d.has_environment || s.lights.empty()) return 0.0f;
-    return s.lights.back().pmf * (slip(SLIP_ENV_PDF_HEMISPHERE) ? 2.0f * kInv4Pi : kInv4Pi);
+    if (s.env_light == RP_NONE) return 0.0f;
+    return s.env_pmf * (slip(SLIP_ENV_PDF_HEMISPHERE) ? 2.0f * kInv4Pi : kInv4Pi);
 }
 
 // Forward-mode derivative of the detached estimator (independent of the
@@ -1097,7 +1684,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
     float prev_density[4] = {1, 1, 1, 1}, prev_pdf[4] = {0, 0, 0, 0};
     float eta_scale[4] = {1, 1, 1, 1};
     bool rr_enabled = g_roulette.load(std::memory_order_relaxed), prev_delta = false;
-    V3 prev_p = o;
+    V3 prev_p = o, prev_n = {0, 0, 0};
     uint32_t count = 1, crossings = 0, medium = cam.medium, mdraw = 0;
     const uint32_t capacity = s.d.max_depth + 2;
     auto add = [&](const float v[4]) {
@@ -1150,9 +1737,11 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 for (int k = 0; k < 4; ++k) prev_pdf[k] = ppdf;
                 prev_delta = false;
                 prev_p = mp;
+                prev_n = {0, 0, 0};
                 // NEE from the medium vertex
                 auto lw = draws(c, out.index, 133 + count - 2);
-                LightSample ls = sample_light(s, mp, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]), sobol_uniform(lw[2]));
+                LightSample ls = sample_light(s, mp, V3{0, 0, 0}, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]),
+                                              sobol_uniform(lw[2]));
                 if (ls.ok) {
                     float ph = hg(slip(SLIP_PHASE_SIGN) ? -dot(wo, ls.wi) : dot(wo, ls.wi), mc.g);
                     float val[4];
@@ -1249,10 +1838,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 float val[4];
                 for (int k = 0; k < 4; ++k) val[k] = beta[k] * spectrum_eval(em.radiance, wl.lambda[k], s.spectra.data());
                 if (!(count == 2 || prev_delta) && !slip(SLIP_EMISSION_UNWEIGHTED)) {
-                    const Light &L = s.lights[s.tri_light[h.tri]];
-                    V3 dv = sp.p - prev_p;
-                    float d2 = dot(dv, dv);
-                    float lp = ((L.pmf * L.inv_area) * d2) / std::fabs(cos_l);
+                    float lp = area_light_pdf(s, s.tri_light[h.tri], prev_p, prev_n, sp.p, sp.n);
                     float lpv[4] = {lp, lp, lp, lp};
                     float a = mixture(prev_density, prev_pdf), b = mixture(prev_density, lpv);
                     float w = a / (a + b);
@@ -1312,6 +1898,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                         for (int k = 0; k < 4; ++k) eta_scale[k] = eta_scale[k] * (etas[k] * etas[k]);
                     prev_delta = hs.delta;
                     prev_p = sp.p;
+                    prev_n = oct_decode(oct_encode(sp.n));
                     V3 wi = fr.world(hs.wi);
                     wi = normalize(wi);
                     medium = outgoing_medium(obj, medium, sp.n, wi);
@@ -1324,7 +1911,8 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
         // direct lighting (hero_direct) with the pre-sampling state
         if (out.status == RP_PATH_SUCCESS && direct_allowed && connectible) {
             auto lw = draws(c, out.index, 133 + count - 2);
-            LightSample ls = sample_light(s, sp.p, wl, sobol_uniform(lw[0]), sobol_uniform(lw[1]), sobol_uniform(lw[2]));
+            LightSample ls = sample_light(s, sp.p, oct_decode(oct_encode(sp.n)), wl, sobol_uniform(lw[0]),
+                                          sobol_uniform(lw[1]), sobol_uniform(lw[2]));
             if (ls.ok) {
                 V3 wil = fr.local(ls.wi);
                 float cosv = slip(SLIP_NEE_NO_COSINE) ? 1.0f : std::fabs(dot(sp.n, ls.wi));
@@ -1495,16 +2083,57 @@ void *or_scene_create(const rp_scene_desc *d) {
 void or_scene_destroy(void *s) { delete static_cast<Scene *>(s); }
 
 uint32_t or_light_count(void *sp) { return uint32_t(static_cast<Scene *>(sp)->lights.size()); }
-void or_light_table(void *sp, uint32_t *kind, uint32_t *index, float *pmf, float *cdf, float *inv_area) {
+// Light table: kind, triangle / spot index, power-sampler pmf, alias
+// threshold, 1/area, light-BVH leaf node.
+void or_light_table(void *sp, uint32_t *kind, uint32_t *index, float *pmf, float *threshold, float *inv_area,
+                    uint32_t *leaf) {
     auto *s = static_cast<Scene *>(sp);
     for (size_t i = 0; i < s->lights.size(); ++i) {
         kind[i] = s->lights[i].kind;
         index[i] = s->lights[i].index;
         pmf[i] = s->lights[i].pmf;
-        cdf[i] = s->lights[i].cdf;
+        threshold[i] = s->lights[i].threshold;
         inv_area[i] = s->lights[i].inv_area;
+        leaf[i] = s->lights[i].leaf;
     }
 }
+uint32_t or_light_node_count(void *sp) { return uint32_t(static_cast<Scene *>(sp)->lnodes.size()); }
+// Light selection probes: the light chosen for (p, n, u) with its pmf, and
+// the pmf of a given light.
+uint32_t or_light_select(void *sp, const float p[3], const float n[3], float u, float *pmf) {
+    Selected sel = light_select(*static_cast<Scene *>(sp), {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, u);
+    *pmf = sel.pmf;
+    return sel.light;
+}
+float or_light_pmf(void *sp, const float p[3], const float n[3], uint32_t id) {
+    return light_pmf(*static_cast<Scene *>(sp), {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, id);
+}
+// A full light sample from (p, n): light id (RP_NONE if none), point (or
+// direction for the environment), pdf (selection pmf included), delta flag.
+uint32_t or_light_sample(void *sp, const float p[3], const float n[3], float ul, float u0, float u1, float out[3],
+                         float *pdf, int *delta) {
+    auto *s = static_cast<Scene *>(sp);
+    Wavelengths wl = sample_wavelengths(0.5f, 0);
+    const V3 pp = {p[0], p[1], p[2]}, nn = {n[0], n[1], n[2]};
+    const Selected sel = light_select(*s, pp, nn, ul);
+    LightSample ls = sample_light(*s, pp, nn, wl, ul, u0, u1);
+    *pdf = ls.ok ? ls.pdf : 0.0f;
+    *delta = ls.delta ? 1 : 0;
+    const V3 o = ls.infinite ? ls.wi : ls.p;
+    out[0] = o.x;
+    out[1] = o.y;
+    out[2] = o.z;
+    return ls.ok ? sel.light : RP_NONE;
+}
+// Solid-angle density of reaching emitter point q (geometric normal ng) of
+// light `id` by light sampling from (p, n).
+float or_area_light_pdf(void *sp, uint32_t id, const float p[3], const float n[3], const float q[3],
+                        const float ng[3]) {
+    return area_light_pdf(*static_cast<Scene *>(sp), id, {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, {q[0], q[1], q[2]},
+                          {ng[0], ng[1], ng[2]});
+}
+float or_atan2(float y, float x) { return atan2_f(y, x); }
+float or_asin(float x) { return asin_f(x); }
 
 // Closest hits for caller rays (rp_ray / rp_hit layouts).
 void or_trace_closest(void *sp, const rp_ray *rays, uint64_t n, rp_hit *hits, uint32_t threads) {

@@ -16,6 +16,7 @@
 #include <bit>
 #include <cmath>
 #include <cstdint>
+#include <cfloat>
 #include <cstring>
 #include <vector>
 
@@ -45,6 +46,8 @@ inline float dop(float a, float b, float c, float d) {
 inline V3 cross(V3 a, V3 b) {
     return {dop(a.y, b.z, a.z, b.y), dop(a.z, b.x, a.x, b.z), dop(a.x, b.y, a.y, b.x)};
 }
+// Plain cross product (the light-sampling geometry uses this form).
+inline V3 cross3(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 inline float length(V3 a) { return std::sqrt(dot(a, a)); }
 inline V3 normalize(V3 a) { return a / length(a); }
 
@@ -94,6 +97,73 @@ inline float log_f(float x) {
     float ef = float(e);
     return ef * 0.693145751953125f + (lm + ef * 1.42860682030941723212e-6f);
 }
+// sin and cos of an angle in radians through the turn reduction above.
+inline float sin_f(float x) {
+    float s, c;
+    sincos_turns(x * 0.159154943091895336f, s, c);
+    return s;
+}
+inline float cos_f(float x) {
+    float s, c;
+    sincos_turns(x * 0.159154943091895336f, s, c);
+    return c;
+}
+// atan: |x| > 1 through pi/2 - atan(1/|x|), then z > tan(pi/8) through
+// pi/4 + atan((z-1)/(z+1)), leaving |t| <= tan(pi/8) for the Maclaurin
+// series t - t^3/3 + ... - t^15/15 + t^17/17 (truncation below 3e-9).
+inline float atan_f(float x) {
+    float ax = std::fabs(x);
+    bool inv = ax > 1.0f;
+    float z = inv ? 1.0f / ax : ax;
+    bool shift = z > 0.414213562373095049f;
+    float t = shift ? (z - 1.0f) / (z + 1.0f) : z;
+    float t2 = t * t;
+    float p = t + t * t2 * (-1.0f / 3.0f + t2 * (1.0f / 5.0f + t2 * (-1.0f / 7.0f + t2 * (1.0f / 9.0f +
+              t2 * (-1.0f / 11.0f + t2 * (1.0f / 13.0f + t2 * (-1.0f / 15.0f + t2 * (1.0f / 17.0f))))))));
+    float r = shift ? 0.785398163397448310f + p : p;
+    r = inv ? 1.57079632679489662f - r : r;
+    return std::copysign(r, x);
+}
+inline float atan2_f(float y, float x) {
+    float ax = std::fabs(x), ay = std::fabs(y);
+    if (ax == 0.0f && ay == 0.0f) return std::signbit(x) ? std::copysign(3.14159265358979324f, y) : std::copysign(0.0f, y);
+    float r = ay <= ax ? atan_f(ay / ax) : 1.57079632679489662f - atan_f(ax / ay);
+    if (std::signbit(x)) r = 3.14159265358979324f - r;
+    return std::copysign(r, y);
+}
+inline float asin_f(float x) { return atan2_f(x, std::sqrt(std::max(0.0f, (1.0f - x) * (1.0f + x)))); }
+
+// Light-sampling context normal: a unit normal through a 2 x 16-bit
+// octahedral code (snorm, round half up) and back, so that the normal a
+// vertex samples lights with and the one the next vertex's MIS weight uses
+// are the same bits.  Code kNoNormal (media, the camera) decodes to zero.
+constexpr uint32_t kNoNormal = 0xffffffffu;
+inline float sign_nz(float v) { return v < 0.0f ? -1.0f : 1.0f; }
+inline uint32_t oct_encode(V3 n) {
+    const float l1 = std::fabs(n.x) + std::fabs(n.y) + std::fabs(n.z);
+    float px = n.x / l1, py = n.y / l1;
+    if (n.z < 0.0f) {
+        const float qx = (1.0f - std::fabs(py)) * sign_nz(px), qy = (1.0f - std::fabs(px)) * sign_nz(py);
+        px = qx;
+        py = qy;
+    }
+    const int32_t ix = int32_t(std::floor(std::clamp(px, -1.0f, 1.0f) * 32767.0f + 0.5f));
+    const int32_t iy = int32_t(std::floor(std::clamp(py, -1.0f, 1.0f) * 32767.0f + 0.5f));
+    return (uint32_t(ix) & 0xffffu) | (uint32_t(iy) << 16);
+}
+inline V3 oct_decode(uint32_t c) {
+    if (c == kNoNormal) return {0, 0, 0};
+    float px = float(int16_t(c & 0xffffu)) / 32767.0f, py = float(int16_t(c >> 16)) / 32767.0f;
+    const float z = 1.0f - std::fabs(px) - std::fabs(py);
+    if (z < 0.0f) {
+        const float qx = (1.0f - std::fabs(py)) * sign_nz(px), qy = (1.0f - std::fabs(px)) * sign_nz(py);
+        px = qx;
+        py = qy;
+    }
+    const V3 v = {px, py, z};
+    return v / std::sqrt(dot(v, v));
+}
+
 inline float next_up(float v) {
     if (std::isinf(v) && v > 0) return v;
     if (v == -0.0f) v = 0.0f;

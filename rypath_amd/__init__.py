@@ -68,6 +68,7 @@ SYNTH_CORNELL, SYNTH_TERRAIN, SYNTH_BATCH, SYNTH_VOLUME, SYNTH_INVERSE = range(5
 PARAM_REFLECTANCE_BIN, PARAM_ROUGHNESS, PARAM_SIGMA_S_BIN = range(3)
 SCHED_PERSISTENT, SCHED_WAVEFRONT = range(2)
 SAMPLER_INDEPENDENT, SAMPLER_SOBOL, SAMPLER_HALTON, SAMPLER_ZSOBOL = range(4)
+LIGHT_SAMPLER_BVH, LIGHT_SAMPLER_POWER, LIGHT_SAMPLER_UNIFORM = range(3)
 u32, u64, f32, f64 = C.c_uint32, C.c_uint64, C.c_float, C.c_double
 
 
@@ -110,7 +111,8 @@ class SceneDesc(C.Structure):
                 ("materials", C.POINTER(Material)), ("emissions", C.POINTER(Emission)),
                 ("media", C.POINTER(Medium)), ("spots", C.POINTER(SpotLight)), ("spectra", C.POINTER(f32)),
                 ("spectra_count", u32), ("has_environment", u32), ("environment", Spectrum),
-                ("camera", Camera), ("max_depth", u32), ("hero_wavelengths", u32)]
+                ("camera", Camera), ("max_depth", u32), ("hero_wavelengths", u32), ("light_sampler", u32),
+                ("pad", u32)]
 
 
 class BuildOptions(C.Structure):

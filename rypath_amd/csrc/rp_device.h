@@ -31,6 +31,8 @@ RP_DEV float comp(F3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 // std::min / std::max semantics (returns the first argument on ties / NaN).
 RP_DEV float omin(float a, float b) { return (b < a) ? b : a; }
 RP_DEV float omax(float a, float b) { return (a < b) ? b : a; }
+// std::clamp semantics.
+RP_DEV float oclamp(float v, float lo, float hi) { return (v < lo) ? lo : ((hi < v) ? hi : v); }
 RP_DEV float dop(float a, float b, float c, float d) {
     float cd = c * d;
     float err = __builtin_fmaf(-c, d, cd);
@@ -38,6 +40,8 @@ RP_DEV float dop(float a, float b, float c, float d) {
     return r + err;
 }
 RP_DEV F3 cross(F3 a, F3 b) { return {dop(a.y, b.z, a.z, b.y), dop(a.z, b.x, a.x, b.z), dop(a.x, b.y, a.y, b.x)}; }
+// Plain cross product (the light-sampling geometry uses this form).
+RP_DEV F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 RP_DEV float length(F3 a) { return __builtin_sqrtf(dot(a, a)); }
 RP_DEV F3 normalize(F3 a) { return a / length(a); }
 
@@ -86,6 +90,73 @@ RP_DEV float log_f(float x) {
     float ef = float(e);
     return ef * 0.693145751953125f + (lm + ef * 1.42860682030941723212e-6f);
 }
+// sin and cos of an angle in radians through the turn reduction.
+RP_DEV float sin_f(float x) {
+    float s, c;
+    sincos_turns(x * 0.159154943091895336f, s, c);
+    return s;
+}
+RP_DEV float cos_f(float x) {
+    float s, c;
+    sincos_turns(x * 0.159154943091895336f, s, c);
+    return c;
+}
+// atan: 1/|x| for |x| > 1 (pi/2 minus), then pi/4 + atan((z-1)/(z+1)) above
+// tan(pi/8); the Maclaurin series to t^17 on |t| <= tan(pi/8).
+RP_DEV float atan_f(float x) {
+    const float ax = fabsf(x);
+    const bool inv = ax > 1.0f;
+    const float z = inv ? 1.0f / ax : ax;
+    const bool shift = z > 0.414213562373095049f;
+    const float t = shift ? (z - 1.0f) / (z + 1.0f) : z;
+    const float t2 = t * t;
+    const float p = t + t * t2 * (-1.0f / 3.0f + t2 * (1.0f / 5.0f + t2 * (-1.0f / 7.0f + t2 * (1.0f / 9.0f +
+                    t2 * (-1.0f / 11.0f + t2 * (1.0f / 13.0f + t2 * (-1.0f / 15.0f + t2 * (1.0f / 17.0f))))))));
+    float r = shift ? 0.785398163397448310f + p : p;
+    r = inv ? 1.57079632679489662f - r : r;
+    return copysignf(r, x);
+}
+RP_DEV float atan2_f(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    if (ax == 0.0f && ay == 0.0f) return signbit(x) ? copysignf(3.14159265358979324f, y) : copysignf(0.0f, y);
+    float r = ay <= ax ? atan_f(ay / ax) : 1.57079632679489662f - atan_f(ax / ay);
+    if (signbit(x)) r = 3.14159265358979324f - r;
+    return copysignf(r, y);
+}
+RP_DEV float asin_f(float x) { return atan2_f(x, __builtin_sqrtf(omax(0.0f, (1.0f - x) * (1.0f + x)))); }
+
+// Light-sampling context normal: a unit normal through a 2 x 16-bit
+// octahedral code (snorm, round half up) and back, so that a vertex's light
+// sampling and the next vertex's MIS weight use the same normal bits; the
+// code also fits the path state's spare word.  kNoNormal (media, camera)
+// decodes to zero.
+constexpr uint32_t kNoNormal = 0xffffffffu;
+RP_DEV float sign_nz(float v) { return v < 0.0f ? -1.0f : 1.0f; }
+RP_DEV uint32_t oct_encode(F3 n) {
+    const float l1 = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    float px = n.x / l1, py = n.y / l1;
+    if (n.z < 0.0f) {
+        const float qx = (1.0f - fabsf(py)) * sign_nz(px), qy = (1.0f - fabsf(px)) * sign_nz(py);
+        px = qx;
+        py = qy;
+    }
+    const int32_t ix = int32_t(floorf(oclamp(px, -1.0f, 1.0f) * 32767.0f + 0.5f));
+    const int32_t iy = int32_t(floorf(oclamp(py, -1.0f, 1.0f) * 32767.0f + 0.5f));
+    return (uint32_t(ix) & 0xffffu) | (uint32_t(iy) << 16);
+}
+RP_DEV F3 oct_decode(uint32_t c) {
+    if (c == kNoNormal) return F3{0, 0, 0};
+    float px = float(int16_t(c & 0xffffu)) / 32767.0f, py = float(int16_t(c >> 16)) / 32767.0f;
+    const float z = 1.0f - fabsf(px) - fabsf(py);
+    if (z < 0.0f) {
+        const float qx = (1.0f - fabsf(py)) * sign_nz(px), qy = (1.0f - fabsf(px)) * sign_nz(py);
+        px = qx;
+        py = qy;
+    }
+    const F3 v = {px, py, z};
+    return v / __builtin_sqrtf(dot(v, v));
+}
+
 RP_DEV float next_up(float v) {
     if (__builtin_isinf(v) && v > 0) return v;
     if (v == -0.0f) v = 0.0f;
@@ -102,10 +173,26 @@ RP_DEV float next_down(float v) {
 }
 
 // ======================================================== scene on device
+// One light (64 B).  kind 0 triangle, 1 spot, 2 environment, 3 quad (a
+// rectangle p00 + u ex + v ey made of two triangles); index: the leaf-order
+// position of the (first) triangle or the spot index; leaf: the light-BVH
+// node holding the light; pmf / threshold / alias: the power sampler's alias
+// table; rect: the quad passes the binary32 rectangle test.
 struct LightEntry {
-    uint32_t kind, index; // index: leaf-order triangle position / spot index
-    float pmf, cdf, inv_area;
-    uint32_t pad[3];
+    uint32_t kind, index, leaf, rect;
+    float pmf, threshold;
+    uint32_t alias;
+    float inv_area;
+    float p00[3], ex[3], ey[3];
+    uint32_t pad;
+};
+// Light-BVH node (64 B): bounds, direction cone, emission cutoff, power
+// normalised by the tree's total; the first child of interior node i is
+// i + 1; flags bit 0 = two-sided.
+struct LightNode {
+    float low[3], high[3], dir[3];
+    float cos_outer, cos_emission, power;
+    uint32_t child, parent, light, flags;
 };
 
 struct Scene {
@@ -121,6 +208,10 @@ struct Scene {
     const LightEntry *lights;
     uint32_t light_count;
     uint32_t has_environment;
+    const LightNode *light_nodes;
+    uint32_t light_node_count;
+    uint32_t light_sampler; // rp_light_sampler
+    uint32_t env_light;     // the environment's light id (RP_NONE: none)
     const float4 *cie;          // 471 rows (x, y, z, 0)
     const uint32_t *sobol_dir;  // transposed directions: [52][1024]
     const uint64_t *sobol_intervals, *sobol_inverse;
@@ -954,53 +1045,301 @@ struct LightSample {
     float pdf, dist;
     bool delta, infinite, ok;
 };
-RP_DEV uint32_t select_light(const Scene &S, float u) {
-    uint32_t lo = 0, hi = S.light_count;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) / 2;
-        if (u < S.lights[mid].cdf) hi = mid;
-        else lo = mid + 1;
+// --- light selection (shaders/light_sampling.h:19-127; PBRT BVHLightSampler,
+// PowerLightSampler, UniformLightSampler).  A receiving point p with normal
+// n (zero in media).
+RP_DEV F3 node_vec(const float v[3]) { return F3{v[0], v[1], v[2]}; }
+RP_DEV bool nonzero(F3 n) { return n.x != 0.0f || n.y != 0.0f || n.z != 0.0f; }
+RP_DEV float light_importance(const LightNode &b, F3 p, F3 n) {
+    const F3 lo = node_vec(b.low), hi = node_vec(b.high);
+    const F3 c = (lo + hi) * 0.5f;
+    const F3 delta = p - c;
+    const float r2 = dot(hi - lo, hi - lo) * 0.25f, d2 = dot(delta, delta);
+    const float radius = __builtin_sqrtf(r2), dist = __builtin_sqrtf(d2);
+    const float cb = d2 <= r2 ? -1.0f : __builtin_sqrtf(omax(0.0f, 1.0f - r2 / d2));
+    const float sb = __builtin_sqrtf(omax(0.0f, 1.0f - cb * cb));
+    const F3 wi = dist > 0 ? delta / dist : F3{0, 0, 1};
+    float cw = oclamp(dot(node_vec(b.dir), wi), -1.0f, 1.0f);
+    if (b.flags & 1u) cw = fabsf(cw);
+    const float sw = __builtin_sqrtf(omax(0.0f, 1.0f - cw * cw));
+    const float co = b.cos_outer, so = __builtin_sqrtf(omax(0.0f, 1.0f - co * co));
+    const float cx = cw > co ? 1.0f : cw * co + sw * so, sx = cw > co ? 0.0f : sw * co - cw * so;
+    const float cp = cx > cb ? 1.0f : cx * cb + sx * sb;
+    if (cp <= b.cos_emission) return 0.0f;
+    float value = b.power * cp / omax(d2, omax(radius, 1.17549435e-38f));
+    if (nonzero(n)) {
+        const float ci = oclamp(fabsf(dot(wi, n)), 0.0f, 1.0f);
+        const float si = __builtin_sqrtf(omax(0.0f, 1.0f - ci * ci));
+        value = value * (ci > cb ? 1.0f : ci * cb + si * sb);
     }
-    return lo < S.light_count - 1 ? lo : S.light_count - 1;
+    return omax(0.0f, value);
 }
-RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, float ul, float u0, float u1) {
+struct Selected {
+    uint32_t light;
+    float pmf;
+};
+RP_DEV Selected light_select(const Scene &S, F3 p, F3 n, float u) {
+    const uint32_t count = S.light_count;
+    if (count == 0) return {kEmpty, 0.0f};
+    u = oclamp(u, 0.0f, 0x1.fffffep-1f);
+    if (S.light_sampler == RP_LIGHT_SAMPLER_UNIFORM) {
+        const uint32_t i = uint32_t(u * float(count));
+        return {i < count - 1 ? i : count - 1, 1.0f / float(count)};
+    }
+    if (S.light_sampler == RP_LIGHT_SAMPLER_POWER) {
+        const float v = u * float(count);
+        uint32_t i = uint32_t(v);
+        i = i < count - 1 ? i : count - 1;
+        const uint32_t id = v - float(i) < S.lights[i].threshold ? i : S.lights[i].alias;
+        return {id, S.lights[id].pmf};
+    }
+    const uint32_t inf = S.env_light != kEmpty ? 1u : 0u, tree = S.light_node_count ? 1u : 0u;
+    const float infinite = float(inf) / float(inf + tree);
+    if (u < infinite) return {S.env_light, 1.0f / float(inf + tree)};
+    if (!tree) return {kEmpty, 0.0f};
+    u = omin((u - infinite) / (1.0f - infinite), 0x1.fffffep-1f);
+    float pmf = 1.0f - infinite;
+    uint32_t i = 0;
+    for (uint32_t steps = 0; steps < S.light_node_count; ++steps) {
+        const LightNode node = S.light_nodes[i];
+        if (node.light != kEmpty) {
+            if (light_importance(node, p, n) <= 0.0f) return {kEmpty, 0.0f};
+            return {node.light, pmf};
+        }
+        const float a = light_importance(S.light_nodes[i + 1], p, n);
+        const float b = light_importance(S.light_nodes[node.child], p, n);
+        const float sum = a + b;
+        if (!(sum > 0.0f)) return {kEmpty, 0.0f};
+        const float q = a / sum;
+        if (u < q) {
+            pmf = pmf * q;
+            u = omin(u / q, 0x1.fffffep-1f);
+            i = i + 1;
+        } else {
+            pmf = pmf * (1.0f - q);
+            u = omin((u - q) / (1.0f - q), 0x1.fffffep-1f);
+            i = node.child;
+        }
+    }
+    return {kEmpty, 0.0f};
+}
+RP_DEV float light_pmf(const Scene &S, F3 p, F3 n, uint32_t id) {
+    const uint32_t count = S.light_count;
+    if (id >= count) return 0.0f;
+    if (S.light_sampler == RP_LIGHT_SAMPLER_UNIFORM) return 1.0f / float(count);
+    if (S.light_sampler == RP_LIGHT_SAMPLER_POWER) return S.lights[id].pmf;
+    const uint32_t inf = S.env_light != kEmpty ? 1u : 0u, tree = S.light_node_count ? 1u : 0u;
+    if (id == S.env_light) return 1.0f / float(inf + tree);
+    uint32_t i = S.lights[id].leaf;
+    if (i >= S.light_node_count) return 0.0f;
+    if (light_importance(S.light_nodes[i], p, n) <= 0.0f) return 0.0f;
+    float pmf = 1.0f - float(inf) / float(inf + tree);
+    for (uint32_t steps = 0; steps < S.light_node_count; ++steps) {
+        const uint32_t parent = S.light_nodes[i].parent;
+        if (parent == kEmpty) return pmf;
+        const float a = light_importance(S.light_nodes[parent + 1], p, n);
+        const float b = light_importance(S.light_nodes[S.light_nodes[parent].child], p, n);
+        const float sum = a + b;
+        if (!(sum > 0.0f)) return 0.0f;
+        pmf = pmf * (i == parent + 1 ? a / sum : b / sum);
+        i = parent;
+    }
+    return 0.0f;
+}
+
+// --- quad emitters: the spherical-rectangle proposal (Urena, Fajardo, King
+// 2013; src/light_geometry_sampling_shared.h:174-260,
+// shaders/light_geometry_sampling.h:118-209), warped by the receiver's corner
+// cosines (bilinear) when a normal is given; uniform over the area when the
+// rectangle subtends 1e-4 sr or less.
+struct Quad {
+    F3 p00, ex, ey;
+    float inv_area;
+    bool rect;
+};
+RP_DEV Quad quad_of(const LightEntry &L) {
+    return {node_vec(L.p00), node_vec(L.ex), node_vec(L.ey), L.inv_area, L.rect != 0};
+}
+RP_DEV F3 unit(F3 v) { return v / length(v); }
+RP_DEV float angle_between(F3 a, F3 b) {
+    return dot(a, b) < 0 ? kPi - 2.0f * asin_f(omin(1.0f, length(a + b) / 2.0f))
+                         : 2.0f * asin_f(omin(1.0f, length(a - b) / 2.0f));
+}
+RP_DEV float spherical_triangle_area(F3 a, F3 b, F3 c) {
+    return fabsf(2.0f * atan2_f(dot(a, cross3(b, c)), 1.0f + dot(a, b) + dot(a, c) + dot(b, c)));
+}
+RP_DEV float triangle_solid_angle(F3 p0, F3 p1, F3 p2, F3 r) {
+    const F3 a = p0 - r, b = p1 - r, c = p2 - r;
+    if (!(dot(a, a) > 0) || !(dot(b, b) > 0) || !(dot(c, c) > 0)) return 0.0f;
+    return spherical_triangle_area(unit(a), unit(b), unit(c));
+}
+RP_DEV float quad_solid_angle(const Quad &Q, F3 r) {
+    const F3 p10 = Q.p00 + Q.ex, p01 = Q.p00 + Q.ey, p11 = Q.p00 + Q.ex + Q.ey;
+    return triangle_solid_angle(Q.p00, p10, p11, r) + triangle_solid_angle(Q.p00, p11, p01, r);
+}
+struct Bilinear {
+    float a, b, c, d; // weights at (0,0), (1,0), (0,1), (1,1)
+};
+RP_DEV Bilinear quad_cosines(const Quad &Q, F3 r, F3 n) {
+    const F3 p10 = Q.p00 + Q.ex, p01 = Q.p00 + Q.ey, p11 = Q.p00 + Q.ex + Q.ey;
+    return {omax(0.01f, fabsf(dot(unit(Q.p00 - r), n))), omax(0.01f, fabsf(dot(unit(p10 - r), n))),
+            omax(0.01f, fabsf(dot(unit(p01 - r), n))), omax(0.01f, fabsf(dot(unit(p11 - r), n)))};
+}
+RP_DEV Bilinear bilinear_normalized(Bilinear w) {
+    const float m = omax(omax(w.a, w.b), omax(w.c, w.d));
+    return m > 0 ? Bilinear{w.a / m, w.b / m, w.c / m, w.d / m} : Bilinear{1, 1, 1, 1};
+}
+RP_DEV float linear_warp(float u, float a, float b) {
+    if (u == 0.0f && a == 0.0f) return 0.0f;
+    return omin(0x1.fffffep-1f, u * (a + b) / (a + __builtin_sqrtf((1.0f - u) * a * a + u * b * b)));
+}
+RP_DEV void bilinear_warp(float &u0, float &u1, Bilinear w) {
+    w = bilinear_normalized(w);
+    const float y = linear_warp(u1, w.a + w.b, w.c + w.d);
+    const float x = linear_warp(u0, (1.0f - y) * w.a + y * w.c, (1.0f - y) * w.b + y * w.d);
+    u0 = x;
+    u1 = y;
+}
+RP_DEV float bilinear_density(float x, float y, Bilinear w) {
+    if (x < 0 || x > 1 || y < 0 || y > 1) return 0.0f;
+    w = bilinear_normalized(w);
+    return 4.0f * ((1.0f - x) * (1.0f - y) * w.a + x * (1.0f - y) * w.b + (1.0f - x) * y * w.c + x * y * w.d) /
+           (w.a + w.b + w.c + w.d);
+}
+struct RectFrame {
+    float x0, x1, y0, y1, z0, b0, b1, g2, g3, area;
+};
+RP_DEV RectFrame rect_frame(const Quad &Q, F3 r) {
+    RectFrame f;
+    const F3 fx = unit(Q.ex), fy = unit(Q.ey);
+    F3 fz = cross3(fx, fy);
+    const F3 delta = Q.p00 - r;
+    f.x0 = dot(delta, fx);
+    f.x1 = f.x0 + length(Q.ex);
+    f.y0 = dot(delta, fy);
+    f.y1 = f.y0 + length(Q.ey);
+    f.z0 = dot(delta, fz);
+    if (f.z0 > 0) f.z0 = -f.z0;
+    const F3 v00 = {f.x0, f.y0, f.z0}, v10 = {f.x1, f.y0, f.z0}, v11 = {f.x1, f.y1, f.z0}, v01 = {f.x0, f.y1, f.z0};
+    const F3 n0 = unit(cross3(v00, v10)), n2 = unit(cross3(v11, v01)), n3 = unit(cross3(v01, v00));
+    f.g2 = angle_between(-n2, n3);
+    f.g3 = angle_between(-n3, n0);
+    f.b0 = n0.z;
+    f.b1 = n2.z;
+    f.area = quad_solid_angle(Q, r);
+    return f;
+}
+RP_DEV void rect_sample(const RectFrame &f, float s0, float s1, float &u, float &v) {
+    const float au = s0 * f.area - f.g2 - f.g3;
+    const float fu = (cos_f(au) * f.b0 - f.b1) / sin_f(au);
+    const float cu = omin(0x1.fffffep-1f,
+                          omax(-0x1.fffffep-1f, copysignf(1.0f / __builtin_sqrtf(fu * fu + f.b0 * f.b0), fu)));
+    const float xu = omin(f.x1, omax(f.x0, -cu * f.z0 / __builtin_sqrtf(omax(0.0f, 1.0f - cu * cu))));
+    const float dd = __builtin_sqrtf(xu * xu + f.z0 * f.z0);
+    const float h0 = f.y0 / __builtin_sqrtf(dd * dd + f.y0 * f.y0), h1 = f.y1 / __builtin_sqrtf(dd * dd + f.y1 * f.y1);
+    const float hv = h0 + s1 * (h1 - h0);
+    const float yv = omin(f.y1, omax(f.y0, hv * dd / __builtin_sqrtf(omax(0.0f, 1.0f - hv * hv))));
+    u = oclamp((xu - f.x0) / (f.x1 - f.x0), 0.0f, 1.0f);
+    v = oclamp((yv - f.y0) / (f.y1 - f.y0), 0.0f, 1.0f);
+}
+RP_DEV void rect_inverse(const RectFrame &f, float u, float v, float &s0, float &s1) {
+    float xu = f.x0 + u * (f.x1 - f.x0);
+    const float yv = f.y0 + v * (f.y1 - f.y0);
+    if (xu == 0.0f) xu = 1e-10f;
+    const float fusq = omax(0.0f, 1.0f + f.z0 * f.z0 / (xu * xu) - f.b0 * f.b0);
+    const float fu = copysignf(__builtin_sqrtf(fusq), xu);
+    const float root = __builtin_sqrtf(omax(0.0f, f.b0 * f.b0 - f.b1 * f.b1 + fusq));
+    float au = atan2_f(-f.b1 * fu - copysignf(f.b0 * root, fu * f.b0), f.b0 * f.b1 - root * fabsf(fu));
+    if (au > 0) au = au - 2.0f * kPi;
+    s0 = oclamp((au + f.g2 + f.g3) / f.area, 0.0f, 1.0f);
+    const float dd2 = xu * xu + f.z0 * f.z0;
+    const float h0 = f.y0 / __builtin_sqrtf(dd2 + f.y0 * f.y0), h1 = f.y1 / __builtin_sqrtf(dd2 + f.y1 * f.y1);
+    const float hv = yv / __builtin_sqrtf(dd2 + yv * yv);
+    s1 = oclamp((hv - h0) / (h1 - h0), 0.0f, 1.0f);
+}
+// Solid-angle density of the quad proposal at rectangle coordinates (u, v),
+// point q, from r with receiver normal n.
+RP_DEV float quad_pdf(const Quad &Q, F3 r, F3 n, float u, float v, F3 q) {
+    const F3 delta = q - r;
+    const float d2 = dot(delta, delta);
+    if (!(d2 > 0)) return 0.0f;
+    const float omega = quad_solid_angle(Q, r);
+    if (Q.rect && omega > 1e-4f) {
+        const RectFrame f = rect_frame(Q, r);
+        float s0, s1;
+        rect_inverse(f, u, v, s0, s1);
+        return (nonzero(n) ? bilinear_density(s0, s1, quad_cosines(Q, r, n)) : 1.0f) / omega;
+    }
+    const F3 nl = unit(cross3(Q.ex, Q.ey));
+    const float c = fabsf(dot(nl, delta / __builtin_sqrtf(d2)));
+    return c > 0 ? Q.inv_area * d2 / c : 0.0f;
+}
+RP_DEV void quad_coords(const Quad &Q, F3 q, float &u, float &v) {
+    const F3 d = q - Q.p00;
+    u = oclamp(dot(d, Q.ex) / dot(Q.ex, Q.ex), 0.0f, 1.0f);
+    v = oclamp(dot(d, Q.ey) / dot(Q.ey, Q.ey), 0.0f, 1.0f);
+}
+
+// A light sample from p (normal n, zero in media): selection by the scene's
+// light sampler, then the light's proposal; pdf includes the selection pmf
+// (for a spot light it is that pmf alone).
+RP_DEV LightSample sample_light(const Scene &S, F3 p, F3 n, const Wavelengths &wl, float ul, float u0, float u1) {
     LightSample ls;
     ls.ok = ls.delta = ls.infinite = false;
     ls.pdf = 0;
     ls.dist = 0;
     for (int k = 0; k < 4; ++k) ls.le[k] = 0;
     ls.p = ls.n = ls.err = ls.wi = F3{0, 0, 0};
-    if (S.light_count == 0) return ls;
-    uint32_t li = select_light(S, ul);
-    const LightEntry L = S.lights[li];
-    if (!(L.pmf > 0)) return ls;
-    if (L.kind == 0) {
-        F3 a, b, c;
-        uint32_t id;
-        load_tri(S, L.index, a, b, c, id);
-        float b0, b1;
-        if (u0 < u1) {
-            b0 = u0 * 0.5f;
-            b1 = u1 - b0;
+    const Selected sel = light_select(S, p, n, ul);
+    if (sel.light == kEmpty || !(sel.pmf > 0)) return ls;
+    const LightEntry &L = S.lights[sel.light];
+    if (L.kind == 0 || L.kind == 3) {
+        float pdf_omega;
+        if (L.kind == 0) {
+            F3 a, b, c;
+            uint32_t id;
+            load_tri(S, L.index, a, b, c, id);
+            float b0, b1;
+            if (u0 < u1) {
+                b0 = u0 * 0.5f;
+                b1 = u1 - b0;
+            } else {
+                b1 = u1 * 0.5f;
+                b0 = u0 - b1;
+            }
+            float b2 = 1.0f - b0 - b1;
+            ls.p = b0 * a + b1 * b + b2 * c;
+            ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
+            ls.n = normalize(cross(a - c, b - c));
+            const F3 dv = ls.p - p;
+            const float d2 = dot(dv, dv);
+            const float cl = fabsf(dot(ls.n, dv / __builtin_sqrtf(d2)));
+            pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
         } else {
-            b1 = u1 * 0.5f;
-            b0 = u0 - b1;
+            const Quad Q = quad_of(L);
+            float su = omin(u0, 0x1.fffffep-1f), sv = omin(u1, 0x1.fffffep-1f), qu = su, qv = sv;
+            const float omega = quad_solid_angle(Q, p);
+            if (Q.rect && omega > 1e-4f) {
+                if (nonzero(n)) bilinear_warp(su, sv, quad_cosines(Q, p, n));
+                rect_sample(rect_frame(Q, p), su, sv, qu, qv);
+            }
+            const F3 a = qu * Q.ex, b = qv * Q.ey;
+            ls.p = Q.p00 + a + b;
+            ls.err = gamma_n(6) * (vabs(Q.p00) + vabs(a) + vabs(b));
+            ls.n = unit(cross3(Q.ex, Q.ey));
+            pdf_omega = quad_pdf(Q, p, n, qu, qv, ls.p);
         }
-        float b2 = 1.0f - b0 - b1;
-        ls.p = b0 * a + b1 * b + b2 * c;
-        ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
-        ls.n = normalize(cross(a - c, b - c));
-        F3 dv = ls.p - p;
-        float d2 = dot(dv, dv);
+        const F3 dv = ls.p - p;
+        const float d2 = dot(dv, dv);
         if (!(d2 > 0)) return ls;
         ls.dist = __builtin_sqrtf(d2);
         ls.wi = dv / ls.dist;
-        float cos_l = dot(ls.n, -ls.wi);
-        uint32_t obj = __float_as_uint(S.tris[3 * L.index + 1].w);
+        const float cos_l = dot(ls.n, -ls.wi);
+        const uint32_t obj = __float_as_uint(S.tris[3 * L.index + 1].w);
         const rp_emission &em = S.emissions[S.objects[obj].emission];
         if (!em.two_sided && !(cos_l > 0)) return ls;
         if (cos_l == 0.0f) return ls;
-        ls.pdf = ((L.pmf * L.inv_area) * d2) / fabsf(cos_l);
+        ls.pdf = sel.pmf * pdf_omega;
         for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(em.radiance, wl.lambda[k], S.spectra);
         ls.ok = ls.pdf > 0 && __builtin_isfinite(ls.pdf);
         return ls;
@@ -1020,7 +1359,7 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, flo
         float fall = t * t * (3.0f - 2.0f * t);
         for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(sp.intensity, wl.lambda[k], S.spectra) * fall / d2;
         ls.p = pos;
-        ls.pdf = L.pmf;
+        ls.pdf = sel.pmf;
         ls.delta = true;
         ls.ok = true;
         return ls;
@@ -1031,10 +1370,27 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, const Wavelengths &wl, flo
     sincos_turns(u1, sn, cs);
     ls.wi = {r * cs, r * sn, z};
     ls.infinite = true;
-    ls.pdf = L.pmf * kInv4Pi;
+    ls.pdf = sel.pmf * kInv4Pi;
     for (int k = 0; k < 4; ++k) ls.le[k] = spectrum_eval(S.environment, wl.lambda[k], S.spectra);
     ls.ok = true;
     return ls;
+}
+
+// Solid-angle density of light sampling from (p, n) toward emitter point q
+// (geometric normal ng) of light `id`: selection pmf x proposal density.
+RP_DEV float area_light_pdf(const Scene &S, uint32_t id, F3 p, F3 n, F3 q, F3 ng) {
+    const float pmf = light_pmf(S, p, n, id);
+    const LightEntry &L = S.lights[id];
+    if (L.kind == 3) {
+        const Quad Q = quad_of(L);
+        float u, v;
+        quad_coords(Q, q, u, v);
+        return pmf * quad_pdf(Q, p, n, u, v, q);
+    }
+    const F3 dv = q - p;
+    const float d2 = dot(dv, dv);
+    const float cl = fabsf(dot(ng, dv / __builtin_sqrtf(d2)));
+    return cl > 0 ? pmf * (L.inv_area * d2 / cl) : 0.0f;
 }
 
 // Visibility with transmittance through null interfaces and media.

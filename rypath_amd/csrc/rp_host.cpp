@@ -3,11 +3,13 @@
 #include "rp_internal.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <dlfcn.h>
 #include <fstream>
+#include <functional>
 #include <mutex>
 #include <stdexcept>
 
@@ -242,57 +244,432 @@ float scene_bounding_radius(const rp_scene_desc &d) {
     return d.vertex_count ? float(std::sqrt(r2)) : 0.0f;
 }
 
-static float avg_spectrum(const rp_spectrum &s, const float *pool) {
-    double acc = 0;
-    for (int i = 0; i < 471; ++i) acc += double(spectrum_eval_host(s, float(360 + i), pool));
-    return float(acc / 471.0);
+// Light tables for next-event estimation (the reference's preparation,
+// src/metal_render.mm:540-640 and 1560-1780, and build_light_sampling,
+// src/light_sampling.cpp:226-571; PBRT v4's BVHLightSampler and
+// PowerLightSampler): one light per emissive triangle, per quad (two
+// consecutive triangles (a,b,c), (a,c,d) of one emissive object forming a
+// rectangle), per spot and for the environment; their power alias table;
+// the light BVH over the finite lights.  Double precision, in the order the
+// reference builds them.
+namespace {
+
+// The hero wavelength sampler's log and exp (binary32, the device's series).
+float host_exp_f(float x) {
+    if (x != x) return x;
+    if (!(x > -104.0f)) return 0.0f;
+    if (x > 88.72f) return INFINITY;
+    float k = std::floor(x * 1.44269504088896341f + 0.5f);
+    float r = (x - k * 0.693145751953125f) - k * 1.42860682030941723212e-6f;
+    float p = 1.0f + r * (1.0f + r * (0.5f + r * (1.0f / 6.0f + r * (1.0f / 24.0f + r * (1.0f / 120.0f + r * (1.0f / 720.0f + r * (1.0f / 5040.0f)))))));
+    return std::ldexp(p, int(k));
+}
+float host_log_f(float x) {
+    if (x != x || x < 0.0f) return NAN;
+    if (x == 0.0f) return -INFINITY;
+    if (x == INFINITY) return x;
+    int e;
+    float m = std::frexp(x, &e);
+    if (m < 0.70710678118654752f) { m = m * 2.0f; e -= 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float lm = s * (2.0f + s2 * (2.0f / 3.0f + s2 * (2.0f / 5.0f + s2 * (2.0f / 7.0f + s2 * (2.0f / 9.0f)))));
+    float ef = float(e);
+    return ef * 0.693145751953125f + (lm + ef * 1.42860682030941723212e-6f);
+}
+// The four visible-sampled hero wavelengths at u = 1/2 and their densities.
+void visible_half(float lambda[4], float pdf[4]) {
+    for (int k = 0; k < 4; ++k) {
+        uint32_t coord = uint32_t(0.5f * 16777216.0f);
+        float s = float((coord + uint32_t(k) * 4194304u) & 0x00ffffffu) * 0x1p-24f;
+        float a = 0.85691062f - 1.82750197f * s;
+        float at = 0.5f * host_log_f((1.0f + a) / (1.0f - a));
+        float l = 538.0f - 138.888889f * at;
+        l = std::min(std::max(l, 360.0f), 830.0f);
+        float e = host_exp_f(0.0072f * (l - 538.0f));
+        float ch = 0.5f * (e + 1.0f / e);
+        lambda[k] = l;
+        pdf[k] = 0.0039398042f / (ch * ch);
+    }
+}
+// Power proxy of a spectrum: the visible-sampled estimate at u = 1/2 of its
+// integral over nanometres (zero: the plain sum over 360..830 nm).
+double spectrum_power(const rp_spectrum &sp, const float *pool) {
+    float lambda[4], pdf[4];
+    visible_half(lambda, pdf);
+    double v = 0;
+    for (int k = 0; k < 4; ++k) v += double(spectrum_eval_host(sp, std::round(lambda[k]), pool)) / double(pdf[k]) / 4.0;
+    if (v == 0)
+        for (int i = 0; i < 471; ++i) v += double(spectrum_eval_host(sp, float(360 + i), pool));
+    return v;
+}
+double spectrum_max(const rp_spectrum &sp, const float *pool) {
+    double m = 0;
+    for (int i = 0; i < 471; ++i) m = std::max(m, double(spectrum_eval_host(sp, float(360 + i), pool)));
+    return m;
 }
 
-std::vector<LightEntry> build_light_table(const rp_scene_desc &d, float scene_radius, std::vector<uint32_t> &tri_light) {
-    const double pi = double(3.14159265358979323846f);
-    std::vector<LightEntry> lights;
-    std::vector<double> power;
+struct Bounds {
+    double power = 0, bounds_power = -1;
+    float low[3] = {0, 0, 0}, high[3] = {0, 0, 0}, dir[3] = {0, 0, 1};
+    float cos_outer = -1, cos_emission = 0;
+    bool two_sided = true, infinite = false;
+};
+using D3 = std::array<double, 3>;
+D3 dsub(const D3 &a, const D3 &b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+double ddot(const D3 &a, const D3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+double dlen(const D3 &a) { return std::sqrt(ddot(a, a)); }
+D3 dcross(const D3 &a, const D3 &b) {
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+
+// Union: boxes, powers, two-sidedness, the smaller emission cosine, and
+// the smallest direction cone holding both cones.
+Bounds unite(const Bounds &a, const Bounds &b) {
+    if (a.power == 0) return b;
+    if (b.power == 0) return a;
+    Bounds r = a;
+    r.power = a.power + b.power;
+    r.two_sided = a.two_sided || b.two_sided;
+    r.cos_emission = std::min(a.cos_emission, b.cos_emission);
+    for (int i = 0; i < 3; ++i) {
+        r.low[i] = std::min(a.low[i], b.low[i]);
+        r.high[i] = std::max(a.high[i], b.high[i]);
+    }
+    const double pi = 3.14159265358979323846;
+    const double ta = std::acos(std::clamp(double(a.cos_outer), -1.0, 1.0));
+    const double tb = std::acos(std::clamp(double(b.cos_outer), -1.0, 1.0));
+    const D3 wa = {a.dir[0], a.dir[1], a.dir[2]}, wb = {b.dir[0], b.dir[1], b.dir[2]};
+    const double between = std::acos(std::clamp(ddot(wa, wb), -1.0, 1.0));
+    if (std::min(between + tb, pi) <= ta) return r;
+    if (std::min(between + ta, pi) <= tb) {
+        for (int i = 0; i < 3; ++i) r.dir[i] = b.dir[i];
+        r.cos_outer = b.cos_outer;
+        return r;
+    }
+    const double theta = (ta + tb + between) / 2 + 1e-6;
+    if (theta >= pi) {
+        r.cos_outer = -1;
+        r.dir[0] = r.dir[1] = 0;
+        r.dir[2] = 1;
+        return r;
+    }
+    D3 t;
+    for (int i = 0; i < 3; ++i) t[i] = wb[i] - ddot(wa, wb) * wa[i];
+    double tl = dlen(t);
+    if (tl < 1e-12) {
+        const int axis = std::fabs(wa[0]) < 0.8 ? 0 : 1;
+        D3 e = {0, 0, 0};
+        e[axis] = 1;
+        for (int i = 0; i < 3; ++i) t[i] = e[i] - wa[axis] * wa[i];
+        tl = dlen(t);
+    }
+    const double turn = theta - ta;
+    for (int i = 0; i < 3; ++i) r.dir[i] = float(std::cos(turn) * wa[i] + std::sin(turn) * t[i] / tl);
+    const double dl = std::sqrt(double(r.dir[0]) * r.dir[0] + double(r.dir[1]) * r.dir[1] + double(r.dir[2]) * r.dir[2]);
+    for (int i = 0; i < 3; ++i) r.dir[i] = float(r.dir[i] / dl);
+    r.cos_outer = std::nextafter(float(std::cos(theta)), -INFINITY);
+    return r;
+}
+// Split cost: power x the cone's angular measure x surface area, scaled by
+// the parent's aspect ratio along `axis`.
+double split_cost(const Bounds &b, const Bounds &all, int axis) {
+    if (b.power == 0) return 0;
+    const double pi = 3.14159265358979323846;
+    const double o = std::acos(std::clamp(double(b.cos_outer), -1.0, 1.0));
+    const double e = std::acos(std::clamp(double(b.cos_emission), -1.0, 1.0));
+    const double w = std::min(o + e, pi);
+    const double omega = 2 * pi * (1 - b.cos_outer) +
+                         pi / 2 * (2 * w * std::sin(o) - std::cos(o - 2 * w) - 2 * o * std::sin(o) + b.cos_outer);
+    double d[3], ad[3];
+    for (int i = 0; i < 3; ++i) {
+        d[i] = double(b.high[i]) - double(b.low[i]);
+        ad[i] = double(all.high[i]) - double(all.low[i]);
+    }
+    if (ad[axis] <= 0) return INFINITY;
+    const double longest = std::max(ad[0], std::max(ad[1], ad[2]));
+    return b.power * omega * (longest / ad[axis]) * 2 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]);
+}
+// Rectangle classification of p = (p00, p10, p01, p11) in double
+// (src/light_sampling.cpp:294-311) and the sampler's binary32 test.
+bool rectangle_f64(const D3 p[4]) {
+    const D3 n = dcross(dsub(p[1], p[0]), dsub(p[2], p[0]));
+    const double nl = dlen(n), diag = dlen(dsub(p[3], p[0]));
+    bool ok = nl > 0 && diag > 0 && std::fabs(ddot(dsub(p[3], p[0]), n) / (diag * nl)) <= 1e-5;
+    D3 c = {0, 0, 0};
+    for (int i = 0; i < 4; ++i)
+        for (int a = 0; a < 3; ++a) c[a] += p[i][a] / 4;
+    const double d0 = ddot(dsub(p[0], c), dsub(p[0], c));
+    const int next[4] = {2, 0, 3, 1};
+    for (int i = 0; i < 4; ++i) {
+        const D3 dl = dsub(p[i], c);
+        ok = ok && dlen(dsub(p[i], p[next[i]])) > 0 && d0 > 0 && std::fabs(ddot(dl, dl) - d0) / d0 <= 1e-4;
+    }
+    return ok;
+}
+struct F3h {
+    float x, y, z;
+};
+F3h fsub(F3h a, F3h b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+F3h fadd(F3h a, F3h b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+float fdot(F3h a, F3h b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+bool rectangle_f32(F3h p00, F3h p10, F3h p01, F3h p11) {
+    const F3h ex = fsub(p10, p00), ey = fsub(p01, p00);
+    const float x2 = fdot(ex, ex), y2 = fdot(ey, ey);
+    if (!(x2 > 0 && y2 > 0)) return false;
+    const F3h e = fsub(p11, fadd(fadd(p00, ex), ey));
+    return fdot(e, e) <= 0x1p-40f * std::max(x2, y2) && std::fabs(fdot(ex, ey)) <= 0x1p-20f * std::sqrt(x2 * y2);
+}
+
+} // namespace
+
+LightTables build_light_tables(const rp_scene_desc &d, float scene_radius, std::vector<uint32_t> &tri_light) {
+    const double pi = 3.14159265358979323846;
+    LightTables out;
     tri_light.assign(d.triangle_count, RP_NONE);
+    std::vector<Bounds> in;
+    auto vert = [&](uint32_t i) {
+        return F3h{d.positions[3 * size_t(i)], d.positions[3 * size_t(i) + 1], d.positions[3 * size_t(i) + 2]};
+    };
+    auto dv = [](F3h v) { return D3{double(v.x), double(v.y), double(v.z)}; };
+    auto box_of = [](Bounds &b, const F3h *c, int n) {
+        for (int a = 0; a < 3; ++a) {
+            b.low[a] = INFINITY;
+            b.high[a] = -INFINITY;
+        }
+        for (int k = 0; k < n; ++k) {
+            const float v[3] = {c[k].x, c[k].y, c[k].z};
+            for (int a = 0; a < 3; ++a) {
+                b.low[a] = std::min(b.low[a], v[a]);
+                b.high[a] = std::max(b.high[a], v[a]);
+            }
+        }
+    };
     for (uint32_t t = 0; t < d.triangle_count; ++t) {
-        const rp_object &obj = d.objects[d.tri_object[t]];
+        const uint32_t o = d.tri_object[t];
+        const rp_object &obj = d.objects[o];
         if (obj.emission == RP_NONE) continue;
         const rp_emission &em = d.emissions[obj.emission];
-        const float *a = d.positions + 3 * size_t(d.indices[3 * t]);
-        const float *b = d.positions + 3 * size_t(d.indices[3 * t + 1]);
-        const float *c = d.positions + 3 * size_t(d.indices[3 * t + 2]);
-        double ex = double(b[0]) - a[0], ey = double(b[1]) - a[1], ez = double(b[2]) - a[2];
-        double fx = double(c[0]) - a[0], fy = double(c[1]) - a[1], fz = double(c[2]) - a[2];
-        double cx = ey * fz - ez * fy, cy = ez * fx - ex * fz, cz = ex * fy - ey * fx;
-        double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        const double mean = spectrum_power(em.radiance, d.spectra), bound = spectrum_max(em.radiance, d.spectra);
+        const uint32_t *ia = d.indices + 3 * size_t(t);
+        if (t + 1 < d.triangle_count && d.tri_object[t + 1] == o && ia[3] == ia[0] && ia[4] == ia[2]) {
+            const F3h c[4] = {vert(ia[0]), vert(ia[1]), vert(ia[5]), vert(ia[2])}; // p00, p10, p01, p11
+            const D3 p[4] = {dv(c[0]), dv(c[1]), dv(c[2]), dv(c[3])};
+            if (rectangle_f64(p) && rectangle_f32(c[0], c[1], c[2], c[3])) {
+                const double area = dlen(dsub(p[2], p[0])) * dlen(dsub(p[1], p[0]));
+                Bounds b;
+                box_of(b, c, 4);
+                b.power = pi * area * mean * (em.two_sided ? 2.0 : 1.0);
+                b.bounds_power = pi * area * bound;
+                b.two_sided = em.two_sided != 0;
+                const D3 ns[4] = {dcross(dsub(p[1], p[0]), dsub(p[2], p[0])), dcross(dsub(p[1], p[0]), dsub(p[3], p[1])),
+                                  dcross(dsub(p[3], p[2]), dsub(p[2], p[0])), dcross(dsub(p[3], p[2]), dsub(p[3], p[1]))};
+                D3 axis = {0, 0, 0};
+                bool degenerate = false;
+                for (const D3 &n : ns) {
+                    const double l = dlen(n);
+                    if (!(l > 0)) degenerate = true;
+                    else
+                        for (int a = 0; a < 3; ++a) axis[a] += n[a] / l;
+                }
+                const double al = dlen(axis);
+                if (!degenerate && al > 0) {
+                    double cmin = 1;
+                    for (const D3 &n : ns) cmin = std::min(cmin, (ddot(axis, n) / al) / dlen(n));
+                    for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a] / al);
+                    b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
+                }
+                LightEntry L{};
+                L.kind = 3;
+                L.index = t;
+                L.leaf = RP_NONE;
+                L.rect = 1;
+                L.inv_area = float(1.0 / area);
+                const F3h ex = fsub(c[1], c[0]), ey = fsub(c[2], c[0]);
+                const float pv[9] = {c[0].x, c[0].y, c[0].z, ex.x, ex.y, ex.z, ey.x, ey.y, ey.z};
+                for (int k = 0; k < 3; ++k) {
+                    L.p00[k] = pv[k];
+                    L.ex[k] = pv[3 + k];
+                    L.ey[k] = pv[6 + k];
+                }
+                tri_light[t] = tri_light[t + 1] = uint32_t(out.lights.size());
+                out.lights.push_back(L);
+                in.push_back(b);
+                ++t;
+                continue;
+            }
+        }
+        const F3h c[3] = {vert(ia[0]), vert(ia[1]), vert(ia[2])};
+        const D3 n = dcross(dsub(dv(c[1]), dv(c[0])), dsub(dv(c[2]), dv(c[0])));
+        const double area = 0.5 * dlen(n);
         if (!(area > 0)) continue;
-        double p = pi * area * double(avg_spectrum(em.radiance, d.spectra)) * (em.two_sided ? 2.0 : 1.0);
-        tri_light[t] = uint32_t(lights.size());
-        lights.push_back({0u, t, 0.0f, 0.0f, float(1.0 / area), {0, 0, 0}});
-        power.push_back(p);
+        Bounds b;
+        box_of(b, c, 3);
+        b.power = pi * area * mean * (em.two_sided ? 2.0 : 1.0);
+        b.bounds_power = pi * area * bound;
+        b.two_sided = em.two_sided != 0;
+        for (int a = 0; a < 3; ++a) b.dir[a] = float(n[a] / (2 * area));
+        b.cos_outer = std::nextafter(1.0f, 0.0f);
+        LightEntry L{};
+        L.kind = 0;
+        L.index = t;
+        L.leaf = RP_NONE;
+        L.inv_area = float(1.0 / area);
+        tri_light[t] = uint32_t(out.lights.size());
+        out.lights.push_back(L);
+        in.push_back(b);
     }
     for (uint32_t i = 0; i < d.spot_count; ++i) {
         const rp_spot_light &sp = d.spots[i];
-        double p = double(avg_spectrum(sp.intensity, d.spectra)) * 2.0 * pi *
-                   (1.0 - 0.5 * (double(sp.cos_falloff_start) + double(sp.cos_falloff_end)));
-        lights.push_back({1u, i, 0.0f, 0.0f, 0.0f, {0, 0, 0}});
-        power.push_back(p);
+        Bounds b;
+        for (int a = 0; a < 3; ++a) b.low[a] = b.high[a] = sp.position[a];
+        b.power = 2 * pi * ((1 - double(sp.cos_falloff_start)) + (double(sp.cos_falloff_start) - sp.cos_falloff_end) / 2) *
+                  spectrum_power(sp.intensity, d.spectra);
+        b.bounds_power = 4 * pi * spectrum_max(sp.intensity, d.spectra);
+        b.two_sided = false;
+        float ce = float(std::cos(std::acos(double(sp.cos_falloff_end)) - std::acos(double(sp.cos_falloff_start))));
+        if (ce == 1) ce = sp.cos_falloff_end != sp.cos_falloff_start ? 0.999f : std::nextafter(1.0f, 0.0f);
+        for (int a = 0; a < 3; ++a) b.dir[a] = sp.direction[a];
+        b.cos_outer = float(std::cos(std::min(pi, std::acos(std::clamp(double(sp.cos_falloff_start), -1.0, 1.0)) + 2e-6)));
+        b.cos_outer = std::max(-1.0f, std::nextafter(b.cos_outer, -INFINITY));
+        b.cos_emission = ce;
+        LightEntry L{};
+        L.kind = 1;
+        L.index = i;
+        L.leaf = RP_NONE;
+        out.lights.push_back(L);
+        in.push_back(b);
     }
     if (d.has_environment) {
-        double r = scene_radius;
-        double p = pi * r * r * 4.0 * pi * double(avg_spectrum(d.environment, d.spectra));
-        lights.push_back({2u, 0u, 0.0f, 0.0f, 0.0f, {0, 0, 0}});
-        power.push_back(p);
+        Bounds b;
+        b.infinite = true;
+        const double r = scene_radius;
+        b.power = 4 * pi * pi * r * r * spectrum_power(d.environment, d.spectra);
+        out.env_light = uint32_t(out.lights.size());
+        LightEntry L{};
+        L.kind = 2;
+        L.leaf = RP_NONE;
+        out.lights.push_back(L);
+        in.push_back(b);
     }
-    double total = 0;
-    for (double p : power) total += p;
-    double cum = 0;
-    for (size_t i = 0; i < lights.size(); ++i) {
-        cum += power[i];
-        lights[i].pmf = total > 0 ? float(power[i] / total) : 0.0f;
-        lights[i].cdf = total > 0 ? float(cum / total) : 0.0f;
+    const size_t n = in.size();
+    if (n == 0) return out;
+    double total = 0, bounds_total = 0;
+    std::vector<double> prob(n);
+    std::vector<Bounds> bv = in;
+    std::vector<uint32_t> finite;
+    for (size_t i = 0; i < n; ++i) {
+        bv[i].power = in[i].bounds_power < 0 ? in[i].power : in[i].bounds_power;
+        if (bv[i].power == 0 && in[i].power > 0) bv[i].power = in[i].power;
+        prob[i] = in[i].power == 0 && bv[i].power > 0 ? bv[i].power : in[i].power;
+        total += prob[i];
+        if (!in[i].infinite) bounds_total += bv[i].power;
+        if (!in[i].infinite && bv[i].power > 0) finite.push_back(uint32_t(i));
     }
-    if (!lights.empty() && total > 0) lights.back().cdf = 1.0f;
-    return lights;
+    // power alias table (Walker / Vose with small and large stacks)
+    std::vector<uint32_t> small, large;
+    for (size_t i = 0; i < n; ++i) {
+        const double pr = total > 0 ? prob[i] / total : 1.0 / double(n);
+        out.lights[i].pmf = float(pr);
+        out.lights[i].threshold = 1;
+        out.lights[i].alias = uint32_t(i);
+        prob[i] = pr * double(n);
+        (prob[i] < 1 ? small : large).push_back(uint32_t(i));
+    }
+    while (!small.empty() && !large.empty()) {
+        const uint32_t a = small.back();
+        small.pop_back();
+        const uint32_t b = large.back();
+        large.pop_back();
+        out.lights[a].threshold = float(prob[a]);
+        out.lights[a].alias = b;
+        prob[b] -= 1 - prob[a];
+        (prob[b] < 1 ? small : large).push_back(b);
+    }
+    for (uint32_t i : small) out.lights[i].threshold = 1, out.lights[i].alias = i;
+    for (uint32_t i : large) out.lights[i].threshold = 1, out.lights[i].alias = i;
+    // light BVH: 12 centroid buckets per axis, splits 1..10
+    auto bucket_of = [&](uint32_t id, int axis, const double lo[3], const double hi[3]) {
+        const double c = (double(bv[id].low[axis]) + bv[id].high[axis]) / 2;
+        return std::min(11, int(12 * (c - lo[axis]) / (hi[axis] - lo[axis])));
+    };
+    std::function<uint32_t(size_t, size_t, uint32_t, unsigned)> build = [&](size_t b, size_t e, uint32_t parent,
+                                                                              unsigned depth) -> uint32_t {
+        const uint32_t index = uint32_t(out.nodes.size());
+        out.nodes.emplace_back();
+        Bounds all;
+        double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t k = b; k < e; ++k) {
+            const Bounds &v = bv[finite[k]];
+            all = unite(all, v);
+            for (int a = 0; a < 3; ++a) {
+                const double c = (double(v.low[a]) + v.high[a]) / 2;
+                clo[a] = std::min(clo[a], c);
+                chi[a] = std::max(chi[a], c);
+            }
+        }
+        LightNodeHost node{};
+        for (int a = 0; a < 3; ++a) {
+            node.low[a] = all.low[a];
+            node.high[a] = all.high[a];
+            node.dir[a] = all.dir[a];
+        }
+        node.cos_outer = all.cos_outer;
+        node.cos_emission = all.cos_emission;
+        node.power = float(all.power / bounds_total);
+        node.parent = parent;
+        node.light = RP_NONE;
+        node.child = RP_NONE;
+        node.flags = all.two_sided ? 1u : 0u;
+        if (e - b == 1) {
+            node.light = finite[b];
+            out.lights[finite[b]].leaf = index;
+        } else {
+            double best = INFINITY;
+            int best_axis = 0, best_split = 0;
+            bool found = false;
+            if (depth < 48)
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (clo[axis] == chi[axis]) continue;
+                    Bounds bucket[12];
+                    for (size_t k = b; k < e; ++k) {
+                        const uint32_t id = finite[k];
+                        const int q = bucket_of(id, axis, clo, chi);
+                        bucket[q] = unite(bucket[q], bv[id]);
+                    }
+                    for (int split = 1; split < 11; ++split) {
+                        Bounds left, right;
+                        for (int k = 0; k <= split; ++k) left = unite(left, bucket[k]);
+                        for (int k = split + 1; k < 12; ++k) right = unite(right, bucket[k]);
+                        const double c = split_cost(left, all, axis) + split_cost(right, all, axis);
+                        if (c > 0 && c < best) {
+                            best = c;
+                            best_axis = axis;
+                            best_split = split;
+                            found = true;
+                        }
+                    }
+                }
+            size_t mid = b + (e - b) / 2;
+            if (found) {
+                auto it = std::stable_partition(finite.begin() + ptrdiff_t(b), finite.begin() + ptrdiff_t(e),
+                                                [&](uint32_t id) { return bucket_of(id, best_axis, clo, chi) <= best_split; });
+                const size_t cand = size_t(it - finite.begin());
+                if (cand > b && cand < e) mid = cand;
+            }
+            build(b, mid, index, depth + 1);
+            node.child = build(mid, e, index, depth + 1);
+        }
+        out.nodes[index] = node;
+        return index;
+    };
+    if (!finite.empty()) build(0, finite.size(), RP_NONE, 0);
+    if (out.env_light != RP_NONE) {
+        out.env_pmf = d.light_sampler == RP_LIGHT_SAMPLER_UNIFORM ? 1.0f / float(n)
+                      : d.light_sampler == RP_LIGHT_SAMPLER_POWER  ? out.lights[out.env_light].pmf
+                                                                    : 1.0f / float(1 + (out.nodes.empty() ? 0 : 1));
+    }
+    return out;
 }
 
 // ------------------------------------------------------------- scene copy
@@ -336,6 +713,7 @@ std::string validate_desc(const rp_scene_desc &d) {
     // BSDF blocks 4.. and NEE blocks 133.. stay disjoint and inside 1024
     // dimensions (src/sampling_contract.h:16-28, dielectric_max_depth = 113).
     if (d.max_depth > 113) return "max_depth beyond the Sobol dimension layout (113)";
+    if (d.light_sampler > RP_LIGHT_SAMPLER_UNIFORM) return "unknown light_sampler";
     for (uint32_t t = 0; t < d.triangle_count; ++t) {
         for (int k = 0; k < 3; ++k)
             if (d.indices[3 * t + k] >= d.vertex_count) return "triangle index out of range";

@@ -83,15 +83,30 @@ inline float decode_lo(float origin, uint8_t e, uint8_t q) {
 }
 
 // ------------------------------------------------------------- lights
+// One light: kind 0 triangle, 1 spot, 2 environment, 3 quad (p00 + u ex + v ey);
+// index: scene triangle id (a quad's first triangle) or spot index; leaf: its
+// light-BVH node; pmf / threshold / alias: the power sampler's alias table.
 struct LightEntry {
-    uint32_t kind;  // 0 triangle, 1 spot, 2 environment
-    uint32_t index; // triangle id / spot index / 0
-    float pmf, cdf, inv_area;
-    uint32_t pad[3];
+    uint32_t kind, index, leaf, rect;
+    float pmf, threshold;
+    uint32_t alias;
+    float inv_area;
+    float p00[3], ex[3], ey[3];
 };
-// Power-proportional light table (PBRT PowerLightSampler semantics, fixed order:
-// emissive triangles by id, then spots, then the environment).
-std::vector<LightEntry> build_light_table(const rp_scene_desc &d, float scene_radius, std::vector<uint32_t> &tri_light);
+struct LightNodeHost {
+    float low[3], high[3], dir[3];
+    float cos_outer, cos_emission, power;
+    uint32_t child, parent, light, flags;
+};
+struct LightTables {
+    std::vector<LightEntry> lights;
+    std::vector<LightNodeHost> nodes; // node i + 1 is the first child of interior node i
+    uint32_t env_light = RP_NONE;
+    float env_pmf = 0; // the environment's selection probability
+};
+// Light tables of a scene for its light_sampler; tri_light maps each
+// triangle to its light (RP_NONE when not emissive).
+LightTables build_light_tables(const rp_scene_desc &d, float scene_radius, std::vector<uint32_t> &tri_light);
 float scene_bounding_radius(const rp_scene_desc &d);
 
 // ------------------------------------------------------------- owned scene copy

@@ -83,6 +83,7 @@ struct rp_scene {
     DevBuf<rp_spot_light> spots;
     DevBuf<float> spectra;
     DevBuf<rpd::LightEntry> lights;
+    DevBuf<rpd::LightNode> light_nodes;
     DevBuf<float4> cie;
     DevBuf<uint32_t> sobol_dir;
     DevBuf<uint64_t> sobol_intervals, sobol_inverse;
@@ -296,7 +297,8 @@ void upload_scene(rp_scene *s) {
     // leaf-order triangles: (v0, scene id), (v1, object), (v2, light)
     std::vector<uint32_t> tri_light;
     float radius = scene_bounding_radius(d);
-    std::vector<LightEntry> lights = build_light_table(d, radius, tri_light);
+    LightTables lt = build_light_tables(d, radius, tri_light);
+    const std::vector<LightEntry> &lights = lt.lights;
     std::vector<uint32_t> leaf_pos(d.triangle_count);
     for (uint32_t p = 0; p < d.triangle_count; ++p) leaf_pos[s->bvh.tri_order[p]] = p;
     std::vector<float4> tris(3 * size_t(d.triangle_count));
@@ -313,12 +315,40 @@ void upload_scene(rp_scene *s) {
     }
     std::vector<rpd::LightEntry> dl(lights.size());
     for (size_t i = 0; i < lights.size(); ++i) {
-        dl[i].kind = lights[i].kind;
-        dl[i].index = lights[i].kind == 0 ? leaf_pos[lights[i].index] : lights[i].index;
-        dl[i].pmf = lights[i].pmf;
-        dl[i].cdf = lights[i].cdf;
-        dl[i].inv_area = lights[i].inv_area;
-        dl[i].pad[0] = dl[i].pad[1] = dl[i].pad[2] = 0;
+        const LightEntry &h = lights[i];
+        rpd::LightEntry &e = dl[i];
+        e = rpd::LightEntry{};
+        e.kind = h.kind;
+        // area lights address their (first) triangle by its leaf position
+        e.index = (h.kind == 0 || h.kind == 3) ? leaf_pos[h.index] : h.index;
+        e.leaf = h.leaf;
+        e.rect = h.rect;
+        e.pmf = h.pmf;
+        e.threshold = h.threshold;
+        e.alias = h.alias;
+        e.inv_area = h.inv_area;
+        for (int k = 0; k < 3; ++k) {
+            e.p00[k] = h.p00[k];
+            e.ex[k] = h.ex[k];
+            e.ey[k] = h.ey[k];
+        }
+    }
+    std::vector<rpd::LightNode> dn(lt.nodes.size());
+    for (size_t i = 0; i < dn.size(); ++i) {
+        const LightNodeHost &h = lt.nodes[i];
+        rpd::LightNode &e = dn[i];
+        for (int k = 0; k < 3; ++k) {
+            e.low[k] = h.low[k];
+            e.high[k] = h.high[k];
+            e.dir[k] = h.dir[k];
+        }
+        e.cos_outer = h.cos_outer;
+        e.cos_emission = h.cos_emission;
+        e.power = h.power;
+        e.child = h.child;
+        e.parent = h.parent;
+        e.light = h.light;
+        e.flags = h.flags;
     }
     s->tris.upload(tris);
     s->nodes.upload(s->bvh.nodes);
@@ -329,6 +359,7 @@ void upload_scene(rp_scene *s) {
     s->spots.upload(s->host.spots);
     s->spectra.upload(s->host.spectra);
     s->lights.upload(dl);
+    s->light_nodes.upload(dn);
     std::vector<float4> cie(471);
     const auto &ct = cie_xyz();
     for (int i = 0; i < 471; ++i) cie[i] = make_float4(ct[i][0], ct[i][1], ct[i][2], 0.0f);
@@ -355,6 +386,10 @@ void upload_scene(rp_scene *s) {
     S.spectra = s->spectra.p;
     S.lights = s->lights.p;
     S.light_count = uint32_t(dl.size());
+    S.light_nodes = s->light_nodes.p;
+    S.light_node_count = uint32_t(dn.size());
+    S.light_sampler = d.light_sampler;
+    S.env_light = lt.env_light;
     S.has_environment = d.has_environment;
     S.cie = s->cie.p;
     S.sobol_dir = s->sobol_dir.p;
@@ -371,13 +406,14 @@ void upload_scene(rp_scene *s) {
     for (const auto &mt : s->host.materials)
         if (mt.kind == RP_MATERIAL_INTERFACE) S.has_interfaces = 1;
     S.has_media = d.medium_count ? 1 : 0;
-    S.env_pdf = (d.has_environment && !dl.empty()) ? dl.back().pmf * rpd::kInv4Pi : 0.0f;
+    S.env_pdf = lt.env_light != RP_NONE ? lt.env_pmf * rpd::kInv4Pi : 0.0f;
     s->info.emitter_count = 0;
     for (auto &l : dl)
-        if (l.kind == 0) ++s->info.emitter_count;
+        if (l.kind == 0 || l.kind == 3) ++s->info.emitter_count;
     s->info.light_count = uint32_t(dl.size());
     s->info.device_bytes = s->tris.n * sizeof(float4) + s->nodes.n * sizeof(QNode) + s->spectra.n * 4 +
-                           s->sobol_dir.n * 4 + s->lights.n * sizeof(rpd::LightEntry);
+                           s->sobol_dir.n * 4 + s->lights.n * sizeof(rpd::LightEntry) +
+                           s->light_nodes.n * sizeof(rpd::LightNode);
 }
 
 rpd::PathParams path_params(uint64_t seed, uint32_t randomization, uint32_t emergency, uint32_t boundary,

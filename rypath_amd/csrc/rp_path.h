@@ -25,6 +25,7 @@ struct PathState {
     float tmax;
     float beta[4], density[4], prev_density[4], prev_pdf[4], eta_scale[4], L[4];
     F3 prev_p;
+    uint32_t prev_nc; // the previous vertex's light-sampling normal (octahedral code; kNoNormal in media)
     uint32_t pixel;
     uint64_t sample, index;
     uint32_t w2;           // wavelength Sobol word (dimension 2)
@@ -72,6 +73,7 @@ RP_DEV void path_init(const Scene &S, const PathParams &P, uint32_t pixel, uint6
         ps.L[k] = 0.0f;
     }
     ps.prev_p = ps.o;
+    ps.prev_nc = kNoNormal;
     ps.count = 1;
     ps.crossings = 0;
     ps.medium = S.camera.medium;
@@ -183,8 +185,10 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
             for (int k = 0; k < 4; ++k) ps.prev_pdf[k] = ppdf;
             ps.flags &= ~kFlagPrevDelta;
             ps.prev_p = mp;
+            ps.prev_nc = kNoNormal;
             const uint4 lw = sobol_scramble4(lraw, 133 + ps.count - 2, P.mode, seed_bits);
-            LightSample ls = sample_light(S, mp, wl, sobol_uniform(lw.x), sobol_uniform(lw.y), sobol_uniform(lw.z));
+            LightSample ls = sample_light(S, mp, F3{0, 0, 0}, wl, sobol_uniform(lw.x), sobol_uniform(lw.y),
+                                          sobol_uniform(lw.z));
             if (ls.ok) {
                 float ph = hg(dot(wo, ls.wi), mc.g);
                 bool any = false;
@@ -263,10 +267,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
             float val[4];
             for (int k = 0; k < 4; ++k) val[k] = ps.beta[k] * spectrum_eval(em.radiance, wl.lambda[k], S.spectra);
             if (!(ps.count == 2 || prev_delta)) {
-                const LightEntry L = S.lights[light_id];
-                F3 dv = sp.p - ps.prev_p;
-                float d2 = dot(dv, dv);
-                float lp = ((L.pmf * L.inv_area) * d2) / fabsf(cos_l);
+                float lp = area_light_pdf(S, light_id, ps.prev_p, oct_decode(ps.prev_nc), sp.p, sp.n);
                 float lpv[4] = {lp, lp, lp, lp};
                 float a = mixture(ps.prev_density, ps.prev_pdf), b = mixture(ps.prev_density, lpv);
                 float w = a / (a + b);
@@ -328,6 +329,7 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                 if (hs.delta) ps.flags |= kFlagPrevDelta;
                 else ps.flags &= ~kFlagPrevDelta;
                 ps.prev_p = sp.p;
+                ps.prev_nc = oct_encode(sp.n);
                 F3 wi = normalize(fr.world(hs.wi));
                 ps.medium = outgoing_medium(obj, ps.medium, sp.n, wi);
                 ps.o = offset_origin(sp.p, sp.err, sp.n, wi);
@@ -338,7 +340,8 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
     }
     if (ps.status == RP_PATH_SUCCESS && direct_allowed && connectible) {
         const uint4 lw = sobol_scramble4(lraw, 133 + ps.count - 2, P.mode, seed_bits);
-        LightSample ls = sample_light(S, sp.p, wl, sobol_uniform(lw.x), sobol_uniform(lw.y), sobol_uniform(lw.z));
+        LightSample ls = sample_light(S, sp.p, oct_decode(oct_encode(sp.n)), wl, sobol_uniform(lw.x),
+                                      sobol_uniform(lw.y), sobol_uniform(lw.z));
         if (ls.ok) {
             F3 wil = fr.local(ls.wi);
             float cosv = fabsf(dot(sp.n, ls.wi));

@@ -189,6 +189,7 @@ struct Translator {
     std::vector<rp_spot_light> spots;
     rp_scene_desc desc{};
     rp_pbrt_info info{};
+    uint32_t light_sampler = RP_LIGHT_SAMPLER_BVH;
     std::map<std::string, uint32_t> named_materials, named_media;
     std::vector<Graphics> stack;
     Graphics g;
@@ -672,6 +673,13 @@ struct Translator {
                 if (type != "path" && type != "volpath") lx.fail("unsupported integrator '" + type + "'");
                 info.volumetric = type == "volpath";
                 info.max_depth = uint32_t(ps.f("maxdepth", 5));
+                // PBRT's light sampler choice (default "bvh")
+                const std::string ls = ps.s("lightsampler", "bvh");
+                light_sampler = ls == "bvh" ? RP_LIGHT_SAMPLER_BVH
+                                : ls == "power" ? RP_LIGHT_SAMPLER_POWER
+                                : ls == "uniform" ? RP_LIGHT_SAMPLER_UNIFORM
+                                                  : ~0u;
+                if (light_sampler == ~0u) lx.fail("unknown lightsampler '" + ls + "'");
             } else if (d == "WorldBegin") {
                 in_world = true;
                 g.ctm = identity();
@@ -829,6 +837,7 @@ struct Translator {
         }
         desc.max_depth = info.max_depth;
         desc.hero_wavelengths = 0;
+        desc.light_sampler = light_sampler;
         desc.vertex_count = uint32_t(positions.size() / 3);
         desc.triangle_count = uint32_t(tri_object.size());
         desc.object_count = uint32_t(objects.size());

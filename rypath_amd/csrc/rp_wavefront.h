@@ -28,6 +28,7 @@ RP_DEV void load_state(const WfState &B, uint32_t i, PathState &ps) {
     ps.o = {a.x, a.y, a.z};
     ps.tmax = a.w;
     ps.d = {b.x, b.y, b.z};
+    ps.prev_nc = __float_as_uint(b.w);
     float4 v;
     v = B.beta[i];
     ps.beta[0] = v.x, ps.beta[1] = v.y, ps.beta[2] = v.z, ps.beta[3] = v.w;
@@ -64,7 +65,7 @@ RP_DEV void load_state(const WfState &B, uint32_t i, PathState &ps) {
 template <bool kIds = true>
 RP_DEV void store_state(const WfState &B, uint32_t i, const PathState &ps) {
     B.o_t[i] = make_float4(ps.o.x, ps.o.y, ps.o.z, ps.tmax);
-    B.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, 0.0f);
+    B.d[i] = make_float4(ps.d.x, ps.d.y, ps.d.z, __uint_as_float(ps.prev_nc));
     B.beta[i] = make_float4(ps.beta[0], ps.beta[1], ps.beta[2], ps.beta[3]);
     B.density[i] = make_float4(ps.density[0], ps.density[1], ps.density[2], ps.density[3]);
     B.prev_density[i] = make_float4(ps.prev_density[0], ps.prev_density[1], ps.prev_density[2], ps.prev_density[3]);

@@ -53,7 +53,16 @@ class Oracle:
             "or_scene_create": (C.c_void_p, [C.POINTER(rp.SceneDesc)]),
             "or_scene_destroy": (None, [C.c_void_p]),
             "or_light_count": (u32, [C.c_void_p]),
-            "or_light_table": (None, [C.c_void_p] + [C.c_void_p] * 5),
+            "or_light_table": (None, [C.c_void_p] + [C.c_void_p] * 6),
+            "or_light_node_count": (u32, [C.c_void_p]),
+            "or_light_select": (u32, [C.c_void_p, C.POINTER(f32), C.POINTER(f32), f32, C.POINTER(f32)]),
+            "or_light_pmf": (f32, [C.c_void_p, C.POINTER(f32), C.POINTER(f32), u32]),
+            "or_light_sample": (u32, [C.c_void_p, C.POINTER(f32), C.POINTER(f32), f32, f32, f32, C.POINTER(f32),
+                                      C.POINTER(f32), C.POINTER(C.c_int)]),
+            "or_area_light_pdf": (f32, [C.c_void_p, u32, C.POINTER(f32), C.POINTER(f32), C.POINTER(f32),
+                                        C.POINTER(f32)]),
+            "or_atan2": (f32, [f32, f32]),
+            "or_asin": (f32, [f32]),
             "or_trace_closest": (None, [C.c_void_p, C.c_void_p, u64, C.c_void_p, u32]),
             "or_sample": (C.c_int, [C.c_void_p, C.POINTER(rp.Request), C.c_void_p, u32]),
             "or_render_frame": (C.c_int, [C.c_void_p, u64, u32, u64, u32, C.c_void_p, u32, C.POINTER(u64)]),
@@ -122,9 +131,33 @@ class OracleScene:
     def light_table(self):
         n = self.orc.L.or_light_count(self.h)
         arrs = [np.zeros(n, np.uint32), np.zeros(n, np.uint32), np.zeros(n, np.float32), np.zeros(n, np.float32),
-                np.zeros(n, np.float32)]
+                np.zeros(n, np.float32), np.zeros(n, np.uint32)]
         self.orc.L.or_light_table(self.h, *[a.ctypes.data_as(C.c_void_p) for a in arrs])
-        return dict(zip(["kind", "index", "pmf", "cdf", "inv_area"], arrs))
+        return dict(zip(["kind", "index", "pmf", "threshold", "inv_area", "leaf"], arrs))
+
+    def light_node_count(self):
+        return self.orc.L.or_light_node_count(self.h)
+
+    @staticmethod
+    def _v(x):
+        return (C.c_float * 3)(*[float(v) for v in x])
+
+    def light_select(self, p, n, u):
+        pmf = C.c_float()
+        i = self.orc.L.or_light_select(self.h, self._v(p), self._v(n), u, C.byref(pmf))
+        return i, pmf.value
+
+    def light_pmf(self, p, n, light):
+        return self.orc.L.or_light_pmf(self.h, self._v(p), self._v(n), light)
+
+    def light_sample(self, p, n, ul, u0, u1):
+        out, pdf, delta = (C.c_float * 3)(), C.c_float(), C.c_int()
+        i = self.orc.L.or_light_sample(self.h, self._v(p), self._v(n), ul, u0, u1, out, C.byref(pdf),
+                                       C.byref(delta))
+        return i, np.array(out[:], np.float64), pdf.value, bool(delta.value)
+
+    def area_light_pdf(self, light, p, n, q, ng):
+        return self.orc.L.or_area_light_pdf(self.h, light, self._v(p), self._v(n), self._v(q), self._v(ng))
 
     def trace_closest(self, rays, threads=0):
         rp = self.orc.rp
