@@ -25,7 +25,17 @@ struct Light {
     float inv_area;  // triangle / quad
     V3 p00, ex, ey;  // quad: p00 + u ex + v ey, u, v in [0, 1]
     bool rect;       // quad: passes the binary32 rectangle test (spherical sampling allowed)
+    V3 ux, uy, uz;   // quad: unit edge directions and their cross product (binary32)
+    float lx, ly;    // quad: edge lengths
 };
+// The quad's unit frame and edge lengths, as the host precomputes them.
+static void quad_axes(Light &L) {
+    L.lx = std::sqrt(dot(L.ex, L.ex));
+    L.ly = std::sqrt(dot(L.ey, L.ey));
+    L.ux = L.ex * (1.0f / L.lx);
+    L.uy = L.ey * (1.0f / L.ly);
+    L.uz = cross3(L.ux, L.uy);
+}
 // Light-BVH node (PBRT v4 LightBounds + tree links): bounds, direction cone,
 // emission cutoff, power normalised by the tree's total; flags bit 0 = two-sided.
 struct LightNode {
@@ -792,7 +802,9 @@ static void build_lights(Scene &s) {
                     for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a] / al);
                     b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
                 }
-                Light L{3u, t, t + 1, RP_NONE, 0, 1, 0, float(1.0 / area), c[0], c[1] - c[0], c[2] - c[0], true};
+                Light L{3u, t, t + 1, RP_NONE, 0, 1, 0, float(1.0 / area), c[0], c[1] - c[0], c[2] - c[0], true,
+                        {}, {}, {}, 0, 0};
+                quad_axes(L);
                 s.tri_light[t] = s.tri_light[t + 1] = uint32_t(s.lights.size());
                 s.lights.push_back(L);
                 in.push_back(b);
@@ -813,7 +825,7 @@ static void build_lights(Scene &s) {
         for (int a = 0; a < 3; ++a) b.dir[a] = float(n[a] / (2 * area));
         b.cos_outer = std::nextafter(1.0f, 0.0f);
         s.tri_light[t] = uint32_t(s.lights.size());
-        s.lights.push_back({0u, t, RP_NONE, RP_NONE, 0, 1, 0, float(1.0 / area), {}, {}, {}, false});
+        s.lights.push_back({0u, t, RP_NONE, RP_NONE, 0, 1, 0, float(1.0 / area), {}, {}, {}, false, {}, {}, {}, 0, 0});
         in.push_back(b);
     }
     for (uint32_t i = 0; i < s.spots.size(); ++i) {
@@ -830,7 +842,7 @@ static void build_lights(Scene &s) {
         b.cos_outer = float(std::cos(std::min(pi, std::acos(std::clamp(double(sp.cos_falloff_start), -1.0, 1.0)) + 2e-6)));
         b.cos_outer = std::max(-1.0f, std::nextafter(b.cos_outer, -INFINITY));
         b.cos_emission = ce;
-        s.lights.push_back({1u, i, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false});
+        s.lights.push_back({1u, i, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false, {}, {}, {}, 0, 0});
         in.push_back(b);
     }
     if (s.d.has_environment) {
@@ -839,7 +851,7 @@ static void build_lights(Scene &s) {
         const double r = s.scene_radius;
         b.power = 4 * pi * pi * r * r * spectrum_power(s, s.d.environment);
         s.env_light = uint32_t(s.lights.size());
-        s.lights.push_back({2u, 0u, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false});
+        s.lights.push_back({2u, 0u, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false, {}, {}, {}, 0, 0});
         in.push_back(b);
     }
     const size_t n = in.size();
@@ -1162,7 +1174,7 @@ static float light_importance(const LightNode &b, V3 p, V3 n) {
     const float radius = std::sqrt(r2), dist = std::sqrt(d2);
     const float cb = d2 <= r2 ? -1.0f : std::sqrt(std::max(0.0f, 1.0f - r2 / d2));
     const float sb = std::sqrt(std::max(0.0f, 1.0f - cb * cb));
-    const V3 wi = dist > 0 ? delta / dist : V3{0, 0, 1};
+    const V3 wi = dist > 0 ? delta * (1.0f / dist) : V3{0, 0, 1};
     float cw = std::clamp(dot(V3{b.dir[0], b.dir[1], b.dir[2]}, wi), -1.0f, 1.0f);
     if (b.flags & 1u) cw = std::fabs(cw);
     const float sw = std::sqrt(std::max(0.0f, 1.0f - cw * cw));
@@ -1253,31 +1265,18 @@ static float light_pmf(const Scene &s, V3 p, V3 n, uint32_t id) {
 // as adapted in src/light_geometry_sampling_shared.h:174-260 and
 // shaders/light_geometry_sampling.h:118-209), warped by the receiver's
 // corner cosines (bilinear, PBRT SampleBilinear) when a normal is given.
-static V3 unit(V3 v) { return v / length(v); }
+static V3 unit(V3 v) { return v * (1.0f / std::sqrt(dot(v, v))); }
 static float angle_between(V3 a, V3 b) {
     return dot(a, b) < 0 ? kPi - 2.0f * asin_f(std::min(1.0f, length(a + b) / 2.0f))
                          : 2.0f * asin_f(std::min(1.0f, length(a - b) / 2.0f));
 }
+// Area of the spherical triangle of unit vectors a, b, c (Van Oosterom-Strackee).
 static float spherical_triangle_area(V3 a, V3 b, V3 c) {
     return std::fabs(2.0f * atan2_f(dot(a, cross3(b, c)), 1.0f + dot(a, b) + dot(a, c) + dot(b, c)));
-}
-static float triangle_solid_angle(V3 p0, V3 p1, V3 p2, V3 r) {
-    const V3 a = p0 - r, b = p1 - r, c = p2 - r;
-    if (!(dot(a, a) > 0) || !(dot(b, b) > 0) || !(dot(c, c) > 0)) return 0.0f;
-    return spherical_triangle_area(unit(a), unit(b), unit(c));
-}
-static float quad_solid_angle(const Light &L, V3 r) {
-    const V3 p10 = L.p00 + L.ex, p01 = L.p00 + L.ey, p11 = L.p00 + L.ex + L.ey;
-    return triangle_solid_angle(L.p00, p10, p11, r) + triangle_solid_angle(L.p00, p11, p01, r);
 }
 struct Bilinear {
     float a, b, c, d; // weights at (0,0), (1,0), (0,1), (1,1)
 };
-static Bilinear quad_cosines(const Light &L, V3 r, V3 n) {
-    const V3 p10 = L.p00 + L.ex, p01 = L.p00 + L.ey, p11 = L.p00 + L.ex + L.ey;
-    return {std::max(0.01f, std::fabs(dot(unit(L.p00 - r), n))), std::max(0.01f, std::fabs(dot(unit(p10 - r), n))),
-            std::max(0.01f, std::fabs(dot(unit(p01 - r), n))), std::max(0.01f, std::fabs(dot(unit(p11 - r), n)))};
-}
 static Bilinear bilinear_normalized(Bilinear w) {
     const float m = std::max(std::max(w.a, w.b), std::max(w.c, w.d));
     return m > 0 ? Bilinear{w.a / m, w.b / m, w.c / m, w.d / m} : Bilinear{1, 1, 1, 1};
@@ -1300,24 +1299,25 @@ static float bilinear_density(float x, float y, Bilinear w) {
     return 4.0f * ((1.0f - x) * (1.0f - y) * w.a + x * (1.0f - y) * w.b + (1.0f - x) * y * w.c + x * y * w.d) /
            (w.a + w.b + w.c + w.d);
 }
+// The rectangle seen from r in its own frame (x along ex, y along ey, z
+// toward r negative): extents, the spherical-rectangle constants, the solid
+// angle, and the unit directions to the four corners.
 struct RectFrame {
-    V3 x, y, z;
-    float x0, x1, y0, y1, z0, b0, b1, g2, g3, area;
+    float x0, x1, y0, y1, z0, b0, b1, g2, g3, area, zs;
+    V3 w00, w10, w01, w11;
 };
 static RectFrame rect_frame(const Light &L, V3 r) {
     RectFrame f;
-    f.x = unit(L.ex);
-    f.y = unit(L.ey);
-    f.z = cross3(f.x, f.y);
     const V3 delta = L.p00 - r;
-    f.x0 = dot(delta, f.x);
-    f.x1 = f.x0 + length(L.ex);
-    f.y0 = dot(delta, f.y);
-    f.y1 = f.y0 + length(L.ey);
-    f.z0 = dot(delta, f.z);
+    f.x0 = dot(delta, L.ux);
+    f.x1 = f.x0 + L.lx;
+    f.y0 = dot(delta, L.uy);
+    f.y1 = f.y0 + L.ly;
+    f.z0 = dot(delta, L.uz);
+    f.zs = 1.0f;
     if (f.z0 > 0) {
-        f.z = -f.z;
         f.z0 = -f.z0;
+        f.zs = -1.0f;
     }
     const V3 v00 = {f.x0, f.y0, f.z0}, v10 = {f.x1, f.y0, f.z0}, v11 = {f.x1, f.y1, f.z0}, v01 = {f.x0, f.y1, f.z0};
     const V3 n0 = unit(cross3(v00, v10)), n2 = unit(cross3(v11, v01)), n3 = unit(cross3(v01, v00));
@@ -1325,8 +1325,18 @@ static RectFrame rect_frame(const Light &L, V3 r) {
     f.g3 = angle_between(-n3, n0);
     f.b0 = n0.z;
     f.b1 = n2.z;
-    f.area = quad_solid_angle(L, r);
+    f.w00 = unit(v00);
+    f.w10 = unit(v10);
+    f.w01 = unit(v01);
+    f.w11 = unit(v11);
+    f.area = spherical_triangle_area(f.w00, f.w10, f.w11) + spherical_triangle_area(f.w00, f.w11, f.w01);
     return f;
+}
+// Corner cosines |n . w| (floored at 0.01) of the receiver normal n.
+static Bilinear rect_cosines(const RectFrame &f, const Light &L, V3 n) {
+    const V3 nl = {dot(n, L.ux), dot(n, L.uy), f.zs * dot(n, L.uz)};
+    return {std::max(0.01f, std::fabs(dot(f.w00, nl))), std::max(0.01f, std::fabs(dot(f.w10, nl))),
+            std::max(0.01f, std::fabs(dot(f.w01, nl))), std::max(0.01f, std::fabs(dot(f.w11, nl)))};
 }
 // Square sample -> rectangle coordinates (u along ex, v along ey).
 static void rect_sample(const RectFrame &f, float s0, float s1, float &u, float &v) {
@@ -1357,23 +1367,27 @@ static void rect_inverse(const RectFrame &f, float u, float v, float &s0, float 
     const float hv = yv / std::sqrt(dd2 + yv * yv);
     s1 = std::clamp((hv - h0) / (h1 - h0), 0.0f, 1.0f);
 }
-// Solid-angle density of the quad proposal at rectangle coordinates (u, v)
-// (point q on the quad) seen from r with receiver normal n.
-static float quad_pdf(const Light &L, V3 r, V3 n, float u, float v, V3 q) {
+// Area density of a quad point q seen from r, as solid-angle density.
+static float quad_area_pdf(const Light &L, V3 r, V3 q) {
     const V3 delta = q - r;
     const float d2 = dot(delta, delta);
     if (!(d2 > 0)) return 0.0f;
-    const float omega = quad_solid_angle(L, r);
-    if (L.rect && omega > 1e-4f) {
-        const RectFrame f = rect_frame(L, r);
-        float s0, s1;
-        rect_inverse(f, u, v, s0, s1);
-        const bool has_n = (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) && !slip(SLIP_RECT_PDF_UNWARPED);
-        return (has_n ? bilinear_density(s0, s1, quad_cosines(L, r, n)) : 1.0f) / omega;
-    }
-    const V3 nl = unit(cross3(L.ex, L.ey));
-    const float c = std::fabs(dot(nl, delta / std::sqrt(d2)));
+    const float c = std::fabs(dot(L.uz, delta * (1.0f / std::sqrt(d2))));
     return c > 0 ? L.inv_area * d2 / c : 0.0f;
+}
+// Solid-angle density of the quad proposal at rectangle coordinates (u, v)
+// (point q on the quad) seen from r with receiver normal n.
+static float quad_pdf(const Light &L, V3 r, V3 n, float u, float v, V3 q) {
+    if (L.rect) {
+        const RectFrame f = rect_frame(L, r);
+        if (f.area > 1e-4f) {
+            float s0, s1;
+            rect_inverse(f, u, v, s0, s1);
+            const bool has_n = (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) && !slip(SLIP_RECT_PDF_UNWARPED);
+            return (has_n ? bilinear_density(s0, s1, rect_cosines(f, L, n)) : 1.0f) / f.area;
+        }
+    }
+    return quad_area_pdf(L, r, q);
 }
 static void quad_coords(const Light &L, V3 q, float &u, float &v) {
     const V3 d = q - L.p00;
@@ -1414,17 +1428,29 @@ static LightSample sample_light(const Scene &s, V3 p, V3 n, const Wavelengths &w
             pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
         } else {
             float su = std::min(u0, 0x1.fffffep-1f), sv = std::min(u1, 0x1.fffffep-1f), qu = su, qv = sv;
-            const float omega = quad_solid_angle(L, p);
-            if (L.rect && omega > 1e-4f) {
-                if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) bilinear_warp(su, sv, quad_cosines(L, p, n));
-                rect_sample(rect_frame(L, p), su, sv, qu, qv);
+            bool spherical = false;
+            if (L.rect) {
+                // spherical rectangle above 1e-4 sr: the density is the warp's
+                // bilinear density over the solid angle, at the warped sample
+                const RectFrame f = rect_frame(L, p);
+                if (f.area > 1e-4f) {
+                    float dens = 1.0f;
+                    if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) {
+                        const Bilinear w = rect_cosines(f, L, n);
+                        bilinear_warp(su, sv, w);
+                        if (!slip(SLIP_RECT_PDF_UNWARPED)) dens = bilinear_density(su, sv, w);
+                    }
+                    rect_sample(f, su, sv, qu, qv);
+                    pdf_omega = dens / f.area;
+                    spherical = true;
+                }
             }
             const V3 a = qu * L.ex, b = qv * L.ey;
             ls.p = L.p00 + a + b;
             ls.err = gamma_n(6) * (vabs(L.p00) + vabs(a) + vabs(b));
-            ls.n = unit(cross3(L.ex, L.ey));
+            ls.n = L.uz;
             tri = L.index;
-            pdf_omega = quad_pdf(L, p, n, qu, qv, ls.p);
+            if (!spherical) pdf_omega = quad_area_pdf(L, p, ls.p);
         }
         const V3 dv = ls.p - p;
         const float d2 = dot(dv, dv);

@@ -173,25 +173,31 @@ RP_DEV float next_down(float v) {
 }
 
 // ======================================================== scene on device
-// One light (64 B).  kind 0 triangle, 1 spot, 2 environment, 3 quad (a
+// One light (128 B).  kind 0 triangle, 1 spot, 2 environment, 3 quad (a
 // rectangle p00 + u ex + v ey made of two triangles); index: the leaf-order
 // position of the (first) triangle or the spot index; leaf: the light-BVH
 // node holding the light; pmf / threshold / alias: the power sampler's alias
-// table; rect: the quad passes the binary32 rectangle test.
+// table; rect: the quad passes the binary32 rectangle test; ux, uy, uz, lx,
+// ly: the quad's unit frame and edge lengths.
 struct LightEntry {
     uint32_t kind, index, leaf, rect;
     float pmf, threshold;
     uint32_t alias;
     float inv_area;
     float p00[3], ex[3], ey[3];
-    uint32_t pad;
+    float ux[3], uy[3], uz[3];
+    float lx, ly;
+    uint32_t pad[2];
 };
-// Light-BVH node (64 B): bounds, direction cone, emission cutoff, power
-// normalised by the tree's total; the first child of interior node i is
-// i + 1; flags bit 0 = two-sided.
+// Light-BVH node (64 B): the bounding sphere of its box (centre, squared
+// radius, radius), direction cone (axis, cosine and sine of its half-angle),
+// emission cutoff, power normalised by the
+// tree's total; the first child of interior node i is i + 1; flags bit 0 =
+// two-sided.
 struct LightNode {
-    float low[3], high[3], dir[3];
-    float cos_outer, cos_emission, power;
+    float center[3], r2, radius;
+    float dir[3];
+    float cos_outer, sin_outer, cos_emission, power;
     uint32_t child, parent, light, flags;
 };
 
@@ -1051,18 +1057,16 @@ struct LightSample {
 RP_DEV F3 node_vec(const float v[3]) { return F3{v[0], v[1], v[2]}; }
 RP_DEV bool nonzero(F3 n) { return n.x != 0.0f || n.y != 0.0f || n.z != 0.0f; }
 RP_DEV float light_importance(const LightNode &b, F3 p, F3 n) {
-    const F3 lo = node_vec(b.low), hi = node_vec(b.high);
-    const F3 c = (lo + hi) * 0.5f;
-    const F3 delta = p - c;
-    const float r2 = dot(hi - lo, hi - lo) * 0.25f, d2 = dot(delta, delta);
-    const float radius = __builtin_sqrtf(r2), dist = __builtin_sqrtf(d2);
+    const F3 delta = p - node_vec(b.center);
+    const float r2 = b.r2, d2 = dot(delta, delta);
+    const float radius = b.radius, dist = __builtin_sqrtf(d2);
     const float cb = d2 <= r2 ? -1.0f : __builtin_sqrtf(omax(0.0f, 1.0f - r2 / d2));
     const float sb = __builtin_sqrtf(omax(0.0f, 1.0f - cb * cb));
-    const F3 wi = dist > 0 ? delta / dist : F3{0, 0, 1};
+    const F3 wi = dist > 0 ? delta * (1.0f / dist) : F3{0, 0, 1};
     float cw = oclamp(dot(node_vec(b.dir), wi), -1.0f, 1.0f);
     if (b.flags & 1u) cw = fabsf(cw);
     const float sw = __builtin_sqrtf(omax(0.0f, 1.0f - cw * cw));
-    const float co = b.cos_outer, so = __builtin_sqrtf(omax(0.0f, 1.0f - co * co));
+    const float co = b.cos_outer, so = b.sin_outer;
     const float cx = cw > co ? 1.0f : cw * co + sw * so, sx = cw > co ? 0.0f : sw * co - cw * so;
     const float cp = cx > cb ? 1.0f : cx * cb + sx * sb;
     if (cp <= b.cos_emission) return 0.0f;
@@ -1152,15 +1156,7 @@ RP_DEV float light_pmf(const Scene &S, F3 p, F3 n, uint32_t id) {
 // shaders/light_geometry_sampling.h:118-209), warped by the receiver's corner
 // cosines (bilinear) when a normal is given; uniform over the area when the
 // rectangle subtends 1e-4 sr or less.
-struct Quad {
-    F3 p00, ex, ey;
-    float inv_area;
-    bool rect;
-};
-RP_DEV Quad quad_of(const LightEntry &L) {
-    return {node_vec(L.p00), node_vec(L.ex), node_vec(L.ey), L.inv_area, L.rect != 0};
-}
-RP_DEV F3 unit(F3 v) { return v / length(v); }
+RP_DEV F3 unit(F3 v) { return v * (1.0f / __builtin_sqrtf(dot(v, v))); }
 RP_DEV float angle_between(F3 a, F3 b) {
     return dot(a, b) < 0 ? kPi - 2.0f * asin_f(omin(1.0f, length(a + b) / 2.0f))
                          : 2.0f * asin_f(omin(1.0f, length(a - b) / 2.0f));
@@ -1168,23 +1164,9 @@ RP_DEV float angle_between(F3 a, F3 b) {
 RP_DEV float spherical_triangle_area(F3 a, F3 b, F3 c) {
     return fabsf(2.0f * atan2_f(dot(a, cross3(b, c)), 1.0f + dot(a, b) + dot(a, c) + dot(b, c)));
 }
-RP_DEV float triangle_solid_angle(F3 p0, F3 p1, F3 p2, F3 r) {
-    const F3 a = p0 - r, b = p1 - r, c = p2 - r;
-    if (!(dot(a, a) > 0) || !(dot(b, b) > 0) || !(dot(c, c) > 0)) return 0.0f;
-    return spherical_triangle_area(unit(a), unit(b), unit(c));
-}
-RP_DEV float quad_solid_angle(const Quad &Q, F3 r) {
-    const F3 p10 = Q.p00 + Q.ex, p01 = Q.p00 + Q.ey, p11 = Q.p00 + Q.ex + Q.ey;
-    return triangle_solid_angle(Q.p00, p10, p11, r) + triangle_solid_angle(Q.p00, p11, p01, r);
-}
 struct Bilinear {
     float a, b, c, d; // weights at (0,0), (1,0), (0,1), (1,1)
 };
-RP_DEV Bilinear quad_cosines(const Quad &Q, F3 r, F3 n) {
-    const F3 p10 = Q.p00 + Q.ex, p01 = Q.p00 + Q.ey, p11 = Q.p00 + Q.ex + Q.ey;
-    return {omax(0.01f, fabsf(dot(unit(Q.p00 - r), n))), omax(0.01f, fabsf(dot(unit(p10 - r), n))),
-            omax(0.01f, fabsf(dot(unit(p01 - r), n))), omax(0.01f, fabsf(dot(unit(p11 - r), n)))};
-}
 RP_DEV Bilinear bilinear_normalized(Bilinear w) {
     const float m = omax(omax(w.a, w.b), omax(w.c, w.d));
     return m > 0 ? Bilinear{w.a / m, w.b / m, w.c / m, w.d / m} : Bilinear{1, 1, 1, 1};
@@ -1206,28 +1188,42 @@ RP_DEV float bilinear_density(float x, float y, Bilinear w) {
     return 4.0f * ((1.0f - x) * (1.0f - y) * w.a + x * (1.0f - y) * w.b + (1.0f - x) * y * w.c + x * y * w.d) /
            (w.a + w.b + w.c + w.d);
 }
+// The rectangle seen from r in its own frame: extents, spherical-rectangle
+// constants, solid angle, unit directions to the four corners.
 struct RectFrame {
-    float x0, x1, y0, y1, z0, b0, b1, g2, g3, area;
+    float x0, x1, y0, y1, z0, b0, b1, g2, g3, area, zs;
+    F3 w00, w10, w01, w11;
 };
-RP_DEV RectFrame rect_frame(const Quad &Q, F3 r) {
+RP_DEV RectFrame rect_frame(const LightEntry &L, F3 r) {
     RectFrame f;
-    const F3 fx = unit(Q.ex), fy = unit(Q.ey);
-    F3 fz = cross3(fx, fy);
-    const F3 delta = Q.p00 - r;
-    f.x0 = dot(delta, fx);
-    f.x1 = f.x0 + length(Q.ex);
-    f.y0 = dot(delta, fy);
-    f.y1 = f.y0 + length(Q.ey);
-    f.z0 = dot(delta, fz);
-    if (f.z0 > 0) f.z0 = -f.z0;
+    const F3 delta = node_vec(L.p00) - r;
+    f.x0 = dot(delta, node_vec(L.ux));
+    f.x1 = f.x0 + L.lx;
+    f.y0 = dot(delta, node_vec(L.uy));
+    f.y1 = f.y0 + L.ly;
+    f.z0 = dot(delta, node_vec(L.uz));
+    f.zs = 1.0f;
+    if (f.z0 > 0) {
+        f.z0 = -f.z0;
+        f.zs = -1.0f;
+    }
     const F3 v00 = {f.x0, f.y0, f.z0}, v10 = {f.x1, f.y0, f.z0}, v11 = {f.x1, f.y1, f.z0}, v01 = {f.x0, f.y1, f.z0};
     const F3 n0 = unit(cross3(v00, v10)), n2 = unit(cross3(v11, v01)), n3 = unit(cross3(v01, v00));
     f.g2 = angle_between(-n2, n3);
     f.g3 = angle_between(-n3, n0);
     f.b0 = n0.z;
     f.b1 = n2.z;
-    f.area = quad_solid_angle(Q, r);
+    f.w00 = unit(v00);
+    f.w10 = unit(v10);
+    f.w01 = unit(v01);
+    f.w11 = unit(v11);
+    f.area = spherical_triangle_area(f.w00, f.w10, f.w11) + spherical_triangle_area(f.w00, f.w11, f.w01);
     return f;
+}
+RP_DEV Bilinear rect_cosines(const RectFrame &f, const LightEntry &L, F3 n) {
+    const F3 nl = {dot(n, node_vec(L.ux)), dot(n, node_vec(L.uy)), f.zs * dot(n, node_vec(L.uz))};
+    return {omax(0.01f, fabsf(dot(f.w00, nl))), omax(0.01f, fabsf(dot(f.w10, nl))), omax(0.01f, fabsf(dot(f.w01, nl))),
+            omax(0.01f, fabsf(dot(f.w11, nl)))};
 }
 RP_DEV void rect_sample(const RectFrame &f, float s0, float s1, float &u, float &v) {
     const float au = s0 * f.area - f.g2 - f.g3;
@@ -1257,27 +1253,31 @@ RP_DEV void rect_inverse(const RectFrame &f, float u, float v, float &s0, float 
     const float hv = yv / __builtin_sqrtf(dd2 + yv * yv);
     s1 = oclamp((hv - h0) / (h1 - h0), 0.0f, 1.0f);
 }
-// Solid-angle density of the quad proposal at rectangle coordinates (u, v),
-// point q, from r with receiver normal n.
-RP_DEV float quad_pdf(const Quad &Q, F3 r, F3 n, float u, float v, F3 q) {
+// Area density of quad point q seen from r, as solid-angle density.
+RP_DEV float quad_area_pdf(const LightEntry &L, F3 r, F3 q) {
     const F3 delta = q - r;
     const float d2 = dot(delta, delta);
     if (!(d2 > 0)) return 0.0f;
-    const float omega = quad_solid_angle(Q, r);
-    if (Q.rect && omega > 1e-4f) {
-        const RectFrame f = rect_frame(Q, r);
-        float s0, s1;
-        rect_inverse(f, u, v, s0, s1);
-        return (nonzero(n) ? bilinear_density(s0, s1, quad_cosines(Q, r, n)) : 1.0f) / omega;
-    }
-    const F3 nl = unit(cross3(Q.ex, Q.ey));
-    const float c = fabsf(dot(nl, delta / __builtin_sqrtf(d2)));
-    return c > 0 ? Q.inv_area * d2 / c : 0.0f;
+    const float c = fabsf(dot(node_vec(L.uz), delta * (1.0f / __builtin_sqrtf(d2))));
+    return c > 0 ? L.inv_area * d2 / c : 0.0f;
 }
-RP_DEV void quad_coords(const Quad &Q, F3 q, float &u, float &v) {
-    const F3 d = q - Q.p00;
-    u = oclamp(dot(d, Q.ex) / dot(Q.ex, Q.ex), 0.0f, 1.0f);
-    v = oclamp(dot(d, Q.ey) / dot(Q.ey, Q.ey), 0.0f, 1.0f);
+// Solid-angle density of the quad proposal at rectangle coordinates (u, v),
+// point q, from r with receiver normal n.
+RP_DEV float quad_pdf(const LightEntry &L, F3 r, F3 n, float u, float v, F3 q) {
+    if (L.rect) {
+        const RectFrame f = rect_frame(L, r);
+        if (f.area > 1e-4f) {
+            float s0, s1;
+            rect_inverse(f, u, v, s0, s1);
+            return (nonzero(n) ? bilinear_density(s0, s1, rect_cosines(f, L, n)) : 1.0f) / f.area;
+        }
+    }
+    return quad_area_pdf(L, r, q);
+}
+RP_DEV void quad_coords(const LightEntry &L, F3 q, float &u, float &v) {
+    const F3 d = q - node_vec(L.p00), ex = node_vec(L.ex), ey = node_vec(L.ey);
+    u = oclamp(dot(d, ex) / dot(ex, ex), 0.0f, 1.0f);
+    v = oclamp(dot(d, ey) / dot(ey, ey), 0.0f, 1.0f);
 }
 
 // A light sample from p (normal n, zero in media): selection by the scene's
@@ -1316,18 +1316,29 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, F3 n, const Wavelengths &w
             const float cl = fabsf(dot(ls.n, dv / __builtin_sqrtf(d2)));
             pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
         } else {
-            const Quad Q = quad_of(L);
             float su = omin(u0, 0x1.fffffep-1f), sv = omin(u1, 0x1.fffffep-1f), qu = su, qv = sv;
-            const float omega = quad_solid_angle(Q, p);
-            if (Q.rect && omega > 1e-4f) {
-                if (nonzero(n)) bilinear_warp(su, sv, quad_cosines(Q, p, n));
-                rect_sample(rect_frame(Q, p), su, sv, qu, qv);
+            bool spherical = false;
+            if (L.rect) {
+                // spherical rectangle above 1e-4 sr: the warp's bilinear
+                // density over the solid angle, at the warped sample
+                const RectFrame f = rect_frame(L, p);
+                if (f.area > 1e-4f) {
+                    float dens = 1.0f;
+                    if (nonzero(n)) {
+                        const Bilinear w = rect_cosines(f, L, n);
+                        bilinear_warp(su, sv, w);
+                        dens = bilinear_density(su, sv, w);
+                    }
+                    rect_sample(f, su, sv, qu, qv);
+                    pdf_omega = dens / f.area;
+                    spherical = true;
+                }
             }
-            const F3 a = qu * Q.ex, b = qv * Q.ey;
-            ls.p = Q.p00 + a + b;
-            ls.err = gamma_n(6) * (vabs(Q.p00) + vabs(a) + vabs(b));
-            ls.n = unit(cross3(Q.ex, Q.ey));
-            pdf_omega = quad_pdf(Q, p, n, qu, qv, ls.p);
+            const F3 a = qu * node_vec(L.ex), b = qv * node_vec(L.ey);
+            ls.p = node_vec(L.p00) + a + b;
+            ls.err = gamma_n(6) * (vabs(node_vec(L.p00)) + vabs(a) + vabs(b));
+            ls.n = node_vec(L.uz);
+            if (!spherical) pdf_omega = quad_area_pdf(L, p, ls.p);
         }
         const F3 dv = ls.p - p;
         const float d2 = dot(dv, dv);
@@ -1382,10 +1393,9 @@ RP_DEV float area_light_pdf(const Scene &S, uint32_t id, F3 p, F3 n, F3 q, F3 ng
     const float pmf = light_pmf(S, p, n, id);
     const LightEntry &L = S.lights[id];
     if (L.kind == 3) {
-        const Quad Q = quad_of(L);
         float u, v;
-        quad_coords(Q, q, u, v);
-        return pmf * quad_pdf(Q, p, n, u, v, q);
+        quad_coords(L, q, u, v);
+        return pmf * quad_pdf(L, p, n, u, v, q);
     }
     const F3 dv = q - p;
     const float d2 = dot(dv, dv);

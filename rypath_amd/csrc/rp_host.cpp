@@ -492,6 +492,18 @@ LightTables build_light_tables(const rp_scene_desc &d, float scene_radius, std::
                     L.ex[k] = pv[3 + k];
                     L.ey[k] = pv[6 + k];
                 }
+                // unit frame in binary32: u = e * (1 / |e|), z = u_x x u_y
+                L.lx = std::sqrt(fdot(ex, ex));
+                L.ly = std::sqrt(fdot(ey, ey));
+                const float rx = 1.0f / L.lx, ry = 1.0f / L.ly;
+                const F3h ux = {ex.x * rx, ex.y * rx, ex.z * rx}, uy = {ey.x * ry, ey.y * ry, ey.z * ry};
+                const float uz[3] = {ux.y * uy.z - ux.z * uy.y, ux.z * uy.x - ux.x * uy.z, ux.x * uy.y - ux.y * uy.x};
+                const float uxa[3] = {ux.x, ux.y, ux.z}, uya[3] = {uy.x, uy.y, uy.z};
+                for (int k = 0; k < 3; ++k) {
+                    L.ux[k] = uxa[k];
+                    L.uy[k] = uya[k];
+                    L.uz[k] = uz[k];
+                }
                 tri_light[t] = tri_light[t + 1] = uint32_t(out.lights.size());
                 out.lights.push_back(L);
                 in.push_back(b);

@@ -92,6 +92,8 @@ struct LightEntry {
     uint32_t alias;
     float inv_area;
     float p00[3], ex[3], ey[3];
+    float ux[3], uy[3], uz[3]; // quad: unit edge directions and their cross product
+    float lx, ly;              // quad: edge lengths
 };
 struct LightNodeHost {
     float low[3], high[3], dir[3];

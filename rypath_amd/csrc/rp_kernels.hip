@@ -331,18 +331,28 @@ void upload_scene(rp_scene *s) {
             e.p00[k] = h.p00[k];
             e.ex[k] = h.ex[k];
             e.ey[k] = h.ey[k];
+            e.ux[k] = h.ux[k];
+            e.uy[k] = h.uy[k];
+            e.uz[k] = h.uz[k];
         }
+        e.lx = h.lx;
+        e.ly = h.ly;
     }
     std::vector<rpd::LightNode> dn(lt.nodes.size());
     for (size_t i = 0; i < dn.size(); ++i) {
         const LightNodeHost &h = lt.nodes[i];
         rpd::LightNode &e = dn[i];
+        // bounding sphere of the box in binary32: centre (lo + hi) / 2,
+        // squared radius |hi - lo|^2 / 4 (the expressions the oracle evaluates)
+        const float dx = h.high[0] - h.low[0], dy = h.high[1] - h.low[1], dz = h.high[2] - h.low[2];
         for (int k = 0; k < 3; ++k) {
-            e.low[k] = h.low[k];
-            e.high[k] = h.high[k];
+            e.center[k] = (h.low[k] + h.high[k]) * 0.5f;
             e.dir[k] = h.dir[k];
         }
+        e.r2 = (dx * dx + dy * dy + dz * dz) * 0.25f;
+        e.radius = std::sqrt(e.r2);
         e.cos_outer = h.cos_outer;
+        e.sin_outer = std::sqrt(std::max(0.0f, 1.0f - h.cos_outer * h.cos_outer));
         e.cos_emission = h.cos_emission;
         e.power = h.power;
         e.child = h.child;
