@@ -60,7 +60,10 @@ typedef enum rp_spectrum_kind {
     RP_SPECTRUM_SIGMOID = 1,  /* PBRT RGB sigmoid s(x), x = fma(fma(c0,l,c1),l,c2), l in nm */
     RP_SPECTRUM_TABLE = 2,    /* 471 samples at `table` (offset in floats into spectra pool) */
     RP_SPECTRUM_CAUCHY = 3,   /* c0 + c1 / (l_um * l_um) */
-    RP_SPECTRUM_LINEAR = 4    /* linear from c0 at 360 nm to c1 at 830 nm */
+    RP_SPECTRUM_LINEAR = 4,   /* linear from c0 at 360 nm to c1 at 830 nm */
+    RP_SPECTRUM_KNOTS8 = 5    /* piecewise linear through 8 values at `table` (offset into the pool) at the
+                                 reference's knots 360, 420, 470, 520, 570, 620, 700, 830 nm
+                                 (SpectralMaterial::reflectance, src/render.hpp:14,108) */
 } rp_spectrum_kind;
 
 typedef struct rp_spectrum {
@@ -145,8 +148,24 @@ typedef struct rp_scene_desc {
     uint32_t max_depth;          /* PBRT maxdepth: scattering events */
     uint32_t hero_wavelengths;   /* 0 = uniform (1/470), 1 = visible (PBRT fit) */
     uint32_t light_sampler;      /* rp_light_sampler (RenderScene::light_sampler, src/render.hpp:269) */
-    uint32_t pad;
+    uint32_t transport;          /* rp_transport (RenderScene::spectral_transport, src/render.hpp:270) */
 } rp_scene_desc;
+
+/* Spectral transport (SpectralTransport, src/render.hpp:213).  HERO: four
+ * hero wavelengths with spectral balance MIS (the default).  SCALAR: one
+ * wavelength per path, uniform on 360-830 nm, the reference's scalar
+ * estimator (shaders/scalar_transport.h:446-899) for diffuse surfaces:
+ * light sampling with power-heuristic MIS (beta = 2), cosine-hemisphere
+ * continuation, survival 3/4 from the third scattering on; film weight
+ * 470 CIE(round lambda) L.  SCALAR_DEMO: the reference's demo model
+ * (docs/rendering.md:41-60): the environment radiance as the only light,
+ * sampled over the uniform sphere, no depth limit (emergency only), film
+ * weight L CIE(lambda) 470 / Y (linear CIE interpolation, Y = 106.857).
+ * Scalar scenes render with rp_sample_scalar and with rp_render_frame* in
+ * RP_MODE_MEGAKERNEL. */
+typedef enum rp_transport {
+    RP_TRANSPORT_HERO = 0, RP_TRANSPORT_SCALAR = 1, RP_TRANSPORT_SCALAR_DEMO = 2
+} rp_transport;
 
 /* LightSamplerKind (src/render.hpp:208): how next-event estimation picks a light.
  * BVH: PBRT's light bounding-volume hierarchy, importance by position and
@@ -249,6 +268,10 @@ typedef struct rp_frame_config {
 #define RP_FRAME_NO_ROULETTE 2u
 /* Wavefront: every bounce through the queue kernels (no latency tail). */
 #define RP_FRAME_NO_TAIL 4u
+/* Scalar transport only: the independent Philox4x32-10 sampler (the demo's
+ * and PBRT's "independent"; counter (pixel, sample low, sample high, block),
+ * key = the seed's two words) instead of Sobol. */
+#define RP_FRAME_INDEPENDENT_SAMPLER 8u
 
 typedef struct rp_frame_stats {
     double seconds;                 /* device time of the frame (events) */
@@ -295,10 +318,27 @@ int rp_scene_create(const rp_scene_desc *desc, const rp_build_options *options, 
 int rp_scene_info(const rp_scene *scene, rp_build_info *info);
 void rp_scene_destroy(rp_scene *scene);
 
+/* RenderSample (src/render.hpp:292-301): one scalar path. */
+typedef struct rp_render_sample {
+    uint32_t pixel;
+    uint32_t status;             /* rp_path_status */
+    uint64_t sample;
+    float wavelength;            /* nm */
+    float radiance;              /* L(lambda) along the path */
+    uint32_t scatterings;
+    uint32_t random_words[4];    /* the camera / wavelength words (Philox block 0 or Sobol dims 0-3) */
+    float xyz[3];                /* the film weight of the path (diagnostic) */
+} rp_render_sample;
+
 /* Per-path records for request->pixel_count * request->sample_count paths,
  * lane = pixel_offset + pixel_count * sample_offset (src/render.metal:240-242).
  * `out` is host memory with room for that many records. */
 int rp_sample(rp_scene *scene, const rp_request *request, rp_path_record *out, uint64_t capacity);
+/* Scalar-transport scenes: RenderSample records (MetalRenderScene::sample's
+ * `samples`, src/metal_render.hpp:35), lane = pixel_offset + pixel_count *
+ * sample_offset.  request->sampler_kind is RP_SAMPLER_SOBOL or
+ * RP_SAMPLER_INDEPENDENT.  Hero scenes return RP_INVALID (use rp_sample). */
+int rp_sample_scalar(rp_scene *scene, const rp_request *request, rp_render_sample *out, uint64_t capacity);
 
 /* One frame of W*H*spp paths.  Film: per-pixel mean XYZ (float32, W*H*3) of the
  * samples [sample_begin, sample_begin+spp), reduced in increasing sample order

@@ -23,7 +23,8 @@ typedef enum rp_synthetic_kind {
     RP_SYNTH_TERRAIN = 1,     /* fBm heightfield + icospheres, 40/30/30 materials, 16 quads + env */
     RP_SYNTH_BATCH = 2,       /* terrain-like scene sized to target_triangles */
     RP_SYNTH_VOLUME = 3,      /* displaced icosphere, dielectric 1.33 boundary, HG medium, spot + env */
-    RP_SYNTH_INVERSE = 4      /* Cornell box, conductor boxes, fog box (null interface) */
+    RP_SYNTH_INVERSE = 4,     /* Cornell box, conductor boxes, fog box (null interface) */
+    RP_SYNTH_DEMO = 5         /* reference demo: floor, two walls, three boxes, unit environment, scalar demo model */
 } rp_synthetic_kind;
 
 typedef struct rp_synthetic_options {

@@ -1985,6 +1985,228 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
     return out;
 }
 
+// ============================================================ scalar transport
+// One wavelength per path: the reference's scalar estimator for diffuse
+// surfaces (shaders/scalar_transport.h:446-899; docs/rendering.md:41-60),
+// restated.  Draw blocks (render.metal:154-177 draw_words): block 0 camera
+// x/y and wavelength, scattering b uses block 1 + 2b (continuation x/y,
+// roulette word z) and block 2 + 2b (light selection and position).  The
+// independent sampler is Philox4x32-10 with counter (pixel, sample low,
+// sample high, block) and key (seed low, seed high); Sobol blocks are the
+// four dimensions 4 block .. 4 block + 3.
+struct ScalarOut {
+    uint32_t status = RP_PATH_SUCCESS;
+    float wavelength = 0, radiance = 0;
+    uint32_t scatterings = 0;
+    uint32_t words[4] = {0, 0, 0, 0};
+    uint32_t primary = RP_NONE;
+    uint64_t ext_rays = 0, shadow_rays = 0;
+};
+// Power heuristic a^2 / (a^2 + b^2) on values scaled by their maximum
+// (render.metal:1901-1909).
+static float power_heuristic(float a, float b) {
+    const float scale = std::max(a, b);
+    if (!(scale > 0) || !std::isfinite(scale)) return NAN;
+    a = a / scale;
+    b = b / scale;
+    return a * a / (a * a + b * b);
+}
+// Uniform sphere and cosine hemisphere about n (render.metal:821-836).
+static V3 sphere_dir(float u, float v) {
+    const float z = 1.0f - 2.0f * u;
+    const float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+    float sn, cs;
+    sincos_turns(v, sn, cs);
+    return {r * cs, r * sn, z};
+}
+static V3 hemisphere_dir(V3 n, float u, float v) {
+    const float r = std::sqrt(u);
+    float sn, cs;
+    sincos_turns(v, sn, cs);
+    const V3 axis = std::fabs(n.z) < 0.999f ? V3{0, 0, 1} : V3{0, 1, 0};
+    const V3 t = normalize(cross3(axis, n));
+    const V3 b = cross3(n, t);
+    return normalize((r * cs) * t + (r * sn) * b + std::sqrt(1.0f - u) * n);
+}
+// Demo MIS weights with p_light = 1/(4 pi), p_bsdf = cos/pi (render.metal:838-848).
+static float demo_light_weight(float c) {
+    const float q = 4.0f * c;
+    return 1.0f / (1.0f + q * q);
+}
+static float demo_bsdf_weight(float c) {
+    const float q = 4.0f * c;
+    const float q2 = q * q;
+    return q2 / (1.0f + q2);
+}
+// Light-sampling factor f cos / p_light times the power-heuristic weight
+// (none for delta lights) (render.metal:2212-2219).
+static float scattering_direct(float f_cos, float bsdf_pdf, float light_pdf, bool delta) {
+    if (!(f_cos >= 0) || !(bsdf_pdf >= 0) || !(light_pdf > 0) || !std::isfinite(f_cos) || !std::isfinite(bsdf_pdf) ||
+        !std::isfinite(light_pdf))
+        return NAN;
+    if (f_cos == 0) return 0;
+    return (f_cos / light_pdf) * (delta ? 1.0f : power_heuristic(light_pdf, bsdf_pdf));
+}
+// Demo film weight: CIE linearly interpolated at lambda times 470 / Y
+// (src/render.cpp:2131-2145; Y = the trapezoid integral of y-bar).
+static void demo_sensor(float lambda, float L, float xyz[3]) {
+    const auto &cie = cie_table();
+    const float off = lambda - 360.0f;
+    const int lo = std::min(std::max(int(off), 0), 470);
+    const int hi = std::min(lo + 1, 470);
+    const float f = off - float(lo);
+    const float scale = 470.0f / 106.856914916766900f;
+    for (int c = 0; c < 3; ++c) xyz[c] = ((cie.xyz[lo][c] + (cie.xyz[hi][c] - cie.xyz[lo][c]) * f) * scale) * L;
+}
+
+static ScalarOut trace_scalar(const Ctx &c, uint32_t pixel, uint64_t sample, bool independent) {
+    const Scene &s = *c.s;
+    ScalarOut out;
+    const bool demo = s.d.transport == RP_TRANSPORT_SCALAR_DEMO;
+    const uint32_t W = s.d.camera.width, H = s.d.camera.height;
+    const uint32_t px = pixel % W, py = pixel / W;
+    const uint32_t seed_lo = uint32_t(c.seed), seed_hi = uint32_t(c.seed >> 32);
+    uint64_t index = 0;
+    auto block = [&](uint32_t b) -> std::array<uint32_t, 4> {
+        if (independent) return philox({pixel, uint32_t(sample), uint32_t(sample >> 32), b}, {seed_lo, seed_hi});
+        return draws(c, index, b);
+    };
+    auto su = [&](uint32_t w) { return independent ? philox_u01(w) : sobol_uniform(w); };
+    std::array<uint32_t, 4> w0;
+    if (independent) w0 = block(0);
+    else {
+        index = sobol_index(s.m, px, py, sample);
+        w0 = {sobol_word(index, 0, RP_RANDOMIZE_NONE, 0), sobol_word(index, 1, RP_RANDOMIZE_NONE, 0),
+              sobol_word(index, 2, c.mode, seed_lo), sobol_word(index, 3, c.mode, seed_lo)};
+    }
+    for (int k = 0; k < 4; ++k) out.words[k] = w0[k];
+    const float lambda = 360.0f + 470.0f * su(w0[2]);
+    out.wavelength = lambda;
+    Wavelengths wl;
+    for (int k = 0; k < 4; ++k) wl.lambda[k] = lambda, wl.pdf[k] = 1.0f / 470.0f;
+    const float ux = independent ? su(w0[0]) : sobol_camera_uniform(w0[0], s.m, px);
+    const float uy = independent ? su(w0[1]) : sobol_camera_uniform(w0[1], s.m, py);
+    const float fx = (float(px) + ux) / float(W), fy = (float(py) + uy) / float(H);
+    const rp_camera &cam = s.d.camera;
+    V3 d = {cam.corner[0] + fx * cam.dx[0] + fy * cam.dy[0], cam.corner[1] + fx * cam.dx[1] + fy * cam.dy[1],
+            cam.corner[2] + fx * cam.dx[2] + fy * cam.dy[2]};
+    const float scale = std::max(std::fabs(d.x), std::max(std::fabs(d.y), std::fabs(d.z)));
+    d = normalize(d / scale);
+    V3 o = {cam.origin[0], cam.origin[1], cam.origin[2]};
+    const float E = s.d.has_environment ? spectrum_eval(s.d.environment, lambda, s.spectra.data()) : 0.0f;
+    float throughput = 1.0f, escape_weight = 1.0f, prev_pdf = 0.0f;
+    V3 prev_p = o, prev_n = {0, 0, 0};
+    uint32_t crossings = 0;
+    float L = 0.0f;
+    for (;;) {
+        Hit h = closest(s, o, d, INFINITY);
+        ++out.ext_rays;
+        if (out.scatterings == 0 && crossings == 0 && out.primary == RP_NONE) out.primary = h.tri;
+        if (h.tri == RP_NONE) {
+            if (!demo && out.scatterings != 0)
+                escape_weight = power_heuristic(prev_pdf, s.env_light == RP_NONE ? 0.0f
+                                                          : light_pmf(s, prev_p, prev_n, s.env_light) * kInv4Pi);
+            L = L + throughput * escape_weight * E;
+            break;
+        }
+        const uint32_t bounce = out.scatterings;
+        const SurfacePoint sp = surface_point(s, h);
+        const rp_object &obj = s.objects[s.tri_object[h.tri]];
+        const rp_material &mat = s.materials[obj.material];
+        if (mat.kind == RP_MATERIAL_INTERFACE) {
+            if (crossings++ >= c.boundary) { out.status = RP_PATH_BOUNDARY_EXHAUSTION; break; }
+            o = offset_origin(sp.p, sp.err, sp.n, d);
+            continue;
+        }
+        if (obj.emission != RP_NONE && s.tri_light[h.tri] != RP_NONE) {
+            const auto &em = s.emissions[obj.emission];
+            const float Le = spectrum_eval(em.radiance, lambda, s.spectra.data());
+            if (Le > 0.0f && (em.two_sided || dot(sp.n, -d) >= 0.0f)) {
+                float weight = 1.0f;
+                if (bounce != 0)
+                    weight = power_heuristic(prev_pdf, area_light_pdf(s, s.tri_light[h.tri], prev_p, prev_n, sp.p, sp.n));
+                if (!std::isfinite(weight)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                L = L + throughput * (weight * Le);
+            }
+        }
+        if (!demo && bounce >= s.d.max_depth) break;
+        if (mat.kind != RP_MATERIAL_DIFFUSE) { out.status = RP_PATH_INVALID_HIT; break; }
+        const float rho = spectrum_eval(mat.reflectance, lambda, s.spectra.data());
+        if (!std::isfinite(rho) || !(rho >= 0.0f && rho <= 1.0f)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+        if (rho == 0.0f) break;
+        if (bounce >= c.emergency) { out.status = RP_PATH_EMERGENCY_DEPTH; break; }
+        const V3 n = dot(sp.n, d) > 0.0f ? -sp.n : sp.n;
+        const auto lw = block(2 + 2 * bounce), bw = block(1 + 2 * bounce);
+        if (demo) {
+            const V3 ldir = sphere_dir(su(lw[0]), su(lw[1]));
+            const float lc = dot(n, ldir);
+            if (E > 0.0f && lc > 0.0f) {
+                float tr[4];
+                uint32_t cr = 0;
+                ++out.shadow_rays;
+                if (transmittance(c, offset_origin(sp.p, sp.err, sp.n, ldir), ldir, INFINITY, RP_NONE, wl, tr, cr))
+                    L = L + throughput * rho * ((4.0f * lc) * demo_light_weight(lc)) * E * tr[0];
+            }
+        } else {
+            const LightSample ls = sample_light(s, sp.p, sp.n, wl, su(lw[0]), su(lw[1]), su(lw[2]));
+            if (ls.ok && ls.pdf > 0 && ls.le[0] > 0) {
+                const float cosv = dot(n, ls.wi);
+                if (cosv > 0) {
+                    const float f_cos = rho * cosv / kPi, bsdf_pdf = cosv / kPi;
+                    const float factor = scattering_direct(f_cos, bsdf_pdf, ls.pdf, ls.delta);
+                    if (!std::isfinite(factor)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+                    const V3 from = offset_origin(sp.p, sp.err, sp.n, ls.wi);
+                    V3 sd;
+                    float tm;
+                    if (ls.infinite) { sd = ls.wi; tm = INFINITY; }
+                    else {
+                        const V3 to = ls.delta ? ls.p : offset_origin(ls.p, ls.err, ls.n, from - ls.p);
+                        sd = to - from;
+                        tm = 1.0f - 1e-4f;
+                    }
+                    float tr[4];
+                    uint32_t cr = 0;
+                    ++out.shadow_rays;
+                    if (transmittance(c, from, sd, tm, RP_NONE, wl, tr, cr))
+                        L = L + throughput * factor * ls.le[0] * tr[0];
+                }
+            }
+        }
+        const V3 wi = hemisphere_dir(n, su(bw[0]), su(bw[1]));
+        throughput = throughput * rho;
+        if (!std::isfinite(throughput) || throughput < 0.0f) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+        ++out.scatterings;
+        if (out.scatterings >= 3) {
+            if (bw[2] >= 0xc0000000u) break;
+            throughput = slip(SLIP_ROULETTE_UNCOMPENSATED) ? throughput : throughput / 0.75f;
+        }
+        const float cosine = dot(n, wi);
+        if (!(cosine > 0.0f)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
+        prev_pdf = cosine / kPi;
+        escape_weight = demo ? (slip(SLIP_EMISSION_UNWEIGHTED) ? 1.0f : demo_bsdf_weight(cosine))
+                             : power_heuristic(prev_pdf, 1.0f / (4.0f * kPi));
+        prev_p = sp.p;
+        prev_n = sp.n;
+        o = offset_origin(sp.p, sp.err, sp.n, wi);
+        d = wi;
+        crossings = 0;
+    }
+    out.radiance = L;
+    return out;
+}
+static void scalar_xyz(const Scene &s, const ScalarOut &p, float xyz[3]) {
+    xyz[0] = xyz[1] = xyz[2] = 0.0f;
+    if (p.status != RP_PATH_SUCCESS) return;
+    if (s.d.transport == RP_TRANSPORT_SCALAR_DEMO) {
+        demo_sensor(p.wavelength, p.radiance, xyz);
+        return;
+    }
+    const auto &cie = cie_table();
+    int i = int(std::floor(p.wavelength + 0.5f)) - 360;
+    i = std::min(std::max(i, 0), 470);
+    for (int ch = 0; ch < 3; ++ch) xyz[ch] = cie.xyz[i][ch] * (470.0f * p.radiance);
+}
+
 // Hero sensor: sum_k cie(lambda_k) * (L_k / pdf_k / 4)  (shaders/frame_sensor.h:32-53).
 static void sensor_xyz(const PathOut &p, float xyz[3]) {
     xyz[0] = xyz[1] = xyz[2] = 0.0f;
@@ -2236,9 +2458,11 @@ int or_sample(void *sp, const rp_request *rq, rp_path_record *out, uint32_t thre
 
 // Full frame: per-pixel mean XYZ of samples [sample_begin, sample_begin+spp),
 // Neumaier-compensated double sums in increasing sample order (src/render.cpp:2326-2407).
-int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
-                    float *xyz, uint32_t threads, uint64_t *rays) {
+int or_render_frame_flags(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                          float *xyz, uint32_t threads, uint64_t *rays, uint32_t flags) {
     auto *s = static_cast<Scene *>(sp);
+    const bool scalar = s->d.transport != RP_TRANSPORT_HERO;
+    const bool independent = (flags & RP_FRAME_INDEPENDENT_SAMPLER) != 0;
     Ctx c{s, seed, randomization, 1024u, 4096u};
     uint32_t npx = s->d.camera.width * s->d.camera.height;
     if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
@@ -2253,10 +2477,16 @@ int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sa
             for (uint32_t pixel = b; pixel < e; ++pixel) {
                 double sum[3] = {0, 0, 0}, corr[3] = {0, 0, 0};
                 for (uint32_t k = 0; k < spp; ++k) {
-                    PathOut p = trace_path(c, pixel, sample_begin + k);
-                    local_rays += p.ext_rays + p.shadow_rays;
                     float v[3] = {0, 0, 0};
-                    if (p.status == RP_PATH_SUCCESS) sensor_xyz(p, v);
+                    if (scalar) {
+                        const ScalarOut p = trace_scalar(c, pixel, sample_begin + k, independent);
+                        local_rays += p.ext_rays + p.shadow_rays;
+                        scalar_xyz(*s, p, v);
+                    } else {
+                        PathOut p = trace_path(c, pixel, sample_begin + k);
+                        local_rays += p.ext_rays + p.shadow_rays;
+                        if (p.status == RP_PATH_SUCCESS) sensor_xyz(p, v);
+                    }
                     for (int ch = 0; ch < 3; ++ch) {
                         double x = double(v[ch]);
                         double t = sum[ch] + x;
@@ -2275,6 +2505,49 @@ int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sa
     work();
     for (auto &t : pool) t.join();
     if (rays) *rays = ray_count.load();
+    return 0;
+}
+
+int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                    float *xyz, uint32_t threads, uint64_t *rays) {
+    return or_render_frame_flags(sp, seed, randomization, sample_begin, spp, xyz, threads, rays, 0);
+}
+
+// Scalar-transport records (rp_render_sample) for request rq; the sampler
+// is rq->sampler_kind (Sobol or independent).
+int or_sample_scalar(void *sp, const rp_request *rq, rp_render_sample *out, uint32_t threads) {
+    auto *s = static_cast<Scene *>(sp);
+    Ctx c{s, rq->seed, rq->randomization, rq->emergency_limit ? rq->emergency_limit : 1024u,
+          rq->boundary_limit ? rq->boundary_limit : 4096u};
+    const bool independent = rq->sampler_kind == RP_SAMPLER_INDEPENDENT;
+    const uint64_t n = uint64_t(rq->pixel_count) * rq->sample_count;
+    if (threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::atomic<uint64_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            const uint64_t b = next.fetch_add(256);
+            if (b >= n) break;
+            const uint64_t e = std::min(n, b + 256);
+            for (uint64_t lane = b; lane < e; ++lane) {
+                const uint32_t pixel = rq->pixel_begin + uint32_t(lane % rq->pixel_count);
+                const uint64_t sample = rq->sample_begin + lane / rq->pixel_count;
+                const ScalarOut p = trace_scalar(c, pixel, sample, independent);
+                rp_render_sample &r = out[lane];
+                r.pixel = pixel;
+                r.status = p.status;
+                r.sample = sample;
+                r.wavelength = p.wavelength;
+                r.radiance = p.radiance;
+                r.scatterings = p.scatterings;
+                for (int k = 0; k < 4; ++k) r.random_words[k] = p.words[k];
+                scalar_xyz(*s, p, r.xyz);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
     return 0;
 }
 

@@ -60,11 +60,13 @@ def _teardown():
 
 NONE = 0xFFFFFFFF
 SUCCESS, NOT_BUILT, NO_DEVICE, FAILURE, INVALID, BUSY = range(6)
-SPECTRUM_CONSTANT, SPECTRUM_SIGMOID, SPECTRUM_TABLE, SPECTRUM_CAUCHY, SPECTRUM_LINEAR = range(5)
+SPECTRUM_CONSTANT, SPECTRUM_SIGMOID, SPECTRUM_TABLE, SPECTRUM_CAUCHY, SPECTRUM_LINEAR, SPECTRUM_KNOTS8 = range(6)
+TRANSPORT_HERO, TRANSPORT_SCALAR, TRANSPORT_SCALAR_DEMO = range(3)
+FRAME_INDEPENDENT_SAMPLER = 8
 MATERIAL_DIFFUSE, MATERIAL_CONDUCTOR, MATERIAL_DIELECTRIC, MATERIAL_INTERFACE = range(4)
 RANDOMIZE_NONE, RANDOMIZE_PERMUTE, RANDOMIZE_FASTOWEN, RANDOMIZE_OWEN = range(4)
 MODE_MEGAKERNEL, MODE_WAVEFRONT, MODE_WAVEFRONT_SORTED = range(3)
-SYNTH_CORNELL, SYNTH_TERRAIN, SYNTH_BATCH, SYNTH_VOLUME, SYNTH_INVERSE = range(5)
+SYNTH_CORNELL, SYNTH_TERRAIN, SYNTH_BATCH, SYNTH_VOLUME, SYNTH_INVERSE, SYNTH_DEMO = range(6)
 PARAM_REFLECTANCE_BIN, PARAM_ROUGHNESS, PARAM_SIGMA_S_BIN = range(3)
 SCHED_PERSISTENT, SCHED_WAVEFRONT = range(2)
 SAMPLER_INDEPENDENT, SAMPLER_SOBOL, SAMPLER_HALTON, SAMPLER_ZSOBOL = range(4)
@@ -112,7 +114,7 @@ class SceneDesc(C.Structure):
                 ("media", C.POINTER(Medium)), ("spots", C.POINTER(SpotLight)), ("spectra", C.POINTER(f32)),
                 ("spectra_count", u32), ("has_environment", u32), ("environment", Spectrum),
                 ("camera", Camera), ("max_depth", u32), ("hero_wavelengths", u32), ("light_sampler", u32),
-                ("pad", u32)]
+                ("transport", u32)]
 
 
 class BuildOptions(C.Structure):
@@ -184,6 +186,11 @@ RECORD_DTYPE = np.dtype([
     ("splat_begin", "<u4"), ("splat_count", "<u4"),
 ])
 assert RECORD_DTYPE.itemsize == 112
+RENDER_SAMPLE_DTYPE = np.dtype([
+    ("pixel", "<u4"), ("status", "<u4"), ("sample", "<u8"), ("wavelength", "<f4"), ("radiance", "<f4"),
+    ("scatterings", "<u4"), ("random_words", "<u4", 4), ("xyz", "<f4", 3),
+])
+assert RENDER_SAMPLE_DTYPE.itemsize == 56
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmax", "<f4"), ("direction", "<f4", 3), ("pad", "<u4")])
 HIT_DTYPE = np.dtype([("triangle", "<u4"), ("t", "<f4"), ("b1", "<f4"), ("b2", "<f4")])
 
@@ -204,6 +211,7 @@ _sig("rp_scene_destroy", None, [C.c_void_p])
 _sig("rp_sample", C.c_int, [C.c_void_p, C.POINTER(Request), C.c_void_p, u64])
 _sig("rp_render_frame", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
 _sig("rp_render_frame_device", C.c_int, [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
+_sig("rp_sample_scalar", C.c_int, [C.c_void_p, C.POINTER(Request), C.c_void_p, u64])
 _sig("rp_render_frame_device_noalloc", C.c_int,
      [C.c_void_p, C.POINTER(FrameConfig), C.c_void_p, C.POINTER(FrameStats)])
 _sig("rp_trace_closest", C.c_int, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.POINTER(f64)])
@@ -258,7 +266,7 @@ _sig("rp_frame_release", None, [C.c_void_p])
 
 EXPORTED = [
     "rp_api_version", "rp_device_count", "rp_last_error", "rp_scene_create", "rp_scene_info", "rp_scene_destroy",
-    "rp_sample", "rp_render_frame", "rp_render_frame_device", "rp_render_frame_device_noalloc", "rp_trace_closest", "rp_trace_any",
+    "rp_sample", "rp_sample_scalar", "rp_render_frame", "rp_render_frame_device", "rp_render_frame_device_noalloc", "rp_trace_closest", "rp_trace_any",
     "rp_probe_read_bandwidth",
     "rp_batch_create", "rp_batch_render", "rp_batch_destroy", "rp_autotune", "rp_set_params", "rp_backward",
     "rp_backward_device",
@@ -401,6 +409,13 @@ class Scene:
         n = req.pixel_count * req.sample_count
         out = np.zeros(n, dtype=RECORD_DTYPE)
         _check("rp_sample", _lib.rp_sample(self._h, C.byref(req), out.ctypes.data_as(C.c_void_p), n))
+        return out
+
+    def sample_scalar(self, req):
+        """RenderSample records of a scalar-transport scene (rp_sample_scalar)."""
+        n = req.pixel_count * req.sample_count
+        out = np.zeros(n, dtype=RENDER_SAMPLE_DTYPE)
+        _check("rp_sample_scalar", _lib.rp_sample_scalar(self._h, C.byref(req), out.ctypes.data_as(C.c_void_p), n))
         return out
 
     def render_frame(self, cfg):

@@ -1,6 +1,7 @@
 // `rypath` command line (counterpart of src/main.cpp:429-530 --render-demo,
 // :532-800 --render-pbrt, :885-899 --check-pbrt): renders a PBRT-v4 scene file
-// or the Cornell demo on the GPU and publishes a linear-sRGB OpenEXR image.
+// or the reference's demo scene (scalar demo model, Philox sampler) on the
+// GPU and publishes a linear-sRGB OpenEXR image.
 // Exit status: 0 success, 1 failure (bad arguments, invalid scene, render or
 // write error), 2 unavailable (no HIP device), as src/main.cpp:215-230.
 #include "../../include/rp_api.h"
@@ -57,8 +58,8 @@ void usage() {
 
 struct Options {
     std::string scene, output, checkpoint, resume;
-    uint64_t samples = 0, seed = 0, width = 256, height = 256, batch = 0;
-    bool have_seed = false, half = false, timings = false;
+    uint64_t samples = 0, seed = 0, width = 256, height = 192, batch = 0;
+    bool have_seed = false, have_mode = false, half = false, timings = false;
     double exposure = 1, interval = 60;
     uint32_t mode = RP_MODE_WAVEFRONT; // the fastest mode on MI355X; films are identical in every mode
     int hero = -1;
@@ -109,6 +110,7 @@ int parse_options(int argc, char **argv, int first, Options &o) {
             else if (m == "wavefront") o.mode = RP_MODE_WAVEFRONT;
             else if (m == "sorted") o.mode = RP_MODE_WAVEFRONT_SORTED;
             else return fail("--mode requires megakernel, wavefront or sorted");
+            o.have_mode = true;
         } else if (k == "--hero-wavelengths") {
             std::string m = v;
             if (m == "uniform") o.hero = 0;
@@ -127,7 +129,7 @@ bool ends_with(const std::string &s, const char *suffix) {
 }
 
 int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t seed, uint32_t randomization,
-           const std::string &source) {
+           const std::string &source, uint32_t frame_flags = 0) {
     auto t0 = std::chrono::steady_clock::now();
     // The film accumulates batches of samples; a resumed or checkpointed
     // render equals the uninterrupted one bit for bit.  A resumed film must
@@ -181,6 +183,7 @@ int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t s
         cfg.sample_begin = fi.samples;
         cfg.seed = seed;
         cfg.randomization = randomization;
+        cfg.flags = frame_flags;
         rp_frame_stats stats{};
         st = rp_film_accumulate(scene, film, &cfg, nullptr, &stats);
         if (st != RP_SUCCESS) {
@@ -267,21 +270,27 @@ int render_pbrt(int argc, char **argv, bool check_only) {
     return rc;
 }
 
+// The reference's demo (src/main.cpp:429-530): demo-v1 scene, scalar demo
+// model, Philox4x32-10 independent sampler, 256 x 192, 128 samples, seed 1.
 int render_demo(int argc, char **argv) {
     Options o;
     if (int e = parse_options(argc, argv, 2, o)) return e;
     if (o.output.empty()) return fail("--render-demo requires --output PATH");
     if (!ends_with(o.output, ".exr")) return fail("--render-demo output must have the .exr extension");
+    if (o.hero >= 0) return fail("--hero-wavelengths requires effective hero transport");
+    if (o.have_mode && o.mode != RP_MODE_MEGAKERNEL) return fail("the demo's scalar transport renders as a megakernel");
+    o.mode = RP_MODE_MEGAKERNEL;
     rp_synthetic_options so{};
     so.width = uint32_t(o.width);
     so.height = uint32_t(o.height);
     rp_synthetic *syn = nullptr;
-    int st = rp_synthetic_create(RP_SYNTH_CORNELL, o.have_seed ? o.seed : 1, &so, &syn);
+    int st = rp_synthetic_create(RP_SYNTH_DEMO, 0, &so, &syn);
     if (st != RP_SUCCESS) return status_exit(st, "demo scene");
     rp_scene_desc desc = *rp_synthetic_desc(syn);
-    if (o.hero >= 0) desc.hero_wavelengths = uint32_t(o.hero);
-    int rc = render(desc, o, o.samples ? uint32_t(o.samples) : 64, o.have_seed ? o.seed : 1, RP_RANDOMIZE_FASTOWEN,
-                    "cornell-demo");
+    std::printf("Scene: demo-v1\nSampler: philox4x32-10-v1\nResolution: %ux%u\n", desc.camera.width,
+                desc.camera.height);
+    int rc = render(desc, o, o.samples ? uint32_t(o.samples) : 128, o.have_seed ? o.seed : 1, RP_RANDOMIZE_NONE,
+                    "demo-v1", RP_FRAME_INDEPENDENT_SAMPLER);
     rp_synthetic_destroy(syn);
     return rc;
 }

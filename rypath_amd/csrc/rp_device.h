@@ -225,6 +225,7 @@ struct Scene {
     rp_spectrum environment;
     uint32_t max_depth, hero_mode, m;
     uint32_t has_interfaces, has_media;
+    uint32_t transport; // rp_transport
     float env_pdf;
 };
 
@@ -396,6 +397,20 @@ RP_DEV float spectrum_eval(const rp_spectrum &s, float l, const float *pool) {
     case RP_SPECTRUM_LINEAR: {
         float t = (l - 360.0f) / 470.0f;
         v = (1.0f - t) * s.c[0] + t * s.c[1];
+        break;
+    }
+    case RP_SPECTRUM_KNOTS8: {
+        // piecewise linear through the 8 pool values at 360, 420, 470, 520,
+        // 570, 620, 700 and 830 nm (x + (y - x) t between neighbours)
+        const float *r = pool + s.table;
+        const float knot[8] = {360.0f, 420.0f, 470.0f, 520.0f, 570.0f, 620.0f, 700.0f, 830.0f};
+        v = r[7];
+        for (int i = 1; i < 8; ++i)
+            if (l <= knot[i]) {
+                const float t = (l - knot[i - 1]) / (knot[i] - knot[i - 1]);
+                v = r[i - 1] + (r[i] - r[i - 1]) * t;
+                break;
+            }
         break;
     }
     default: v = __builtin_nanf("");

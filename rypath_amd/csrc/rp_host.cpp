@@ -223,6 +223,20 @@ float spectrum_eval_host(const rp_spectrum &s, float l, const float *pool) {
         v = (1.0f - t) * s.c[0] + t * s.c[1];
         break;
     }
+    case RP_SPECTRUM_KNOTS8: {
+        // piecewise linear through the 8 pool values at 360, 420, 470, 520,
+        // 570, 620, 700 and 830 nm (x + (y - x) t between neighbours)
+        const float *r = pool + s.table;
+        const float knot[8] = {360.0f, 420.0f, 470.0f, 520.0f, 570.0f, 620.0f, 700.0f, 830.0f};
+        v = r[7];
+        for (int i = 1; i < 8; ++i)
+            if (l <= knot[i]) {
+                const float t = (l - knot[i - 1]) / (knot[i] - knot[i - 1]);
+                v = r[i - 1] + (r[i] - r[i - 1]) * t;
+                break;
+            }
+        break;
+    }
     default: v = NAN;
     }
     return v * s.scale;
@@ -708,7 +722,9 @@ void HostScene::copy_from(const rp_scene_desc &d) {
 }
 
 static std::string check_spectrum(const rp_spectrum &s, const rp_scene_desc &d, const char *what) {
-    if (s.kind > RP_SPECTRUM_LINEAR) return std::string(what) + ": unknown spectrum kind";
+    if (s.kind > RP_SPECTRUM_KNOTS8) return std::string(what) + ": unknown spectrum kind";
+    if (s.kind == RP_SPECTRUM_KNOTS8 && (uint64_t(s.table) + 8 > d.spectra_count))
+        return std::string(what) + ": knot values outside the pool";
     if (s.kind == RP_SPECTRUM_TABLE && (uint64_t(s.table) + 471 > d.spectra_count))
         return std::string(what) + ": spectrum table outside the pool";
     if (!std::isfinite(s.scale)) return std::string(what) + ": non-finite scale";
@@ -726,6 +742,9 @@ std::string validate_desc(const rp_scene_desc &d) {
     // dimensions (src/sampling_contract.h:16-28, dielectric_max_depth = 113).
     if (d.max_depth > 113) return "max_depth beyond the Sobol dimension layout (113)";
     if (d.light_sampler > RP_LIGHT_SAMPLER_UNIFORM) return "unknown light_sampler";
+    if (d.transport > RP_TRANSPORT_SCALAR_DEMO) return "unknown transport";
+    if (d.transport == RP_TRANSPORT_SCALAR_DEMO && !d.has_environment)
+        return "the scalar demo model needs an environment (its only light)";
     for (uint32_t t = 0; t < d.triangle_count; ++t) {
         for (int k = 0; k < 3; ++k)
             if (d.indices[3 * t + k] >= d.vertex_count) return "triangle index out of range";
@@ -762,6 +781,20 @@ std::string validate_desc(const rp_scene_desc &d) {
     if (d.has_environment) {
         std::string s = check_spectrum(d.environment, d, "environment");
         if (!s.empty()) return s;
+    }
+    // The scalar estimators scatter only at diffuse surfaces in vacuum
+    // (shaders/scalar_transport.h:446-899); the demo model lights only by
+    // the environment (docs/rendering.md:41-47).
+    if (d.transport != RP_TRANSPORT_HERO) {
+        for (uint32_t m = 0; m < d.material_count; ++m)
+            if (d.materials[m].kind != RP_MATERIAL_DIFFUSE && d.materials[m].kind != RP_MATERIAL_INTERFACE)
+                return "the scalar transport scatters only at diffuse surfaces";
+        if (d.medium_count != 0 || d.camera.medium != RP_NONE) return "the scalar transport has no participating media";
+    }
+    if (d.transport == RP_TRANSPORT_SCALAR_DEMO) {
+        if (d.spot_count != 0) return "the scalar demo model has no spot lights";
+        for (uint32_t o = 0; o < d.object_count; ++o)
+            if (d.objects[o].emission != RP_NONE) return "the scalar demo model has no emitters";
     }
     return {};
 }

@@ -21,6 +21,7 @@
 #include "rp_wavefront.h"
 #include "rp_batch.h"
 #include "rp_adjoint.h"
+#include "rp_scalar.h"
 
 static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 128 && sizeof(rp_batch_config) == 32 &&
                   sizeof(rp_request) == 72 && sizeof(rp_path_record) == 112,
@@ -92,6 +93,7 @@ struct rp_scene {
     DevBuf<float> path_xyz;
     DevBuf<double> film;
     DevBuf<rp_path_record> records;
+    DevBuf<rp_render_sample> scalar_records;
     DevBuf<unsigned long long> counters;
     DevBuf<rp_ray> rays;
     DevBuf<rp_hit> hits;
@@ -199,6 +201,59 @@ __global__ void __launch_bounds__(kBlock) k_mega(rpd::Scene S, rpd::PathParams P
         sh = r.shadow_rays;
     }
     count_rays(counters, ext, sh, failed);
+}
+
+// Scalar-transport megakernel: one thread = one scalar path; per-path film
+// weight planar as in k_mega.
+__global__ void __launch_bounds__(kBlock) k_scalar_mega(rpd::Scene S, rpd::PathParams P, uint32_t npx,
+                                                         uint64_t sample_begin, uint64_t n, uint32_t independent,
+                                                         float *xyz, unsigned long long *counters) {
+    __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
+    __shared__ float s_t[rpd::kShortStack * kBlock];
+    rpd::Stack st;
+    LdsStack<kBlock>::init(st, s_node, s_t);
+    uint64_t lane = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    uint32_t ext = 0, sh = 0, failed = 0;
+    if (lane < n) {
+        rpd::ScalarResult r;
+        rpd::trace_scalar<true>(S, P, st, uint32_t(lane % npx), sample_begin + lane / npx, independent != 0, r);
+        float v[3];
+        rpd::scalar_xyz(S, r, v);
+        failed = r.status != RP_PATH_SUCCESS;
+        xyz[lane] = v[0];
+        xyz[n + lane] = v[1];
+        xyz[2 * n + lane] = v[2];
+        ext = r.ext_rays;
+        sh = r.shadow_rays;
+    }
+    count_rays(counters, ext, sh, failed);
+}
+
+// Scalar RenderSample records; B = the request's workgroup width.
+template <int B>
+__global__ void __launch_bounds__(B) k_scalar_records(rpd::Scene S, rpd::PathParams P, uint32_t pixel_begin,
+                                                      uint32_t pixel_count, uint64_t sample_begin, uint64_t n,
+                                                      uint32_t independent, rp_render_sample *out) {
+    __shared__ uint32_t s_node[rpd::kShortStack * B];
+    __shared__ float s_t[rpd::kShortStack * B];
+    rpd::Stack st;
+    LdsStack<B>::init(st, s_node, s_t);
+    uint64_t lane = uint64_t(blockIdx.x) * B + threadIdx.x;
+    if (lane >= n) return;
+    const uint32_t pixel = pixel_begin + uint32_t(lane % pixel_count);
+    const uint64_t sample = sample_begin + lane / pixel_count;
+    rpd::ScalarResult r;
+    rpd::trace_scalar<true>(S, P, st, pixel, sample, independent != 0, r);
+    rp_render_sample rec;
+    rec.pixel = pixel;
+    rec.status = r.status;
+    rec.sample = sample;
+    rec.wavelength = r.wavelength;
+    rec.radiance = r.radiance;
+    rec.scatterings = r.scatterings;
+    for (int k = 0; k < 4; ++k) rec.random_words[k] = r.words[k];
+    rpd::scalar_xyz(S, r, rec.xyz);
+    out[lane] = rec;
 }
 
 // Atomic-free film: each pixel adds its samples in increasing sample order
@@ -416,6 +471,7 @@ void upload_scene(rp_scene *s) {
     for (const auto &mt : s->host.materials)
         if (mt.kind == RP_MATERIAL_INTERFACE) S.has_interfaces = 1;
     S.has_media = d.medium_count ? 1 : 0;
+    S.transport = d.transport;
     S.env_pdf = lt.env_light != RP_NONE ? lt.env_pmf * rpd::kInv4Pi : 0.0f;
     s->info.emitter_count = 0;
     for (auto &l : dl)
@@ -508,6 +564,12 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     if (spp == 0) throw std::runtime_error("spp must be positive");
     if (cfg.mode > RP_MODE_WAVEFRONT_SORTED) throw std::runtime_error("unknown mode");
     if (cfg.randomization > RP_RANDOMIZE_OWEN) throw std::runtime_error("unknown randomization");
+    const bool scalar = s->host.desc.transport != RP_TRANSPORT_HERO;
+    if (scalar && cfg.mode != RP_MODE_MEGAKERNEL)
+        throw std::runtime_error("scalar transport renders in RP_MODE_MEGAKERNEL");
+    if (!scalar && (cfg.flags & RP_FRAME_INDEPENDENT_SAMPLER))
+        throw std::runtime_error("the independent sampler is for scalar transport");
+    const uint32_t independent = (cfg.flags & RP_FRAME_INDEPENDENT_SAMPLER) ? 1u : 0u;
     HIP_CHECK(hipSetDevice(s->device));
     rpd::PathParams P = path_params(cfg.seed, cfg.randomization, 0, 0, cfg.flags);
     const uint64_t max_chunk_paths = uint64_t(1) << 26;
@@ -539,7 +601,10 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         uint32_t ns = std::min(chunk, spp - s0);
         uint64_t n = uint64_t(ns) * npx;
         s->path_xyz.alloc(3 * n);
-        if (cfg.mode == RP_MODE_MEGAKERNEL) {
+        if (scalar) {
+            hipLaunchKernelGGL(k_scalar_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                               s->dev, P, npx, cfg.sample_begin + s0, n, independent, s->path_xyz.p, s->counters.p);
+        } else if (cfg.mode == RP_MODE_MEGAKERNEL) {
             hipLaunchKernelGGL(k_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream, s->dev, P,
                                npx, cfg.sample_begin + s0, n, s->path_xyz.p, s->counters.p);
         } else {
@@ -757,7 +822,7 @@ unsigned char *noalloc_scratch(NoAllocSlot &slot) {
 // RenderRequest validation (src/metal_render.mm:1930-1946, src/sampling.hpp:29-31):
 // raster size, sampler kind and randomization, the pixel and sample ranges
 // (and the fixed sample budget when given), and the workgroup width.
-bool request_valid(const rp_scene *s, const rp_request &rq, const char *who) {
+bool request_valid(const rp_scene *s, const rp_request &rq, const char *who, bool scalar = false) {
     const rp_camera &cam = s->host.desc.camera;
     const uint32_t npx = cam.width * cam.height;
     auto bad = [&](const std::string &why) {
@@ -767,7 +832,11 @@ bool request_valid(const rp_scene *s, const rp_request &rq, const char *who) {
     if ((rq.width || rq.height) && (rq.width != cam.width || rq.height != cam.height))
         return bad("request raster " + std::to_string(rq.width) + "x" + std::to_string(rq.height) +
                    " differs from the scene camera's " + std::to_string(cam.width) + "x" + std::to_string(cam.height));
-    if (rq.sampler_kind != RP_SAMPLER_SOBOL) return bad("sampler kind not supported by this backend (Sobol only)");
+    if (scalar ? (s->host.desc.transport == RP_TRANSPORT_HERO) : (s->host.desc.transport != RP_TRANSPORT_HERO))
+        return bad(scalar ? "the scene uses hero transport (rp_sample)" : "the scene uses scalar transport (rp_sample_scalar)");
+    if (rq.sampler_kind != RP_SAMPLER_SOBOL && !(scalar && rq.sampler_kind == RP_SAMPLER_INDEPENDENT))
+        return bad(scalar ? "sampler kind not supported by this backend (Sobol or independent)"
+                          : "sampler kind not supported by this backend (Sobol only)");
     if (rq.randomization > RP_RANDOMIZE_OWEN) return bad("unknown randomization");
     if (rq.pixel_count == 0 || rq.sample_count == 0 || uint64_t(rq.pixel_begin) + rq.pixel_count > npx)
         return bad("pixel range outside the film or empty request");
@@ -894,6 +963,44 @@ int rp_sample(rp_scene *s, const rp_request *rq, rp_path_record *out, uint64_t c
     }
 }
 
+int rp_sample_scalar(rp_scene *s, const rp_request *rq, rp_render_sample *out, uint64_t capacity) {
+    if (!s || !rq || !out) {
+        set_error("rp_sample_scalar: null argument");
+        return RP_INVALID;
+    }
+    if (!request_valid(s, *rq, "rp_sample_scalar", true)) return RP_INVALID;
+    const uint64_t n = uint64_t(rq->pixel_count) * rq->sample_count;
+    if (capacity < n) {
+        set_error("rp_sample_scalar: output capacity too small");
+        return RP_INVALID;
+    }
+    try {
+        std::lock_guard<std::mutex> lock(s->exec);
+        HIP_CHECK(hipSetDevice(s->device));
+        s->scalar_records.alloc(n);
+        rpd::PathParams P = path_params(rq->seed, rq->randomization, rq->emergency_limit, rq->boundary_limit);
+        const uint32_t B = rq->threadgroup_width ? rq->threadgroup_width : uint32_t(kBlock);
+        const uint32_t independent = rq->sampler_kind == RP_SAMPLER_INDEPENDENT ? 1u : 0u;
+        const dim3 grid(uint32_t((n + B - 1) / B));
+        if (B == 64)
+            hipLaunchKernelGGL(k_scalar_records<64>, grid, dim3(64), 0, s->stream, s->dev, P, rq->pixel_begin,
+                               rq->pixel_count, rq->sample_begin, n, independent, s->scalar_records.p);
+        else if (B == 256)
+            hipLaunchKernelGGL(k_scalar_records<256>, grid, dim3(256), 0, s->stream, s->dev, P, rq->pixel_begin,
+                               rq->pixel_count, rq->sample_begin, n, independent, s->scalar_records.p);
+        else
+            hipLaunchKernelGGL(k_scalar_records<128>, grid, dim3(128), 0, s->stream, s->dev, P, rq->pixel_begin,
+                               rq->pixel_count, rq->sample_begin, n, independent, s->scalar_records.p);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(out, s->scalar_records.p, n * sizeof(rp_render_sample), hipMemcpyDeviceToHost,
+                                 s->stream));
+        HIP_CHECK(hipStreamSynchronize(s->stream));
+        return RP_SUCCESS;
+    } catch (const std::exception &e) {
+        return fail("rp_sample_scalar", e);
+    }
+}
+
 int rp_render_frame_device(rp_scene *s, const rp_frame_config *cfg, float *xyz_device, rp_frame_stats *stats) {
     if (!s || !cfg || !xyz_device) {
         set_error("rp_render_frame_device: null argument");
@@ -942,6 +1049,10 @@ int rp_render_frame_device_noalloc(rp_scene *s, const rp_frame_config *cfg, floa
         set_error(std::string(who) + ": spp must be positive and randomization known");
         return RP_INVALID;
     }
+    if (s->host.desc.transport == RP_TRANSPORT_HERO && (cfg->flags & RP_FRAME_INDEPENDENT_SAMPLER)) {
+        set_error(std::string(who) + ": the independent sampler is for scalar transport");
+        return RP_INVALID;
+    }
     const rp_camera &cam = s->host.desc.camera;
     const uint64_t npx = uint64_t(cam.width) * cam.height;
     const size_t film_off = kNoAllocCounterBytes;
@@ -979,8 +1090,13 @@ int rp_render_frame_device_noalloc(rp_scene *s, const rp_frame_config *cfg, floa
         for (uint32_t s0 = 0; s0 < cfg->spp; s0 += chunk) {
             const uint32_t ns = std::min(chunk, cfg->spp - s0);
             const uint64_t n = uint64_t(ns) * npx;
-            hipLaunchKernelGGL(k_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream, s->dev, P,
-                               uint32_t(npx), cfg->sample_begin + s0, n, xyz, counters);
+            if (s->host.desc.transport != RP_TRANSPORT_HERO)
+                hipLaunchKernelGGL(k_scalar_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                                   s->dev, P, uint32_t(npx), cfg->sample_begin + s0, n,
+                                   (cfg->flags & RP_FRAME_INDEPENDENT_SAMPLER) ? 1u : 0u, xyz, counters);
+            else
+                hipLaunchKernelGGL(k_mega, dim3(uint32_t((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s->stream,
+                                   s->dev, P, uint32_t(npx), cfg->sample_begin + s0, n, xyz, counters);
             hipLaunchKernelGGL(k_film_add, dim3(uint32_t((npx + 255) / 256)), dim3(256), 0, s->stream, uint32_t(npx), ns,
                                n, xyz, film);
         }
@@ -1531,6 +1647,8 @@ int rp_batch_create(const rp_batch_item *items, uint32_t count, rp_batch **out) 
         for (uint32_t i = 0; i < count; ++i) {
             const rp_batch_item &it = items[i];
             if (!it.scene || it.spp == 0) throw std::runtime_error("batch item without scene or spp");
+            if (it.scene->host.desc.transport != RP_TRANSPORT_HERO)
+                throw std::runtime_error("the batch pool renders hero-transport scenes");
             if (it.scene->device != b->device) throw std::runtime_error("batch scenes on different devices");
             const rp_camera &cam = it.scene->host.desc.camera;
             uint64_t paths = uint64_t(cam.width) * cam.height * it.spp;
@@ -1767,6 +1885,10 @@ int backward_impl(const char *name, rp_scene *s, const rp_frame_config *cfg, con
     }
     if (count > rpd::kMaxParams) {
         set_error(std::string(name) + ": at most 256 parameters");
+        return RP_INVALID;
+    }
+    if (s->host.desc.transport != RP_TRANSPORT_HERO) {
+        set_error(std::string(name) + ": the adjoint differentiates hero-transport scenes");
         return RP_INVALID;
     }
     try {

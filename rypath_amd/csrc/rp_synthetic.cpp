@@ -464,6 +464,86 @@ void make_inverse(Builder &b, uint64_t seed, const rp_synthetic_options &o) {
     b.desc.hero_wavelengths = 0;
 }
 
+// The reference's demo scene (src/render.cpp:2060-2106): floor and two
+// walls as quads triangulated (0,1,2),(0,2,3) in the given winding, one unit
+// cube mesh instanced three times by box_transform (src/render.cpp:821-827,
+// rows {c w, 0, s d, x}, {0, h, 0, y}, {-s w, 0, c d, z}), six diffuse
+// materials as reflectance at 360/420/470/520/570/620/700/830 nm, a unit
+// environment and a 42-degree vertical field of view from (4.1, 3, 6.2).
+void make_demo(Builder &b, const rp_synthetic_options &o) {
+    const uint32_t w = o.width ? o.width : 256, h = o.height ? o.height : 192;
+    static const float knots[6][8] = {
+        {0.54f, 0.54f, 0.55f, 0.58f, 0.61f, 0.63f, 0.65f, 0.65f},
+        {0.69f, 0.68f, 0.67f, 0.65f, 0.64f, 0.62f, 0.61f, 0.61f},
+        {0.34f, 0.34f, 0.36f, 0.43f, 0.55f, 0.61f, 0.63f, 0.63f},
+        {0.10f, 0.10f, 0.16f, 0.52f, 0.40f, 0.09f, 0.06f, 0.06f},
+        {0.07f, 0.08f, 0.09f, 0.13f, 0.39f, 0.73f, 0.72f, 0.72f},
+        {0.32f, 0.47f, 0.72f, 0.23f, 0.11f, 0.08f, 0.09f, 0.09f},
+    };
+    uint32_t obj[6];
+    for (int m = 0; m < 6; ++m) {
+        rp_spectrum r{};
+        r.kind = RP_SPECTRUM_KNOTS8;
+        r.table = uint32_t(b.spectra.size());
+        r.scale = 1;
+        b.spectra.insert(b.spectra.end(), knots[m], knots[m] + 8);
+        obj[m] = b.object(b.material(diffuse(r)));
+    }
+    using P3 = std::array<float, 3>;
+    auto mesh_quad = [&](const P3 (&c)[4], uint32_t object) {
+        uint32_t v[4];
+        for (int i = 0; i < 4; ++i) {
+            b.positions.insert(b.positions.end(), c[i].begin(), c[i].end());
+            v[i] = uint32_t(b.positions.size() / 3 - 1);
+        }
+        b.tri(v[0], v[1], v[2], object);
+        b.tri(v[0], v[2], v[3], object);
+    };
+    mesh_quad({{-3, 0, -3}, {-3, 0, 3}, {3, 0, 3}, {3, 0, -3}}, obj[0]);
+    mesh_quad({{-3, 0, -3}, {3, 0, -3}, {3, 3.2f, -3}, {-3, 3.2f, -3}}, obj[1]);
+    mesh_quad({{-3, 0, 3}, {-3, 0, -3}, {-3, 3.2f, -3}, {-3, 3.2f, 3}}, obj[2]);
+    static const float cube[8][3] = {{-0.5f, 0, -0.5f}, {0.5f, 0, -0.5f}, {0.5f, 0, 0.5f}, {-0.5f, 0, 0.5f},
+                                     {-0.5f, 1, -0.5f}, {0.5f, 1, -0.5f}, {0.5f, 1, 0.5f}, {-0.5f, 1, 0.5f}};
+    static const uint32_t cube_tris[12][3] = {{0, 1, 2}, {0, 2, 3}, {4, 6, 5}, {4, 7, 6}, {0, 4, 5}, {0, 5, 1},
+                                              {1, 5, 6}, {1, 6, 2}, {2, 6, 7}, {2, 7, 3}, {3, 7, 4}, {3, 4, 0}};
+    struct DemoBox {
+        float offset[3], scale[3], angle;
+    };
+    static const DemoBox boxes[3] = {{{-1.25f, -0.05f, -0.6f}, {1.25f, 2.25f, 1.25f}, 0.18f},
+                                     {{0.25f, -0.05f, 0.8f}, {1.45f, 1.08f, 1.3f}, -0.24f},
+                                     {{1.4f, -0.05f, -1.5f}, {1.07f, 1.55f, 1.06f}, 0.44f}};
+    for (int i = 0; i < 3; ++i) {
+        const DemoBox &bx = boxes[i];
+        const float c = std::cos(bx.angle), s = std::sin(bx.angle);
+        const float m[12] = {c * bx.scale[0], 0, s * bx.scale[2], bx.offset[0], 0, bx.scale[1], 0, bx.offset[1],
+                             -s * bx.scale[0], 0, c * bx.scale[2], bx.offset[2]};
+        const uint32_t base = uint32_t(b.positions.size() / 3);
+        for (const auto &p : cube)
+            for (int r = 0; r < 3; ++r)
+                b.positions.push_back(m[4 * r] * p[0] + m[4 * r + 1] * p[1] + m[4 * r + 2] * p[2] + m[4 * r + 3]);
+        for (const auto &t : cube_tris) b.tri(base + t[0], base + t[1], base + t[2], obj[3 + i]);
+    }
+    // Vertical field of view whatever the aspect (the demo camera).
+    const D3 eye = {4.1f, 3.0f, 6.2f}, target = {0.0f, 1.0f, -0.45f}, up = {0, 1, 0};
+    const D3 f = normalize(target - eye), r = normalize(cross(f, up)), u = cross(r, f);
+    const double th = std::tan(42.0 * 0.5 * 3.14159265358979323846 / 180.0), asp = double(w) / double(h);
+    const D3 corner = f - (th * asp) * r + th * u, dx = (2 * th * asp) * r, dy = (-2 * th) * u;
+    rp_camera cam{};
+    cam.origin[0] = float(eye.x), cam.origin[1] = float(eye.y), cam.origin[2] = float(eye.z);
+    cam.corner[0] = float(corner.x), cam.corner[1] = float(corner.y), cam.corner[2] = float(corner.z);
+    cam.dx[0] = float(dx.x), cam.dx[1] = float(dx.y), cam.dx[2] = float(dx.z);
+    cam.dy[0] = float(dy.x), cam.dy[1] = float(dy.y), cam.dy[2] = float(dy.z);
+    cam.width = w;
+    cam.height = h;
+    cam.medium = RP_NONE;
+    b.desc.camera = cam;
+    b.desc.has_environment = 1;
+    b.desc.environment = constant(1.0f);
+    b.desc.transport = RP_TRANSPORT_SCALAR_DEMO;
+    b.desc.max_depth = o.max_depth ? o.max_depth : 113;
+    b.desc.hero_wavelengths = 0;
+}
+
 } // namespace
 } // namespace rp
 
@@ -505,6 +585,7 @@ int rp_synthetic_create(uint32_t kind, uint64_t seed, const rp_synthetic_options
         }
         case RP_SYNTH_VOLUME: make_volume(b, seed, o); break;
         case RP_SYNTH_INVERSE: make_inverse(b, seed, o); break;
+        case RP_SYNTH_DEMO: make_demo(b, o); break;
         default:
             delete s;
             set_error("rp_synthetic_create: unknown kind");

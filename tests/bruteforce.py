@@ -10,7 +10,9 @@ numbers or sampling scheme with the oracle or the device: pixel positions,
 wavelengths and directions come from numpy's PCG64 stream, one wavelength
 per path, uniform on [360, 830] nm, with the sensor weight
 470 CIE(round lambda) -- the same nearest-nm sensor the hero estimator
-integrates.
+integrates -- or, with sensor="demo", the reference demo model's sensor:
+CIE linearly interpolated at lambda times 470 / Y, Y the trapezoid integral
+of y-bar (tests/render_reference.cpp:18-33, 266-272).
 """
 import numpy as np
 
@@ -69,7 +71,16 @@ def _cosine(n, rng):
     return t * (s * np.cos(phi))[:, None] + b * (s * np.sin(phi))[:, None] + n * c[:, None]
 
 
-def render(builder, desc, paths_per_pixel, seed, max_scatterings=1024):
+def demo_sensor(lam):
+    """CIE at lambda by linear interpolation between 1 nm rows, times 470 / Y."""
+    y_norm = ((tc.CIE[1:, 1] + tc.CIE[:-1, 1]) * 0.5).sum()
+    off = np.asarray(lam, np.float64) - 360.0
+    row = np.minimum(off.astype(np.int64), 469)
+    f = (off - row)[:, None]
+    return (470.0 / y_norm) * ((1 - f) * tc.CIE[row] + f * tc.CIE[row + 1])
+
+
+def render(builder, desc, paths_per_pixel, seed, max_scatterings=1024, sensor="nearest"):
     """Per-path raw XYZ (float64) and pixel index for every pixel of the
     builder's camera, and the largest number of scatterings any path made."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -111,6 +122,8 @@ def render(builder, desc, paths_per_pixel, seed, max_scatterings=1024):
         d[alive] = _cosine(nrm, rng)
     else:
         raise RuntimeError("brute-force estimator exceeded its scattering limit")
+    if sensor == "demo":
+        return demo_sensor(lam) * radiance[:, None], pix, deepest
     row = np.clip(np.floor(lam + 0.5).astype(np.int64) - 360, 0, 470)
     xyz = 470.0 * tc.CIE[row] * radiance[:, None]
     return xyz, pix, deepest

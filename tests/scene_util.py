@@ -43,6 +43,15 @@ class SceneBuilder:
         self.pool.extend(values.tolist())
         return spectrum(rp.SPECTRUM_TABLE, table=off)
 
+    def knots8(self, values):
+        """Piecewise-linear spectrum through 8 values at 360, 420, 470, 520,
+        570, 620, 700 and 830 nm (the reference's spectral material)."""
+        values = np.asarray(values, np.float32)
+        assert values.shape == (8,)
+        off = len(self.pool)
+        self.pool.extend(values.tolist())
+        return spectrum(rp.SPECTRUM_KNOTS8, table=off)
+
     def binned(self, bins):
         """471-sample table, piecewise constant on len(bins) equal wavelength bins
         (nm index i belongs to bin i * n // 471)."""
@@ -110,6 +119,13 @@ class SceneBuilder:
         self.positions.extend(v.tolist() for v in p)
         self.indices += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
         self.tri_object += [obj, obj]
+
+    def mesh(self, points, triangles, obj):
+        """Triangles over `points` exactly as given (no winding change)."""
+        base = len(self.positions)
+        self.positions.extend(np.asarray(v, np.float64).tolist() for v in points)
+        self.indices += [[base + a, base + b, base + c] for a, b, c in triangles]
+        self.tri_object += [obj] * len(triangles)
 
     def box(self, lo, hi, obj, inward=True):
         lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)

@@ -83,9 +83,11 @@ def test_cli_render_demo_writes_film(rp, gpu_available, tmp_path):
     rgb, e = exr_util.read_rgb(out)
     assert (e["width"], e["height"]) == (64, 48)
     assert e["strings"]["rypath.samples"] == "16"
-    # same film through the library: the CLI renders the sorted wavefront frame
-    s = rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=64, height=48)
-    xyz, _ = rp.Scene(s.desc).render_frame(rp.frame_config(16, mode=rp.MODE_WAVEFRONT_SORTED, seed=1))
+    assert "Sampler: philox4x32-10-v1" in r.stdout
+    # same film through the library: the demo scene, scalar megakernel, Philox
+    s = rp.Synthetic(rp.SYNTH_DEMO, width=64, height=48)
+    xyz, _ = rp.Scene(s.desc).render_frame(rp.frame_config(
+        16, mode=rp.MODE_MEGAKERNEL, seed=1, randomization=rp.RANDOMIZE_NONE, flags=rp.FRAME_INDEPENDENT_SAMPLER))
     np.testing.assert_allclose(rgb, (xyz.astype(np.float64) @ M_SRGB.T).astype(np.float32), rtol=1e-5, atol=1e-6)
 
 

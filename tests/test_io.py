@@ -371,12 +371,13 @@ def test_cli_without_device_exits_2(tmp_path):
 def test_cli_resume_rejects_a_checkpoint_of_another_film(tmp_path):
     """A complete checkpoint (as many samples as requested) of another film
     size or sampler is rejected before anything renders or is written; the
-    demo renders 256x256 by default."""
+    demo renders 256x192 by default without Sobol randomization."""
     from tests.test_checkpoint import encode, sample_film
 
-    for w, h, samples, seed, rnd, note in [(5, 3, 4, 1, 2, "the checkpoint's film is 5x3"),
-                                           (256, 256, 4, 7, 2, "another seed or randomization"),
-                                           (256, 256, 9, 1, 2, "more samples than requested")]:
+    for w, h, samples, seed, rnd, note in [(5, 3, 4, 1, 0, "the checkpoint's film is 5x3"),
+                                           (256, 192, 4, 7, 0, "another seed or randomization"),
+                                           (256, 192, 4, 1, 2, "another seed or randomization"),
+                                           (256, 192, 9, 1, 0, "more samples than requested")]:
         _, _, _, _, _, ident, _, _ = sample_film()
         planes = np.ones((6, w * h)) * samples
         planes[1::2] = 0

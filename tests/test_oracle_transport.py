@@ -2,7 +2,9 @@
 
 The expectations come from closed forms and float64 quadrature in
 tests/transport_cases.py, which shares no code with the oracle or the device:
-a white furnace (sum_k rho^k Le), an environment-lit floor (rho E), a spot
+a white furnace (sum_k rho^k Le) in a cube and in a long tunnel tiled into
+unit quad lights (far tiles seen at grazing angles), an environment-lit
+floor (rho E), a spot
 light over a floor (inverse square, cosine and cone smoothstep, per pixel),
 an absorbing slab (Beer-Lambert, per nm) and single scattering in a
 homogeneous medium under a spot light (Henyey-Greenstein, per pixel) for
@@ -13,7 +15,8 @@ Each planted slip (or_plant_slip, a test switch of the oracle) reproduces one
 plausible estimator mistake -- a dropped MIS weight, a missing light pmf,
 uncompensated roulette, a flipped phase-function sign, a missing
 transmittance, a wrong environment pdf, a missing falloff or cosine, hero
-transmittance on every lane -- and the slip tests require that at least one
+transmittance on every lane, a light-BVH pmf without the receiver normal, an
+unwarped quad proposal density -- and the slip tests require that at least one
 known-answer gate fails with it.  Every result is deterministic (fixed Sobol
 seed), so these are exact regression checks, not flaky statistics.
 """
@@ -28,6 +31,8 @@ CASES = {
     "furnace_d3": (lambda: tc.furnace(3), 512, True),
     "furnace_d6": (lambda: tc.furnace(6), 512, True),
     "furnace_d6_no_roulette": (lambda: tc.furnace(6), 512, False),
+    "furnace_tunnel_d1": (lambda: tc.furnace_tunnel(1), 2048, True),
+    "furnace_tunnel_d3": (lambda: tc.furnace_tunnel(3), 2048, True),
     "env_floor": (lambda: tc.env_floor(), 512, True),
     "spot_plane": (lambda: tc.spot_plane(False), 1024, True),
     "spot_plane_env": (lambda: tc.spot_plane(True), 1024, True),
@@ -50,6 +55,8 @@ SLIPS = {
     8: ("surface NEE without the receiver cosine", ["spot_plane", "env_floor", "furnace_d3"]),
     9: ("hero sigma_t for every lane's transmittance", ["slab"]),
     10: ("continuation weight without |cos|", ["furnace_d3", "env_floor"]),
+    11: ("light-BVH pmf at emission hits without the receiver normal", ["furnace_tunnel_d1", "furnace_tunnel_d3"]),
+    12: ("quad proposal density without its cosine warp", ["furnace_tunnel_d1", "furnace_d1", "furnace_d3"]),
 }
 
 _built = {}

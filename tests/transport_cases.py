@@ -48,6 +48,9 @@ def spectrum_value(s, pool, lam):
     lam = np.asarray(lam, np.float64)
     if s.kind == rp.SPECTRUM_CONSTANT:
         v = np.full(lam.shape, s.c[0])
+    elif s.kind == rp.SPECTRUM_KNOTS8:
+        knots = np.array([360, 420, 470, 520, 570, 620, 700, 830], np.float64)
+        v = np.interp(lam, knots, np.asarray(pool, np.float64)[s.table:s.table + 8])
     elif s.kind == rp.SPECTRUM_LINEAR:
         t = (lam - 360.0) / 470.0
         v = (1 - t) * s.c[0] + t * s.c[1]
@@ -97,6 +100,42 @@ def furnace(max_depth, rho=(0.35, 0.75), le=(0.6, 1.4), width=8, height=8):
         return spectrum_value(ls, b.pool, lam) * sum(r ** k for k in range(max_depth + 1))
 
     return Case(b, desc, xyz_integral(radiance), label=f"furnace depth {max_depth}")
+
+
+def furnace_tunnel(max_depth, rho=(0.35, 0.75), le=(0.6, 1.4), width=8, height=8, half=(1.0, 0.5, 6.0)):
+    """Closed box |x| <= hx, |y| <= hy, |z| <= hz whose walls are tiled into
+    unit-sized inward-emitting quads (one light each), so that far tiles are
+    seen at grazing angles; the furnace expectation is shape-independent."""
+    b = SceneBuilder(width, height, max_depth)
+    m = b.diffuse(b.linear(*rho))
+    e = b.emission(b.linear(*le))
+    obj = b.object(m, e)
+    h = np.asarray(half, np.float64)
+    for axis in range(3):
+        u, v = (axis + 1) % 3, (axis + 2) % 3
+        nu, nv = max(1, int(round(2 * h[u]))), max(1, int(round(2 * h[v])))
+        for side in (-1.0, 1.0):
+            for i in range(nu):
+                for j in range(nv):
+                    u0, u1 = -h[u] + 2 * h[u] * i / nu, -h[u] + 2 * h[u] * (i + 1) / nu
+                    v0, v1 = -h[v] + 2 * h[v] * j / nv, -h[v] + 2 * h[v] * (j + 1) / nv
+                    corners = []
+                    for a_, b_ in ((u0, v0), (u1, v0), (u1, v1), (u0, v1)):
+                        q = np.zeros(3)
+                        q[axis], q[u], q[v] = side * h[axis], a_, b_
+                        corners.append(q)
+                    facing = np.zeros(3)
+                    facing[axis] = -side
+                    b.quad(*corners, obj, facing=facing)
+    b.look_at((0.2, -0.1, -5.0), (0.0, 0.0, 6.0), (0, 1, 0), 40)
+    desc = b.build()
+    rs, ls = desc.materials[0].reflectance, desc.emissions[0].radiance
+
+    def radiance(lam):
+        r = spectrum_value(rs, b.pool, lam)
+        return spectrum_value(ls, b.pool, lam) * sum(r ** k for k in range(max_depth + 1))
+
+    return Case(b, desc, xyz_integral(radiance), label=f"tunnel furnace depth {max_depth}")
 
 
 def env_floor(max_depth=2, rho=(0.2, 0.8), env=(1.3, 0.5), width=8, height=8):
