@@ -552,7 +552,13 @@ def main():
         inv = InverseScene()
         dt_inv = inv.step_seconds(10, 2)
         adjoint = dict(inverse_line(args, dt_inv, inv.W, inv.H, inv.spp, len(inv.params)), steps=10, warmup=2,
-                       gradient_check="bench.py --workload inverse --fd-trials 6 --fd-rough-trials 32")
+                       gradient_check={
+                           "step": 1e-3, "target_rel_err": 0.02, "groups": ["walls", "roughness", "sigma_s"],
+                           "test": "tests/test_gpu_adjoint.py::"
+                                   "test_adjoint_within_2_percent_of_central_differences_at_1e_3",
+                           "method": "GPU adjoint against central differences of the forward render with "
+                                     "every sampling decision frozen at theta (common paths)",
+                           "moving_paths": "bench.py --workload inverse --fd-trials 6 --fd-rough-trials 32"})
         for k in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "data"):
             adjoint.pop(k, None)
         print(f"[bench] inverse step: {dt_inv * 1e3:.2f} ms ({1 / dt_inv:.1f} gradients/s)", file=sys.stderr,

@@ -56,15 +56,34 @@ int rp_write_exr(const char *path, uint32_t width, uint32_t height, const float 
  * the samples [0, samples) of one scene, seed and randomization.
  * rp_film_accumulate renders the next config->spp samples; config->sample_begin
  * must equal the film's sample count, and after the first accumulation the
- * seed, randomization and scene identity must match the film's.  A render
+ * seed, sampler, randomization and scene identity must match the film's.  A render
  * resumed from a checkpoint equals the uninterrupted render bit for bit.
  *
- * Checkpoint file (format 1, little endian):
- *   "RPAFILM\n", u32 format = 1, u32 width, u32 height, u32 randomization,
- *   u64 seed, u64 samples, u8[32] scene identity, u32 segment count,
- *   segments { u64 begin, u64 end, u32 n, n bytes device }, then
- *   6 * width * height f64 planes (sum X, corr X, sum Y, corr Y, sum Z, corr Z),
- *   then the SHA-256 of all preceding bytes.
+ * Checkpoint files are the reference's RYPCHK format 3 (src/checkpoint.cpp:
+ * 17-122, src/checkpoint_contract.h:14), little endian:
+ *   header (44 B): "RYPCHK\r\n", u32 contract words format 3, identity 12,
+ *     renderer 29, sampling 11, accumulation 2, u64 metadata bytes,
+ *     u64 payload bytes (216 per pixel);
+ *   metadata: u32 width, u32 height, u64 job samples, u64 batch size,
+ *     u64 seed, u32 sampler kind (0 independent, 1 Sobol), u32 randomization,
+ *     i32 material seed, u32 film model (0 demo, 1 PBRT), f64 exposure,
+ *     u8 packet, u8 compatibility output, f64 max component, u32 filter kind,
+ *     f32 x radius, f32 y radius, f32 sigma, u8[32] scene identity,
+ *     u32 segment count, segments { u64 begin, u64 end, 4 x (u32 n, n bytes):
+ *     build, device, architecture, os };
+ *   pixels: f64 sum[3], f64 correction[3], u64 count, f64[18] splat and
+ *     compatibility sums, f64 weight sum, f64 weight correction;
+ *   SHA-256 of all preceding bytes.
+ * A film written here uses the one-pixel box filter (radius 0.5, unit
+ * weight), no splats and no clamped compatibility sums; hero films are
+ * packet films (weight sum = sample count), scalar films are not.  Reading a
+ * format-3 file refuses what this film cannot hold: another filter,
+ * splat or compatibility sums, compatibility output, Halton or ZSobol
+ * samples, more than 2^28 pixels.  The scene identity is this library's
+ * SHA-256 of its scene description (rp_scene_identity), so a film written by
+ * the reference reads here but binds to no scene of this library, and the
+ * reference likewise refuses to continue a film written here.  The earlier
+ * format 1 of this library ("RPAFILM\n") is still read.
  * Files are published through a sibling temporary, fsync and rename. */
 typedef struct rp_film rp_film;
 typedef struct rp_film_info {
@@ -72,6 +91,11 @@ typedef struct rp_film_info {
     uint64_t samples, seed;
     uint32_t randomization, segments;
     uint8_t scene_identity[32]; /* SHA-256 of the scene description (zero until bound) */
+    uint32_t sampler_kind;      /* rp_sampler_kind of the samples held */
+    uint32_t model;             /* 0 demo (scalar demo transport), 1 PBRT */
+    uint64_t job_samples, batch_size; /* the job settings a checkpoint records (0 = unset) */
+    uint32_t packet;            /* unit-weight reconstruction sums (hero transport) */
+    uint32_t format;            /* checkpoint format the film was read from, 0 if created */
 } rp_film_info;
 int rp_film_create(uint32_t width, uint32_t height, rp_film **out);
 /* Whether `film` can continue rendering `scene` with this seed and
@@ -85,6 +109,11 @@ int rp_film_accumulate(rp_scene *scene, rp_film *film, const rp_frame_config *co
                        rp_frame_stats *stats);
 int rp_film_resolve(const rp_film *film, float *xyz_host);
 int rp_film_get_info(const rp_film *film, rp_film_info *info);
+/* Records the render job's total samples and batch size, which a checkpoint
+ * stores as the reference's RenderSettings.samples and batch_size
+ * (src/checkpoint.cpp:87-89); unset, the film's own sample count is
+ * written.  job_samples must be at least the film's samples. */
+int rp_film_set_job(rp_film *film, uint64_t job_samples, uint64_t batch_size);
 int rp_checkpoint_write(const rp_film *film, const char *path);
 int rp_checkpoint_read(const char *path, rp_film **out);
 void rp_film_destroy(rp_film *film);

@@ -68,6 +68,14 @@ struct Scene {
     std::vector<uint32_t> tri_light; // triangle -> light index or RP_NONE
     float scene_radius = 0;
     uint32_t m = 0;
+    // Test switch (or_scene_set_sampling): the scene whose materials and
+    // media drive every sampling decision and density -- BSDF and
+    // free-flight sampling, hero-lane pdfs, MIS weights -- while this scene's
+    // values are evaluated (BSDF values, scattering coefficients,
+    // transmittance).  Null: this scene for both.  With it, central
+    // differences of the forward render in the parameters keep every path
+    // fixed (the detached estimator the adjoint differentiates).
+    const Scene *sampling = nullptr;
     V3 vert(uint32_t i) const { return {positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]}; }
     void tri(uint32_t t, V3 &a, V3 &b, V3 &c) const {
         a = vert(indices[3 * t]);
@@ -1680,6 +1688,7 @@ static std::atomic<bool> g_roulette{true};
 template <class T>
 static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &trk) {
     const Scene &s = *c.s;
+    const Scene &s0 = s.sampling ? *s.sampling : s;
     PathOut out;
     const uint32_t W = s.d.camera.width, H = s.d.camera.height;
     const uint32_t px = pixel % W, py = pixel / W;
@@ -1728,16 +1737,17 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
         // ---- homogeneous medium along the segment (hero free flight)
         if (medium != RP_NONE) {
             MediumCoeff mc = medium_coeff(s, medium, wl);
+            const MediumCoeff mc0 = &s0 == &s ? mc : medium_coeff(s0, medium, wl);
             auto mw = medium_draws(c, pixel, sample, mdraw);
             float uf = philox_u01(mw[0]);
-            float tf = mc.st[0] > 0 ? -log_f(1.0f - uf) / mc.st[0] : INFINITY;
+            float tf = mc0.st[0] > 0 ? -log_f(1.0f - uf) / mc0.st[0] : INFINITY;
             if (tf < seg) {
                 // scattering event in the medium at distance tf
                 float num[4], pdf[4];
                 for (int k = 0; k < 4; ++k) {
                     float tr = exp_f(-mc.st[k] * tf);
                     num[k] = mc.ss[k] * tr;
-                    pdf[k] = mc.st[k] * tr;
+                    pdf[k] = mc0.st[k] * exp_f(-mc0.st[k] * tf);
                 }
                 if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
                 trk.medium_factor(medium, wl, tf, mc.ss);
@@ -1816,7 +1826,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 for (int k = 0; k < 4; ++k) {
                     float tr = exp_f(-mc.st[k] * seg);
                     num[k] = slip(SLIP_HERO_TRANSMITTANCE) ? exp_f(-mc.st[0] * seg) : tr;
-                    pdf[k] = tr;
+                    pdf[k] = &s0 == &s ? tr : exp_f(-mc0.st[k] * seg);
                 }
                 if (!advance(density, beta, num, pdf)) { out.status = RP_PATH_INVALID_ARITHMETIC; break; }
                 trk.medium_pass(medium, wl, seg);
@@ -1878,9 +1888,11 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
         if (count - 2 >= c.emergency) { out.status = RP_PATH_EMERGENCY_DEPTH; break; }
         Frame fr = frame_from_z(sp.n);
         V3 wol = fr.local(wo);
-        Lane lanes[4];
+        Lane lanes[4], lanes0[4];
         for (int k = 0; k < 4; ++k) lanes[k] = prepare_lane(s, mat, wl.lambda[k]);
-        bool connectible = !lanes[0].smooth || lanes[0].kind == RP_MATERIAL_DIFFUSE;
+        for (int k = 0; k < 4; ++k)
+            lanes0[k] = &s0 == &s ? lanes[k] : prepare_lane(s0, s0.materials[obj.material], wl.lambda[k]);
+        bool connectible = !lanes0[0].smooth || lanes0[0].kind == RP_MATERIAL_DIFFUSE;
         auto dr = draws(c, out.index, 4 + count - 2);
         float direct_density[4], direct_beta[4];
         std::memcpy(direct_density, density, sizeof density);
@@ -1890,25 +1902,25 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
         bool cont = direct_allowed;
         if (cont) {
             float uc = sobol_uniform(dr[0]), u0 = sobol_uniform(dr[1]), u1 = sobol_uniform(dr[2]);
-            Sampled hs = sample_lane(lanes[0], wol, uc, u0, u1);
+            Sampled hs = sample_lane(lanes0[0], wol, uc, u0, u1);
             if (!hs.ok) cont = false;
             else {
                 float num[4], pdf[4], etas[4];
                 if (hs.delta) {
                     for (int k = 0; k < 4; ++k) {
-                        if (k == 0) {
-                            float m, p;
-                            dirac_lane(lanes[0], wol, hs.wi, uc, u0, u1, m, p);
-                            num[0] = m;
-                            pdf[0] = hs.pdf;
-                        } else dirac_lane(lanes[k], wol, hs.wi, uc, u0, u1, num[k], pdf[k]);
+                        float m, p;
+                        dirac_lane(lanes[k], wol, hs.wi, uc, u0, u1, m, p);
+                        num[k] = m;
+                        pdf[k] = p;
+                        if (&s0 != &s) dirac_lane(lanes0[k], wol, hs.wi, uc, u0, u1, m, pdf[k]);
+                        if (k == 0) pdf[0] = hs.pdf;
                         etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
                     }
                 } else {
                     for (int k = 0; k < 4; ++k) {
                         Eval e = eval_lane(lanes[k], wol, hs.wi);
                         num[k] = slip(SLIP_CONTINUATION_NO_COSINE) ? e.f : e.f * std::fabs(hs.wi.z);
-                        pdf[k] = e.pdf;
+                        pdf[k] = &s0 == &s ? e.pdf : eval_lane(lanes0[k], wol, hs.wi).pdf;
                         etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
                     }
                     pdf[0] = hs.pdf;
@@ -1946,7 +1958,7 @@ static PathOut trace_path_t(const Ctx &c, uint32_t pixel, uint64_t sample, T &tr
                 bool any = false;
                 for (int k = 0; k < 4; ++k) {
                     Eval e = eval_lane(lanes[k], wol, wil);
-                    bp[k] = e.pdf;
+                    bp[k] = &s0 == &s ? e.pdf : eval_lane(lanes0[k], wol, wil).pdf;
                     val[k] = ((direct_beta[k] * e.f) * (ls.le[k] / ls.pdf)) * cosv;
                     any = any || val[k] > 0;
                 }
@@ -2348,6 +2360,12 @@ void or_light_table(void *sp, uint32_t *kind, uint32_t *index, float *pmf, float
 uint32_t or_light_node_count(void *sp) { return uint32_t(static_cast<Scene *>(sp)->lnodes.size()); }
 // Light selection probes: the light chosen for (p, n, u) with its pmf, and
 // the pmf of a given light.
+// Test switch: `sampling` (an oracle scene of the same geometry, or NULL)
+// drives the sampling decisions and densities of `sp` (Scene::sampling).
+void or_scene_set_sampling(void *sp, void *sampling) {
+    static_cast<Scene *>(sp)->sampling = static_cast<const Scene *>(sampling);
+}
+
 uint32_t or_light_select(void *sp, const float p[3], const float n[3], float u, float *pmf) {
     Selected sel = light_select(*static_cast<Scene *>(sp), {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, u);
     *pmf = sel.pmf;

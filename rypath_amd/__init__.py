@@ -245,7 +245,8 @@ _sig("rp_write_exr", C.c_int, [C.c_char_p, u32, u32, C.POINTER(f32), f64, u32, C
                                C.POINTER(C.c_char_p), u32])
 class FilmInfo(C.Structure):
     _fields_ = [("width", u32), ("height", u32), ("samples", u64), ("seed", u64), ("randomization", u32),
-                ("segments", u32), ("scene_identity", C.c_uint8 * 32)]
+                ("segments", u32), ("scene_identity", C.c_uint8 * 32), ("sampler_kind", u32), ("model", u32),
+                ("job_samples", u64), ("batch_size", u64), ("packet", u32), ("format", u32)]
 
 
 _sig("rp_film_create", C.c_int, [u32, u32, C.POINTER(C.c_void_p)])
@@ -254,6 +255,7 @@ _sig("rp_film_accumulate", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FrameConf
                                      C.POINTER(FrameStats)])
 _sig("rp_film_resolve", C.c_int, [C.c_void_p, C.c_void_p])
 _sig("rp_film_get_info", C.c_int, [C.c_void_p, C.POINTER(FilmInfo)])
+_sig("rp_film_set_job", C.c_int, [C.c_void_p, u64, u64])
 _sig("rp_checkpoint_write", C.c_int, [C.c_void_p, C.c_char_p])
 _sig("rp_checkpoint_read", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)])
 _sig("rp_film_destroy", None, [C.c_void_p])
@@ -273,7 +275,7 @@ EXPORTED = [
     "rp_synthetic_create", "rp_synthetic_desc", "rp_synthetic_destroy",
     "rp_pbrt_load", "rp_pbrt_load_string", "rp_pbrt_desc", "rp_pbrt_get_info", "rp_pbrt_destroy",
     "rp_write_exr", "rp_frame_submit", "rp_frame_poll", "rp_frame_wait", "rp_frame_release",
-    "rp_film_create", "rp_film_check", "rp_film_accumulate", "rp_film_resolve", "rp_film_get_info", "rp_checkpoint_write",
+    "rp_film_create", "rp_film_check", "rp_film_accumulate", "rp_film_resolve", "rp_film_get_info", "rp_film_set_job", "rp_checkpoint_write",
     "rp_checkpoint_read", "rp_film_destroy", "rp_scene_identity", "rp_sha256",
 ]
 
@@ -551,6 +553,10 @@ class Film:
         out = {f: getattr(i, f) for f, _ in FilmInfo._fields_}
         out["scene_identity"] = bytes(i.scene_identity)
         return out
+
+    def set_job(self, job_samples, batch_size):
+        """Records the job's total samples and batch size for checkpoints (rp_film_set_job)."""
+        _check("rp_film_set_job", _lib.rp_film_set_job(self._h, job_samples, batch_size))
 
     def check(self, scene, seed, randomization):
         """Raises unless this film can continue `scene` with (seed, randomization) (rp_film_check)."""

@@ -151,6 +151,9 @@ int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t s
         mismatch = "the checkpoint holds more samples than requested";
     else if (fi.samples && (fi.seed != seed || fi.randomization != randomization))
         mismatch = "the checkpoint was rendered with another seed or randomization";
+    else if (fi.samples && fi.sampler_kind != ((frame_flags & RP_FRAME_INDEPENDENT_SAMPLER) ? uint32_t(RP_SAMPLER_INDEPENDENT)
+                                                                                             : uint32_t(RP_SAMPLER_SOBOL)))
+        mismatch = "the checkpoint holds samples of another sampler";
     if (!mismatch.empty()) {
         rp_film_destroy(film);
         return fail("checkpoint does not match the scene or sampler: " + mismatch);
@@ -174,6 +177,7 @@ int render(const rp_scene_desc &desc, const Options &o, uint32_t spp, uint64_t s
         return status_exit(st, "resume");
     }
     const uint64_t batch = o.batch ? o.batch : (o.checkpoint.empty() ? spp : std::max<uint64_t>(1, spp / 16));
+    rp_film_set_job(film, spp, batch); // recorded in checkpoints; spp >= the film's samples was checked above
     rp_frame_stats total{};
     auto last_write = std::chrono::steady_clock::now();
     while (fi.samples < spp) {

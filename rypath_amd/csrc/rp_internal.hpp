@@ -138,17 +138,25 @@ std::string validate_film(const rp_film &f);
 } // namespace rp
 
 // Running raw-XYZ film (include/rp_io.h): the compensated per-pixel sums of
-// the samples [0, samples), bound to one scene, seed and randomization.
+// the samples [0, samples), bound to one scene, seed, sampler and
+// randomization, with the settings a checkpoint records beside it.
 struct rp_film {
     uint32_t width = 0, height = 0;
     uint64_t samples = 0;
     uint64_t seed = 0;
     uint32_t randomization = 0;
+    uint32_t sampler = RP_SAMPLER_SOBOL; // rp_sampler_kind of the samples held
+    uint32_t model = 1;                  // film model: 0 demo (scalar demo transport), 1 PBRT
+    bool packet = true;                  // unit-weight reconstruction sums (hero transport)
+    uint64_t job_samples = 0, batch_size = 0; // the render job's settings (0 = unset)
+    int32_t material_seed = 0;
+    double exposure = 1.0, max_component = __builtin_inf();
     std::array<uint8_t, 32> identity{}; // scene identity (valid when samples > 0)
     std::vector<double> planes;         // 6 * width * height: sum X, corr X, sum Y, corr Y, sum Z, corr Z
     struct Segment {
         uint64_t begin, end;
-        std::string device;
+        std::string build, device, architecture, os;
     };
-    std::vector<Segment> segments; // provenance: sample ranges and the device that rendered them
+    std::vector<Segment> segments; // provenance: sample ranges and what rendered them
+    uint32_t format = 0;           // checkpoint format the film was read from (0: created here)
 };

@@ -105,7 +105,7 @@ struct rp_scene {
     std::shared_ptr<rp::FrameSession> frames;
     std::array<uint8_t, 32> identity{}; // scene_identity(desc), computed on first use
     bool has_identity = false;
-    std::string device_name;            // provenance of film segments
+    std::string device_name, device_arch; // provenance of film segments
     ~rp_scene();
 };
 
@@ -675,7 +675,9 @@ std::array<uint8_t, 32> identity_locked(rp_scene *s) {
 // Whether film f can continue scene s with (seed, randomization): the
 // resume contract of src/render_job.cpp:125-140 (size, sampler, identity).
 // The caller holds s->exec.
-bool film_compatible(rp_scene *s, const rp_film *f, uint64_t seed, uint32_t randomization, const char *who) {
+// sampler: the rp_sampler_kind about to render, or UINT32_MAX to leave it unchecked
+bool film_compatible(rp_scene *s, const rp_film *f, uint64_t seed, uint32_t randomization, const char *who,
+                     uint32_t sampler = UINT32_MAX) {
     const rp_camera &cam = s->host.desc.camera;
     if (f->width != cam.width || f->height != cam.height) {
         set_error(std::string(who) + ": the film is " + std::to_string(f->width) + "x" + std::to_string(f->height) +
@@ -685,6 +687,10 @@ bool film_compatible(rp_scene *s, const rp_film *f, uint64_t seed, uint32_t rand
     if (f->samples == 0) return true;
     if (seed != f->seed || randomization != f->randomization) {
         set_error(std::string(who) + ": seed and randomization must match the film's");
+        return false;
+    }
+    if (sampler != UINT32_MAX && sampler != f->sampler) {
+        set_error(std::string(who) + ": the film holds samples of another sampler");
         return false;
     }
     if (identity_locked(s) != f->identity) {
@@ -1148,7 +1154,14 @@ int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, floa
     const rp_camera &cam = s->host.desc.camera;
     try {
         std::lock_guard<std::mutex> lock(s->exec);
-        if (!film_compatible(s, f, cfg->seed, cfg->randomization, "rp_film_accumulate")) return RP_INVALID;
+        const uint32_t sampler =
+            (cfg->flags & RP_FRAME_INDEPENDENT_SAMPLER) ? uint32_t(RP_SAMPLER_INDEPENDENT) : uint32_t(RP_SAMPLER_SOBOL);
+        if (!film_compatible(s, f, cfg->seed, cfg->randomization, "rp_film_accumulate", sampler)) return RP_INVALID;
+        if (f->job_samples && f->samples + cfg->spp > f->job_samples) {
+            set_error("rp_film_accumulate: the frame would exceed the film's job of " + std::to_string(f->job_samples) +
+                      " samples");
+            return RP_INVALID;
+        }
         if (cfg->sample_begin != f->samples) {
             set_error("rp_film_accumulate: sample_begin must equal the film's sample count " +
                       std::to_string(f->samples));
@@ -1159,7 +1172,8 @@ int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, floa
         if (s->device_name.empty()) {
             hipDeviceProp_t prop{};
             HIP_CHECK(hipGetDeviceProperties(&prop, s->device));
-            s->device_name = std::string(prop.name) + " " + prop.gcnArchName;
+            s->device_name = prop.name;
+            s->device_arch = prop.gcnArchName;
         }
         const uint32_t npx = cam.width * cam.height;
         DevBuf<float> out;
@@ -1174,11 +1188,18 @@ int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, floa
         f->samples = carry.samples;
         f->seed = cfg->seed;
         f->randomization = cfg->randomization;
+        f->sampler = sampler;
+        f->model = s->host.desc.transport == RP_TRANSPORT_SCALAR_DEMO ? 0u : 1u;
+        f->packet = s->host.desc.transport == RP_TRANSPORT_HERO;
         f->identity = id;
-        if (!f->segments.empty() && f->segments.back().device == s->device_name && f->segments.back().end == begin)
-            f->segments.back().end = f->samples;
+        const rp_film::Segment seg{begin, f->samples, "rypath_amd C ABI " + std::to_string(RP_API_VERSION),
+                                   s->device_name, s->device_arch, "Linux"};
+        rp_film::Segment *last = f->segments.empty() ? nullptr : &f->segments.back();
+        if (last && last->end == begin && last->build == seg.build && last->device == seg.device &&
+            last->architecture == seg.architecture && last->os == seg.os)
+            last->end = f->samples;
         else
-            f->segments.push_back({begin, f->samples, s->device_name});
+            f->segments.push_back(seg);
         return RP_SUCCESS;
     } catch (const std::exception &e) {
         return fail("rp_film_accumulate", e);

@@ -1,7 +1,7 @@
 """Shared helpers for derivative checks of the inverse-rendering scene: write
 parameter values into a scene description's tables (the host-side meaning of
-rp_set_params) and estimate directional derivatives by central differences
-over several independent sample sets."""
+rp_set_params) and estimate directional derivatives by central differences,
+over several independent sample sets or with the sampling frozen."""
 import numpy as np
 
 
@@ -49,3 +49,27 @@ def direction(values, group, seed):
 def mean_se(x):
     x = np.asarray(x, np.float64)
     return float(x.mean()), float(x.std(ddof=1) / np.sqrt(len(x)))
+
+
+def frozen_sampling_differences(orc, dp, synth, w, v, h, spp, seed):
+    """Central difference (L(theta + h v) - L(theta - h v)) / 2h of the loss
+    L = sum(w * film) of the oracle's forward render, with every sampling
+    decision and density (BSDF and free-flight sampling, hero-lane pdfs, MIS
+    weights) taken at theta (or_scene_set_sampling) and only the evaluated
+    values moved.  Every path stays the same, so the quotient has no
+    discontinuity noise, and since the estimator's expectation is the
+    rendered integral at every parameter value, its derivative is the
+    derivative of the image.  Returns (difference quotient, base scene)."""
+    dp.restore()
+    base = orc.scene(synth.desc)
+    losses = []
+    for sgn in (+1, -1):
+        dp.apply((dp.values + sgn * h * v).astype(np.float32))
+        s = orc.scene(synth.desc)
+        s.set_sampling(base)
+        img, _ = s.render_frame(spp, seed=seed)
+        s.set_sampling(None)
+        del s
+        losses.append(float((img.astype(np.float64) * w).sum()))
+    dp.restore()
+    return (losses[0] - losses[1]) / (2 * h), base

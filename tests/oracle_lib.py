@@ -52,6 +52,7 @@ class Oracle:
             "or_intersect_triangle": (C.c_int, [C.POINTER(f32), C.POINTER(f32), f32, C.POINTER(f32), C.POINTER(f32)]),
             "or_scene_create": (C.c_void_p, [C.POINTER(rp.SceneDesc)]),
             "or_scene_destroy": (None, [C.c_void_p]),
+            "or_scene_set_sampling": (None, [C.c_void_p, C.c_void_p]),
             "or_light_count": (u32, [C.c_void_p]),
             "or_light_table": (None, [C.c_void_p] + [C.c_void_p] * 6),
             "or_light_node_count": (u32, [C.c_void_p]),
@@ -130,6 +131,12 @@ class OracleScene:
         if getattr(self, "h", None):
             self.orc.L.or_scene_destroy(self.h)
             self.h = None
+
+    def set_sampling(self, other):
+        """Test switch: `other` (an oracle scene of the same geometry, or None)
+        drives this scene's sampling decisions and densities."""
+        self._sampling = other  # keeps it alive while referenced
+        self.orc.L.or_scene_set_sampling(self.h, other.h if other is not None else None)
 
     def light_table(self):
         n = self.orc.L.or_light_count(self.h)

@@ -133,6 +133,42 @@ def test_checkpoint_resume_equals_uninterrupted(rp, gpu_available, tmp_path):
     assert "another scene" in str(e.value)
 
 
+def test_checkpoint_records_transport_and_sampler(rp, gpu_available, tmp_path):
+    """Format-3 files written after accumulation carry the film model, packet
+    flag and sampler of the transport that rendered them, and a film of one
+    sampler does not continue with the other."""
+    from tests.test_checkpoint import decode3
+
+    demo = rp.Scene(rp.Synthetic(rp.SYNTH_DEMO, width=24, height=16).desc)
+    film = rp.Film(24, 16)
+    film.set_job(8, 4)
+    cfg = rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, seed=3, randomization=rp.RANDOMIZE_NONE,
+                          flags=rp.FRAME_INDEPENDENT_SAMPLER)
+    film.accumulate(demo, cfg)
+    film.write(tmp_path / "demo.ckpt")
+    d = decode3((tmp_path / "demo.ckpt").read_bytes())
+    assert (d["sampler"], d["model"], d["packet"], d["job"], d["batch"]) == (0, 0, 0, 8, 4)
+    assert (d["pixels"]["count"] == 4).all() and (d["pixels"]["w"] == 0).all()
+    seg = d["segments"][0]
+    assert seg[:2] == (0, 4) and seg[2].startswith("rypath_amd") and "gfx950" in seg[4]
+    with pytest.raises(rp.RypathError) as e:  # Sobol samples onto an independent-sampler film
+        film.accumulate(demo, rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, sample_begin=4, seed=3,
+                                              randomization=rp.RANDOMIZE_NONE))
+    assert "sampler" in str(e.value)
+    with pytest.raises(rp.RypathError):  # beyond the job's 8 samples
+        film.accumulate(demo, rp.frame_config(8, mode=rp.MODE_MEGAKERNEL, sample_begin=4, seed=3,
+                                              randomization=rp.RANDOMIZE_NONE, flags=rp.FRAME_INDEPENDENT_SAMPLER))
+    cornell = rp.Scene(rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=24, height=16).desc)
+    hero = rp.Film(24, 16)
+    hero.accumulate(cornell, rp.frame_config(3, mode=rp.MODE_WAVEFRONT, seed=3))
+    hero.write(tmp_path / "hero.ckpt")
+    d = decode3((tmp_path / "hero.ckpt").read_bytes())
+    assert (d["sampler"], d["model"], d["packet"], d["job"]) == (1, 1, 1, 3)
+    assert (d["pixels"]["w"][:, 0] == 3).all()
+    back = rp.Film.read(tmp_path / "hero.ckpt")
+    np.testing.assert_array_equal(back.resolve(), hero.resolve())
+
+
 def test_cli_checkpoint_and_resume(gpu_available, tmp_path):
     ck = tmp_path / "run.ckpt"
     full = tmp_path / "full.exr"

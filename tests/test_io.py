@@ -372,17 +372,20 @@ def test_cli_resume_rejects_a_checkpoint_of_another_film(tmp_path):
     """A complete checkpoint (as many samples as requested) of another film
     size or sampler is rejected before anything renders or is written; the
     demo renders 256x192 by default without Sobol randomization."""
-    from tests.test_checkpoint import encode, sample_film
+    from tests.test_checkpoint import encode3, sample_film
 
-    for w, h, samples, seed, rnd, note in [(5, 3, 4, 1, 0, "the checkpoint's film is 5x3"),
-                                           (256, 192, 4, 7, 0, "another seed or randomization"),
-                                           (256, 192, 4, 1, 2, "another seed or randomization"),
-                                           (256, 192, 9, 1, 0, "more samples than requested")]:
+    for w, h, samples, seed, rnd, sampler, note in [
+            (5, 3, 4, 1, 0, 0, "the checkpoint's film is 5x3"),
+            (256, 192, 4, 7, 0, 0, "another seed or randomization"),
+            (256, 192, 4, 1, 2, 0, "another seed or randomization"),
+            (256, 192, 4, 1, 0, 1, "another sampler"),
+            (256, 192, 9, 1, 0, 0, "more samples than requested")]:
         _, _, _, _, _, ident, _, _ = sample_film()
         planes = np.ones((6, w * h)) * samples
         planes[1::2] = 0
-        ckpt = tmp_path / f"c{w}_{seed}_{samples}.ckpt"
-        ckpt.write_bytes(encode(w, h, samples, seed, rnd, ident, [(0, samples, "MI355X gfx950")], planes))
+        ckpt = tmp_path / f"c{w}_{seed}_{samples}_{rnd}_{sampler}.ckpt"
+        segs = [(0, samples, "b", "MI355X", "gfx950", "Linux")]
+        ckpt.write_bytes(encode3(w, h, samples, seed, rnd, ident, segs, planes, sampler=sampler, model=0, packet=0))
         out = tmp_path / "x.exr"
         r = run_cli("--render-demo", "--output", out, "--samples", 4, "--seed", 1, "--resume", ckpt)
         assert r.returncode == 1, r.stderr
