@@ -111,8 +111,9 @@ int rp_film_resolve(const rp_film *film, float *xyz_host);
 int rp_film_get_info(const rp_film *film, rp_film_info *info);
 /* Records the render job's total samples and batch size, which a checkpoint
  * stores as the reference's RenderSettings.samples and batch_size
- * (src/checkpoint.cpp:87-89); unset, the film's own sample count is
- * written.  job_samples must be at least the film's samples. */
+ * (src/checkpoint.cpp:87-89); unset, or once the film holds more samples,
+ * the film's own sample count is written as the job's.  job_samples must be
+ * at least the film's samples. */
 int rp_film_set_job(rp_film *film, uint64_t job_samples, uint64_t batch_size);
 int rp_checkpoint_write(const rp_film *film, const char *path);
 int rp_checkpoint_read(const char *path, rp_film **out);

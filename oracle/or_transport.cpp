@@ -2476,8 +2476,11 @@ int or_sample(void *sp, const rp_request *rq, rp_path_record *out, uint32_t thre
 
 // Full frame: per-pixel mean XYZ of samples [sample_begin, sample_begin+spp),
 // Neumaier-compensated double sums in increasing sample order (src/render.cpp:2326-2407).
-int or_render_frame_flags(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
-                          float *xyz, uint32_t threads, uint64_t *rays, uint32_t flags) {
+} // extern "C"
+
+template <typename Out>
+static int render_frame_impl(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                             Out *xyz, uint32_t threads, uint64_t *rays, uint32_t flags) {
     auto *s = static_cast<Scene *>(sp);
     const bool scalar = s->d.transport != RP_TRANSPORT_HERO;
     const bool independent = (flags & RP_FRAME_INDEPENDENT_SAMPLER) != 0;
@@ -2513,7 +2516,7 @@ int or_render_frame_flags(void *sp, uint64_t seed, uint32_t randomization, uint6
                         sum[ch] = t;
                     }
                 }
-                for (int ch = 0; ch < 3; ++ch) xyz[3 * size_t(pixel) + ch] = float((sum[ch] + corr[ch]) / double(spp));
+                for (int ch = 0; ch < 3; ++ch) xyz[3 * size_t(pixel) + ch] = Out((sum[ch] + corr[ch]) / double(spp));
             }
         }
         ray_count += local_rays;
@@ -2524,6 +2527,19 @@ int or_render_frame_flags(void *sp, uint64_t seed, uint32_t randomization, uint6
     for (auto &t : pool) t.join();
     if (rays) *rays = ray_count.load();
     return 0;
+}
+
+extern "C" {
+
+int or_render_frame_flags(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                          float *xyz, uint32_t threads, uint64_t *rays, uint32_t flags) {
+    return render_frame_impl(sp, seed, randomization, sample_begin, spp, xyz, threads, rays, flags);
+}
+
+// The film's per-pixel means in binary64 (no final rounding to binary32).
+int or_render_frame_f64(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,
+                        double *xyz, uint32_t threads, uint64_t *rays, uint32_t flags) {
+    return render_frame_impl(sp, seed, randomization, sample_begin, spp, xyz, threads, rays, flags);
 }
 
 int or_render_frame(void *sp, uint64_t seed, uint32_t randomization, uint64_t sample_begin, uint32_t spp,

@@ -109,8 +109,9 @@ bool fsync_retry(int fd) {
 }
 
 // The render settings a format-3 file records for the film: the job's
-// sample budget and batch (the film's own count when the job set none).
-uint64_t job_samples(const rp_film &f) { return f.job_samples ? f.job_samples : std::max<uint64_t>(f.samples, 1); }
+// sample budget and batch (the film's own count when the job set none or the
+// film has grown past it).
+uint64_t job_samples(const rp_film &f) { return std::max<uint64_t>(std::max(f.job_samples, f.samples), 1); }
 uint64_t job_batch(const rp_film &f) { return f.batch_size ? f.batch_size : std::max<uint64_t>(f.samples, 1); }
 
 // Format 3 metadata (src/checkpoint.cpp:85-107).
@@ -369,7 +370,6 @@ std::string validate_film(const rp_film &f) {
     if (f.planes.size() != 6 * npx) return "film planes do not match its dimensions";
     if (f.randomization > RP_RANDOMIZE_OWEN) return "invalid film sampler randomization";
     if (f.sampler != RP_SAMPLER_INDEPENDENT && f.sampler != RP_SAMPLER_SOBOL) return "invalid film sampler";
-    if (f.job_samples && f.samples > f.job_samples) return "the film holds more samples than its job";
     if (f.segments.size() > kMaxSegments) return "film exceeds 1024 provenance segments";
     uint64_t end = 0;
     for (const auto &s : f.segments) {

@@ -1157,11 +1157,6 @@ int rp_film_accumulate(rp_scene *s, rp_film *f, const rp_frame_config *cfg, floa
         const uint32_t sampler =
             (cfg->flags & RP_FRAME_INDEPENDENT_SAMPLER) ? uint32_t(RP_SAMPLER_INDEPENDENT) : uint32_t(RP_SAMPLER_SOBOL);
         if (!film_compatible(s, f, cfg->seed, cfg->randomization, "rp_film_accumulate", sampler)) return RP_INVALID;
-        if (f->job_samples && f->samples + cfg->spp > f->job_samples) {
-            set_error("rp_film_accumulate: the frame would exceed the film's job of " + std::to_string(f->job_samples) +
-                      " samples");
-            return RP_INVALID;
-        }
         if (cfg->sample_begin != f->samples) {
             set_error("rp_film_accumulate: sample_begin must equal the film's sample count " +
                       std::to_string(f->samples));

@@ -53,7 +53,8 @@ def mean_se(x):
 
 def frozen_sampling_differences(orc, dp, synth, w, v, h, spp, seed):
     """Central difference (L(theta + h v) - L(theta - h v)) / 2h of the loss
-    L = sum(w * film) of the oracle's forward render, with every sampling
+    L = sum(w * film) of the oracle's forward render (binary64 film means),
+    with every sampling
     decision and density (BSDF and free-flight sampling, hero-lane pdfs, MIS
     weights) taken at theta (or_scene_set_sampling) and only the evaluated
     values moved.  Every path stays the same, so the quotient has no
@@ -67,9 +68,21 @@ def frozen_sampling_differences(orc, dp, synth, w, v, h, spp, seed):
         dp.apply((dp.values + sgn * h * v).astype(np.float32))
         s = orc.scene(synth.desc)
         s.set_sampling(base)
-        img, _ = s.render_frame(spp, seed=seed)
+        img, _ = s.render_frame_f64(spp, seed=seed)
         s.set_sampling(None)
         del s
         losses.append(float((img.astype(np.float64) * w).sum()))
     dp.restore()
     return (losses[0] - losses[1]) / (2 * h), base
+
+
+def steepest_relative_direction(values, group, pilot):
+    """Direction for a relative step in one group: v_i = sign(pilot_i) |theta_i|,
+    so theta + h v moves every parameter of the group by the fraction h of its
+    value, each the way that `pilot` (a gradient from an independent sample
+    set) says increases the loss.  In a random direction a group's terms can
+    cancel to a directional derivative far below what the group carries."""
+    sl = GROUPS[group]
+    v = np.zeros(len(values))
+    v[sl] = np.sign(pilot[sl]) * np.abs(values[sl])
+    return v

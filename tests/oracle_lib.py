@@ -69,6 +69,8 @@ class Oracle:
             "or_render_frame": (C.c_int, [C.c_void_p, u64, u32, u64, u32, C.c_void_p, u32, C.POINTER(u64)]),
             "or_render_frame_flags": (C.c_int, [C.c_void_p, u64, u32, u64, u32, C.c_void_p, u32, C.POINTER(u64),
                                                 u32]),
+            "or_render_frame_f64": (C.c_int, [C.c_void_p, u64, u32, u64, u32, C.c_void_p, u32, C.POINTER(u64),
+                                              u32]),
             "or_sample_scalar": (C.c_int, [C.c_void_p, C.POINTER(rp.Request), C.c_void_p, u32]),
             "or_set_roulette": (None, [C.c_int]),
             "or_plant_slip": (None, [C.c_int]),
@@ -183,6 +185,14 @@ class OracleScene:
         out = np.zeros(n, dtype=rp.RECORD_DTYPE)
         self.orc.L.or_sample(self.h, C.byref(req), out.ctypes.data_as(C.c_void_p), threads)
         return out
+
+    def render_frame_f64(self, spp, seed=1, randomization=2, sample_begin=0, threads=0, flags=0):
+        """Film means in binary64 (or_render_frame_f64)."""
+        out = np.zeros((self.height, self.width, 3), dtype=np.float64)
+        rays = C.c_uint64()
+        self.orc.L.or_render_frame_f64(self.h, seed, randomization, sample_begin, spp,
+                                       out.ctypes.data_as(C.c_void_p), threads, C.byref(rays), flags)
+        return out, rays.value
 
     def render_frame(self, spp, seed=1, randomization=2, sample_begin=0, threads=0, flags=0):
         out = np.zeros((self.height, self.width, 3), dtype=np.float32)

@@ -138,7 +138,8 @@ def test_adjoint_within_2_percent_of_central_differences_at_1e_3(rp, orc, invers
     """The 2 % gradient target at the stated step h = 1e-3, per parameter
     group (walls: 48 reflectance bins, roughness: 2 conductors, sigma_s: 8
     medium bins).  The GPU path-replay adjoint of L = sum(w * film) along a
-    random direction in the group against central differences of the
+    direction that moves every parameter of the group by 0.1 % of its value
+    (signs from an independent pilot adjoint) against central differences of the
     forward render with common random numbers in the strongest form: the
     oracle renders theta +- h v with every sampling decision and density
     frozen at theta (grad_util.frozen_sampling_differences), so each path
@@ -151,7 +152,10 @@ def test_adjoint_within_2_percent_of_central_differences_at_1e_3(rp, orc, invers
     params, values = rp.inverse_params(synth)
     dp = grad_util.DescParams(rp, synth, params, values)
     w = np.random.default_rng(1).uniform(0.5, 1.5, (32, 32, 3)).astype(np.float32)
-    v = grad_util.direction(values, group, 4)
+    scene.set_params(params, values)
+    pilot, _ = scene.backward(rp.frame_config(64, mode=rp.MODE_WAVEFRONT, seed=3, flags=rp.FRAME_NO_ROULETTE),
+                              params, w)
+    v = grad_util.steepest_relative_direction(values, group, pilot)
     orc.L.or_set_roulette.argtypes = [C.c_int]
     orc.L.or_set_roulette(0)
     try:

@@ -155,9 +155,11 @@ def test_checkpoint_records_transport_and_sampler(rp, gpu_available, tmp_path):
         film.accumulate(demo, rp.frame_config(4, mode=rp.MODE_MEGAKERNEL, sample_begin=4, seed=3,
                                               randomization=rp.RANDOMIZE_NONE))
     assert "sampler" in str(e.value)
-    with pytest.raises(rp.RypathError):  # beyond the job's 8 samples
-        film.accumulate(demo, rp.frame_config(8, mode=rp.MODE_MEGAKERNEL, sample_begin=4, seed=3,
-                                              randomization=rp.RANDOMIZE_NONE, flags=rp.FRAME_INDEPENDENT_SAMPLER))
+    film.accumulate(demo, rp.frame_config(8, mode=rp.MODE_MEGAKERNEL, sample_begin=4, seed=3,
+                                          randomization=rp.RANDOMIZE_NONE, flags=rp.FRAME_INDEPENDENT_SAMPLER))
+    film.write(tmp_path / "demo12.ckpt")  # grown past its job of 8: the count is written as the job
+    d = decode3((tmp_path / "demo12.ckpt").read_bytes())
+    assert (d["job"], d["segments"][0][:2]) == (12, (0, 12))
     cornell = rp.Scene(rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=24, height=16).desc)
     hero = rp.Film(24, 16)
     hero.accumulate(cornell, rp.frame_config(3, mode=rp.MODE_WAVEFRONT, seed=3))

@@ -86,14 +86,17 @@ def test_reflectance_derivative_exact_without_roulette(orc, rp, no_roulette):
 
 @pytest.mark.parametrize("group", ["walls", "roughness", "sigma_s"])
 def test_forward_mode_matches_frozen_sampling_differences_at_1e_3(orc, rp, no_roulette, group):
-    """At the stated step h = 1e-3, central differences of the forward render
-    with the sampling frozen at the base parameters (every path fixed, see
+    """At the stated step h = 1e-3 (every parameter of the group moved by
+    0.1 % of its value, along the signs of an independent pilot gradient),
+    central differences of the forward render with the sampling frozen at
+    the base parameters (every path fixed, see
     grad_util.frozen_sampling_differences) equal the forward-mode derivative
     of the detached estimator to within 2e-3 relative, for each group: the
     2 % gradient target met with a margin of ten."""
     synth, params, values, dp = inverse_scene(rp)
     w = np.random.default_rng(1).uniform(0.5, 1.5, (W, W, 3)).astype(np.float32)
-    v = grad_util.direction(values, group, 4)
+    pilot = oracle_lib.oracle_gradient(orc, orc.scene(synth.desc), params, w, spp=64, seed=3)
+    v = grad_util.steepest_relative_direction(values, group, pilot)
     fd, base = grad_util.frozen_sampling_differences(orc, dp, synth, w, v, 1e-3, 256, 7)
     ad = float(oracle_lib.oracle_gradient(orc, base, params, w, spp=256, seed=7) @ v)
     print(f"[{group}] forward mode {ad:.6e} frozen-sampling central difference {fd:.6e} "
