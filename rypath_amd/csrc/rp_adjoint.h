@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, 
     const uint32_t np = G.nparams;
     for (uint32_t q = threadIdx.x; q < np; q += kWfBlock) srow[q] = 0.0f;
     __syncthreads();
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, kWfBlock);
     AdjointHook hk;
     hk.G = &G;
@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_tail(Scene S, PathParams P, Gr
     extern __shared__ float sgrad[];
     const uint32_t np = G.nparams;
     clear_block_grad<kWfBlock>(sgrad, np);
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, kWfBlock);
     AdjointHook hk;
     hk.G = &G;

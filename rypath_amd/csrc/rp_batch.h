@@ -86,7 +86,7 @@ __global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scene
                                                           unsigned long long *work, uint64_t total) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, BS);
     const uint32_t lane = threadIdx.x & 63;
     unsigned long long ext = 0, sh = 0, failed = 0;
@@ -221,7 +221,7 @@ template <int BS>
 __global__ void __launch_bounds__(BS) k_pool_extend(PoolView V, uint32_t p) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, BS);
     const uint32_t n = V.c->qn[p], nb = (n + BS - 1) / BS;
     if (blockIdx.x >= nb) return;
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(BS) k_pool_shadow(PoolView V, uint32_t p) {
     __shared__ float s_t[kShortStack * BS];
     const uint32_t n = V.c->sn, nb = (n + BS - 1) / BS;
     if (blockIdx.x >= nb) return;
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, BS);
     const uint32_t i = blockIdx.x * BS + threadIdx.x;
     const bool valid = i < n;
@@ -447,7 +447,7 @@ template <bool kInterfaces>
 __global__ void __launch_bounds__(256) k_pool_shadow_mixed(PoolView V, uint32_t p) {
     __shared__ uint32_t s_node[kShortStack * 256];
     __shared__ float s_t[kShortStack * 256];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, 256);
     const uint32_t n = V.c->ovs;
     uint32_t rays = 0;
@@ -504,7 +504,7 @@ template <bool kInterfaces>
 __global__ void __launch_bounds__(kWfBlock) k_pool_tail(PoolView V, uint32_t p, uint32_t n) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, kWfBlock);
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t ext = 0, shr = 0, failed = 0;

@@ -510,13 +510,20 @@ static_assert(kShortStack >= 1 && kShortStack <= kMaxStack, "short stack size");
 typedef __attribute__((address_space(3))) uint32_t LdsU32;
 typedef __attribute__((address_space(3))) float LdsF32;
 
+// Per-lane overflow entries, a private array of their own: kept apart from
+// the Stack's scalar fields so that those stay in registers (a struct that
+// holds a dynamically indexed array lives whole in scratch, stack pointer
+// included, and every push and pop then waits on scratch).
+struct SpillStack {
+    uint32_t node[kSpillStack];
+    float t[kSpillStack];
+};
 struct Stack {
     LdsU32 *lds_node; // this lane's column of [kShortStack][block]
     LdsF32 *lds_t;
     int stride;       // block size
     int sp;
-    uint32_t spill_node[kSpillStack];
-    float spill_t[kSpillStack];
+    SpillStack *spill; // set where the Stack is declared (RP_STACK)
     // node, t: the block's __shared__ arrays of kShortStack * block entries.
     RP_DEV void init(uint32_t *node, float *t, int block) {
         lds_node = (LdsU32 *)(node + threadIdx.x);
@@ -529,8 +536,8 @@ struct Stack {
             lds_node[sp * stride] = n;
             lds_t[sp * stride] = t;
         } else {
-            spill_node[sp - kShortStack] = n;
-            spill_t[sp - kShortStack] = t;
+            spill->node[sp - kShortStack] = n;
+            spill->t[sp - kShortStack] = t;
         }
         ++sp;
     }
@@ -540,11 +547,16 @@ struct Stack {
             n = lds_node[sp * stride];
             t = lds_t[sp * stride];
         } else {
-            n = spill_node[sp - kShortStack];
-            t = spill_t[sp - kShortStack];
+            n = spill->node[sp - kShortStack];
+            t = spill->t[sp - kShortStack];
         }
     }
 };
+// Declares traversal stack `name` with its overflow array.
+#define RP_STACK(name)                                                                                                 \
+    ::rpd::Stack name;                                                                                                 \
+    ::rpd::SpillStack name##_spill;                                                                                    \
+    name.spill = &name##_spill
 
 RP_DEV float byte_of(uint32_t w, int i) { return float((w >> (8 * i)) & 0xffu); }
 

@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(B) k_records(rpd::Scene S, rpd::PathParams P, 
                                                rp_path_record *out) {
     __shared__ uint32_t s_node[rpd::kShortStack * B];
     __shared__ float s_t[rpd::kShortStack * B];
-    rpd::Stack st;
+    RP_STACK(st);
     LdsStack<B>::init(st, s_node, s_t);
     uint64_t lane = uint64_t(blockIdx.x) * B + threadIdx.x;
     if (lane >= n) return;
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_mega(rpd::Scene S, rpd::PathParams P
                                                   uint64_t n, float *xyz, unsigned long long *counters) {
     __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
     __shared__ float s_t[rpd::kShortStack * kBlock];
-    rpd::Stack st;
+    RP_STACK(st);
     LdsStack<kBlock>::init(st, s_node, s_t);
     uint64_t lane = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     uint32_t ext = 0, sh = 0, failed = 0;
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_scalar_mega(rpd::Scene S, rpd::PathP
                                                          float *xyz, unsigned long long *counters) {
     __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
     __shared__ float s_t[rpd::kShortStack * kBlock];
-    rpd::Stack st;
+    RP_STACK(st);
     LdsStack<kBlock>::init(st, s_node, s_t);
     uint64_t lane = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     uint32_t ext = 0, sh = 0, failed = 0;
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(B) k_scalar_records(rpd::Scene S, rpd::PathPar
                                                       uint32_t independent, rp_render_sample *out) {
     __shared__ uint32_t s_node[rpd::kShortStack * B];
     __shared__ float s_t[rpd::kShortStack * B];
-    rpd::Stack st;
+    RP_STACK(st);
     LdsStack<B>::init(st, s_node, s_t);
     uint64_t lane = uint64_t(blockIdx.x) * B + threadIdx.x;
     if (lane >= n) return;
@@ -310,7 +310,7 @@ __global__ void k_film_finish(uint32_t npx, uint64_t samples, const double *film
 __global__ void __launch_bounds__(kBlock) k_trace_closest(rpd::Scene S, const rp_ray *rays, uint64_t n, rp_hit *hits) {
     __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
     __shared__ float s_t[rpd::kShortStack * kBlock];
-    rpd::Stack st;
+    RP_STACK(st);
     LdsStack<kBlock>::init(st, s_node, s_t);
     uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) k_trace_closest(rpd::Scene S, const rp
 __global__ void __launch_bounds__(kBlock) k_trace_any(rpd::Scene S, const rp_ray *rays, uint64_t n, uint8_t *occ) {
     __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
     __shared__ float s_t[rpd::kShortStack * kBlock];
-    rpd::Stack st;
+    RP_STACK(st);
     LdsStack<kBlock>::init(st, s_node, s_t);
     uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;

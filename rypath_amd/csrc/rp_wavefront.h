@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
                                                           unsigned long long *counters) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, kWfBlock);
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t nodes = 0, tris = 0, spills = 0;
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
                                                           unsigned long long *counters) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, kWfBlock);
     uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     TraversalCounts tc = {0, 0, 0};
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_tail(Scene S, PathParams P, WfS
                                                        uint32_t npx, uint64_t sample_begin) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
-    Stack st;
+    RP_STACK(st);
     st.init(s_node, s_t, kWfBlock);
     const uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
     uint32_t ext = 0, shr = 0;
