@@ -893,6 +893,69 @@ RP_DEV Eval eval_lane(const Lane &l, F3 wo, F3 wi) {
     }
     return e;
 }
+// eval_lane for the four hero lanes of one material: the terms that depend
+// only on the directions and the roughness (half vector, Smith Lambda, D,
+// the denominators) once, the wavelength-dependent factors (Fresnel, albedo,
+// a dielectric's refracted half vector) per lane, and a lane whose optical
+// constants equal lane 0's takes lane 0's result.  Each e[k] is bit for bit
+// eval_lane(l[k], wo, wi): the same operations on the same values.
+RP_DEV void eval_hero(const Lane l[4], F3 wo, F3 wi, Eval e[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e[k] = {0.0f, 0.0f};
+    const uint32_t kind = l[0].kind;
+    if (kind == RP_MATERIAL_DIFFUSE) {
+        if (!(wo.z * wi.z > 0)) return;
+        const float pdf = fabsf(wi.z) * kInvPi;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = {l[k].r * kInvPi, pdf};
+        return;
+    }
+    if (wo.z == 0.0f || wi.z == 0.0f) return;
+    if (kind == RP_MATERIAL_CONDUCTOR) {
+        // a conductor lane is smooth by its roughness alone: all lanes alike
+        if (l[0].smooth || !(wo.z * wi.z > 0)) return;
+        const F3 h = wo + wi;
+        if (dot(h, h) == 0.0f) return;
+        const F3 m = normalize(h);
+        const float c = fabsf(dot(wo, m));
+        const float lo = ggx_lambda(l[0], wo), li = ggx_lambda(l[0], wi), D = ggx_d(l[0], m);
+        const float den = 4.0f * fabsf(wo.z) * fabsf(wi.z) * (1.0f + lo + li);
+        const float pdf = D / (4.0f * fabsf(wo.z) * (1.0f + lo));
+        const float F0 = fresnel_complex(c, l[0].eta, l[0].k);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float F = (k == 0 || (l[k].eta == l[0].eta && l[k].k == l[0].k)) ? F0
+                                                                                    : fresnel_complex(c, l[k].eta, l[k].k);
+            e[k] = {D * F / den, pdf};
+        }
+        return;
+    }
+    if (wo.z * wi.z > 0) { // dielectric reflection: the half vector is the lanes' own
+        const F3 h = wi * 1.0f + wo;
+        if (dot(h, h) == 0.0f) return;
+        F3 m = normalize(h);
+        if (m.z < 0) m = -m;
+        const float dmi = dot(m, wi), dmo = dot(m, wo);
+        if (dmi * wi.z < 0 || dmo * wo.z < 0) return;
+        const float lo = ggx_lambda(l[0], wo), li = ggx_lambda(l[0], wi), D = ggx_d(l[0], m);
+        const float G = 1.0f / (1.0f + lo + li);
+        const float pdf_wm = D / (fabsf(wo.z) * (1.0f + lo)) * fabsf(dmo);
+        const float DG = D * G, den = fabsf(4.0f * wi.z * wo.z), pw = pdf_wm / (4.0f * fabsf(dmo));
+        const float F0 = fresnel_dielectric(dmo, l[0].eta);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (l[k].smooth) continue;
+            const float F = (k == 0 || l[k].eta == l[0].eta) ? F0 : fresnel_dielectric(dmo, l[k].eta);
+            const float R = F, T = 1.0f - F;
+            e[k] = {DG * F / den, pw * (R / (R + T))};
+        }
+        return;
+    }
+    e[0] = eval_lane(l[0], wo, wi);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) e[k] = l[k].eta == l[0].eta ? e[0] : eval_lane(l[k], wo, wi);
+}
+
 struct Sampled {
     F3 wi;
     float pdf, eta;

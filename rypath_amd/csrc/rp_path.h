@@ -309,10 +309,11 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
                     etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
                 }
             } else {
+                Eval ev[4];
+                eval_hero(lanes, wol, hs.wi, ev);
                 for (int k = 0; k < 4; ++k) {
-                    Eval e = eval_lane(lanes[k], wol, hs.wi);
-                    num[k] = e.f * fabsf(hs.wi.z);
-                    pdf[k] = e.pdf;
+                    num[k] = ev[k].f * fabsf(hs.wi.z);
+                    pdf[k] = ev[k].pdf;
                     etas[k] = hs.transmission ? (wol.z > 0 ? lanes[k].eta : 1.0f / lanes[k].eta) : 1.0f;
                 }
                 pdf[0] = hs.pdf;
@@ -347,10 +348,11 @@ RP_DEV bool path_event(const Scene &S, const PathParams &P, PathState &ps, const
             float cosv = fabsf(dot(sp.n, ls.wi));
             float bp[4];
             bool any = false;
+            Eval ev[4];
+            eval_hero(lanes, wol, wil, ev);
             for (int k = 0; k < 4; ++k) {
-                Eval e = eval_lane(lanes[k], wol, wil);
-                bp[k] = e.pdf;
-                sr.val[k] = ((direct_beta[k] * e.f) * (ls.le[k] / ls.pdf)) * cosv;
+                bp[k] = ev[k].pdf;
+                sr.val[k] = ((direct_beta[k] * ev[k].f) * (ls.le[k] / ls.pdf)) * cosv;
                 any = any || sr.val[k] > 0;
             }
             if (any) {
