@@ -44,6 +44,14 @@ struct LightNode {
     uint32_t child, parent, light, flags;
 };
 
+// A light's input to the tables (LightSamplingInput, src/light_sampling.hpp:11-17).
+struct LightInput {
+    double power = 0, bounds_power = -1;
+    float low[3] = {0, 0, 0}, high[3] = {0, 0, 0}, dir[3] = {0, 0, 1};
+    float cos_outer = -1, cos_emission = 0;
+    bool two_sided = true, infinite = false;
+};
+
 struct Scene {
     rp_scene_desc d{};
     std::vector<float> positions;
@@ -63,6 +71,7 @@ struct Scene {
     std::vector<uint32_t> order;
     std::vector<Light> lights;
     std::vector<LightNode> lnodes;   // light BVH (node i + 1 is the first child of an interior node i)
+    std::vector<LightInput> light_inputs; // per light: powers, bounds and cones the tables were built from
     uint32_t env_light = RP_NONE;    // the environment's light id
     float env_pmf = 0;               // its selection probability (context independent)
     std::vector<uint32_t> tri_light; // triangle -> light index or RP_NONE
@@ -627,24 +636,29 @@ static bool slip(Slip v) { return g_slip.load(std::memory_order_relaxed) == v; }
 // 540-640 and 1560-1780: powers, bounds, cones) and build_light_sampling
 // (src/light_sampling.cpp:226-292 unite / cost, 382-571 alias table and BVH),
 // which follow PBRT v4's BVHLightSampler and PowerLightSampler.
-struct LightInput {
-    double power = 0, bounds_power = -1;
-    float low[3] = {0, 0, 0}, high[3] = {0, 0, 0}, dir[3] = {0, 0, 1};
-    float cos_outer = -1, cos_emission = 0;
-    bool two_sided = true, infinite = false;
-};
 
-// Power proxy of a spectrum: the visible-wavelength estimate at u = 1/2 of
-// its integral over nanometres (zero: the plain sum over 360..830 nm).
-static double spectrum_power(const Scene &s, const rp_spectrum &sp) {
-    Wavelengths w = sample_wavelengths(0.5f, 1);
+// Power proxy of a spectrum (light_spectrum_power, src/light_sampling.cpp:17-27):
+// the visible-wavelength estimate at u = 1/2 of its integral over nanometres,
+// with the four wavelengths of the packet sampler (float shifts by 1/4, a
+// shifted value above 1 wrapped; src/packet_spectrum_shared.h:27-58) and the
+// host's binary32 atanh and cosh; zero: the plain sum over 360..830 nm.  Host
+// data, not path arithmetic, so libm is used as the reference's host does.
+static double spectrum_power_pool(const rp_spectrum &sp, const float *pool) {
     double v = 0;
-    for (int k = 0; k < 4; ++k)
-        v += double(spectrum_eval(sp, std::round(w.lambda[k]), s.spectra.data())) / double(w.pdf[k]) / 4.0;
+    for (int k = 0; k < 4; ++k) {
+        float up = 0.5f + float(k) * 0.25f;
+        if (up > 1) up -= 1;
+        float l = 538 - 138.888889f * std::atanh(0.85691062f - 1.82750197f * up);
+        l = std::min(std::max(l, 360.0f), 830.0f);
+        const float c = std::cosh(0.0072f * (l - 538));
+        const float pdf = 0.0039398042f / (c * c);
+        v += double(spectrum_eval(sp, std::round(l), pool)) / pdf / 4;
+    }
     if (v == 0)
-        for (int i = 0; i < 471; ++i) v += double(spectrum_eval(sp, float(360 + i), s.spectra.data()));
+        for (int i = 0; i < 471; ++i) v += double(spectrum_eval(sp, float(360 + i), pool));
     return v;
 }
+static double spectrum_power(const Scene &s, const rp_spectrum &sp) { return spectrum_power_pool(sp, s.spectra.data()); }
 static double spectrum_max(const Scene &s, const rp_spectrum &sp) {
     double m = 0;
     for (int i = 0; i < 471; ++i) m = std::max(m, double(spectrum_eval(sp, float(360 + i), s.spectra.data())));
@@ -754,10 +768,36 @@ static bool rect_f(V3 p00, V3 p10, V3 p01, V3 p11) {
     return dot(e, e) <= 0x1p-40f * std::max(x2, y2) && std::fabs(dot(ex, ey)) <= 0x1p-20f * std::sqrt(x2 * y2);
 }
 
+// The smallest-angle cone about the mean unit normal holding n normals
+// (light_normal_cone, src/light_sampling.cpp:356-381): a degenerate normal or
+// a vanishing mean leaves the full sphere (cos_outer = -1).
+static void normal_cone(LightInput &b, const D3 *ns, int n) {
+    D3 axis = {0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+        const double l = dlen(ns[k]);
+        if (!(l > 0 && std::isfinite(l))) {
+            b.cos_outer = -1;
+            return;
+        }
+        for (int a = 0; a < 3; ++a) axis[a] += ns[k][a] / l;
+    }
+    const double al = dlen(axis);
+    if (!(al > 0)) {
+        b.cos_outer = -1;
+        return;
+    }
+    for (int a = 0; a < 3; ++a) axis[a] /= al;
+    double cmin = 1;
+    for (int k = 0; k < n; ++k) cmin = std::min(cmin, ddot(axis, ns[k]) / dlen(ns[k]));
+    for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a]);
+    b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
+}
+
 static void build_lights(Scene &s) {
     const double pi = 3.14159265358979323846;
     s.lights.clear();
     s.lnodes.clear();
+    s.light_inputs.clear();
     s.env_light = RP_NONE;
     s.tri_light.assign(s.d.triangle_count, RP_NONE);
     std::vector<LightInput> in;
@@ -795,21 +835,7 @@ static void build_lights(Scene &s) {
                 // normal cone of the four corner normals
                 const D3 ns[4] = {dcross(dsub(p[1], p[0]), dsub(p[2], p[0])), dcross(dsub(p[1], p[0]), dsub(p[3], p[1])),
                                   dcross(dsub(p[3], p[2]), dsub(p[2], p[0])), dcross(dsub(p[3], p[2]), dsub(p[3], p[1]))};
-                D3 axis = {0, 0, 0};
-                bool degenerate = false;
-                for (const D3 &n : ns) {
-                    const double l = dlen(n);
-                    if (!(l > 0)) degenerate = true;
-                    else
-                        for (int a = 0; a < 3; ++a) axis[a] += n[a] / l;
-                }
-                const double al = dlen(axis);
-                if (!degenerate && al > 0) {
-                    double cmin = 1;
-                    for (const D3 &n : ns) cmin = std::min(cmin, (ddot(axis, n) / al) / dlen(n));
-                    for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a] / al);
-                    b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
-                }
+                normal_cone(b, ns, 4);
                 Light L{3u, t, t + 1, RP_NONE, 0, 1, 0, float(1.0 / area), c[0], c[1] - c[0], c[2] - c[0], true,
                         {}, {}, {}, 0, 0};
                 quad_axes(L);
@@ -862,6 +888,7 @@ static void build_lights(Scene &s) {
         s.lights.push_back({2u, 0u, RP_NONE, RP_NONE, 0, 1, 0, 0.0f, {}, {}, {}, false, {}, {}, {}, 0, 0});
         in.push_back(b);
     }
+    s.light_inputs = in;
     const size_t n = in.size();
     if (n == 0) return;
     // alias table over the powers (Walker / Vose, small and large stacks)
@@ -2358,6 +2385,69 @@ void or_light_table(void *sp, uint32_t *kind, uint32_t *index, float *pmf, float
     }
 }
 uint32_t or_light_node_count(void *sp) { return uint32_t(static_cast<Scene *>(sp)->lnodes.size()); }
+// What the light tables were built from (one LightSamplingInput per light)
+// and what they hold: alias links, and the light BVH's nodes.
+void or_light_inputs(void *sp, double *power, double *bounds_power, float *low, float *high, float *dir,
+                     float *cos_outer, float *cos_emission, uint8_t *two_sided, uint8_t *infinite) {
+    const auto &in = static_cast<Scene *>(sp)->light_inputs;
+    for (size_t i = 0; i < in.size(); ++i) {
+        power[i] = in[i].power;
+        bounds_power[i] = in[i].bounds_power;
+        for (int a = 0; a < 3; ++a) {
+            low[3 * i + a] = in[i].low[a];
+            high[3 * i + a] = in[i].high[a];
+            dir[3 * i + a] = in[i].dir[a];
+        }
+        cos_outer[i] = in[i].cos_outer;
+        cos_emission[i] = in[i].cos_emission;
+        two_sided[i] = in[i].two_sided ? 1 : 0;
+        infinite[i] = in[i].infinite ? 1 : 0;
+    }
+}
+void or_light_alias(void *sp, uint32_t *alias) {
+    const auto &l = static_cast<Scene *>(sp)->lights;
+    for (size_t i = 0; i < l.size(); ++i) alias[i] = l[i].alias;
+}
+void or_light_nodes(void *sp, float *low, float *high, float *dir, float *power, float *cos_outer, float *cos_emission,
+                    uint32_t *child, uint32_t *parent, uint32_t *light, uint32_t *flags) {
+    const auto &nodes = static_cast<Scene *>(sp)->lnodes;
+    for (size_t k = 0; k < nodes.size(); ++k) {
+        const LightNode &n = nodes[k];
+        for (int a = 0; a < 3; ++a) {
+            low[3 * k + a] = n.low[a];
+            high[3 * k + a] = n.high[a];
+            dir[3 * k + a] = n.dir[a];
+        }
+        power[k] = n.power;
+        cos_outer[k] = n.cos_outer;
+        cos_emission[k] = n.cos_emission;
+        child[k] = n.child;
+        parent[k] = n.parent;
+        light[k] = n.light;
+        flags[k] = n.flags;
+    }
+}
+// Probes of the table helpers: the power proxy of a 471-sample table, the
+// patch rectangle test of p = (p00, p10, p01, p11), the normal cone.
+double or_spectrum_power_table(const float *table) {
+    rp_spectrum sp{};
+    sp.kind = RP_SPECTRUM_TABLE;
+    sp.scale = 1;
+    sp.table = 0;
+    return spectrum_power_pool(sp, table);
+}
+int or_patch_rectangle(const double *p) {
+    const D3 c[4] = {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}, {p[6], p[7], p[8]}, {p[9], p[10], p[11]}};
+    return patch_rectangle(c) ? 1 : 0;
+}
+void or_normal_cone(const double *ns, uint32_t n, float *dir, float *cos_outer) {
+    std::vector<D3> v(n);
+    for (uint32_t i = 0; i < n; ++i) v[i] = {ns[3 * i], ns[3 * i + 1], ns[3 * i + 2]};
+    LightInput b;
+    normal_cone(b, v.data(), int(n));
+    for (int a = 0; a < 3; ++a) dir[a] = b.dir[a];
+    *cos_outer = b.cos_outer;
+}
 // Light selection probes: the light chosen for (p, n, u) with its pmf, and
 // the pmf of a given light.
 // Test switch: `sampling` (an oracle scene of the same geometry, or NULL)

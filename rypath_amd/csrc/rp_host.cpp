@@ -268,51 +268,23 @@ float scene_bounding_radius(const rp_scene_desc &d) {
 // reference builds them.
 namespace {
 
-// The hero wavelength sampler's log and exp (binary32, the device's series).
-float host_exp_f(float x) {
-    if (x != x) return x;
-    if (!(x > -104.0f)) return 0.0f;
-    if (x > 88.72f) return INFINITY;
-    float k = std::floor(x * 1.44269504088896341f + 0.5f);
-    float r = (x - k * 0.693145751953125f) - k * 1.42860682030941723212e-6f;
-    float p = 1.0f + r * (1.0f + r * (0.5f + r * (1.0f / 6.0f + r * (1.0f / 24.0f + r * (1.0f / 120.0f + r * (1.0f / 720.0f + r * (1.0f / 5040.0f)))))));
-    return std::ldexp(p, int(k));
-}
-float host_log_f(float x) {
-    if (x != x || x < 0.0f) return NAN;
-    if (x == 0.0f) return -INFINITY;
-    if (x == INFINITY) return x;
-    int e;
-    float m = std::frexp(x, &e);
-    if (m < 0.70710678118654752f) { m = m * 2.0f; e -= 1; }
-    float s = (m - 1.0f) / (m + 1.0f);
-    float s2 = s * s;
-    float lm = s * (2.0f + s2 * (2.0f / 3.0f + s2 * (2.0f / 5.0f + s2 * (2.0f / 7.0f + s2 * (2.0f / 9.0f)))));
-    float ef = float(e);
-    return ef * 0.693145751953125f + (lm + ef * 1.42860682030941723212e-6f);
-}
-// The four visible-sampled hero wavelengths at u = 1/2 and their densities.
-void visible_half(float lambda[4], float pdf[4]) {
-    for (int k = 0; k < 4; ++k) {
-        uint32_t coord = uint32_t(0.5f * 16777216.0f);
-        float s = float((coord + uint32_t(k) * 4194304u) & 0x00ffffffu) * 0x1p-24f;
-        float a = 0.85691062f - 1.82750197f * s;
-        float at = 0.5f * host_log_f((1.0f + a) / (1.0f - a));
-        float l = 538.0f - 138.888889f * at;
-        l = std::min(std::max(l, 360.0f), 830.0f);
-        float e = host_exp_f(0.0072f * (l - 538.0f));
-        float ch = 0.5f * (e + 1.0f / e);
-        lambda[k] = l;
-        pdf[k] = 0.0039398042f / (ch * ch);
-    }
-}
-// Power proxy of a spectrum: the visible-sampled estimate at u = 1/2 of its
-// integral over nanometres (zero: the plain sum over 360..830 nm).
+// Power proxy of a spectrum (light_spectrum_power, src/light_sampling.cpp:17-27):
+// the visible-wavelength estimate at u = 1/2 of its integral over nanometres,
+// at the packet sampler's four wavelengths (float shifts by 1/4, a shifted
+// value above 1 wrapped; src/packet_spectrum_shared.h:27-58) with the host's
+// binary32 atanh and cosh, as the reference's host computes it; zero: the
+// plain sum over 360..830 nm.
 double spectrum_power(const rp_spectrum &sp, const float *pool) {
-    float lambda[4], pdf[4];
-    visible_half(lambda, pdf);
     double v = 0;
-    for (int k = 0; k < 4; ++k) v += double(spectrum_eval_host(sp, std::round(lambda[k]), pool)) / double(pdf[k]) / 4.0;
+    for (int k = 0; k < 4; ++k) {
+        float up = 0.5f + float(k) * 0.25f;
+        if (up > 1) up -= 1;
+        float l = 538 - 138.888889f * std::atanh(0.85691062f - 1.82750197f * up);
+        l = std::min(std::max(l, 360.0f), 830.0f);
+        const float c = std::cosh(0.0072f * (l - 538));
+        const float pdf = 0.0039398042f / (c * c);
+        v += double(spectrum_eval_host(sp, std::round(l), pool)) / pdf / 4;
+    }
     if (v == 0)
         for (int i = 0; i < 471; ++i) v += double(spectrum_eval_host(sp, float(360 + i), pool));
     return v;
@@ -435,6 +407,31 @@ bool rectangle_f32(F3h p00, F3h p10, F3h p01, F3h p11) {
     return fdot(e, e) <= 0x1p-40f * std::max(x2, y2) && std::fabs(fdot(ex, ey)) <= 0x1p-20f * std::sqrt(x2 * y2);
 }
 
+// The smallest-angle cone about the mean unit normal holding n normals
+// (light_normal_cone, src/light_sampling.cpp:356-381): a degenerate normal or
+// a vanishing mean leaves the full sphere (cos_outer = -1).
+void normal_cone(Bounds &b, const D3 *ns, int n) {
+    D3 axis = {0, 0, 0};
+    for (int k = 0; k < n; ++k) {
+        const double l = dlen(ns[k]);
+        if (!(l > 0 && std::isfinite(l))) {
+            b.cos_outer = -1;
+            return;
+        }
+        for (int a = 0; a < 3; ++a) axis[a] += ns[k][a] / l;
+    }
+    const double al = dlen(axis);
+    if (!(al > 0)) {
+        b.cos_outer = -1;
+        return;
+    }
+    for (int a = 0; a < 3; ++a) axis[a] /= al;
+    double cmin = 1;
+    for (int k = 0; k < n; ++k) cmin = std::min(cmin, ddot(axis, ns[k]) / dlen(ns[k]));
+    for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a]);
+    b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
+}
+
 } // namespace
 
 LightTables build_light_tables(const rp_scene_desc &d, float scene_radius, std::vector<uint32_t> &tri_light) {
@@ -478,21 +475,7 @@ LightTables build_light_tables(const rp_scene_desc &d, float scene_radius, std::
                 b.two_sided = em.two_sided != 0;
                 const D3 ns[4] = {dcross(dsub(p[1], p[0]), dsub(p[2], p[0])), dcross(dsub(p[1], p[0]), dsub(p[3], p[1])),
                                   dcross(dsub(p[3], p[2]), dsub(p[2], p[0])), dcross(dsub(p[3], p[2]), dsub(p[3], p[1]))};
-                D3 axis = {0, 0, 0};
-                bool degenerate = false;
-                for (const D3 &n : ns) {
-                    const double l = dlen(n);
-                    if (!(l > 0)) degenerate = true;
-                    else
-                        for (int a = 0; a < 3; ++a) axis[a] += n[a] / l;
-                }
-                const double al = dlen(axis);
-                if (!degenerate && al > 0) {
-                    double cmin = 1;
-                    for (const D3 &n : ns) cmin = std::min(cmin, (ddot(axis, n) / al) / dlen(n));
-                    for (int a = 0; a < 3; ++a) b.dir[a] = float(axis[a] / al);
-                    b.cos_outer = std::max(-1.0f, std::nextafter(float(std::clamp(cmin, -1.0, 1.0)), -INFINITY));
-                }
+                normal_cone(b, ns, 4);
                 LightEntry L{};
                 L.kind = 3;
                 L.index = t;

@@ -56,6 +56,12 @@ class Oracle:
             "or_light_count": (u32, [C.c_void_p]),
             "or_light_table": (None, [C.c_void_p] + [C.c_void_p] * 6),
             "or_light_node_count": (u32, [C.c_void_p]),
+            "or_light_inputs": (None, [C.c_void_p] + [C.c_void_p] * 9),
+            "or_light_alias": (None, [C.c_void_p, C.c_void_p]),
+            "or_light_nodes": (None, [C.c_void_p] + [C.c_void_p] * 10),
+            "or_spectrum_power_table": (C.c_double, [C.POINTER(f32)]),
+            "or_patch_rectangle": (C.c_int, [C.POINTER(C.c_double)]),
+            "or_normal_cone": (None, [C.POINTER(C.c_double), u32, C.POINTER(f32), C.POINTER(f32)]),
             "or_light_select": (u32, [C.c_void_p, C.POINTER(f32), C.POINTER(f32), f32, C.POINTER(f32)]),
             "or_light_pmf": (f32, [C.c_void_p, C.POINTER(f32), C.POINTER(f32), u32]),
             "or_light_sample": (u32, [C.c_void_p, C.POINTER(f32), C.POINTER(f32), f32, f32, f32, C.POINTER(f32),
@@ -149,6 +155,30 @@ class OracleScene:
 
     def light_node_count(self):
         return self.orc.L.or_light_node_count(self.h)
+
+    def light_inputs(self):
+        """Per light, what the tables were built from (LightSamplingInput)."""
+        n = self.orc.L.or_light_count(self.h)
+        f, d = np.float32, np.float64
+        out = {"power": np.zeros(n, d), "bounds_power": np.zeros(n, d), "low": np.zeros((n, 3), f),
+               "high": np.zeros((n, 3), f), "dir": np.zeros((n, 3), f), "cos_outer": np.zeros(n, f),
+               "cos_emission": np.zeros(n, f), "two_sided": np.zeros(n, np.uint8), "infinite": np.zeros(n, np.uint8)}
+        self.orc.L.or_light_inputs(self.h, *[a.ctypes.data_as(C.c_void_p) for a in out.values()])
+        return out
+
+    def light_alias(self):
+        a = np.zeros(self.orc.L.or_light_count(self.h), np.uint32)
+        self.orc.L.or_light_alias(self.h, a.ctypes.data_as(C.c_void_p))
+        return a
+
+    def light_nodes(self):
+        m = self.orc.L.or_light_node_count(self.h)
+        f, u = np.float32, np.uint32
+        out = {"low": np.zeros((m, 3), f), "high": np.zeros((m, 3), f), "dir": np.zeros((m, 3), f),
+               "power": np.zeros(m, f), "cos_outer": np.zeros(m, f), "cos_emission": np.zeros(m, f),
+               "child": np.zeros(m, u), "parent": np.zeros(m, u), "light": np.zeros(m, u), "flags": np.zeros(m, u)}
+        self.orc.L.or_light_nodes(self.h, *[a.ctypes.data_as(C.c_void_p) for a in out.values()])
+        return out
 
     @staticmethod
     def _v(x):

@@ -1,0 +1,227 @@
+"""The oracle pinned against the reference's own host code.
+
+oracle/_ref/libref.so is built by `make -C oracle ref` from the reference's
+sources where they lie (src/sampling.cpp, src/light_sampling.cpp,
+src/rgb_spectrum.cpp and the headers they include; oracle/ref_driver.cpp
+only calls them).  Where those sources are absent the tests skip.
+
+* Sobol (src/sobol_shared.h:56-154): interval indices, the four scrambled
+  word modes, the dimension hash and both uniform conversions, bit for bit
+  over random arguments.
+* Hero wavelengths (src/hero_spectrum_shared.h:199-244): the uniform mode
+  bit for bit; the visible mode within a few ulps (the oracle's own atanh and
+  cosh series against the host libm the reference calls).
+* Light tables (src/light_sampling.cpp): the power proxy of a spectrum, the
+  patch rectangle test, the normal cone and positional cone, and the alias
+  table and light BVH that build_light_sampling makes from the oracle's
+  per-light inputs, bit for bit, on scenes with many kinds of lights.
+"""
+import numpy as np
+import pytest
+
+import rypath_amd as rp
+from tests.ref_lib import ref
+from tests.test_oracle_lights import many_lights_scene
+
+
+def test_sobol_indices_and_words_equal_the_reference(orc):
+    R = ref()
+    rng = np.random.default_rng(5)
+    for _ in range(4000):
+        m = int(rng.integers(0, 13))
+        x, y = (int(rng.integers(0, 1 << m)), int(rng.integers(0, 1 << m))) if m else (0, 0)
+        s = int(rng.integers(0, 1 << min(52 - 2 * m, 40)))
+        assert orc.sobol_index(m, x, y, s) == R.L.ref_sobol_index(m, x, y, s)
+    for _ in range(6000):
+        idx = int(rng.integers(0, 1 << 40))
+        dim = int(rng.integers(0, 1024))
+        mode = int(rng.integers(0, 4))
+        seed = int(rng.integers(0, 1 << 32))
+        assert orc.sobol_word(idx, dim, mode, seed) == R.L.ref_sobol_word(idx, dim, mode, seed), (idx, dim, mode)
+        assert orc.L.or_dimension_hash(dim, seed) == R.L.ref_dimension_hash(dim, seed)
+    words = np.concatenate([rng.integers(0, 1 << 32, 3000), [0, 1, 0xFFFFFFFF, 0xFFFFFF00, 0xFFFFFF7F]])
+    for w in words.tolist():
+        assert orc.L.or_sobol_uniform(w) == R.L.ref_sobol_uniform(w)
+        m = int(rng.integers(0, 12))
+        px = int(rng.integers(0, 1 << m)) if m else 0
+        assert orc.L.or_camera_uniform(w, m, px) == R.L.ref_camera_uniform(w, m, px)
+
+
+def test_hero_wavelengths_follow_the_reference(orc):
+    R = ref()
+    rng = np.random.default_rng(8)
+    us = np.concatenate([rng.random(2000).astype(np.float32), np.float32([0, 0.25, 0.5, 0.75, 1 - 2 ** -24])])
+    worst = 0.0
+    for u in us.tolist():
+        lam, pdf = orc.wavelengths(u, 0)
+        rl, rpdf = R.wavelengths("ref_hero_uniform", u)
+        assert np.array_equal(np.float32(lam), rl) and np.array_equal(np.float32(pdf), rpdf), u
+        lam, pdf = orc.wavelengths(u, 1)
+        rl, rpdf = R.wavelengths("ref_hero_visible", u)
+        ulps = np.abs(np.float32(lam) - rl) / np.spacing(rl)
+        worst = max(worst, float(ulps.max()))
+        assert np.all(ulps <= 8), (u, lam, rl)
+        assert np.allclose(pdf, rpdf, rtol=2e-5, atol=0), (u, pdf, rpdf)
+    assert worst <= 8
+
+
+def spectra_for_power(rng):
+    """471-sample tables: flat, random, blackbody-like, and a narrow band that
+    none of the four proxy wavelengths sees (the plain-sum fallback)."""
+    lam = np.arange(360, 831, dtype=np.float64)
+    out = [np.full(471, 0.5), rng.random(471)]
+    for T in (3000.0, 4000.0, 6500.0, 8000.0):
+        x = lam * 1e-9
+        b = 1 / (x ** 5 * (np.exp(1.4387769e-2 / (x * T)) - 1))
+        out.append(b / b.max() * rng.uniform(0.5, 20))
+    band = np.zeros(471)
+    band[100:103] = 3.0
+    out.append(band)
+    return [np.ascontiguousarray(s, np.float32) for s in out]
+
+
+def test_light_power_proxy_equals_the_reference(orc):
+    R = ref()
+    import ctypes as C
+
+    for s in spectra_for_power(np.random.default_rng(2)):
+        p = s.ctypes.data_as(C.POINTER(C.c_float))
+        assert orc.L.or_spectrum_power_table(p) == R.L.ref_light_spectrum_power(p)
+
+
+def random_patches(rng):
+    out = []
+    for _ in range(600):
+        c = rng.normal(size=3) * 10 ** rng.uniform(-2, 2)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        v = np.cross(u, rng.normal(size=3))
+        v /= np.linalg.norm(v)
+        w, h = 10 ** rng.uniform(-2, 1, 2)
+        p00, p10, p01 = c, c + w * u, c + h * v
+        p11 = c + w * u + h * v
+        kind = int(rng.integers(0, 4))
+        if kind == 1:  # slightly off a rectangle
+            p11 = p11 + rng.normal(size=3) * (w + h) * 10 ** rng.uniform(-8, -3)
+        elif kind == 2:  # skewed parallelogram
+            p01 = c + h * (v + rng.uniform(0.01, 0.5) * u)
+            p11 = p10 + (p01 - c)
+        elif kind == 3:  # a collapsed edge
+            p10 = p00.copy()
+        out.append(np.ascontiguousarray(np.concatenate([p00, p10, p01, p11]), np.float64))
+    return out
+
+
+def test_patch_rectangle_test_equals_the_reference(orc):
+    R = ref()
+    import ctypes as C
+
+    seen = set()
+    for p in random_patches(np.random.default_rng(4)):
+        ptr = p.ctypes.data_as(C.POINTER(C.c_double))
+        got, want = orc.L.or_patch_rectangle(ptr), R.L.ref_light_patch_rectangle(ptr)
+        assert got == want, p
+        seen.add(want)
+    assert seen == {0, 1}
+
+
+def test_normal_cone_equals_the_reference(orc):
+    R = ref()
+    import ctypes as C
+
+    rng = np.random.default_rng(6)
+    for i in range(800):
+        n = int(rng.integers(1, 7))
+        axis = rng.normal(size=3)
+        ns = axis + rng.normal(size=(n, 3)) * 10 ** rng.uniform(-6, 0.5)
+        ns *= 10 ** rng.uniform(-3, 3, (n, 1))
+        if i % 50 == 0:
+            ns[0] = 0  # a degenerate normal: the full sphere
+        ns = np.ascontiguousarray(ns, np.float64)
+        got = [(C.c_float * 3)(), C.c_float()]
+        want = [(C.c_float * 3)(), C.c_float()]
+        orc.L.or_normal_cone(ns.ctypes.data_as(C.POINTER(C.c_double)), n, got[0], C.byref(got[1]))
+        R.L.ref_light_normal_cone(ns.ctypes.data_as(C.POINTER(C.c_double)), n, want[0], C.byref(want[1]))
+        assert got[1].value == want[1].value and list(got[0]) == list(want[0]), ns
+
+
+def test_spot_cone_equals_the_reference_positional_cone(orc):
+    """A spot light's cone (direction, cos_outer from the falloff start)
+    equals light_positional_cone of a rotation whose third row is the spot
+    direction."""
+    R = ref()
+    import ctypes as C
+
+    from tests.scene_util import SceneBuilder
+
+    rng = np.random.default_rng(9)
+    for _ in range(40):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a = np.cross(d, rng.normal(size=3))
+        a /= np.linalg.norm(a)
+        b = np.cross(d, a)
+        start = float(np.cos(np.radians(rng.uniform(1, 80))))
+        end = float(np.cos(np.arccos(start) + np.radians(rng.uniform(0, 10))))
+        sb = SceneBuilder(4, 4, 2)
+        floor = sb.object(sb.diffuse(sb.constant(0.5)))
+        sb.quad((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1), floor, facing=(0, 1, 0))
+        sb.spot((0, 2, 0), d, start, end, sb.constant(5.0))
+        sb.look_at((0, 3, -3), (0, 0, 0), (0, 1, 0), 40)
+        sc = orc.scene(sb.build())
+        inp = sc.light_inputs()
+        i = int(np.flatnonzero(inp["cos_emission"] != 0)[0]) if np.any(inp["cos_emission"] != 0) else len(
+            inp["power"]) - 1
+        m = (C.c_float * 9)(*np.concatenate([a, b, d]).astype(np.float32).tolist())
+        dirv, co, ce, ts = (C.c_float * 3)(), C.c_float(), C.c_float(), C.c_int()
+        R.L.ref_light_positional_cone(m, np.float32(start), inp["cos_emission"][i], dirv, C.byref(co), C.byref(ce),
+                                      C.byref(ts))
+        assert inp["cos_outer"][i] == np.float32(co.value)
+        assert inp["two_sided"][i] == ts.value == 0
+        assert np.allclose(inp["dir"][i], np.float32(dirv[:]), atol=2e-7)
+
+
+def light_scenes():
+    for sampler in (rp.LIGHT_SAMPLER_BVH, rp.LIGHT_SAMPLER_POWER, rp.LIGHT_SAMPLER_UNIFORM):
+        yield f"many-lights-{sampler}", many_lights_scene(sampler)[1]
+    yield "cornell", rp.Synthetic(rp.SYNTH_CORNELL, seed=1, width=32, height=32).desc
+    yield "volume", rp.Synthetic(rp.SYNTH_VOLUME, seed=1, width=16, height=16, hg_g=0.3).desc
+    for seed in (1000, 1001, 1002):
+        yield f"batch-{seed}", rp.Synthetic(rp.SYNTH_BATCH, seed=seed, width=32, height=32,
+                                            target_triangles=5000).desc
+
+
+@pytest.mark.parametrize("name,desc", list(light_scenes()), ids=lambda v: v if isinstance(v, str) else "")
+def test_light_tables_equal_build_light_sampling(orc, name, desc):
+    R = ref()
+    sc = orc.scene(desc)
+    inp = sc.light_inputs()
+    if len(inp["power"]) == 0:
+        pytest.skip("no lights")
+    alias, nodes = R.build_light_sampling(inp)
+    tab = sc.light_table()
+    assert np.array_equal(tab["pmf"], alias["pmf"])
+    assert np.array_equal(tab["threshold"], alias["threshold"])
+    assert np.array_equal(sc.light_alias(), alias["alias"])
+    assert np.array_equal(tab["leaf"], alias["leaf"])
+    mine = sc.light_nodes()
+    assert len(mine["power"]) == len(nodes["power"])
+    for k in ("low", "high", "dir", "power", "cos_outer", "cos_emission", "parent", "light", "flags"):
+        assert np.array_equal(mine[k], nodes[k]), k
+    interior = nodes["light"] == 0xFFFFFFFF
+    assert np.array_equal(mine["child"][interior], nodes["child"][interior])
+
+
+def test_terrain_light_tables_equal_build_light_sampling(orc):
+    """The 1080p terrain's 16 blackbody quads and its environment."""
+    R = ref()
+    desc = rp.Synthetic(rp.SYNTH_TERRAIN, seed=1, width=64, height=36).desc
+    sc = orc.scene(desc)
+    inp = sc.light_inputs()
+    assert len(inp["power"]) == 17 and inp["infinite"].sum() == 1
+    alias, nodes = R.build_light_sampling(inp)
+    tab = sc.light_table()
+    assert np.array_equal(tab["pmf"], alias["pmf"]) and np.array_equal(tab["leaf"], alias["leaf"])
+    mine = sc.light_nodes()
+    for k in ("low", "high", "dir", "power", "cos_outer", "cos_emission", "parent", "light", "flags"):
+        assert np.array_equal(mine[k], nodes[k]), k
