@@ -356,7 +356,7 @@ __global__ void __launch_bounds__(kWfBlock) k_adj_shadow(Scene S, PathParams P, 
 // vertex is applied once its NEE is in L).  Gradient sums differ from the
 // bounce kernels' only in float summation order.
 template <bool kInterfaces>
-__global__ void __launch_bounds__(kWfBlock) k_adj_tail(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
+__global__ void __launch_bounds__(kWfBlock) RP_PATH_ATTR k_adj_tail(Scene S, PathParams P, GradTables G, AdjWf A, WfState C,
                                                         uint32_t p, uint32_t qn, uint32_t npx, uint64_t sample_begin,
                                                         float *partial) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];

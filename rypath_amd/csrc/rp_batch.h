@@ -82,7 +82,7 @@ RP_DEV void for_scene(const BatchScene *scenes, uint32_t s, Fn &&f) {
 
 // ---------------------------------------------------- persistent megakernel
 template <int BS>
-__global__ void __launch_bounds__(BS) k_batch_persistent(const BatchScene *scenes, uint32_t nscenes,
+__global__ void __launch_bounds__(BS) RP_PATH_ATTR k_batch_persistent(const BatchScene *scenes, uint32_t nscenes,
                                                           unsigned long long *work, uint64_t total) {
     __shared__ uint32_t s_node[kShortStack * BS];
     __shared__ float s_t[kShortStack * BS];
@@ -501,7 +501,7 @@ __global__ void k_pool_retire(PoolView V) {
 // Ray and failure counts go to the pool counters; the host counts the n
 // paths as done.
 template <bool kInterfaces>
-__global__ void __launch_bounds__(kWfBlock) k_pool_tail(PoolView V, uint32_t p, uint32_t n) {
+__global__ void __launch_bounds__(kWfBlock) RP_PATH_ATTR k_pool_tail(PoolView V, uint32_t p, uint32_t n) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
     RP_STACK(st);

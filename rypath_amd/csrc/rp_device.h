@@ -500,6 +500,16 @@ constexpr uint32_t kEmpty = 0xffffffffu;
 #define RP_SHORT_STACK 8
 #endif
 constexpr int kShortStack = RP_SHORT_STACK;
+
+// Whole-path kernels (megakernels, latency tails, the persistent batch) inline
+// the path loop, traversal and light sampling together.  Left to itself the
+// compiler gives them 256 VGPRs plus accumulation registers, one wave per
+// SIMD; two waves per SIMD (at most 256 registers in all) keep twice the
+// memory requests in flight.
+#ifndef RP_PATH_WAVES
+#define RP_PATH_WAVES 2
+#endif
+#define RP_PATH_ATTR __attribute__((amdgpu_waves_per_eu(RP_PATH_WAVES)))
 constexpr int kMaxStack = 64;                      // the BVH builder's bound is checked against this
 constexpr int kSpillStack = kMaxStack - kShortStack; // per-lane overflow (scratch)
 static_assert(kShortStack >= 1 && kShortStack <= kMaxStack, "short stack size");

@@ -138,7 +138,7 @@ __device__ __forceinline__ void count_rays(unsigned long long *counters, uint32_
 // Per-path records; B = the request's workgroup width (the records do not
 // depend on it).
 template <int B>
-__global__ void __launch_bounds__(B) k_records(rpd::Scene S, rpd::PathParams P, uint32_t pixel_begin,
+__global__ void __launch_bounds__(B) RP_PATH_ATTR k_records(rpd::Scene S, rpd::PathParams P, uint32_t pixel_begin,
                                                uint32_t pixel_count, uint64_t sample_begin, uint64_t n,
                                                rp_path_record *out) {
     __shared__ uint32_t s_node[rpd::kShortStack * B];
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(B) k_records(rpd::Scene S, rpd::PathParams P, 
 
 // Megakernel: one thread = one complete path; per-path XYZ written planar
 // (x plane, y plane, z plane) for the film reduction.
-__global__ void __launch_bounds__(kBlock) k_mega(rpd::Scene S, rpd::PathParams P, uint32_t npx, uint64_t sample_begin,
+__global__ void __launch_bounds__(kBlock) RP_PATH_ATTR k_mega(rpd::Scene S, rpd::PathParams P, uint32_t npx, uint64_t sample_begin,
                                                   uint64_t n, float *xyz, unsigned long long *counters) {
     __shared__ uint32_t s_node[rpd::kShortStack * kBlock];
     __shared__ float s_t[rpd::kShortStack * kBlock];
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(kBlock) k_mega(rpd::Scene S, rpd::PathParams P
 
 // Scalar-transport megakernel: one thread = one scalar path; per-path film
 // weight planar as in k_mega.
-__global__ void __launch_bounds__(kBlock) k_scalar_mega(rpd::Scene S, rpd::PathParams P, uint32_t npx,
+__global__ void __launch_bounds__(kBlock) RP_PATH_ATTR k_scalar_mega(rpd::Scene S, rpd::PathParams P, uint32_t npx,
                                                          uint64_t sample_begin, uint64_t n, uint32_t independent,
                                                          float *xyz, unsigned long long *counters) {
     __shared__ uint32_t s_node[rpd::kShortStack * kBlock];

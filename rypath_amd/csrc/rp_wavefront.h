@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
 // contribution is added after the event, as the shadow kernel adds it), so
 // the radiance is unchanged bit for bit.
 template <bool kInterfaces>
-__global__ void __launch_bounds__(kWfBlock) k_wf_tail(Scene S, PathParams P, WfState C, FrameWf F, uint32_t qn,
+__global__ void __launch_bounds__(kWfBlock) RP_PATH_ATTR k_wf_tail(Scene S, PathParams P, WfState C, FrameWf F, uint32_t qn,
                                                        uint32_t npx, uint64_t sample_begin) {
     __shared__ uint32_t s_node[kShortStack * kWfBlock];
     __shared__ float s_t[kShortStack * kWfBlock];
