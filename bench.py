@@ -67,16 +67,19 @@ WORKLOADS = {
 PMC_SUMMARY = ROOT / "profiles" / "terrain_1080p_hbm_traffic.md"
 
 
+TRAVERSAL_KERNELS = ("void rpd::k_wf_extend<false", "void rpd::k_wf_trace<false", "void rpd::k_wf_shadow<false")
+
+
 def pmc_traffic():
-    """HBM bytes per closest-hit launch of the 1080p terrain wavefront frame,
-    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by
-    its copy calibration).  None when the summary is absent."""
+    """HBM bytes per traversal launch (closest hit, fused closest hit + shadow,
+    shadow alone) of the 1080p terrain wavefront frame, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by its copy
+    calibration).  None when the summary is absent."""
     launches, mib = 0, 0.0
     try:
         for line in PMC_SUMMARY.read_text().splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            # k_wf_extend<false, kGen>: the first bounce's launch and the others
-            if cells and cells[0].startswith("void rpd::k_wf_extend<false"):
+            if cells and cells[0].startswith(TRAVERSAL_KERNELS):
                 launches += int(cells[1])
                 mib += float(cells[4]) + float(cells[5])
     except (OSError, ValueError, IndexError):
@@ -490,13 +493,24 @@ def main():
         cfg = rp.frame_config(args.spp, mode=modes[frame_mode], seed=1, flags=rp.FRAME_COUNT_TRAFFIC)
         cnt = scene.render_frame_device(cfg, film.data_ptr())
         st = results[frame_mode]["stats"]
-        # the instrumented replay traces every bounce in k_wf_extend; the timed
-        # frame's closest-hit kernels traced extend_rays of its rays (the rest
-        # ran in the latency tail), at the replay's bytes per ray
-        per_ray = (RAY_BYTES * cnt["extension_rays"] + NODE_BYTES * cnt["node_fetches"] +
-                   TRI_BYTES * cnt["triangle_tests"]) / cnt["extension_rays"]
-        alg = per_ray * st["extend_rays"]
-        achieved = alg / st["trace_seconds"] / 1e9
+        # The traversal launches of the timed frame: the first bounce's closest
+        # hits (k_wf_extend), every later bounce's closest hits together with
+        # the previous bounce's shadow rays (k_wf_trace), and shadow rays alone
+        # before the latency tail (k_wf_shadow).  The instrumented replay (no
+        # tail: every query in a bounce kernel) gives the algorithmic bytes per
+        # closest-hit and per any-hit query; the timed frame's launches traced
+        # extend_rays and shadow_kernel_rays of them.
+        ext_per_ray = (RAY_BYTES * cnt["extension_rays"] + NODE_BYTES * cnt["node_fetches"] +
+                       TRI_BYTES * cnt["triangle_tests"]) / cnt["extension_rays"]
+        sh_rays = max(cnt["shadow_rays"], 1)
+        sh_per_ray = (SHADOW_RAY_BYTES * cnt["shadow_rays"] + NODE_BYTES * cnt["shadow_node_fetches"] +
+                      TRI_BYTES * cnt["shadow_triangle_tests"]) / sh_rays
+        st = results[frame_mode]["stats"]
+        alg = ext_per_ray * st["extend_rays"] + sh_per_ray * st["shadow_kernel_rays"]
+        t_trav = st["trace_seconds"] + st["shadow_seconds"]
+        launches_ = max(st["extend_launches"] + st["shadow_launches"], 1)
+        achieved = alg / t_trav / 1e9
+        avg_launch = t_trav / launches_
         traffic = pmc_traffic() if (args.workload == "terrain" and (args.width, args.height, args.spp) ==
                                     (1920, 1080, 16) and frame_mode == "wave") else None
         # measured read peaks on this device (rp_probe_read_bandwidth): an
@@ -506,29 +520,11 @@ def main():
         l2_meas = rp.probe_read_bandwidth(24 << 20, 400)
         print(f"[bench] measured read bandwidth: HBM {hbm_meas:.0f} GB/s, L2 {l2_meas:.0f} GB/s",
               file=sys.stderr, flush=True)
-        launches_ = max(st["extend_launches"], 1)
-        avg_launch = st["trace_seconds"] / launches_
-        # any-hit (shadow) kernel: the replay's bytes per shadow ray over the
-        # timed frame's shadow-kernel time
-        sh_rays = max(cnt["shadow_rays"], 1)
-        sh_per_ray = (SHADOW_RAY_BYTES * cnt["shadow_rays"] + NODE_BYTES * cnt["shadow_node_fetches"] +
-                      TRI_BYTES * cnt["shadow_triangle_tests"]) / sh_rays
-        sh_alg = sh_per_ray * st["shadow_kernel_rays"]
-        sh_achieved = sh_alg / st["shadow_seconds"] / 1e9 if st["shadow_seconds"] > 0 else None
-        shadow = {"kernel": "k_wf_shadow (any hit)", "achieved": round(sh_achieved, 1) if sh_achieved else None,
-                  "frac": round(sh_achieved / HBM_PEAK_GBS, 4) if sh_achieved else None,
-                  "per_ray_bytes": round(sh_per_ray, 1),
-                  "nodes_per_ray": round(cnt["shadow_node_fetches"] / sh_rays, 2),
-                  "tris_per_ray": round(cnt["shadow_triangle_tests"] / sh_rays, 2),
-                  "ns_per_ray": round(st["shadow_seconds"] / max(st["shadow_kernel_rays"], 1) * 1e9, 4),
-                  "closest_hit_ns_per_ray": round(st["trace_seconds"] / max(st["extend_rays"], 1) * 1e9, 4),
-                  "avg_launch_ms": round(st["shadow_seconds"] / max(st["shadow_launches"], 1) * 1e3, 3),
-                  "launches": st["shadow_launches"]}
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    # the kernel moves far fewer HBM bytes than it fetches through
-                    # the caches: it is limited by latency and lane divergence,
-                    # not by the HBM roof that `frac` prices it against
+                    # the kernels move far fewer HBM bytes than they fetch through
+                    # the caches: they are limited by latency and lane divergence,
+                    # not by the HBM roof that `frac` prices them against
                     "limiter": "latency/divergence",
                     "frac_by_measured_traffic": (round(traffic / avg_launch / 1e9 / HBM_PEAK_GBS, 4)
                                                  if traffic else None),
@@ -536,16 +532,24 @@ def main():
                     "measured_read_peak_l2": round(l2_meas, 1), "frac_of_measured_l2": round(achieved / l2_meas, 4),
                     "traffic": traffic,
                     "traffic_source": PMC_SUMMARY.relative_to(ROOT).as_posix() if traffic else None,
-                    "algorithmic_bytes_per_launch": round(alg / max(st["extend_launches"], 1)),
-                    "kernel": f"k_wf_extend (closest hit, {frame_mode} frame)",
-                    "per_ray_bytes": round(per_ray, 1),
-                    "tail_extension_rays": st["extension_rays"] - st["extend_rays"],
-                    "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),
-                    "tris_per_ray": round(cnt["triangle_tests"] / cnt["extension_rays"], 2),
+                    "algorithmic_bytes_per_launch": round(alg / launches_),
+                    "kernel": f"BVH traversal ({frame_mode} frame): k_wf_extend (first bounce, closest hit), "
+                              "k_wf_trace (closest hit + the previous bounce's shadow rays), k_wf_shadow "
+                              "(shadow rays alone)",
+                    "launches": {"k_wf_extend": st["extend_launches"] - st["fused_launches"],
+                                 "k_wf_trace": st["fused_launches"], "k_wf_shadow": st["shadow_launches"]},
                     "avg_launch_ms": round(avg_launch * 1e3, 3),
+                    "closest_hit": {"per_ray_bytes": round(ext_per_ray, 1), "rays": st["extend_rays"],
+                                    "nodes_per_ray": round(cnt["node_fetches"] / cnt["extension_rays"], 2),
+                                    "tris_per_ray": round(cnt["triangle_tests"] / cnt["extension_rays"], 2),
+                                    "tail_extension_rays": st["extension_rays"] - st["extend_rays"]},
+                    "any_hit": {"per_ray_bytes": round(sh_per_ray, 1), "rays": st["shadow_kernel_rays"],
+                                "fused_rays": st["fused_shadow_rays"],
+                                "nodes_per_ray": round(cnt["shadow_node_fetches"] / sh_rays, 2),
+                                "tris_per_ray": round(cnt["shadow_triangle_tests"] / sh_rays, 2)},
+                    "ns_per_query": round(t_trav / max(st["extend_rays"] + st["shadow_kernel_rays"], 1) * 1e9, 4),
                     "stack_spills_per_ray": round(cnt["stack_spills"] / max(cnt["extension_rays"] +
-                                                                            cnt["shadow_rays"], 1), 5),
-                    "shadow": shadow}
+                                                                            cnt["shadow_rays"], 1), 5)}
     # the second headline figure: inverse-rendering gradients per second
     adjoint = None
     if args.workload == "terrain":

@@ -278,8 +278,9 @@ typedef struct rp_frame_stats {
     uint64_t paths;                 /* W*H*spp */
     uint64_t extension_rays, shadow_rays;
     uint64_t failed_paths;          /* status != success */
-    double trace_seconds;           /* wavefront: device time in closest-hit kernels */
-    uint64_t extend_launches;       /* wavefront: closest-hit kernel launches */
+    double trace_seconds;           /* wavefront: device time in the launches that trace extension rays
+                                       (closest hit; fused launches also trace shadow rays) */
+    uint64_t extend_launches;       /* wavefront: those launches */
     uint64_t node_fetches, triangle_tests; /* with RP_FRAME_COUNT_TRAFFIC: closest-hit traversal totals */
     uint64_t extend_rays;           /* wavefront: extension rays traced by the closest-hit kernels
                                        (the rest of extension_rays by the latency tail) */
@@ -287,10 +288,14 @@ typedef struct rp_frame_stats {
      * traversal-stack pushes of both kernels that overflowed the LDS short
      * stack into per-lane scratch */
     uint64_t shadow_node_fetches, shadow_triangle_tests, stack_spills;
-    double shadow_seconds;          /* wavefront: device time in the shadow (any-hit) kernels */
-    uint64_t shadow_launches;       /* wavefront: shadow kernel launches */
-    uint64_t shadow_kernel_rays;    /* wavefront: shadow rays traced by the shadow kernels (the rest
-                                       of shadow_rays by the latency tail) */
+    double shadow_seconds;          /* wavefront: device time in the shadow-only (any-hit) launches */
+    uint64_t shadow_launches;       /* wavefront: shadow-only launches */
+    uint64_t shadow_kernel_rays;    /* wavefront: shadow rays traced by the bounce kernels, shadow-only
+                                       and fused (the rest of shadow_rays by the latency tail) */
+    /* wavefront, scenes without null interfaces: launches that trace a bounce's
+     * extension rays together with the previous bounce's shadow rays (counted
+     * in extend_launches and trace_seconds), and the shadow rays they traced */
+    uint64_t fused_launches, fused_shadow_rays;
 } rp_frame_stats;
 
 /* ---- traversal probe ray/hit records ------------------------------------ */

@@ -144,7 +144,8 @@ class FrameStats(C.Structure):
                 ("failed_paths", u64), ("trace_seconds", f64), ("extend_launches", u64), ("node_fetches", u64),
                 ("triangle_tests", u64), ("extend_rays", u64), ("shadow_node_fetches", u64),
                 ("shadow_triangle_tests", u64), ("stack_spills", u64), ("shadow_seconds", f64),
-                ("shadow_launches", u64), ("shadow_kernel_rays", u64)]
+                ("shadow_launches", u64), ("shadow_kernel_rays", u64), ("fused_launches", u64),
+                ("fused_shadow_rays", u64)]
 
 
 class Ray(C.Structure):

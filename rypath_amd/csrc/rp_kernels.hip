@@ -23,7 +23,7 @@
 #include "rp_adjoint.h"
 #include "rp_scalar.h"
 
-static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 128 && sizeof(rp_batch_config) == 32 &&
+static_assert(sizeof(rp_frame_config) == 48 && sizeof(rp_frame_stats) == 144 && sizeof(rp_batch_config) == 32 &&
                   sizeof(rp_request) == 72 && sizeof(rp_path_record) == 112,
               "C ABI struct sizes");
 
@@ -588,7 +588,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
     hipEvent_t e0 = ev.a, e1 = ev.b;
     HIP_CHECK(hipEventRecord(e0, s->stream));
     double trace_seconds = 0, shadow[2] = {0, 0};
-    uint64_t launches = 0, wf_rays[2] = {0, 0};
+    uint64_t launches = 0, wf_rays[2] = {0, 0}, fused[2] = {0, 0};
     // the device's wavefront arena is held until this frame's last kernel has
     // finished (the event synchronisation below happens before the unlock)
     std::unique_lock<std::mutex> arena_lock;
@@ -611,7 +611,7 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
             arena->primal.valid = false;
             trace_seconds += rpd::wavefront_render(s->dev, P, arena->fb, cfg, npx, cfg.sample_begin + s0, n,
                                                    s->path_xyz.p, s->counters.p, s->stream, launches, wf_rays,
-                                                   s0 == 0, shadow);
+                                                   s0 == 0, shadow, fused);
         }
         HIP_CHECK(hipGetLastError());
         hipLaunchKernelGGL(k_film_add, dim3((npx + 255) / 256), dim3(256), 0, s->stream, npx, ns, n, s->path_xyz.p,
@@ -657,6 +657,8 @@ double render_frame_impl(rp_scene *s, const rp_frame_config &cfg, float *xyz_dev
         stats->shadow_seconds = shadow[0];
         stats->shadow_launches = uint64_t(shadow[1]);
         stats->shadow_kernel_rays = wf_rays[1];
+        stats->fused_launches = fused[0];
+        stats->fused_shadow_rays = fused[1];
     }
     return ms * 1e-3;
 }

@@ -226,16 +226,14 @@ __global__ void __launch_bounds__(256) k_wf_generate(Scene S, PathParams P, uint
     else store_state(C, uint32_t(i), ps);
 }
 
-// Closest hit for C[0, qn).  kGen (the first bounce, compact generation):
-// the ray leaves the camera origin with the direction in C.d.
-template <bool kCount, bool kGen = false>
-__global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, FrameWf F, uint32_t qn,
-                                                          unsigned long long *counters) {
-    __shared__ uint32_t s_node[kShortStack * kWfBlock];
-    __shared__ float s_t[kShortStack * kWfBlock];
-    RP_STACK(st);
-    st.init(s_node, s_t, kWfBlock);
-    uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+// Closest hit of position i of C[0, qn).  kGen (the first bounce, compact
+// generation): the ray leaves the camera origin with the direction in C.d.
+// With kCount, node fetches, triangle tests and stack overflows are tallied
+// in counters 3, 4 and 7 (the instrumented replay); ray counts are the queue
+// counts, summed on the host.
+template <bool kCount, bool kGen>
+RP_DEV void extend_ray(const Scene &S, Stack &st, const WfState &C, const FrameWf &F, uint32_t i, uint32_t qn,
+                       unsigned long long *counters) {
     uint32_t nodes = 0, tris = 0, spills = 0;
     if (i < qn) {
         float4 a, b = C.d[i];
@@ -252,14 +250,22 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, Fram
             spills = tc.spills;
         }
     }
-    // ray counts are the queue counts (summed on the host); only the
-    // instrumented replay tallies node fetches, triangle tests and stack
-    // overflows (counter 7, shared with the shadow kernel)
     if (kCount) {
         wave_count(&counters[3 * kCounterStride], nodes);
         wave_count(&counters[4 * kCounterStride], tris);
         wave_count(&counters[7 * kCounterStride], spills);
     }
+}
+
+// Closest hit for C[0, qn).
+template <bool kCount, bool kGen = false>
+__global__ void __launch_bounds__(kWfBlock) k_wf_extend(Scene S, WfState C, FrameWf F, uint32_t qn,
+                                                          unsigned long long *counters) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    RP_STACK(st);
+    st.init(s_node, s_t, kWfBlock);
+    extend_ray<kCount, kGen>(S, st, C, F, blockIdx.x * kWfBlock + threadIdx.x, qn, counters);
 }
 
 RP_DEV uint32_t material_key(const Scene &S, const uint2 *hit_id, uint32_t slot, uint32_t bits) {
@@ -399,18 +405,14 @@ __global__ void __launch_bounds__(kWfBlock) RP_SHADE_ATTR k_wf_shade(Scene S, Pa
     shade_one<kGen>(S, P, C, N, F, valid, valid ? (sorted ? F.perm[i] : i) : 0u, npx, sample_begin, sh);
 }
 
-// Shadow rays of one bounce; adds visible contributions to the survivor's
-// radiance in N or to an ended path's lane.  With kCount (instrumented
-// replay) node fetches, triangle tests and stack overflows are tallied in
-// counters 5, 6 and 7.
-template <bool kInterfaces, bool kCount = false>
-__global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, WfState N, FrameWf F, uint32_t sn,
-                                                          unsigned long long *counters) {
-    __shared__ uint32_t s_node[kShortStack * kWfBlock];
-    __shared__ float s_t[kShortStack * kWfBlock];
-    RP_STACK(st);
-    st.init(s_node, s_t, kWfBlock);
-    uint32_t i = blockIdx.x * kWfBlock + threadIdx.x;
+// Shadow ray i of one bounce's sn; adds a visible contribution to the
+// survivor's radiance in N or to an ended path's lane.  With kCount
+// (instrumented replay) node fetches, triangle tests and stack overflows are
+// tallied in counters 5, 6 and 7; shadow ray counts are the queue counts,
+// summed on the host.
+template <bool kInterfaces, bool kCount>
+RP_DEV void shadow_ray(const Scene &S, const PathParams &P, Stack &st, const WfState &N, const FrameWf &F, uint32_t i,
+                       uint32_t sn, unsigned long long *counters) {
     TraversalCounts tc = {0, 0, 0};
     if (i < sn) {
         float4 o = F.sh_o[i], d = F.sh_d[i], v = F.sh_val[i];
@@ -436,12 +438,40 @@ __global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, W
             *dst = make_float4(ps.L[0], ps.L[1], ps.L[2], ps.L[3]);
         }
     }
-    // shadow ray counts are the queue counts (summed on the host)
     if (kCount) {
         wave_count(&counters[5 * kCounterStride], tc.nodes);
         wave_count(&counters[6 * kCounterStride], tc.tris);
         wave_count(&counters[7 * kCounterStride], tc.spills);
     }
+}
+
+// Shadow rays of one bounce.
+template <bool kInterfaces, bool kCount = false>
+__global__ void __launch_bounds__(kWfBlock) k_wf_shadow(Scene S, PathParams P, WfState N, FrameWf F, uint32_t sn,
+                                                          unsigned long long *counters) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    RP_STACK(st);
+    st.init(s_node, s_t, kWfBlock);
+    shadow_ray<kInterfaces, kCount>(S, P, st, N, F, blockIdx.x * kWfBlock + threadIdx.x, sn, counters);
+}
+
+// One launch for a bounce's closest hits and the previous bounce's shadow
+// rays (scenes without null interfaces): blocks [0, eb) trace the extension
+// rays of C[0, qn), the rest the shadow rays [0, sn), which add to the
+// radiance of C (the previous bounce's survivor set).  Neither half reads what
+// the other writes (hit records against radiance and ended lanes).  The late
+// bounces are latency-bound (a launch lasts as long as its slowest rays), and
+// one launch instead of two overlaps those tails.
+template <bool kCount>
+__global__ void __launch_bounds__(kWfBlock) k_wf_trace(Scene S, PathParams P, WfState C, FrameWf F, uint32_t qn,
+                                                         uint32_t eb, uint32_t sn, unsigned long long *counters) {
+    __shared__ uint32_t s_node[kShortStack * kWfBlock];
+    __shared__ float s_t[kShortStack * kWfBlock];
+    RP_STACK(st);
+    st.init(s_node, s_t, kWfBlock);
+    if (blockIdx.x < eb) extend_ray<kCount, false>(S, st, C, F, blockIdx.x * kWfBlock + threadIdx.x, qn, counters);
+    else shadow_ray<false, kCount>(S, P, st, C, F, (blockIdx.x - eb) * kWfBlock + threadIdx.x, sn, counters);
 }
 
 // ------------------------------------------------------------ latency tail
@@ -484,7 +514,7 @@ __global__ void k_wf_zero_tail(unsigned long long *tail) {
 // 65536 / 131072 / 262144, and the 512x512 volume frame 24.6 / 20.8 / 19.7 /
 // 20.5 ms.
 inline uint32_t tail_limit(const rp_frame_config &cfg) {
-    // the instrumented replay counts every closest-hit query in k_wf_extend
+    // the instrumented replay counts every query in the bounce kernels
     if (cfg.flags & (RP_FRAME_NO_TAIL | RP_FRAME_COUNT_TRAFFIC)) return 0;
     if (cfg.tail_paths) return cfg.tail_paths;
     int dev = 0, cus = 0;
@@ -604,15 +634,19 @@ inline void FrameBuffers::ensure(uint64_t n) {
 // Runs n paths (position i: pixel tile_pixel(i % npx), sample sample_begin +
 // i / npx; npx = W * H) to
 // completion and writes planar per-path XYZ.  Returns device seconds spent in
-// the closest-hit (extend) kernels; adds the extension and shadow ray counts
-// of the bounce kernels (their queue counts) to rays[0] and rays[1]; the
-// latency tail's go to the arena's tail counters, zeroed when first_chunk.
-// With `shadow`, the any-hit (shadow) kernels' device seconds and launches
-// are added to shadow[0] and shadow[1].
+// the launches that trace extension rays (k_wf_extend, and k_wf_trace, which
+// traces the previous bounce's shadow rays in the same launch); adds the
+// extension and shadow ray counts of the bounce kernels (their queue counts)
+// to rays[0] and rays[1]; the latency tail's go to the arena's tail
+// counters, zeroed when first_chunk.  With `shadow`, the device seconds and
+// launches of the shadow-only launches (k_wf_shadow) are added to shadow[0]
+// and shadow[1]; with `fused`, the k_wf_trace launches and the shadow rays
+// they traced to fused[0] and fused[1].
 inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers &FB, const rp_frame_config &cfg,
                                uint32_t npx, uint64_t sample_begin, uint64_t n, float *xyz,
                                unsigned long long *counters, hipStream_t stream, uint64_t &launches,
-                               uint64_t rays[2], bool first_chunk, double *shadow = nullptr) {
+                               uint64_t rays[2], bool first_chunk, double *shadow = nullptr,
+                               uint64_t *fused = nullptr) {
     if (n > (uint64_t(1) << 31)) throw std::runtime_error("wavefront chunk too large");
     FB.ensure(n);
     const FrameWf &F = FB.f;
@@ -634,23 +668,70 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
     uint32_t qn = uint32_t(n);
     EventPair ev, ev_sh; // destroyed on every exit, exceptions included
     hipEvent_t e0 = ev.a, e1 = ev.b;
-    double trace = 0;
-    bool shadow_pending = false;
-    double shadow_seconds = 0;
+    const bool count = (cfg.flags & RP_FRAME_COUNT_TRAFFIC) != 0;
+    // Shadow rays of the previous bounce, pending in F.sh_*: they add to the
+    // radiance of the current set C.  Without null interfaces they ride in the
+    // next bounce's closest-hit launch (k_wf_trace); otherwise, and before the
+    // latency tail or once no path is left, they get a launch of their own.
+    uint32_t pending = 0;
+    double trace = 0, shadow_seconds = 0;
     uint64_t shadow_launches = 0;
+    bool shadow_timed = false; // a shadow launch's events not read yet
+    auto read_shadow_time = [&]() {
+        if (!shadow_timed) return;
+        WF_CHECK(hipEventSynchronize(ev_sh.b));
+        float ms = 0;
+        WF_CHECK(hipEventElapsedTime(&ms, ev_sh.a, ev_sh.b));
+        shadow_seconds += ms * 1e-3;
+        shadow_timed = false;
+    };
+    auto shadow_alone = [&](const WfState &dst, uint32_t sn) {
+        read_shadow_time();
+        const dim3 sg((sn + kWfBlock - 1) / kWfBlock);
+        WF_CHECK(hipEventRecord(ev_sh.a, stream));
+        if (S.has_interfaces && count)
+            hipLaunchKernelGGL((k_wf_shadow<true, true>), sg, dim3(kWfBlock), 0, stream, S, P, dst, F, sn, counters);
+        else if (S.has_interfaces)
+            hipLaunchKernelGGL((k_wf_shadow<true, false>), sg, dim3(kWfBlock), 0, stream, S, P, dst, F, sn, counters);
+        else if (count)
+            hipLaunchKernelGGL((k_wf_shadow<false, true>), sg, dim3(kWfBlock), 0, stream, S, P, dst, F, sn, counters);
+        else
+            hipLaunchKernelGGL((k_wf_shadow<false, false>), sg, dim3(kWfBlock), 0, stream, S, P, dst, F, sn, counters);
+        WF_CHECK(hipEventRecord(ev_sh.b, stream));
+        shadow_timed = true;
+        ++shadow_launches;
+    };
     while (qn > 0) {
         const WfState &C = FB.state[cur], &N = FB.state[1 - cur];
         const dim3 grid((qn + kWfBlock - 1) / kWfBlock);
         if (qn <= tail) {
+            if (pending) shadow_alone(C, pending);
+            pending = 0;
             if (S.has_interfaces)
                 hipLaunchKernelGGL(k_wf_tail<true>, grid, dim3(kWfBlock), 0, stream, S, P, C, F, qn, npx, sample_begin);
             else
                 hipLaunchKernelGGL(k_wf_tail<false>, grid, dim3(kWfBlock), 0, stream, S, P, C, F, qn, npx, sample_begin);
             break;
         }
+        if (pending && S.has_interfaces) {
+            shadow_alone(C, pending);
+            pending = 0;
+        }
         WF_CHECK(hipEventRecord(e0, stream));
-        const bool count = (cfg.flags & RP_FRAME_COUNT_TRAFFIC) != 0;
-        if (gen && count)
+        if (pending) {
+            const uint32_t eb = grid.x, sb = (pending + kWfBlock - 1) / kWfBlock;
+            if (count)
+                hipLaunchKernelGGL((k_wf_trace<true>), dim3(eb + sb), dim3(kWfBlock), 0, stream, S, P, C, F, qn, eb,
+                                   pending, counters);
+            else
+                hipLaunchKernelGGL((k_wf_trace<false>), dim3(eb + sb), dim3(kWfBlock), 0, stream, S, P, C, F, qn, eb,
+                                   pending, counters);
+            if (fused) {
+                fused[0] += 1;
+                fused[1] += pending;
+            }
+            pending = 0;
+        } else if (gen && count)
             hipLaunchKernelGGL((k_wf_extend<true, true>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
         else if (gen)
             hipLaunchKernelGGL((k_wf_extend<false, true>), grid, dim3(kWfBlock), 0, stream, S, C, F, qn, counters);
@@ -680,41 +761,19 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         float ms = 0;
         WF_CHECK(hipEventElapsedTime(&ms, e0, e1));
         trace += ms * 1e-3;
-        if (shadow_pending) { // the previous bounce's shadow launch finished before this bounce's kernels
-            WF_CHECK(hipEventElapsedTime(&ms, ev_sh.a, ev_sh.b));
-            shadow_seconds += ms * 1e-3;
-            shadow_pending = false;
-        }
+        read_shadow_time(); // a shadow launch of this bounce has finished with the stream
         const uint32_t next = FB.host_count[0], sn = FB.host_count[kQueueStride];
         rays[0] += qn;
         rays[1] += sn;
         if (next > qn || sn > qn) {
             throw std::runtime_error("wavefront queue counts out of range");
         }
-        if (sn) {
-            const dim3 sg((sn + kWfBlock - 1) / kWfBlock);
-            WF_CHECK(hipEventRecord(ev_sh.a, stream));
-            if (S.has_interfaces && count)
-                hipLaunchKernelGGL((k_wf_shadow<true, true>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
-            else if (S.has_interfaces)
-                hipLaunchKernelGGL((k_wf_shadow<true, false>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
-            else if (count)
-                hipLaunchKernelGGL((k_wf_shadow<false, true>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
-            else
-                hipLaunchKernelGGL((k_wf_shadow<false, false>), sg, dim3(kWfBlock), 0, stream, S, P, N, F, sn, counters);
-            WF_CHECK(hipEventRecord(ev_sh.b, stream));
-            shadow_pending = true;
-            ++shadow_launches;
-        }
+        pending = sn;
         cur = 1 - cur;
         qn = next;
     }
-    if (shadow_pending) {
-        WF_CHECK(hipEventSynchronize(ev_sh.b));
-        float ms = 0;
-        WF_CHECK(hipEventElapsedTime(&ms, ev_sh.a, ev_sh.b));
-        shadow_seconds += ms * 1e-3;
-    }
+    if (pending) shadow_alone(FB.state[cur], pending); // no path left to trace: the last shadow rays alone
+    read_shadow_time();
     if (shadow) {
         shadow[0] += shadow_seconds;
         shadow[1] += double(shadow_launches);

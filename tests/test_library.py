@@ -40,7 +40,7 @@ def test_struct_layouts(rp):
     assert C.sizeof(rp.Camera) == 64
     assert C.sizeof(rp.Request) == 72
     assert C.sizeof(rp.FrameConfig) == 48
-    assert C.sizeof(rp.FrameStats) == 128
+    assert C.sizeof(rp.FrameStats) == 144
     assert C.sizeof(rp.BatchConfig) == 32
     assert rp.RECORD_DTYPE.itemsize == 112
     assert C.sizeof(rp.Ray) == 32 and C.sizeof(rp.Hit) == 16
