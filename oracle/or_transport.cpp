@@ -2385,6 +2385,62 @@ void or_light_table(void *sp, uint32_t *kind, uint32_t *index, float *pmf, float
     }
 }
 uint32_t or_light_node_count(void *sp) { return uint32_t(static_cast<Scene *>(sp)->lnodes.size()); }
+// The quad proposal of light `id` (a rectangle light) from p with receiver
+// normal n (zero: none), as sample_light draws it: rectangle coordinates
+// (u along ex, v along ey) and the solid-angle density.  Returns 1 when the
+// spherical-rectangle proposal was used, 0 for area sampling, -1 when `id`
+// is not a quad light.
+int or_quad_sample(void *sp, uint32_t id, const float p[3], const float n[3], float u0, float u1, float uv[2],
+                   float *pdf_omega) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    if (id >= s.lights.size() || s.lights[id].kind != 3) return -1;
+    const Light &L = s.lights[id];
+    const V3 r = {p[0], p[1], p[2]}, nn = {n[0], n[1], n[2]};
+    float su = std::min(u0, 0x1.fffffep-1f), sv = std::min(u1, 0x1.fffffep-1f);
+    uv[0] = su;
+    uv[1] = sv;
+    if (L.rect) {
+        const RectFrame f = rect_frame(L, r);
+        if (f.area > 1e-4f) {
+            float dens = 1.0f;
+            if (nn.x != 0.0f || nn.y != 0.0f || nn.z != 0.0f) {
+                const Bilinear w = rect_cosines(f, L, nn);
+                bilinear_warp(su, sv, w);
+                dens = bilinear_density(su, sv, w);
+            }
+            rect_sample(f, su, sv, uv[0], uv[1]);
+            *pdf_omega = dens / f.area;
+            return 1;
+        }
+    }
+    *pdf_omega = quad_area_pdf(L, r, L.p00 + uv[0] * L.ex + uv[1] * L.ey);
+    return 0;
+}
+// The proposal's solid-angle density at rectangle coordinates uv (the
+// emission-hit side of MIS).
+float or_quad_pdf(void *sp, uint32_t id, const float p[3], const float n[3], const float uv[2]) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    if (id >= s.lights.size() || s.lights[id].kind != 3) return -1.0f;
+    const Light &L = s.lights[id];
+    const V3 q = L.p00 + uv[0] * L.ex + uv[1] * L.ey;
+    return quad_pdf(L, {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, uv[0], uv[1], q);
+}
+// A sequence of hero events from unit densities: before step i, the balance
+// weight a / (a + b) of a sample drawn with densities pdf_i against the
+// alternative other_i; then the cumulative update with (num_i, pdf_i).
+// Returns the number of steps whose update succeeded.
+uint32_t or_hero_sequence(uint32_t steps, const float *beta0, const float *num, const float *pdf, const float *other,
+                          float *beta_out, float *weight_out) {
+    float density[4] = {1, 1, 1, 1}, beta[4];
+    for (int k = 0; k < 4; ++k) beta[k] = beta0[k];
+    for (uint32_t i = 0; i < steps; ++i) {
+        const float a = mixture(density, pdf + 4 * i), b = mixture(density, other + 4 * i);
+        weight_out[i] = a / (a + b);
+        if (!advance(density, beta, num + 4 * i, pdf + 4 * i)) return i;
+        for (int k = 0; k < 4; ++k) beta_out[4 * i + k] = beta[k];
+    }
+    return steps;
+}
 // What the light tables were built from (one LightSamplingInput per light)
 // and what they hold: alias links, and the light BVH's nodes.
 void or_light_inputs(void *sp, double *power, double *bounds_power, float *low, float *high, float *dir,

@@ -16,7 +16,13 @@
 //   ref_light_*          src/light_sampling.cpp:17-27 (power proxy), 294-327 (patch
 //                        rectangle / area), 329-381 (positional and normal cones),
 //                        382-571 (alias table and light BVH)
+//   ref_rect_*           src/light_geometry_sampling_shared.h:84-107,183-260 (bilinear
+//                        warp, rectangle frame, sample and inverse), composed as the
+//                        patch branch of shaders/light_geometry_sampling.h:114-209
+//   ref_hero_sequence    src/hero_spectrum_shared.h:102-160 (scaled likelihoods,
+//                        advance, balance)
 #include "hero_spectrum_shared.h"
+#include "light_geometry_sampling_shared.h"
 #include "light_sampling.hpp"
 #include "packet_spectrum_shared.h"
 #include "sampling.hpp"
@@ -162,4 +168,60 @@ REF_API int ref_build_light_sampling(uint32_t n, const double *power, const doub
         node_flags[k] = nd.flags;
     }
     return int(tables->nodes.size());
+}
+
+namespace lg = rypath_light_geometry_math;
+static lg::V3 v3(const float *p) { return lg::V3(p[0], p[1], p[2]); }
+
+// The solid-angle density of the patch proposal of rectangle (p00, p10, p01,
+// p11) (12 floats) at rectangle coordinates uv from `reference` with receiver
+// normal (zero: none), as the reference's pdf() for a patch: -1 when the
+// spherical rectangle is not used (not an affine rectangle, or 1e-4 sr or less).
+REF_API float ref_rect_pdf(const float *reference, const float *corners, const float *normal, const float *uv) {
+    const lg::V3 p[4] = {v3(corners), v3(corners + 3), v3(corners + 6), v3(corners + 9)};
+    const lg::V3 r = v3(reference), n = v3(normal);
+    const float area = lg::rectangle_solid_angle(p[0], p[1], p[2], p[3], r);
+    if (!(lg::rectangle(p[0], p[1], p[2], p[3]) && area > 1e-4f)) return -1.0f;
+    const auto frame = lg::rectangle_frame(r, p[0], p[1] - p[0], p[2] - p[0]);
+    const lg::V2 square = lg::rectangle_inverse(frame, lg::V2(uv[0], uv[1]));
+    return (lg::dot(n, n) > 0 ? lg::bilinear_pdf(square, lg::rectangle_cosines(p[0], p[1], p[2], p[3], r, n)) : 1.f) /
+           area;
+}
+// The patch proposal's sample: rectangle coordinates uv and their density.
+// Returns 0 (nothing written) when the spherical rectangle is not used.
+REF_API int ref_rect_sample(const float *reference, const float *corners, const float *normal, float u0, float u1,
+                            float *uv, float *pdf) {
+    const lg::V3 p[4] = {v3(corners), v3(corners + 3), v3(corners + 6), v3(corners + 9)};
+    const lg::V3 r = v3(reference), n = v3(normal);
+    lg::V2 u(std::min(u0, lg::one_minus_epsilon), std::min(u1, lg::one_minus_epsilon));
+    const float area = lg::rectangle_solid_angle(p[0], p[1], p[2], p[3], r);
+    if (!(lg::rectangle(p[0], p[1], p[2], p[3]) && area > 1e-4f)) return 0;
+    if (lg::dot(n, n) > 0) u = lg::bilinear_sample(u, lg::rectangle_cosines(p[0], p[1], p[2], p[3], r, n));
+    const lg::V2 q = lg::rectangle_sample(lg::rectangle_frame(r, p[0], p[1] - p[0], p[2] - p[0]), u);
+    uv[0] = q.x;
+    uv[1] = q.y;
+    *pdf = ref_rect_pdf(reference, corners, normal, uv);
+    return 1;
+}
+
+// A sequence of hero events from the initial density: before step i the
+// balance weight of densities pdf_i against other_i, then advance with
+// (num_i, pdf_i).  Returns the number of steps whose update was valid.
+REF_API uint32_t ref_hero_sequence(uint32_t steps, const float *beta0, const float *num, const float *pdf,
+                                   const float *other, float *beta_out, float *weight_out) {
+    rypath_hero::Density density = rypath_hero::initial_density();
+    rypath_hero::Vector beta = {beta0[0], beta0[1], beta0[2], beta0[3]};
+    for (uint32_t i = 0; i < steps; ++i) {
+        const rypath_hero::Vector pv = {pdf[4 * i], pdf[4 * i + 1], pdf[4 * i + 2], pdf[4 * i + 3]};
+        const rypath_hero::Vector ov = {other[4 * i], other[4 * i + 1], other[4 * i + 2], other[4 * i + 3]};
+        const rypath_hero::Vector nv = {num[4 * i], num[4 * i + 1], num[4 * i + 2], num[4 * i + 3]};
+        const rypath_hero::Scalar w = rypath_hero::balance(density, pv, ov);
+        weight_out[i] = w.valid ? w.value : -1.0f;
+        const rypath_hero::Update u = rypath_hero::advance(density, beta, nv, pv);
+        if (!u.valid) return i;
+        density = u.density;
+        beta = u.beta;
+        for (int k = 0; k < 4; ++k) beta_out[4 * i + k] = beta[k];
+    }
+    return steps;
 }

@@ -15,6 +15,14 @@ only calls them).  Where those sources are absent the tests skip.
   patch rectangle test, the normal cone and positional cone, and the alias
   table and light BVH that build_light_sampling makes from the oracle's
   per-light inputs, bit for bit, on scenes with many kinds of lights.
+* Quad emitters (src/light_geometry_sampling_shared.h): the spherical
+  rectangle proposal warped by the receiver's corner cosines, its density at
+  the sample and at an emission hit, against the reference's own functions
+  composed as its patch branch composes them (a planted unwarped density
+  fails it).
+* Hero balance bookkeeping (src/hero_spectrum_shared.h:102-160): cumulative
+  densities, throughputs and balance weights over event sequences, bit for
+  bit.
 """
 import numpy as np
 import pytest
@@ -225,3 +233,157 @@ def test_terrain_light_tables_equal_build_light_sampling(orc):
     mine = sc.light_nodes()
     for k in ("low", "high", "dir", "power", "cos_outer", "cos_emission", "parent", "light", "flags"):
         assert np.array_equal(mine[k], nodes[k]), k
+
+
+def one_quad_scene(rng):
+    """A single two-sided rectangle emitter of random size and orientation; its
+    corners (p00, p10, p01, p11) as the light tables take them (triangles
+    (a, b, c) and (a, c, d) give p00 = a, p10 = b, p01 = d, p11 = c)."""
+    from tests.scene_util import SceneBuilder
+
+    b = SceneBuilder(4, 4, 1)
+    c = rng.normal(size=3)
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    v = np.cross(u, rng.normal(size=3))
+    v /= np.linalg.norm(v)
+    w, h = 10 ** rng.uniform(-1, 0.5, 2)
+    e = b.emission(b.constant(1.0), two_sided=True)
+    o = b.object(b.diffuse(b.constant(0.0)), e)
+    b.quad(c - w * u - h * v, c + w * u - h * v, c + w * u + h * v, c - w * u + h * v, o)
+    b.look_at((0, 0, 10), (0, 0, 0), (0, 1, 0), 40)
+    d = b.build()
+    (ia, ib, ic), (_, _, idd) = b.indices[-2], b.indices[-1]
+    P = np.asarray(b.positions, np.float32)
+    corners = np.ascontiguousarray(np.concatenate([P[ia], P[ib], P[idd], P[ic]]), np.float32)
+    return d, corners, c, max(w, h)
+
+
+def quad_cases(seed, count):
+    """(scene, corners, receiver, normal, u) over receivers from 0.3 to 30
+    quad sizes away, with and without a receiver normal."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        d, corners, c, size = one_quad_scene(rng)
+        for _ in range(64):
+            dirn = rng.normal(size=3)
+            dirn /= np.linalg.norm(dirn)
+            p = (c + dirn * size * 10 ** rng.uniform(-0.5, 1.5)).astype(np.float32)
+            n = rng.normal(size=3)
+            n = (n / np.linalg.norm(n)).astype(np.float32) if rng.random() < 0.75 else np.zeros(3, np.float32)
+            out.append((d, corners, p, n, rng.random(2).astype(np.float32)))
+    return out
+
+
+def _quad_compare(orc, R, cases):
+    import ctypes as C
+
+    fp = C.POINTER(C.c_float)
+    orc.L.or_quad_sample.restype, orc.L.or_quad_sample.argtypes = C.c_int, [C.c_void_p, C.c_uint32, fp, fp, C.c_float,
+                                                                            C.c_float, fp, fp]
+    orc.L.or_quad_pdf.restype, orc.L.or_quad_pdf.argtypes = C.c_float, [C.c_void_p, C.c_uint32, fp, fp, fp]
+    R.L.ref_rect_sample.restype, R.L.ref_rect_sample.argtypes = C.c_int, [fp, fp, fp, C.c_float, C.c_float, fp, fp]
+    R.L.ref_rect_pdf.restype, R.L.ref_rect_pdf.argtypes = C.c_float, [fp, fp, fp, fp]
+    P = lambda a: a.ctypes.data_as(fp)  # noqa: E731
+    scenes = {}
+    duv, dpdf, dhit, area, used = [], [], [], [], 0
+    zero = np.zeros(3, np.float32)
+    for d, corners, p, n, u in cases:
+        key = id(d)
+        if key not in scenes:
+            scenes[key] = orc.scene(d)
+        sc = scenes[key]
+        uv_o, pdf_o = np.zeros(2, np.float32), np.zeros(1, np.float32)
+        uv_r, pdf_r = np.zeros(2, np.float32), np.zeros(1, np.float32)
+        mode_o = orc.L.or_quad_sample(sc.h, 0, P(p), P(n), float(u[0]), float(u[1]), P(uv_o), P(pdf_o))
+        mode_r = R.L.ref_rect_sample(P(p), P(corners), P(n), float(u[0]), float(u[1]), P(uv_r), P(pdf_r))
+        assert mode_o in (0, 1)
+        if mode_o != mode_r:
+            continue  # the 1e-4 sr threshold met at a rounding boundary (counted below)
+        if mode_r == 0:
+            continue
+        used += 1
+        duv.append(float(np.max(np.abs(uv_o - uv_r))))
+        dpdf.append(abs(float(pdf_o[0]) / float(pdf_r[0]) - 1))
+        # the emission-hit density at the oracle's sample against the reference's
+        want = R.L.ref_rect_pdf(P(p), P(corners), P(n), P(uv_o))
+        dhit.append(abs(orc.L.or_quad_pdf(sc.h, 0, P(p), P(n), P(uv_o)) / want - 1))
+        area.append(1.0 / R.L.ref_rect_pdf(P(p), P(corners), P(zero), P(uv_o)))  # the solid angle
+    return used, np.array(duv), np.array(dpdf), np.array(dhit), np.array(area)
+
+
+def test_rectangle_proposal_follows_the_reference(orc):
+    """The quad proposal (spherical rectangle warped by the receiver's corner
+    cosines) against the reference's own rectangle_frame / rectangle_sample /
+    rectangle_inverse / bilinear_sample / bilinear_pdf, composed as its patch
+    branch composes them.
+
+    Where the rectangle subtends 0.01 sr or more the samples and densities
+    agree to 2e-4.  Below that both sides lose digits in the same place: the
+    sample's first coordinate comes from u A - g2 - g3, with g2 + g3 close to
+    pi and A the solid angle, so a difference of one ulp of pi in the angles
+    (the oracle's series for asin, atan2, sin and cos against the host libm
+    the reference calls) moves it by 2^-22 / A.  There the bound is stated in
+    those units."""
+    R = ref()
+    cases = quad_cases(21, 3000)
+    used, duv, dpdf, dhit, area = _quad_compare(orc, R, cases)
+    assert used >= 0.9 * len(cases)
+    well = area >= 1e-2
+    assert well.sum() >= 1000
+    for d in (duv, dpdf, dhit):
+        assert d[well].max() < 2e-4, d[well].max()
+    assert np.max(duv * area / 2 ** -22) < 1000
+    assert np.quantile(dpdf, 0.99) < 1e-3 and np.quantile(dhit, 0.99) < 1e-3
+
+
+def test_rectangle_proposal_pin_sees_an_unwarped_density(orc):
+    """The same comparison with the oracle's cosine-warp density dropped (a
+    planted slip) fails by far more than its tolerance."""
+    R = ref()
+    cases = [c for c in quad_cases(22, 600) if np.any(c[3] != 0)]
+    with orc.planted_slip(12):
+        _, _, dpdf, dhit, _ = _quad_compare(orc, R, cases)
+    assert np.quantile(dhit, 0.5) > 1e-2
+
+
+def hero_sequences(rng, count, steps=8):
+    out = []
+    for _ in range(count):
+        pdf = np.exp(rng.normal(0, 2, (steps, 4))).astype(np.float32)
+        num = (pdf * np.exp(rng.normal(0, 0.5, (steps, 4)))).astype(np.float32)
+        other = np.exp(rng.normal(0, 2, (steps, 4))).astype(np.float32)
+        drop = rng.random((steps, 4)) < 0.1
+        drop[:, 0] = False  # the hero's own density is positive
+        pdf[drop] = 0  # a companion lane that cannot follow (Dirac, or a different Snell direction)
+        other[rng.random((steps, 4)) < 0.05] = 0
+        out.append((np.ones(4, np.float32), num, pdf, other))
+    return out
+
+
+def test_hero_balance_bookkeeping_equals_the_reference(orc):
+    """Cumulative hero densities and throughputs (advance) and the balance
+    weights over (technique x lane) equal the reference's scaled-likelihood
+    arithmetic bit for bit on event sequences of ordinary dynamic range
+    (densities renormalised by powers of two here, as (fraction, exponent)
+    pairs there)."""
+    R = ref()
+    import ctypes as C
+
+    fp = C.POINTER(C.c_float)
+    for L, name in ((orc.L, "or_hero_sequence"), (R.L, "ref_hero_sequence")):
+        fn = getattr(L, name)
+        fn.restype, fn.argtypes = C.c_uint32, [C.c_uint32, fp, fp, fp, fp, fp, fp]
+    P = lambda a: np.ascontiguousarray(a, np.float32).ctypes.data_as(fp)  # noqa: E731
+    rng = np.random.default_rng(31)
+    for beta0, num, pdf, other in hero_sequences(rng, 2000):
+        steps = len(pdf)
+        bo, wo = np.zeros((steps, 4), np.float32), np.zeros(steps, np.float32)
+        br, wr = np.zeros((steps, 4), np.float32), np.zeros(steps, np.float32)
+        args = [P(beta0), P(num), P(pdf), P(other)]
+        no = orc.L.or_hero_sequence(steps, *args, bo.ctypes.data_as(fp), wo.ctypes.data_as(fp))
+        nr = R.L.ref_hero_sequence(steps, *args, br.ctypes.data_as(fp), wr.ctypes.data_as(fp))
+        assert no == nr
+        assert np.array_equal(bo[:no], br[:nr]), (bo[:no], br[:nr])
+        assert np.array_equal(wo[:no], wr[:nr]), (wo[:no], wr[:nr])
