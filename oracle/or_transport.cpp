@@ -343,86 +343,202 @@ static Lane prepare_lane(const Scene &s, const rp_material &m, float lambda) {
     return l;
 }
 
-struct Cx {
-    float re, im;
+// ------------------------------------------------- microfacet BSDFs
+// The reference's binary32 forms (shaders/conductor.h:22-208,
+// shaders/dielectric.h:15-234, adapted from PBRT v4): a scaled unit vector,
+// compensated Fresnel terms, the rationalised refraction coefficient and
+// the compensated VNDF height.  Where the reference divides by a range scale
+// (unit vectors, complex ratios, Lambda's length) this multiplies by one
+// reciprocal: those quantities do not depend on the scale.  A form the
+// reference marks invalid yields NaN here, which fails the path's
+// arithmetic checks.
+struct Sum2 {
+    float s, e;
 };
-static Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-static Cx cdiv(Cx a, Cx b) {
-    float scale = 1.0f / (b.re * b.re + b.im * b.im);
-    return {scale * (a.re * b.re + a.im * b.im), scale * (a.im * b.re - a.re * b.im)};
+// Error-free sum of finite operands (Knuth's TwoSum).
+static Sum2 two_sum(float a, float b) {
+    const float s = a + b, bv = s - a;
+    return {s, (a - (s - bv)) + (b - bv)};
 }
-static float cnorm(Cx a) { return a.re * a.re + a.im * a.im; }
-static Cx csqrt(Cx z) {
-    float n = std::sqrt(cnorm(z));
-    if (n == 0.0f) return {0.0f, 0.0f};
-    float t1 = std::sqrt(0.5f * (n + std::fabs(z.re)));
-    float t2 = 0.5f * z.im / t1;
-    if (z.re >= 0.0f) return {t1, t2};
-    return {std::fabs(t2), std::copysign(t1, z.im)};
+static bool finite3(V3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+// v / |v| after scaling by the largest component (false: zero or not finite).
+static bool unit3(V3 v, V3 &out) {
+    const float s = std::max(std::max(std::fabs(v.x), std::fabs(v.y)), std::fabs(v.z));
+    if (!(s > 0) || !std::isfinite(s)) return false;
+    v = v * (1.0f / s);
+    out = v * (1.0f / std::sqrt(dot(v, v)));
+    return finite3(out);
 }
-// PBRT v4 FrComplex.
-static float fresnel_complex(float cos_i, float eta, float k) {
-    cos_i = std::min(std::max(cos_i, 0.0f), 1.0f);
-    Cx e = {eta, k};
-    float sin2_i = 1.0f - cos_i * cos_i;
-    Cx sin2_t = cdiv({sin2_i, 0.0f}, cmul(e, e));
-    Cx cos_t = csqrt({1.0f - sin2_t.re, -sin2_t.im});
-    Cx ec = {e.re * cos_i, e.im * cos_i};
-    Cx r_parl = cdiv({ec.re - cos_t.re, ec.im - cos_t.im}, {ec.re + cos_t.re, ec.im + cos_t.im});
-    Cx ect = cmul(e, cos_t);
-    Cx r_perp = cdiv({cos_i - ect.re, -ect.im}, {cos_i + ect.re, ect.im});
-    return (cnorm(r_parl) + cnorm(r_perp)) * 0.5f;
+// |a|^2 / |b|^2 of two complex numbers, scaled by b's larger component.
+static bool ratio_norm(float ar, float ai, float br, float bi, float &out) {
+    const float s = std::max(std::fabs(br), std::fabs(bi));
+    if (!(s > 0) || !std::isfinite(s) || !std::isfinite(ar) || !std::isfinite(ai)) return false;
+    const float r = 1.0f / s;
+    ar = ar * r, ai = ai * r, br = br * r, bi = bi * r;
+    out = (ar * ar + ai * ai) / (br * br + bi * bi);
+    return std::isfinite(out);
 }
-// PBRT v4 FrDielectric.
-static float fresnel_dielectric(float cos_i, float eta) {
-    cos_i = std::min(std::max(cos_i, -1.0f), 1.0f);
-    if (cos_i < 0.0f) {
-        eta = 1.0f / eta;
-        cos_i = -cos_i;
+// Conductor Fresnel reflectance at cos(theta) for eta + i k: the complex
+// Snell root sqrt(n^2 - sin^2) with the three squares and their sum
+// compensated (shaders/conductor.h:57-98).
+static float fresnel_complex(float cosine, float eta, float k) {
+    if (!std::isfinite(cosine) || !std::isfinite(eta) || !std::isfinite(k) || eta < 0 || k < 0 ||
+        (eta == 0 && k == 0))
+        return NAN;
+    const float c = std::min(std::max(cosine, 0.0f), 1.0f);
+    if (eta == 1 && k == 0) return 0.0f;
+    if (c == 0) return 1.0f;
+    const float n2x = eta * eta - k * k, n2y = 2 * eta * k;
+    if (!std::isfinite(n2x) || !std::isfinite(n2y)) return NAN;
+    const float eta2 = eta * eta, k2 = k * k, c2 = c * c;
+    const Sum2 ab = two_sum(eta2, -k2), cd = two_sum(c2, -1.0f), total = two_sum(ab.s, cd.s);
+    const float correction = ab.e + cd.e + total.e + std::fma(eta, eta, -eta2) - std::fma(k, k, -k2) +
+                             std::fma(c, c, -c2);
+    const float zx = total.s + correction, zy = n2y;
+    const float scale = std::max(std::fabs(zx), std::fabs(zy));
+    float radius = 0.0f;
+    if (scale != 0) {
+        const float r = 1.0f / scale, sx = zx * r, sy = zy * r;
+        radius = scale * std::sqrt(sx * sx + sy * sy);
     }
-    float sin2_i = 1.0f - cos_i * cos_i;
-    float sin2_t = sin2_i / (eta * eta);
-    if (sin2_t >= 1.0f) return 1.0f;
-    float cos_t = std::sqrt(std::max(0.0f, 1.0f - sin2_t));
-    float r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
-    float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
-    return (r_parl * r_parl + r_perp * r_perp) * 0.5f;
+    if (!std::isfinite(radius)) return NAN;
+    float qr, qi;
+    if (zx >= 0) {
+        qr = std::sqrt(std::max(0.0f, 0.5f * radius + 0.5f * zx));
+        qi = qr > 0 ? zy / (2 * qr) : 0.0f;
+    } else {
+        qi = std::sqrt(std::max(0.0f, 0.5f * radius - 0.5f * zx));
+        qr = qi > 0 ? zy / (2 * qi) : 0.0f;
+    }
+    float rs, rp;
+    if (!ratio_norm(c - qr, 0.0f - qi, c + qr, qi, rs) ||
+        !ratio_norm(n2x * c - qr, n2y * c - qi, n2x * c + qr, n2y * c + qi, rp))
+        return NAN;
+    const float f = (rs + rp) * 0.5f;
+    return std::isfinite(f) && f >= 0 ? f : NAN;
 }
-// Trowbridge-Reitz D and Smith Lambda (shaders/conductor.h:72-90 forms).
+// Dielectric optics at cos(theta) (negative: from inside) for relative index
+// eta (shaders/dielectric.h:29-71): reflectance F, relative index etap of the
+// side the ray enters, transmitted cosine ct, total internal reflection.
+// The Snell discriminant is compensated, with the authored eta kept on both
+// sides.
+struct Optics {
+    float F, etap, ct;
+    bool tir, ok;
+};
+static Optics optics(float cosine, float eta) {
+    Optics o = {0.0f, 1.0f, 0.0f, false, false};
+    if (!std::isfinite(cosine)) return o;
+    o.etap = cosine < 0 ? 1 / eta : eta;
+    const float c = std::min(std::fabs(cosine), 1.0f);
+    if (!(o.etap > 0) || !std::isfinite(o.etap)) return o;
+    if (o.etap == 1) {
+        o.ct = c;
+        o.ok = true;
+        return o;
+    }
+    const float c2 = c * c, e2 = eta * eta;
+    float disc;
+    if (cosine < 0) {
+        const Sum2 a = two_sum(1.0f, -e2);
+        disc = std::fma(e2, c2, a.s) +
+               (a.e + std::fma(eta, eta, -e2) * (c2 - 1) + e2 * std::fma(c, c, -c2));
+    } else {
+        const Sum2 a = two_sum(e2, -1.0f), b = two_sum(a.s, c2);
+        disc = b.s + (a.e + b.e + std::fma(eta, eta, -e2) + std::fma(c, c, -c2));
+    }
+    if (!std::isfinite(disc)) return o;
+    o.tir = disc <= 0;
+    if (o.tir) {
+        o.F = 1.0f;
+        o.ok = true;
+        return o;
+    }
+    o.ct = std::sqrt(disc) / (cosine < 0 ? 1.0f : eta);
+    const float rp = (o.etap * c - o.ct) / (o.etap * c + o.ct);
+    const float rs = (c - o.etap * o.ct) / (c + o.etap * o.ct);
+    o.F = 0.5f * (rp * rp + rs * rs);
+    o.ok = std::isfinite(o.ct) && std::isfinite(o.F) && o.F >= 0 && o.F <= 1;
+    return o;
+}
+static float fresnel_dielectric(float cosine, float eta) {
+    const Optics o = optics(cosine, eta);
+    return o.ok ? o.F : NAN;
+}
+// c / etap - ct written with the authored index on both sides
+// (shaders/dielectric.h:125-130).
+static float refraction_coefficient(float eta, float co, float ct) {
+    const float c = std::fabs(co);
+    return co < 0 ? ((eta - 1) * (eta + 1)) / (eta * c + ct) : (((1 - eta) * (1 + eta)) / eta) / (c + eta * ct);
+}
+// Trowbridge-Reitz D with PBRT's grazing cutoff, and Smith Lambda.
 static float ggx_d(const Lane &l, V3 m) {
     if (m.z * m.z * m.z * m.z < 1e-16f) return 0.0f;
     V3 v = {m.x / l.ax, m.y / l.ay, m.z};
     float t = dot(v, v);
-    return 1.0f / (kPi * l.ax * l.ay * t * t);
+    const float den = kPi * l.ax * l.ay * t * t;
+    if (!std::isfinite(den) || !(den > 0)) return NAN;
+    const float d = 1.0f / den;
+    return d > 0 ? d : NAN;
 }
 static float ggx_lambda(const Lane &l, V3 w) {
-    V3 v = {1.0f, l.ax * w.x / w.z, l.ay * w.y / w.z};
-    return 0.5f * (length(v) - 1.0f);
+    const V3 v = {1.0f, l.ax * w.x / w.z, l.ay * w.y / w.z};
+    const float s = std::max(std::max(std::fabs(v.x), std::fabs(v.y)), std::fabs(v.z));
+    return 0.5f * (s * length(v * (1.0f / s)) - 1.0f);
 }
-// Heitz 2018 visible-normal sampling (PBRT v4 Sample_wm).
-static V3 ggx_sample_wm(const Lane &l, V3 w, float u0, float u1) {
-    V3 wh = normalize(V3{l.ax * w.x, l.ay * w.y, w.z});
+// Heitz 2018 visible-normal sampling; the height over the warped disk keeps
+// 1 - u at the disk's edge (shaders/conductor.h:153-178).  False: invalid.
+static bool ggx_sample_wm(const Lane &l, V3 w, float u0, float u1, V3 &m) {
+    V3 wh;
+    if (!unit3(V3{l.ax * w.x, l.ay * w.y, w.z}, wh)) return false;
     if (wh.z < 0) wh = -wh;
-    V3 t1 = wh.z < 0.99999f ? normalize(V3{-wh.y, wh.x, 0.0f}) : V3{1.0f, 0.0f, 0.0f};
-    V3 t2 = cross(wh, t1);
-    float r = std::sqrt(u0);
+    V3 t1 = {1.0f, 0.0f, 0.0f};
+    if (wh.z < 0.99999f && !unit3(V3{-wh.y, wh.x, 0.0f}, t1)) return false;
+    const V3 t2 = cross3(wh, t1);
+    const float r = std::sqrt(u0);
     float sn, cs;
     sincos_turns(u1, sn, cs);
-    float px = r * cs, py = r * sn;
-    float h = std::sqrt(1.0f - px * px);
-    float a = (1.0f + wh.z) * 0.5f;
-    py = (1.0f - a) * h + a * py;
-    float pz = std::sqrt(std::max(0.0f, 1.0f - (px * px + py * py)));
-    V3 nh = px * t1 + py * t2 + pz * wh;
-    return normalize(V3{l.ax * nh.x, l.ay * nh.y, std::max(1e-6f, nh.z)});
+    const float dx = r * cs, old_y = r * sn;
+    const float h = std::sqrt(std::max(0.0f, 1.0f - dx * dx)), a = (1.0f + wh.z) * 0.5f;
+    const float dy = (1.0f - a) * h + a * old_y;
+    const float old_minus_h = old_y > 0 ? -(1.0f - u0) / (h + old_y) : old_y - h;
+    const float z = std::sqrt(std::max(0.0f, (1.0f - u0) + (1.0f - a) * old_minus_h * (old_y + dy)));
+    const V3 nh = dx * t1 + dy * t2 + z * wh;
+    return unit3(V3{l.ax * nh.x, l.ay * nh.y, std::max(1e-6f, nh.z)}, m);
 }
-
 struct Eval {
     float f = 0, pdf = 0;
 };
-
+// Rough dielectric at (wo, wi) with its half vector (shaders/dielectric.h:72-124).
+static Eval dielectric_eval(const Lane &l, V3 wo, V3 wi) {
+    const float nan = NAN;
+    const bool reflection = (wo.z > 0) == (wi.z > 0), entering = wo.z > 0;
+    const V3 h = reflection ? wo + wi
+                            : (entering ? V3{std::fma(l.eta, wi.x, wo.x), std::fma(l.eta, wi.y, wo.y),
+                                             std::fma(l.eta, wi.z, wo.z)}
+                                        : V3{std::fma(l.eta, wo.x, wi.x), std::fma(l.eta, wo.y, wi.y),
+                                             std::fma(l.eta, wo.z, wi.z)});
+    if (h.x == 0 && h.y == 0 && h.z == 0) return {0.0f, 0.0f};
+    V3 m;
+    if (!unit3(h, m)) return {nan, nan};
+    if (m.z < 0) m = -m;
+    const float co = dot(wo, m), ci = dot(wi, m);
+    if (co * wo.z < 0 || ci * wi.z < 0) return {0.0f, 0.0f};
+    const Optics o = optics(co, l.eta);
+    if (!o.ok) return {nan, nan};
+    const float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), d = ggx_d(l, m);
+    if (!std::isfinite(lo) || !std::isfinite(li) || !std::isfinite(d)) return {nan, nan};
+    if (d == 0) return {0.0f, 0.0f};
+    if (reflection)
+        return {d * o.F / (4 * std::fabs(wo.z) * std::fabs(wi.z) * (1 + lo + li)), d * o.F / (4 * std::fabs(wo.z) * (1 + lo))};
+    const float den = dot(h, m) / (entering ? l.eta : 1.0f);
+    const float jacobian = std::fabs(ci) / (den * den);
+    const float pdf = d * std::fabs(co) / (std::fabs(wo.z) * (1 + lo)) * jacobian * (1 - o.F);
+    const float scale = entering ? 1 / (l.eta * l.eta) : l.eta * l.eta;
+    return {d * (1 - o.F) * std::fabs(ci * co) * scale / (std::fabs(wo.z) * std::fabs(wi.z) * den * den * (1 + lo + li)), pdf};
+}
 static Eval eval_lane(const Lane &l, V3 wo, V3 wi) {
-    Eval e;
+    Eval e = {0.0f, 0.0f};
     if (l.kind == RP_MATERIAL_DIFFUSE) {
         if (!(wo.z * wi.z > 0)) return e;
         e.f = l.r * kInvPi;
@@ -431,66 +547,23 @@ static Eval eval_lane(const Lane &l, V3 wo, V3 wi) {
     }
     if (l.smooth || wo.z == 0.0f || wi.z == 0.0f) return e;
     if (l.kind == RP_MATERIAL_CONDUCTOR) {
-        if (!(wo.z * wi.z > 0)) return e;
-        V3 h = wo + wi;
-        if (dot(h, h) == 0.0f) return e;
-        V3 m = normalize(h);
-        float F = fresnel_complex(std::fabs(dot(wo, m)), l.eta, l.k);
-        float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+        if ((wo.z > 0) != (wi.z > 0)) return e;
+        V3 m;
+        if (!unit3(wo + wi, m)) return {NAN, NAN};
+        const float F = fresnel_complex(std::fabs(dot(wo, m)), l.eta, l.k);
+        const float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
         e.f = D * F / (4.0f * std::fabs(wo.z) * std::fabs(wi.z) * (1.0f + lo + li));
         e.pdf = D / (4.0f * std::fabs(wo.z) * (1.0f + lo));
         return e;
     }
-    // rough dielectric
-    bool reflect = wo.z * wi.z > 0;
-    float etap = 1.0f;
-    if (!reflect) etap = wo.z > 0 ? l.eta : 1.0f / l.eta;
-    V3 h = wi * etap + wo;
-    if (dot(h, h) == 0.0f) return e;
-    V3 m = normalize(h);
-    if (m.z < 0) m = -m;
-    float dmi = dot(m, wi), dmo = dot(m, wo);
-    if (dmi * wi.z < 0 || dmo * wo.z < 0) return e;
-    float F = fresnel_dielectric(dmo, l.eta);
-    float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
-    float G = 1.0f / (1.0f + lo + li);
-    float pdf_wm = D / (std::fabs(wo.z) * (1.0f + lo)) * std::fabs(dmo);
-    float R = F, T = 1.0f - F;
-    if (reflect) {
-        e.f = D * G * F / std::fabs(4.0f * wi.z * wo.z);
-        e.pdf = pdf_wm / (4.0f * std::fabs(dmo)) * (R / (R + T));
-    } else {
-        float s = dmi + dmo / etap;
-        float denom = s * s * wi.z * wo.z;
-        float ft = D * T * G * std::fabs(dmi * dmo / denom);
-        e.f = ft / (etap * etap);
-        e.pdf = pdf_wm * (std::fabs(dmi) / (s * s)) * (T / (R + T));
-    }
-    return e;
+    return dielectric_eval(l, wo, wi);
 }
-
 struct Sampled {
     V3 wi;
     float pdf = 0;
     float eta = 1; // relative eta of a transmission event (1 otherwise)
     bool delta = false, transmission = false, ok = false;
 };
-static bool refract(V3 wi, V3 n, float eta, float &etap, V3 &wt) {
-    float cos_i = dot(n, wi);
-    if (cos_i < 0) {
-        eta = 1.0f / eta;
-        cos_i = -cos_i;
-        n = -n;
-    }
-    float sin2_i = std::max(0.0f, 1.0f - cos_i * cos_i);
-    float sin2_t = sin2_i / (eta * eta);
-    if (sin2_t >= 1.0f) return false;
-    float cos_t = std::sqrt(std::max(0.0f, 1.0f - sin2_t));
-    wt = -wi / eta + (cos_i / eta - cos_t) * n;
-    etap = eta;
-    return true;
-}
-
 // Concentric-disk cosine hemisphere (PBRT v4 SampleCosineHemisphere).
 static V3 cosine_hemisphere(float u0, float u1) {
     float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
@@ -515,6 +588,10 @@ static V3 cosine_hemisphere(float u0, float u1) {
 
 static Sampled sample_lane(const Lane &l, V3 wo, float uc, float u0, float u1) {
     Sampled s;
+    s.wi = V3{0.0f, 0.0f, 0.0f};
+    s.pdf = 0.0f;
+    s.eta = 1.0f;
+    s.delta = s.transmission = s.ok = false;
     if (wo.z == 0.0f) return s;
     if (l.kind == RP_MATERIAL_DIFFUSE) {
         V3 wi = cosine_hemisphere(u0, u1);
@@ -532,63 +609,57 @@ static Sampled sample_lane(const Lane &l, V3 wo, float uc, float u0, float u1) {
             s.ok = true;
             return s;
         }
-        V3 m = ggx_sample_wm(l, wo, u0, u1);
-        V3 wi = 2.0f * dot(wo, m) * m - wo;
-        if (!(wo.z * wi.z > 0)) return s;
-        float lo = ggx_lambda(l, wo);
-        float D = ggx_d(l, m);
-        if (D == 0.0f) return s;
+        V3 m;
+        if (!ggx_sample_wm(l, wo, u0, u1, m)) return s;
+        const V3 wi = -wo + 2.0f * dot(wo, m) * m;
+        if (!finite3(wi) || wi.z == 0.0f || (wo.z > 0) != (wi.z > 0)) return s;
+        const float lo = ggx_lambda(l, wo), D = ggx_d(l, m);
+        if (!(D > 0)) return s;
         s.wi = wi;
         s.pdf = D / (4.0f * std::fabs(wo.z) * (1.0f + lo));
-        s.ok = s.pdf > 0;
+        s.ok = std::isfinite(s.pdf) && s.pdf > 0;
         return s;
     }
-    // dielectric
-    if (l.smooth) {
-        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
-        if (uc < R / (R + T)) {
-            s.wi = {-wo.x, -wo.y, wo.z};
-            s.pdf = R / (R + T);
-        } else {
-            float etap;
-            V3 wt;
-            if (!refract(wo, {0.0f, 0.0f, 1.0f}, l.eta, etap, wt)) return s;
-            s.wi = wt;
-            s.pdf = T / (R + T);
-            s.eta = etap;
-            s.transmission = true;
+    // dielectric (shaders/dielectric.h:131-228): the lobe from the Fresnel
+    // reflectance at the sampled microfacet normal (the geometric normal when
+    // smooth)
+    V3 m = {0.0f, 0.0f, 1.0f};
+    if (!l.smooth && !ggx_sample_wm(l, wo, u0, u1, m)) return s;
+    const float co = dot(wo, m);
+    const Optics o = optics(co, l.eta);
+    if (!o.ok) return s;
+    const bool reflection = uc < o.F;
+    V3 wi;
+    if (reflection) wi = -wo + 2.0f * co * m;
+    else {
+        if (o.tir) return s;
+        const float coefficient = l.eta == 1 ? 0.0f : refraction_coefficient(l.eta, co, o.ct);
+        wi = (co < 0 ? (-l.eta) * wo : (-wo) / l.eta) + coefficient * (co < 0 ? -m : m);
+    }
+    if (!finite3(wi) || wi.z == 0.0f || ((wi.z > 0) == (wo.z > 0)) != reflection) return s;
+    float pdf;
+    if (l.smooth) pdf = reflection ? o.F : 1 - o.F;
+    else {
+        const float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+        if (!std::isfinite(lo) || !std::isfinite(li) || !std::isfinite(D) || D == 0) return s;
+        if (reflection) pdf = D * o.F / (4 * std::fabs(wo.z) * (1 + lo));
+        else {
+            const float den = refraction_coefficient(l.eta, co, o.ct);
+            pdf = D * std::fabs(co) * o.ct * (1 - o.F) / (std::fabs(wo.z) * (1 + lo) * den * den);
         }
-        s.delta = true;
-        s.ok = s.pdf > 0 && s.wi.z != 0.0f;
-        return s;
     }
-    V3 m = ggx_sample_wm(l, wo, u0, u1);
-    float R = fresnel_dielectric(dot(wo, m), l.eta), T = 1.0f - R;
-    float lo = ggx_lambda(l, wo), D = ggx_d(l, m);
-    float pdf_wm = D / (std::fabs(wo.z) * (1.0f + lo)) * std::fabs(dot(wo, m));
-    if (uc < R / (R + T)) {
-        V3 wi = 2.0f * dot(wo, m) * m - wo;
-        if (!(wo.z * wi.z > 0)) return s;
-        s.wi = wi;
-        s.pdf = pdf_wm / (4.0f * std::fabs(dot(wo, m))) * (R / (R + T));
-    } else {
-        float etap;
-        V3 wi;
-        if (!refract(wo, m, l.eta, etap, wi)) return s;
-        if (wo.z * wi.z > 0 || wi.z == 0.0f) return s;
-        float sd = dot(wi, m) + dot(wo, m) / etap;
-        float dwm_dwi = std::fabs(dot(wi, m)) / (sd * sd);
-        s.wi = wi;
-        s.pdf = pdf_wm * dwm_dwi * (T / (R + T));
-        s.eta = etap;
-        s.transmission = true;
-    }
-    s.ok = s.pdf > 0 && D > 0;
+    s.wi = wi;
+    s.pdf = pdf;
+    s.delta = l.smooth;
+    s.transmission = !reflection;
+    s.eta = reflection ? 1.0f : o.etap;
+    s.ok = std::isfinite(pdf) && pdf > 0;
     return s;
 }
-
-// Companion-lane mass and density of a Dirac event (shaders/hero_scattering.h:98-150).
+// A companion lane's mass and density for the hero's Dirac direction wi: the
+// lane keeps mass only if its own specular direction is bit for bit wi.
 static void dirac_lane(const Lane &l, V3 wo, V3 wi, float uc, float u0, float u1, float &mass, float &pdf) {
+    (void)uc;
     mass = pdf = 0.0f;
     if (wo.z == 0.0f || wi.z == 0.0f) return;
     bool transmission = wo.z * wi.z < 0;
@@ -599,13 +670,13 @@ static void dirac_lane(const Lane &l, V3 wo, V3 wi, float uc, float u0, float u1
         return;
     }
     if (l.kind == RP_MATERIAL_DIELECTRIC && l.smooth) {
-        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
-        Sampled c = sample_lane(l, wo, transmission ? R / (R + T) : 0.0f, u0, u1);
-        (void)uc;
+        // the lobe forced by the lobe uniform: F selects transmission, 0 reflection
+        const float F = fresnel_dielectric(wo.z, l.eta);
+        Sampled c = sample_lane(l, wo, transmission ? F : 0.0f, u0, u1);
         if (!c.ok || !(c.wi.x == wi.x && c.wi.y == wi.y && c.wi.z == wi.z)) return;
         pdf = c.pdf;
         float scale = transmission ? 1.0f / (c.eta * c.eta) : 1.0f;
-        mass = (transmission ? T : R) * scale;
+        mass = (transmission ? 1 - F : F) * scale;
     }
 }
 

@@ -21,12 +21,18 @@
 //                        patch branch of shaders/light_geometry_sampling.h:114-209
 //   ref_hero_sequence    src/hero_spectrum_shared.h:102-160 (scaled likelihoods,
 //                        advance, balance)
+//   ref_conductor_*      tests/conductor_reference.hpp:24-103 (double complex Fresnel,
+//                        Smith GGX, Heitz VNDF; the reference's own test model)
+//   ref_dielectric_*     tests/dielectric_reference.hpp:27-139 (double rough dielectric)
 #include "hero_spectrum_shared.h"
 #include "light_geometry_sampling_shared.h"
 #include "light_sampling.hpp"
 #include "packet_spectrum_shared.h"
 #include "sampling.hpp"
 #include "sobol_shared.h"
+// the reference's double-precision test models of its two microfacet BSDFs
+#include "tests/conductor_reference.hpp"
+#include "tests/dielectric_reference.hpp"
 
 #include <array>
 #include <cstdint>
@@ -224,4 +230,39 @@ REF_API uint32_t ref_hero_sequence(uint32_t steps, const float *beta0, const flo
         for (int k = 0; k < 4; ++k) beta_out[4 * i + k] = beta[k];
     }
     return steps;
+}
+
+// The reference's double models: evaluate (f, pdf) and sample (direction,
+// weight f |cos| / pdf, pdf; returns whether the sample was accepted).
+namespace cr = conductor_reference;
+namespace dr = dielectric_reference;
+REF_API double ref_conductor_fresnel(double c, double eta, double k) { return cr::fresnel(c, eta, k); }
+REF_API double ref_dielectric_fresnel(double c, double eta) { return dr::fresnel(c, eta); }
+REF_API void ref_conductor_eval(double eta, double k, double ax, double ay, int remap, const double *wo,
+                                const double *wi, double *f, double *pdf) {
+    const auto e = cr::evaluate(cr::prepare(eta, k, ax, ay, remap != 0), {wo[0], wo[1], wo[2]}, {wi[0], wi[1], wi[2]});
+    *f = e.f;
+    *pdf = e.pdf;
+}
+REF_API int ref_conductor_sample(double eta, double k, double ax, double ay, int remap, const double *wo, double u,
+                                 double v, double *wi, double *weight, double *pdf) {
+    const auto r = cr::sample(cr::prepare(eta, k, ax, ay, remap != 0), {wo[0], wo[1], wo[2]}, u, v);
+    wi[0] = r.i.x, wi[1] = r.i.y, wi[2] = r.i.z;
+    *weight = r.weight;
+    *pdf = r.pdf;
+    return r.accepted ? 1 : 0;
+}
+REF_API void ref_dielectric_eval(double eta, double ax, double ay, int remap, const double *wo, const double *wi,
+                                 double *f, double *pdf) {
+    const auto e = dr::evaluate(dr::prepare(eta, ax, ay, remap != 0), {wo[0], wo[1], wo[2]}, {wi[0], wi[1], wi[2]});
+    *f = e.f;
+    *pdf = e.pdf;
+}
+REF_API int ref_dielectric_sample(double eta, double ax, double ay, int remap, const double *wo, double uc, double u,
+                                  double v, double *wi, double *weight, double *pdf) {
+    const auto r = dr::sample(dr::prepare(eta, ax, ay, remap != 0), {wo[0], wo[1], wo[2]}, uc, u, v);
+    wi[0] = r.i.x, wi[1] = r.i.y, wi[2] = r.i.z;
+    *weight = r.weight;
+    *pdf = r.pdf;
+    return r.accepted ? 1 : 0;
 }

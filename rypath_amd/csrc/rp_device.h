@@ -785,78 +785,200 @@ RP_DEV Lane prepare_lane(const Scene &S, const rp_material &m, float lambda) {
     }
     return l;
 }
-struct Cx {
-    float re, im;
+// ------------------------------------------------- microfacet BSDFs
+// The reference's binary32 forms (shaders/conductor.h:22-208,
+// shaders/dielectric.h:15-234, adapted from PBRT v4): a scaled unit vector,
+// compensated Fresnel terms, the rationalised refraction coefficient and
+// the compensated VNDF height.  Where the reference divides by a range scale
+// (unit vectors, complex ratios, Lambda's length) this multiplies by one
+// reciprocal: those quantities do not depend on the scale.  A form the
+// reference marks invalid yields NaN here, which fails the path's
+// arithmetic checks.
+struct Sum2 {
+    float s, e;
 };
-RP_DEV Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-RP_DEV Cx cdiv(Cx a, Cx b) {
-    float scale = 1.0f / (b.re * b.re + b.im * b.im);
-    return {scale * (a.re * b.re + a.im * b.im), scale * (a.im * b.re - a.re * b.im)};
+// Error-free sum of finite operands (Knuth's TwoSum).
+RP_DEV Sum2 two_sum(float a, float b) {
+    const float s = a + b, bv = s - a;
+    return {s, (a - (s - bv)) + (b - bv)};
 }
-RP_DEV float cnorm(Cx a) { return a.re * a.re + a.im * a.im; }
-RP_DEV Cx csqrt(Cx z) {
-    float n = __builtin_sqrtf(cnorm(z));
-    if (n == 0.0f) return {0.0f, 0.0f};
-    float t1 = __builtin_sqrtf(0.5f * (n + fabsf(z.re)));
-    float t2 = 0.5f * z.im / t1;
-    if (z.re >= 0.0f) return {t1, t2};
-    return {fabsf(t2), copysignf(t1, z.im)};
+RP_DEV bool finite3(F3 v) { return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z); }
+// v / |v| after scaling by the largest component (false: zero or not finite).
+RP_DEV bool unit3(F3 v, F3 &out) {
+    const float s = omax(omax(fabsf(v.x), fabsf(v.y)), fabsf(v.z));
+    if (!(s > 0) || !__builtin_isfinite(s)) return false;
+    v = v * (1.0f / s);
+    out = v * (1.0f / __builtin_sqrtf(dot(v, v)));
+    return finite3(out);
 }
-RP_DEV float fresnel_complex(float cos_i, float eta, float k) {
-    cos_i = omin(omax(cos_i, 0.0f), 1.0f);
-    Cx e = {eta, k};
-    float sin2_i = 1.0f - cos_i * cos_i;
-    Cx sin2_t = cdiv({sin2_i, 0.0f}, cmul(e, e));
-    Cx cos_t = csqrt({1.0f - sin2_t.re, -sin2_t.im});
-    Cx ec = {e.re * cos_i, e.im * cos_i};
-    Cx r_parl = cdiv({ec.re - cos_t.re, ec.im - cos_t.im}, {ec.re + cos_t.re, ec.im + cos_t.im});
-    Cx ect = cmul(e, cos_t);
-    Cx r_perp = cdiv({cos_i - ect.re, -ect.im}, {cos_i + ect.re, ect.im});
-    return (cnorm(r_parl) + cnorm(r_perp)) * 0.5f;
+// |a|^2 / |b|^2 of two complex numbers, scaled by b's larger component.
+RP_DEV bool ratio_norm(float ar, float ai, float br, float bi, float &out) {
+    const float s = omax(fabsf(br), fabsf(bi));
+    if (!(s > 0) || !__builtin_isfinite(s) || !__builtin_isfinite(ar) || !__builtin_isfinite(ai)) return false;
+    const float r = 1.0f / s;
+    ar = ar * r, ai = ai * r, br = br * r, bi = bi * r;
+    out = (ar * ar + ai * ai) / (br * br + bi * bi);
+    return __builtin_isfinite(out);
 }
-RP_DEV float fresnel_dielectric(float cos_i, float eta) {
-    cos_i = omin(omax(cos_i, -1.0f), 1.0f);
-    if (cos_i < 0.0f) {
-        eta = 1.0f / eta;
-        cos_i = -cos_i;
+// Conductor Fresnel reflectance at cos(theta) for eta + i k: the complex
+// Snell root sqrt(n^2 - sin^2) with the three squares and their sum
+// compensated (shaders/conductor.h:57-98).
+RP_DEV float fresnel_complex(float cosine, float eta, float k) {
+    if (!__builtin_isfinite(cosine) || !__builtin_isfinite(eta) || !__builtin_isfinite(k) || eta < 0 || k < 0 ||
+        (eta == 0 && k == 0))
+        return __builtin_nanf("");
+    const float c = omin(omax(cosine, 0.0f), 1.0f);
+    if (eta == 1 && k == 0) return 0.0f;
+    if (c == 0) return 1.0f;
+    const float n2x = eta * eta - k * k, n2y = 2 * eta * k;
+    if (!__builtin_isfinite(n2x) || !__builtin_isfinite(n2y)) return __builtin_nanf("");
+    const float eta2 = eta * eta, k2 = k * k, c2 = c * c;
+    const Sum2 ab = two_sum(eta2, -k2), cd = two_sum(c2, -1.0f), total = two_sum(ab.s, cd.s);
+    const float correction = ab.e + cd.e + total.e + __builtin_fmaf(eta, eta, -eta2) - __builtin_fmaf(k, k, -k2) +
+                             __builtin_fmaf(c, c, -c2);
+    const float zx = total.s + correction, zy = n2y;
+    const float scale = omax(fabsf(zx), fabsf(zy));
+    float radius = 0.0f;
+    if (scale != 0) {
+        const float r = 1.0f / scale, sx = zx * r, sy = zy * r;
+        radius = scale * __builtin_sqrtf(sx * sx + sy * sy);
     }
-    float sin2_i = 1.0f - cos_i * cos_i;
-    float sin2_t = sin2_i / (eta * eta);
-    if (sin2_t >= 1.0f) return 1.0f;
-    float cos_t = __builtin_sqrtf(omax(0.0f, 1.0f - sin2_t));
-    float r_parl = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
-    float r_perp = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
-    return (r_parl * r_parl + r_perp * r_perp) * 0.5f;
+    if (!__builtin_isfinite(radius)) return __builtin_nanf("");
+    float qr, qi;
+    if (zx >= 0) {
+        qr = __builtin_sqrtf(omax(0.0f, 0.5f * radius + 0.5f * zx));
+        qi = qr > 0 ? zy / (2 * qr) : 0.0f;
+    } else {
+        qi = __builtin_sqrtf(omax(0.0f, 0.5f * radius - 0.5f * zx));
+        qr = qi > 0 ? zy / (2 * qi) : 0.0f;
+    }
+    float rs, rp;
+    if (!ratio_norm(c - qr, 0.0f - qi, c + qr, qi, rs) ||
+        !ratio_norm(n2x * c - qr, n2y * c - qi, n2x * c + qr, n2y * c + qi, rp))
+        return __builtin_nanf("");
+    const float f = (rs + rp) * 0.5f;
+    return __builtin_isfinite(f) && f >= 0 ? f : __builtin_nanf("");
 }
+// Dielectric optics at cos(theta) (negative: from inside) for relative index
+// eta (shaders/dielectric.h:29-71): reflectance F, relative index etap of the
+// side the ray enters, transmitted cosine ct, total internal reflection.
+// The Snell discriminant is compensated, with the authored eta kept on both
+// sides.
+struct Optics {
+    float F, etap, ct;
+    bool tir, ok;
+};
+RP_DEV Optics optics(float cosine, float eta) {
+    Optics o = {0.0f, 1.0f, 0.0f, false, false};
+    if (!__builtin_isfinite(cosine)) return o;
+    o.etap = cosine < 0 ? 1 / eta : eta;
+    const float c = omin(fabsf(cosine), 1.0f);
+    if (!(o.etap > 0) || !__builtin_isfinite(o.etap)) return o;
+    if (o.etap == 1) {
+        o.ct = c;
+        o.ok = true;
+        return o;
+    }
+    const float c2 = c * c, e2 = eta * eta;
+    float disc;
+    if (cosine < 0) {
+        const Sum2 a = two_sum(1.0f, -e2);
+        disc = __builtin_fmaf(e2, c2, a.s) +
+               (a.e + __builtin_fmaf(eta, eta, -e2) * (c2 - 1) + e2 * __builtin_fmaf(c, c, -c2));
+    } else {
+        const Sum2 a = two_sum(e2, -1.0f), b = two_sum(a.s, c2);
+        disc = b.s + (a.e + b.e + __builtin_fmaf(eta, eta, -e2) + __builtin_fmaf(c, c, -c2));
+    }
+    if (!__builtin_isfinite(disc)) return o;
+    o.tir = disc <= 0;
+    if (o.tir) {
+        o.F = 1.0f;
+        o.ok = true;
+        return o;
+    }
+    o.ct = __builtin_sqrtf(disc) / (cosine < 0 ? 1.0f : eta);
+    const float rp = (o.etap * c - o.ct) / (o.etap * c + o.ct);
+    const float rs = (c - o.etap * o.ct) / (c + o.etap * o.ct);
+    o.F = 0.5f * (rp * rp + rs * rs);
+    o.ok = __builtin_isfinite(o.ct) && __builtin_isfinite(o.F) && o.F >= 0 && o.F <= 1;
+    return o;
+}
+RP_DEV float fresnel_dielectric(float cosine, float eta) {
+    const Optics o = optics(cosine, eta);
+    return o.ok ? o.F : __builtin_nanf("");
+}
+// c / etap - ct written with the authored index on both sides
+// (shaders/dielectric.h:125-130).
+RP_DEV float refraction_coefficient(float eta, float co, float ct) {
+    const float c = fabsf(co);
+    return co < 0 ? ((eta - 1) * (eta + 1)) / (eta * c + ct) : (((1 - eta) * (1 + eta)) / eta) / (c + eta * ct);
+}
+// Trowbridge-Reitz D with PBRT's grazing cutoff, and Smith Lambda.
 RP_DEV float ggx_d(const Lane &l, F3 m) {
     if (m.z * m.z * m.z * m.z < 1e-16f) return 0.0f;
     F3 v = {m.x / l.ax, m.y / l.ay, m.z};
     float t = dot(v, v);
-    return 1.0f / (kPi * l.ax * l.ay * t * t);
+    const float den = kPi * l.ax * l.ay * t * t;
+    if (!__builtin_isfinite(den) || !(den > 0)) return __builtin_nanf("");
+    const float d = 1.0f / den;
+    return d > 0 ? d : __builtin_nanf("");
 }
 RP_DEV float ggx_lambda(const Lane &l, F3 w) {
-    F3 v = {1.0f, l.ax * w.x / w.z, l.ay * w.y / w.z};
-    return 0.5f * (length(v) - 1.0f);
+    const F3 v = {1.0f, l.ax * w.x / w.z, l.ay * w.y / w.z};
+    const float s = omax(omax(fabsf(v.x), fabsf(v.y)), fabsf(v.z));
+    return 0.5f * (s * length(v * (1.0f / s)) - 1.0f);
 }
-RP_DEV F3 ggx_sample_wm(const Lane &l, F3 w, float u0, float u1) {
-    F3 wh = normalize(F3{l.ax * w.x, l.ay * w.y, w.z});
+// Heitz 2018 visible-normal sampling; the height over the warped disk keeps
+// 1 - u at the disk's edge (shaders/conductor.h:153-178).  False: invalid.
+RP_DEV bool ggx_sample_wm(const Lane &l, F3 w, float u0, float u1, F3 &m) {
+    F3 wh;
+    if (!unit3(F3{l.ax * w.x, l.ay * w.y, w.z}, wh)) return false;
     if (wh.z < 0) wh = -wh;
-    F3 t1 = wh.z < 0.99999f ? normalize(F3{-wh.y, wh.x, 0.0f}) : F3{1.0f, 0.0f, 0.0f};
-    F3 t2 = cross(wh, t1);
-    float r = __builtin_sqrtf(u0);
+    F3 t1 = {1.0f, 0.0f, 0.0f};
+    if (wh.z < 0.99999f && !unit3(F3{-wh.y, wh.x, 0.0f}, t1)) return false;
+    const F3 t2 = cross3(wh, t1);
+    const float r = __builtin_sqrtf(u0);
     float sn, cs;
     sincos_turns(u1, sn, cs);
-    float px = r * cs, py = r * sn;
-    float h = __builtin_sqrtf(1.0f - px * px);
-    float a = (1.0f + wh.z) * 0.5f;
-    py = (1.0f - a) * h + a * py;
-    float pz = __builtin_sqrtf(omax(0.0f, 1.0f - (px * px + py * py)));
-    F3 nh = px * t1 + py * t2 + pz * wh;
-    return normalize(F3{l.ax * nh.x, l.ay * nh.y, omax(1e-6f, nh.z)});
+    const float dx = r * cs, old_y = r * sn;
+    const float h = __builtin_sqrtf(omax(0.0f, 1.0f - dx * dx)), a = (1.0f + wh.z) * 0.5f;
+    const float dy = (1.0f - a) * h + a * old_y;
+    const float old_minus_h = old_y > 0 ? -(1.0f - u0) / (h + old_y) : old_y - h;
+    const float z = __builtin_sqrtf(omax(0.0f, (1.0f - u0) + (1.0f - a) * old_minus_h * (old_y + dy)));
+    const F3 nh = dx * t1 + dy * t2 + z * wh;
+    return unit3(F3{l.ax * nh.x, l.ay * nh.y, omax(1e-6f, nh.z)}, m);
 }
 struct Eval {
     float f, pdf;
 };
+// Rough dielectric at (wo, wi) with its half vector (shaders/dielectric.h:72-124).
+RP_DEV Eval dielectric_eval(const Lane &l, F3 wo, F3 wi) {
+    const float nan = __builtin_nanf("");
+    const bool reflection = (wo.z > 0) == (wi.z > 0), entering = wo.z > 0;
+    const F3 h = reflection ? wo + wi
+                            : (entering ? F3{__builtin_fmaf(l.eta, wi.x, wo.x), __builtin_fmaf(l.eta, wi.y, wo.y),
+                                             __builtin_fmaf(l.eta, wi.z, wo.z)}
+                                        : F3{__builtin_fmaf(l.eta, wo.x, wi.x), __builtin_fmaf(l.eta, wo.y, wi.y),
+                                             __builtin_fmaf(l.eta, wo.z, wi.z)});
+    if (h.x == 0 && h.y == 0 && h.z == 0) return {0.0f, 0.0f};
+    F3 m;
+    if (!unit3(h, m)) return {nan, nan};
+    if (m.z < 0) m = -m;
+    const float co = dot(wo, m), ci = dot(wi, m);
+    if (co * wo.z < 0 || ci * wi.z < 0) return {0.0f, 0.0f};
+    const Optics o = optics(co, l.eta);
+    if (!o.ok) return {nan, nan};
+    const float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), d = ggx_d(l, m);
+    if (!__builtin_isfinite(lo) || !__builtin_isfinite(li) || !__builtin_isfinite(d)) return {nan, nan};
+    if (d == 0) return {0.0f, 0.0f};
+    if (reflection)
+        return {d * o.F / (4 * fabsf(wo.z) * fabsf(wi.z) * (1 + lo + li)), d * o.F / (4 * fabsf(wo.z) * (1 + lo))};
+    const float den = dot(h, m) / (entering ? l.eta : 1.0f);
+    const float jacobian = fabsf(ci) / (den * den);
+    const float pdf = d * fabsf(co) / (fabsf(wo.z) * (1 + lo)) * jacobian * (1 - o.F);
+    const float scale = entering ? 1 / (l.eta * l.eta) : l.eta * l.eta;
+    return {d * (1 - o.F) * fabsf(ci * co) * scale / (fabsf(wo.z) * fabsf(wi.z) * den * den * (1 + lo + li)), pdf};
+}
 RP_DEV Eval eval_lane(const Lane &l, F3 wo, F3 wi) {
     Eval e = {0.0f, 0.0f};
     if (l.kind == RP_MATERIAL_DIFFUSE) {
@@ -867,41 +989,16 @@ RP_DEV Eval eval_lane(const Lane &l, F3 wo, F3 wi) {
     }
     if (l.smooth || wo.z == 0.0f || wi.z == 0.0f) return e;
     if (l.kind == RP_MATERIAL_CONDUCTOR) {
-        if (!(wo.z * wi.z > 0)) return e;
-        F3 h = wo + wi;
-        if (dot(h, h) == 0.0f) return e;
-        F3 m = normalize(h);
-        float F = fresnel_complex(fabsf(dot(wo, m)), l.eta, l.k);
-        float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+        if ((wo.z > 0) != (wi.z > 0)) return e;
+        F3 m;
+        if (!unit3(wo + wi, m)) return {__builtin_nanf(""), __builtin_nanf("")};
+        const float F = fresnel_complex(fabsf(dot(wo, m)), l.eta, l.k);
+        const float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
         e.f = D * F / (4.0f * fabsf(wo.z) * fabsf(wi.z) * (1.0f + lo + li));
         e.pdf = D / (4.0f * fabsf(wo.z) * (1.0f + lo));
         return e;
     }
-    bool reflect = wo.z * wi.z > 0;
-    float etap = 1.0f;
-    if (!reflect) etap = wo.z > 0 ? l.eta : 1.0f / l.eta;
-    F3 h = wi * etap + wo;
-    if (dot(h, h) == 0.0f) return e;
-    F3 m = normalize(h);
-    if (m.z < 0) m = -m;
-    float dmi = dot(m, wi), dmo = dot(m, wo);
-    if (dmi * wi.z < 0 || dmo * wo.z < 0) return e;
-    float F = fresnel_dielectric(dmo, l.eta);
-    float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
-    float G = 1.0f / (1.0f + lo + li);
-    float pdf_wm = D / (fabsf(wo.z) * (1.0f + lo)) * fabsf(dmo);
-    float R = F, T = 1.0f - F;
-    if (reflect) {
-        e.f = D * G * F / fabsf(4.0f * wi.z * wo.z);
-        e.pdf = pdf_wm / (4.0f * fabsf(dmo)) * (R / (R + T));
-    } else {
-        float s = dmi + dmo / etap;
-        float denom = s * s * wi.z * wo.z;
-        float ft = D * T * G * fabsf(dmi * dmo / denom);
-        e.f = ft / (etap * etap);
-        e.pdf = pdf_wm * (fabsf(dmi) / (s * s)) * (T / (R + T));
-    }
-    return e;
+    return dielectric_eval(l, wo, wi);
 }
 // eval_lane for the four hero lanes of one material: the terms that depend
 // only on the directions and the roughness (half vector, Smith Lambda, D,
@@ -923,10 +1020,14 @@ RP_DEV void eval_hero(const Lane l[4], F3 wo, F3 wi, Eval e[4]) {
     if (wo.z == 0.0f || wi.z == 0.0f) return;
     if (kind == RP_MATERIAL_CONDUCTOR) {
         // a conductor lane is smooth by its roughness alone: all lanes alike
-        if (l[0].smooth || !(wo.z * wi.z > 0)) return;
-        const F3 h = wo + wi;
-        if (dot(h, h) == 0.0f) return;
-        const F3 m = normalize(h);
+        if (l[0].smooth || (wo.z > 0) != (wi.z > 0)) return;
+        F3 m;
+        if (!unit3(wo + wi, m)) {
+            const float nan = __builtin_nanf("");
+#pragma unroll
+            for (int k = 0; k < 4; ++k) e[k] = {nan, nan};
+            return;
+        }
         const float c = fabsf(dot(wo, m));
         const float lo = ggx_lambda(l[0], wo), li = ggx_lambda(l[0], wi), D = ggx_d(l[0], m);
         const float den = 4.0f * fabsf(wo.z) * fabsf(wi.z) * (1.0f + lo + li);
@@ -940,30 +1041,44 @@ RP_DEV void eval_hero(const Lane l[4], F3 wo, F3 wi, Eval e[4]) {
         }
         return;
     }
-    if (wo.z * wi.z > 0) { // dielectric reflection: the half vector is the lanes' own
-        const F3 h = wi * 1.0f + wo;
-        if (dot(h, h) == 0.0f) return;
-        F3 m = normalize(h);
+    if (l[0].smooth) { // by roughness all lanes are; by an index of 1 only that lane
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            if (!l[k].smooth) e[k] = dielectric_eval(l[k], wo, wi);
+        return;
+    }
+    const float nan = __builtin_nanf("");
+    if ((wo.z > 0) == (wi.z > 0)) {
+        // dielectric reflection: the half vector, Lambda and D are every
+        // lane's own; only the Fresnel term differs
+        const F3 h = wo + wi;
+        if (h.x == 0 && h.y == 0 && h.z == 0) return;
+        F3 m;
+        if (!unit3(h, m)) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) e[k] = {nan, nan};
+            return;
+        }
         if (m.z < 0) m = -m;
-        const float dmi = dot(m, wi), dmo = dot(m, wo);
-        if (dmi * wi.z < 0 || dmo * wo.z < 0) return;
-        const float lo = ggx_lambda(l[0], wo), li = ggx_lambda(l[0], wi), D = ggx_d(l[0], m);
-        const float G = 1.0f / (1.0f + lo + li);
-        const float pdf_wm = D / (fabsf(wo.z) * (1.0f + lo)) * fabsf(dmo);
-        const float DG = D * G, den = fabsf(4.0f * wi.z * wo.z), pw = pdf_wm / (4.0f * fabsf(dmo));
-        const float F0 = fresnel_dielectric(dmo, l[0].eta);
+        const float co = dot(wo, m), ci = dot(wi, m);
+        if (co * wo.z < 0 || ci * wi.z < 0) return;
+        const float lo = ggx_lambda(l[0], wo), li = ggx_lambda(l[0], wi), d = ggx_d(l[0], m);
+        const bool finite = __builtin_isfinite(lo) && __builtin_isfinite(li) && __builtin_isfinite(d);
+        const float den_f = 4 * fabsf(wo.z) * fabsf(wi.z) * (1 + lo + li), den_p = 4 * fabsf(wo.z) * (1 + lo);
+        const Optics o0 = optics(co, l[0].eta);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (l[k].smooth) continue;
-            const float F = (k == 0 || l[k].eta == l[0].eta) ? F0 : fresnel_dielectric(dmo, l[k].eta);
-            const float R = F, T = 1.0f - F;
-            e[k] = {DG * F / den, pw * (R / (R + T))};
+            const Optics o = (k == 0 || l[k].eta == l[0].eta) ? o0 : optics(co, l[k].eta);
+            if (!o.ok || !finite) e[k] = {nan, nan};
+            else if (d != 0) e[k] = {d * o.F / den_f, d * o.F / den_p};
         }
         return;
     }
-    e[0] = eval_lane(l[0], wo, wi);
+    e[0] = dielectric_eval(l[0], wo, wi);
 #pragma unroll
-    for (int k = 1; k < 4; ++k) e[k] = l[k].eta == l[0].eta ? e[0] : eval_lane(l[k], wo, wi);
+    for (int k = 1; k < 4; ++k)
+        if (!l[k].smooth) e[k] = l[k].eta == l[0].eta ? e[0] : dielectric_eval(l[k], wo, wi);
 }
 
 struct Sampled {
@@ -971,20 +1086,17 @@ struct Sampled {
     float pdf, eta;
     bool delta, transmission, ok;
 };
-RP_DEV bool refract(F3 wi, F3 n, float eta, float &etap, F3 &wt) {
-    float cos_i = dot(n, wi);
-    if (cos_i < 0) {
-        eta = 1.0f / eta;
-        cos_i = -cos_i;
-        n = -n;
-    }
-    float sin2_i = omax(0.0f, 1.0f - cos_i * cos_i);
-    float sin2_t = sin2_i / (eta * eta);
-    if (sin2_t >= 1.0f) return false;
-    float cos_t = __builtin_sqrtf(omax(0.0f, 1.0f - sin2_t));
-    wt = -wi / eta + (cos_i / eta - cos_t) * n;
-    etap = eta;
-    return true;
+// Refraction of wo about the microfacet normal m with the rationalised
+// coefficient (shaders/dielectric.h:176-186); etap is the relative index of
+// the side entered.  False on total internal reflection or an invalid form.
+RP_DEV bool refract(F3 wo, F3 m, float eta, float &etap, F3 &wt) {
+    const float co = dot(wo, m);
+    const Optics o = optics(co, eta);
+    if (!o.ok || o.tir) return false;
+    const float coefficient = eta == 1 ? 0.0f : refraction_coefficient(eta, co, o.ct);
+    wt = (co < 0 ? (-eta) * wo : (-wo) / eta) + coefficient * (co < 0 ? -m : m);
+    etap = o.etap;
+    return finite3(wt);
 }
 RP_DEV F3 cosine_hemisphere(float u0, float u1) {
     float ox = 2.0f * u0 - 1.0f, oy = 2.0f * u1 - 1.0f;
@@ -1008,9 +1120,9 @@ RP_DEV F3 cosine_hemisphere(float u0, float u1) {
 }
 RP_DEV Sampled sample_lane(const Lane &l, F3 wo, float uc, float u0, float u1) {
     Sampled s;
-    s.wi = {0, 0, 0};
-    s.pdf = 0;
-    s.eta = 1;
+    s.wi = F3{0.0f, 0.0f, 0.0f};
+    s.pdf = 0.0f;
+    s.eta = 1.0f;
     s.delta = s.transmission = s.ok = false;
     if (wo.z == 0.0f) return s;
     if (l.kind == RP_MATERIAL_DIFFUSE) {
@@ -1029,59 +1141,55 @@ RP_DEV Sampled sample_lane(const Lane &l, F3 wo, float uc, float u0, float u1) {
             s.ok = true;
             return s;
         }
-        F3 m = ggx_sample_wm(l, wo, u0, u1);
-        F3 wi = 2.0f * dot(wo, m) * m - wo;
-        if (!(wo.z * wi.z > 0)) return s;
-        float lo = ggx_lambda(l, wo);
-        float D = ggx_d(l, m);
-        if (D == 0.0f) return s;
+        F3 m;
+        if (!ggx_sample_wm(l, wo, u0, u1, m)) return s;
+        const F3 wi = -wo + 2.0f * dot(wo, m) * m;
+        if (!finite3(wi) || wi.z == 0.0f || (wo.z > 0) != (wi.z > 0)) return s;
+        const float lo = ggx_lambda(l, wo), D = ggx_d(l, m);
+        if (!(D > 0)) return s;
         s.wi = wi;
         s.pdf = D / (4.0f * fabsf(wo.z) * (1.0f + lo));
-        s.ok = s.pdf > 0;
+        s.ok = __builtin_isfinite(s.pdf) && s.pdf > 0;
         return s;
     }
-    if (l.smooth) {
-        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
-        if (uc < R / (R + T)) {
-            s.wi = {-wo.x, -wo.y, wo.z};
-            s.pdf = R / (R + T);
-        } else {
-            float etap;
-            F3 wt;
-            if (!refract(wo, F3{0.0f, 0.0f, 1.0f}, l.eta, etap, wt)) return s;
-            s.wi = wt;
-            s.pdf = T / (R + T);
-            s.eta = etap;
-            s.transmission = true;
+    // dielectric (shaders/dielectric.h:131-228): the lobe from the Fresnel
+    // reflectance at the sampled microfacet normal (the geometric normal when
+    // smooth)
+    F3 m = {0.0f, 0.0f, 1.0f};
+    if (!l.smooth && !ggx_sample_wm(l, wo, u0, u1, m)) return s;
+    const float co = dot(wo, m);
+    const Optics o = optics(co, l.eta);
+    if (!o.ok) return s;
+    const bool reflection = uc < o.F;
+    F3 wi;
+    if (reflection) wi = -wo + 2.0f * co * m;
+    else {
+        if (o.tir) return s;
+        const float coefficient = l.eta == 1 ? 0.0f : refraction_coefficient(l.eta, co, o.ct);
+        wi = (co < 0 ? (-l.eta) * wo : (-wo) / l.eta) + coefficient * (co < 0 ? -m : m);
+    }
+    if (!finite3(wi) || wi.z == 0.0f || ((wi.z > 0) == (wo.z > 0)) != reflection) return s;
+    float pdf;
+    if (l.smooth) pdf = reflection ? o.F : 1 - o.F;
+    else {
+        const float lo = ggx_lambda(l, wo), li = ggx_lambda(l, wi), D = ggx_d(l, m);
+        if (!__builtin_isfinite(lo) || !__builtin_isfinite(li) || !__builtin_isfinite(D) || D == 0) return s;
+        if (reflection) pdf = D * o.F / (4 * fabsf(wo.z) * (1 + lo));
+        else {
+            const float den = refraction_coefficient(l.eta, co, o.ct);
+            pdf = D * fabsf(co) * o.ct * (1 - o.F) / (fabsf(wo.z) * (1 + lo) * den * den);
         }
-        s.delta = true;
-        s.ok = s.pdf > 0 && s.wi.z != 0.0f;
-        return s;
     }
-    F3 m = ggx_sample_wm(l, wo, u0, u1);
-    float R = fresnel_dielectric(dot(wo, m), l.eta), T = 1.0f - R;
-    float lo = ggx_lambda(l, wo), D = ggx_d(l, m);
-    float pdf_wm = D / (fabsf(wo.z) * (1.0f + lo)) * fabsf(dot(wo, m));
-    if (uc < R / (R + T)) {
-        F3 wi = 2.0f * dot(wo, m) * m - wo;
-        if (!(wo.z * wi.z > 0)) return s;
-        s.wi = wi;
-        s.pdf = pdf_wm / (4.0f * fabsf(dot(wo, m))) * (R / (R + T));
-    } else {
-        float etap;
-        F3 wi;
-        if (!refract(wo, m, l.eta, etap, wi)) return s;
-        if (wo.z * wi.z > 0 || wi.z == 0.0f) return s;
-        float sd = dot(wi, m) + dot(wo, m) / etap;
-        float dwm_dwi = fabsf(dot(wi, m)) / (sd * sd);
-        s.wi = wi;
-        s.pdf = pdf_wm * dwm_dwi * (T / (R + T));
-        s.eta = etap;
-        s.transmission = true;
-    }
-    s.ok = s.pdf > 0 && D > 0;
+    s.wi = wi;
+    s.pdf = pdf;
+    s.delta = l.smooth;
+    s.transmission = !reflection;
+    s.eta = reflection ? 1.0f : o.etap;
+    s.ok = __builtin_isfinite(pdf) && pdf > 0;
     return s;
 }
+// A companion lane's mass and density for the hero's Dirac direction wi: the
+// lane keeps mass only if its own specular direction is bit for bit wi.
 RP_DEV void dirac_lane(const Lane &l, F3 wo, F3 wi, float u0, float u1, float &mass, float &pdf) {
     mass = pdf = 0.0f;
     if (wo.z == 0.0f || wi.z == 0.0f) return;
@@ -1093,12 +1201,13 @@ RP_DEV void dirac_lane(const Lane &l, F3 wo, F3 wi, float u0, float u1, float &m
         return;
     }
     if (l.kind == RP_MATERIAL_DIELECTRIC && l.smooth) {
-        float R = fresnel_dielectric(wo.z, l.eta), T = 1.0f - R;
-        Sampled c = sample_lane(l, wo, transmission ? R / (R + T) : 0.0f, u0, u1);
+        // the lobe forced by the lobe uniform: F selects transmission, 0 reflection
+        const float F = fresnel_dielectric(wo.z, l.eta);
+        Sampled c = sample_lane(l, wo, transmission ? F : 0.0f, u0, u1);
         if (!c.ok || !(c.wi.x == wi.x && c.wi.y == wi.y && c.wi.z == wi.z)) return;
         pdf = c.pdf;
         float scale = transmission ? 1.0f / (c.eta * c.eta) : 1.0f;
-        mass = (transmission ? T : R) * scale;
+        mass = (transmission ? 1 - F : F) * scale;
     }
 }
 
