@@ -1313,14 +1313,16 @@ RP_DEV Selected light_select(const Scene &S, F3 p, F3 n, float u) {
     u = omin((u - infinite) / (1.0f - infinite), 0x1.fffffep-1f);
     float pmf = 1.0f - infinite;
     uint32_t i = 0;
+    // The chosen child's record and importance are carried down, so a level
+    // costs one round trip (the two children's records, loaded together).
+    // A chosen child has positive importance; only a root leaf is tested.
+    LightNode node = S.light_nodes[0];
+    if (node.light != kEmpty)
+        return light_importance(node, p, n) <= 0.0f ? Selected{kEmpty, 0.0f} : Selected{node.light, pmf};
     for (uint32_t steps = 0; steps < S.light_node_count; ++steps) {
-        const LightNode node = S.light_nodes[i];
-        if (node.light != kEmpty) {
-            if (light_importance(node, p, n) <= 0.0f) return {kEmpty, 0.0f};
-            return {node.light, pmf};
-        }
-        const float a = light_importance(S.light_nodes[i + 1], p, n);
-        const float b = light_importance(S.light_nodes[node.child], p, n);
+        const LightNode na = S.light_nodes[i + 1], nb = S.light_nodes[node.child];
+        const float a = light_importance(na, p, n);
+        const float b = light_importance(nb, p, n);
         const float sum = a + b;
         if (!(sum > 0.0f)) return {kEmpty, 0.0f};
         const float q = a / sum;
@@ -1328,11 +1330,14 @@ RP_DEV Selected light_select(const Scene &S, F3 p, F3 n, float u) {
             pmf = pmf * q;
             u = omin(u / q, 0x1.fffffep-1f);
             i = i + 1;
+            node = na;
         } else {
             pmf = pmf * (1.0f - q);
             u = omin((u - q) / (1.0f - q), 0x1.fffffep-1f);
             i = node.child;
+            node = nb;
         }
+        if (node.light != kEmpty) return {node.light, pmf};
     }
     return {kEmpty, 0.0f};
 }
