@@ -416,8 +416,8 @@ def batch_workload(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (paths shard by pixel or sample "
-              f"range; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: this build renders on one GPU "
+              f"(paths would shard by pixel tiles or sample ranges, one film reduction per frame)", file=sys.stderr)
         sys.exit(2)
     import torch
 
