@@ -1501,6 +1501,93 @@ static void quad_coords(const Light &L, V3 q, float &u, float &v) {
     v = std::clamp(dot(d, L.ey) / dot(L.ey, L.ey), 0.0f, 1.0f);
 }
 
+// --- triangle emitters: the spherical-triangle proposal (Arvo 1995, as
+// PBRT v4's SampleSphericalTriangle with the reference's rationalised
+// 1 - cos(beta'); src/light_geometry_sampling_shared.h:109-176,
+// shaders/light_geometry_sampling.h:114-209) for triangles subtending 3e-4
+// to 6.22 sr, warped by the receiver's vertex cosines when a normal is
+// given; otherwise uniform over the area, (b1, b2) = (1 - sqrt u0, u1 sqrt u0).
+static bool positive(V3 v) { return dot(v, v) > 0; }
+static float tri_solid_angle(V3 p0, V3 p1, V3 p2, V3 r) {
+    const V3 a = p0 - r, b = p1 - r, c = p2 - r;
+    if (!positive(a) || !positive(b) || !positive(c)) return 0.0f;
+    return spherical_triangle_area(unit(a), unit(b), unit(c));
+}
+static bool spherical_tri(float area) { return area >= 3e-4f && area <= 6.22f; }
+// Bilinear weights of the vertex cosines: (b, b, a, c) over the square.
+static Bilinear tri_cosines(V3 p0, V3 p1, V3 p2, V3 r, V3 n) {
+    const float wa = std::max(0.01f, std::fabs(dot(n, unit(p0 - r))));
+    const float wb = std::max(0.01f, std::fabs(dot(n, unit(p1 - r))));
+    const float wc = std::max(0.01f, std::fabs(dot(n, unit(p2 - r))));
+    return {wb, wb, wa, wc};
+}
+// Square sample -> barycentrics (b1, b2) of p1 and p2 on the triangle seen
+// from r.  False: a degenerate configuration.
+static bool tri_sample(V3 p0, V3 p1, V3 p2, V3 r, float u0, float u1, float &b1, float &b2) {
+    const V3 a = unit(p0 - r), b = unit(p1 - r), c = unit(p2 - r);
+    const V3 nab = cross3(a, b), nca = cross3(c, a);
+    if (!positive(nab) || !positive(nca)) return false;
+    const float alpha = angle_between(unit(nab), -unit(nca));
+    const float area = spherical_triangle_area(a, b, c);
+    const float ca = cos_f(alpha), sa = sin_f(alpha);
+    const float subarea = u0 * area;
+    const float sh = sin_f(subarea / 2), st = sin_f(subarea), ct_area = cos_f(subarea);
+    const float one_minus_ct = 2 * sh * sh;
+    const float sp = sa * ct_area - ca * st;
+    const float cp = -ca * ct_area - sa * st;
+    const float k1 = -2 * sin_f(alpha - subarea / 2) * sh;
+    const float k2 = sa * (0.5f * dot(a - b, a - b) - one_minus_ct) - ca * st;
+    const float divisor = (k2 * sp + k1 * cp) * sa;
+    if (divisor == 0) return false;
+    const float omc = std::min(2.0f, std::max(0.0f, (-k2 * one_minus_ct - k1 * st) / divisor));
+    const float cosbp = 1 - omc;
+    const V3 arc = c - dot(c, a) * a;
+    if (!positive(arc)) return false;
+    const V3 cprime = cosbp * a + std::sqrt(std::max(0.0f, omc * (2 - omc))) * unit(arc);
+    const float omct = u1 * 0.5f * dot(cprime - b, cprime - b);
+    const float ct = 1 - omct;
+    const V3 tangent = cprime - dot(cprime, b) * b;
+    if (!positive(tangent)) return false;
+    const V3 dir = ct * b + std::sqrt(std::max(0.0f, omct * (2 - omct))) * unit(tangent);
+    const V3 e1 = p1 - p0, e2 = p2 - p0, s1 = cross3(dir, e2);
+    const float denom = dot(s1, e1);
+    if (denom == 0) return false;
+    const V3 s = r - p0;
+    b1 = std::min(1.0f, std::max(0.0f, dot(s, s1) / denom));
+    b2 = std::min(1.0f, std::max(0.0f, dot(dir, cross3(s, e1)) / denom));
+    if (b1 + b2 > 1) {
+        const float sum = b1 + b2;
+        b1 = b1 / sum;
+        b2 = b2 / sum;
+    }
+    return std::isfinite(b1) && std::isfinite(b2);
+}
+// Direction dir from r (a point of the triangle) -> square sample.
+static void tri_inverse(V3 p0, V3 p1, V3 p2, V3 r, V3 dir, float &s0, float &s1) {
+    const V3 a = unit(p0 - r), b = unit(p1 - r), c = unit(p2 - r);
+    V3 cp = unit(cross3(cross3(b, dir), cross3(c, a)));
+    if (dot(cp, a + c) < 0) cp = -cp;
+    s0 = std::min(1.0f, std::max(0.0f, spherical_triangle_area(a, b, cp) / spherical_triangle_area(a, b, c)));
+    s1 = std::min(1.0f, std::max(0.0f, dot(dir - b, dir - b) / dot(cp - b, cp - b)));
+}
+// Solid-angle density of the triangle proposal at point q (geometric normal
+// ng) from r with receiver normal n.
+static float tri_pdf(const Light &L, V3 p0, V3 p1, V3 p2, V3 r, V3 n, V3 q, V3 ng) {
+    const V3 dv = q - r;
+    const float d2 = dot(dv, dv);
+    if (!(d2 > 0)) return 0.0f;
+    const float area = tri_solid_angle(p0, p1, p2, r);
+    if (spherical_tri(area)) {
+        float s0, s1;
+        tri_inverse(p0, p1, p2, r, unit(dv), s0, s1);
+        const bool has_n = (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) && !slip(SLIP_RECT_PDF_UNWARPED);
+        return (has_n ? bilinear_density(s0, s1, tri_cosines(p0, p1, p2, r, n)) : 1.0f) / area;
+    }
+    const float cl = std::fabs(dot(ng, dv / std::sqrt(d2)));
+    return cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+}
+
+
 // A light sample from point p (normal n; zero in media): selection by the
 // scene's light sampler, then the light's own proposal.  pdf includes the
 // selection probability; for delta lights it is that probability alone.
@@ -1515,23 +1602,36 @@ static LightSample sample_light(const Scene &s, V3 p, V3 n, const Wavelengths &w
         if (L.kind == 0) {
             V3 a, b, c;
             s.tri(L.index, a, b, c);
-            float b0, b1;
-            if (u0 < u1) {
-                b0 = u0 * 0.5f;
-                b1 = u1 - b0;
+            float su = std::min(u0, 0x1.fffffep-1f), sv = std::min(u1, 0x1.fffffep-1f), b1, b2;
+            const float area = tri_solid_angle(a, b, c, p);
+            bool spherical = false;
+            if (spherical_tri(area)) {
+                // the warp's bilinear density over the solid angle, at the warped sample
+                float dens = 1.0f;
+                if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f) {
+                    const Bilinear w = tri_cosines(a, b, c, p, n);
+                    bilinear_warp(su, sv, w);
+                    if (!slip(SLIP_RECT_PDF_UNWARPED)) dens = bilinear_density(su, sv, w);
+                }
+                if (!tri_sample(a, b, c, p, su, sv, b1, b2)) return ls;
+                pdf_omega = dens / area;
+                spherical = true;
             } else {
-                b1 = u1 * 0.5f;
-                b0 = u0 - b1;
+                const float root = std::sqrt(su);
+                b1 = 1.0f - root;
+                b2 = sv * root;
             }
-            float b2 = 1.0f - b0 - b1;
+            const float b0 = 1.0f - b1 - b2;
             ls.p = b0 * a + b1 * b + b2 * c;
             ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
             ls.n = normalize(cross(a - c, b - c));
             tri = L.index;
-            const V3 dv = ls.p - p;
-            const float d2 = dot(dv, dv);
-            const float cl = std::fabs(dot(ls.n, dv / std::sqrt(d2)));
-            pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+            if (!spherical) {
+                const V3 dv = ls.p - p;
+                const float d2 = dot(dv, dv);
+                const float cl = std::fabs(dot(ls.n, dv / std::sqrt(d2)));
+                pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+            }
         } else {
             float su = std::min(u0, 0x1.fffffep-1f), sv = std::min(u1, 0x1.fffffep-1f), qu = su, qv = sv;
             bool spherical = false;
@@ -1621,10 +1721,9 @@ static float area_light_pdf(const Scene &s, uint32_t id, V3 p, V3 n, V3 q, V3 ng
         quad_coords(L, q, u, v);
         return pmf * quad_pdf(L, p, n, u, v, q);
     }
-    const V3 dv = q - p;
-    const float d2 = dot(dv, dv);
-    const float cl = std::fabs(dot(ng, dv / std::sqrt(d2)));
-    return cl > 0 ? pmf * (L.inv_area * d2 / cl) : 0.0f;
+    V3 a, b, c;
+    s.tri(L.index, a, b, c);
+    return pmf * tri_pdf(L, a, b, c, p, n, q, ng);
 }
 
 static float env_pdf(const Scene &s) {
@@ -2495,6 +2594,48 @@ float or_quad_pdf(void *sp, uint32_t id, const float p[3], const float n[3], con
     const Light &L = s.lights[id];
     const V3 q = L.p00 + uv[0] * L.ex + uv[1] * L.ey;
     return quad_pdf(L, {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, uv[0], uv[1], q);
+}
+// The triangle proposal of light `id` (an emissive triangle) from p with
+// receiver normal n: barycentrics (b1, b2) and the solid-angle density.
+// Returns 1 for the spherical-triangle proposal, 0 for area sampling, -1
+// when `id` is not a triangle light or the sample failed.
+int or_tri_sample(void *sp, uint32_t id, const float p[3], const float n[3], float u0, float u1, float bary[2],
+                  float *pdf_omega) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    if (id >= s.lights.size() || s.lights[id].kind != 0) return -1;
+    const Light &L = s.lights[id];
+    V3 a, b, c;
+    s.tri(L.index, a, b, c);
+    const V3 r = {p[0], p[1], p[2]}, nn = {n[0], n[1], n[2]};
+    float su = std::min(u0, 0x1.fffffep-1f), sv = std::min(u1, 0x1.fffffep-1f);
+    const float area = tri_solid_angle(a, b, c, r);
+    if (spherical_tri(area)) {
+        float dens = 1.0f;
+        if (nn.x != 0.0f || nn.y != 0.0f || nn.z != 0.0f) {
+            const Bilinear w = tri_cosines(a, b, c, r, nn);
+            bilinear_warp(su, sv, w);
+            dens = bilinear_density(su, sv, w);
+        }
+        if (!tri_sample(a, b, c, r, su, sv, bary[0], bary[1])) return -1;
+        *pdf_omega = dens / area;
+        return 1;
+    }
+    const float root = std::sqrt(su);
+    bary[0] = 1.0f - root;
+    bary[1] = sv * root;
+    const V3 q = (1.0f - bary[0] - bary[1]) * a + bary[0] * b + bary[1] * c;
+    *pdf_omega = tri_pdf(L, a, b, c, r, nn, q, normalize(cross(a - c, b - c)));
+    return 0;
+}
+// The proposal's density at the triangle point with barycentrics (b1, b2).
+float or_tri_pdf(void *sp, uint32_t id, const float p[3], const float n[3], const float bary[2]) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    if (id >= s.lights.size() || s.lights[id].kind != 0) return -1.0f;
+    const Light &L = s.lights[id];
+    V3 a, b, c;
+    s.tri(L.index, a, b, c);
+    const V3 q = (1.0f - bary[0] - bary[1]) * a + bary[0] * b + bary[1] * c;
+    return tri_pdf(L, a, b, c, {p[0], p[1], p[2]}, {n[0], n[1], n[2]}, q, normalize(cross(a - c, b - c)));
 }
 // A sequence of hero events from unit densities: before step i, the balance
 // weight a / (a + b) of a sample drawn with densities pdf_i against the

@@ -16,9 +16,10 @@
 //   ref_light_*          src/light_sampling.cpp:17-27 (power proxy), 294-327 (patch
 //                        rectangle / area), 329-381 (positional and normal cones),
 //                        382-571 (alias table and light BVH)
-//   ref_rect_*           src/light_geometry_sampling_shared.h:84-107,183-260 (bilinear
-//                        warp, rectangle frame, sample and inverse), composed as the
-//                        patch branch of shaders/light_geometry_sampling.h:114-209
+//   ref_rect_*, ref_tri_* src/light_geometry_sampling_shared.h:84-260 (bilinear
+//                        warp, spherical rectangle and triangle samples and inverses),
+//                        composed as the patch and triangle branches of
+//                        shaders/light_geometry_sampling.h:114-209
 //   ref_hero_sequence    src/hero_spectrum_shared.h:102-160 (scaled likelihoods,
 //                        advance, balance)
 //   ref_conductor_*      tests/conductor_reference.hpp:24-103 (double complex Fresnel,
@@ -265,4 +266,42 @@ REF_API int ref_dielectric_sample(double eta, double ax, double ay, int remap, c
     *weight = r.weight;
     *pdf = r.pdf;
     return r.accepted ? 1 : 0;
+}
+
+// The triangle proposal of triangle (p0, p1, p2) (9 floats) from `reference`
+// with receiver normal (zero: none), as the triangle branch of the
+// reference's sample() composes it: barycentrics (b1, b2) of p1 and p2, and
+// the density pdf() gives at the sampled point.  Returns 1 for the
+// spherical triangle, 0 for area sampling (bary set, pdf untouched), -1 when
+// the spherical sample fails.
+REF_API float ref_tri_pdf(const float *reference, const float *corners, const float *normal, const float *q);
+REF_API int ref_tri_sample(const float *reference, const float *corners, const float *normal, float u0, float u1,
+                           float *bary, float *pdf) {
+    const lg::V3 p0 = v3(corners), p1 = v3(corners + 3), p2 = v3(corners + 6), r = v3(reference), n = v3(normal);
+    lg::V2 u(std::min(u0, lg::one_minus_epsilon), std::min(u1, lg::one_minus_epsilon));
+    const float area = lg::triangle_solid_angle(p0, p1, p2, r);
+    if (!lg::spherical_triangle(area)) {
+        const lg::V2 uv = lg::triangle_uniform_uv(u);
+        bary[0] = uv.x;
+        bary[1] = uv.y;
+        return 0;
+    }
+    if (lg::dot(n, n) > 0) u = lg::bilinear_sample(u, lg::triangle_cosines(p0, p1, p2, r, n));
+    lg::V2 uv;
+    if (!lg::triangle_sample(p0, p1, p2, r, u, uv)) return -1;
+    bary[0] = uv.x;
+    bary[1] = uv.y;
+    const lg::V3 q = (1 - uv.x - uv.y) * p0 + uv.x * p1 + uv.y * p2;
+    const float qf[3] = {q.x, q.y, q.z};
+    *pdf = ref_tri_pdf(reference, corners, normal, qf);
+    return 1;
+}
+// pdf() of the triangle branch at point q: -1 when the spherical triangle is
+// not used.
+REF_API float ref_tri_pdf(const float *reference, const float *corners, const float *normal, const float *q) {
+    const lg::V3 p0 = v3(corners), p1 = v3(corners + 3), p2 = v3(corners + 6), r = v3(reference), n = v3(normal);
+    const float area = lg::triangle_solid_angle(p0, p1, p2, r);
+    if (!lg::spherical_triangle(area)) return -1.0f;
+    const lg::V2 square = lg::triangle_inverse(p0, p1, p2, r, lg::unit(v3(q) - r));
+    return (lg::dot(n, n) > 0 ? lg::bilinear_pdf(square, lg::triangle_cosines(p0, p1, p2, r, n)) : 1.f) / area;
 }

@@ -1494,6 +1494,91 @@ RP_DEV void quad_coords(const LightEntry &L, F3 q, float &u, float &v) {
     v = oclamp(dot(d, ey) / dot(ey, ey), 0.0f, 1.0f);
 }
 
+// --- triangle emitters: the spherical-triangle proposal (Arvo 1995, as
+// PBRT v4's SampleSphericalTriangle with the reference's rationalised
+// 1 - cos(beta'); src/light_geometry_sampling_shared.h:109-176,
+// shaders/light_geometry_sampling.h:114-209) for triangles subtending 3e-4
+// to 6.22 sr, warped by the receiver's vertex cosines when a normal is
+// given; otherwise uniform over the area, (b1, b2) = (1 - sqrt u0, u1 sqrt u0).
+RP_DEV bool positive(F3 v) { return dot(v, v) > 0; }
+RP_DEV float tri_solid_angle(F3 p0, F3 p1, F3 p2, F3 r) {
+    const F3 a = p0 - r, b = p1 - r, c = p2 - r;
+    if (!positive(a) || !positive(b) || !positive(c)) return 0.0f;
+    return spherical_triangle_area(unit(a), unit(b), unit(c));
+}
+RP_DEV bool spherical_tri(float area) { return area >= 3e-4f && area <= 6.22f; }
+// Bilinear weights of the vertex cosines: (b, b, a, c) over the square.
+RP_DEV Bilinear tri_cosines(F3 p0, F3 p1, F3 p2, F3 r, F3 n) {
+    const float wa = omax(0.01f, fabsf(dot(n, unit(p0 - r))));
+    const float wb = omax(0.01f, fabsf(dot(n, unit(p1 - r))));
+    const float wc = omax(0.01f, fabsf(dot(n, unit(p2 - r))));
+    return {wb, wb, wa, wc};
+}
+// Square sample -> barycentrics (b1, b2) of p1 and p2 on the triangle seen
+// from r.  False: a degenerate configuration.
+RP_DEV bool tri_sample(F3 p0, F3 p1, F3 p2, F3 r, float u0, float u1, float &b1, float &b2) {
+    const F3 a = unit(p0 - r), b = unit(p1 - r), c = unit(p2 - r);
+    const F3 nab = cross3(a, b), nca = cross3(c, a);
+    if (!positive(nab) || !positive(nca)) return false;
+    const float alpha = angle_between(unit(nab), -unit(nca));
+    const float area = spherical_triangle_area(a, b, c);
+    const float ca = cos_f(alpha), sa = sin_f(alpha);
+    const float subarea = u0 * area;
+    const float sh = sin_f(subarea / 2), st = sin_f(subarea), ct_area = cos_f(subarea);
+    const float one_minus_ct = 2 * sh * sh;
+    const float sp = sa * ct_area - ca * st;
+    const float cp = -ca * ct_area - sa * st;
+    const float k1 = -2 * sin_f(alpha - subarea / 2) * sh;
+    const float k2 = sa * (0.5f * dot(a - b, a - b) - one_minus_ct) - ca * st;
+    const float divisor = (k2 * sp + k1 * cp) * sa;
+    if (divisor == 0) return false;
+    const float omc = omin(2.0f, omax(0.0f, (-k2 * one_minus_ct - k1 * st) / divisor));
+    const float cosbp = 1 - omc;
+    const F3 arc = c - dot(c, a) * a;
+    if (!positive(arc)) return false;
+    const F3 cprime = cosbp * a + __builtin_sqrtf(omax(0.0f, omc * (2 - omc))) * unit(arc);
+    const float omct = u1 * 0.5f * dot(cprime - b, cprime - b);
+    const float ct = 1 - omct;
+    const F3 tangent = cprime - dot(cprime, b) * b;
+    if (!positive(tangent)) return false;
+    const F3 dir = ct * b + __builtin_sqrtf(omax(0.0f, omct * (2 - omct))) * unit(tangent);
+    const F3 e1 = p1 - p0, e2 = p2 - p0, s1 = cross3(dir, e2);
+    const float denom = dot(s1, e1);
+    if (denom == 0) return false;
+    const F3 s = r - p0;
+    b1 = omin(1.0f, omax(0.0f, dot(s, s1) / denom));
+    b2 = omin(1.0f, omax(0.0f, dot(dir, cross3(s, e1)) / denom));
+    if (b1 + b2 > 1) {
+        const float sum = b1 + b2;
+        b1 = b1 / sum;
+        b2 = b2 / sum;
+    }
+    return __builtin_isfinite(b1) && __builtin_isfinite(b2);
+}
+// Direction dir from r (a point of the triangle) -> square sample.
+RP_DEV void tri_inverse(F3 p0, F3 p1, F3 p2, F3 r, F3 dir, float &s0, float &s1) {
+    const F3 a = unit(p0 - r), b = unit(p1 - r), c = unit(p2 - r);
+    F3 cp = unit(cross3(cross3(b, dir), cross3(c, a)));
+    if (dot(cp, a + c) < 0) cp = -cp;
+    s0 = omin(1.0f, omax(0.0f, spherical_triangle_area(a, b, cp) / spherical_triangle_area(a, b, c)));
+    s1 = omin(1.0f, omax(0.0f, dot(dir - b, dir - b) / dot(cp - b, cp - b)));
+}
+// Solid-angle density of the triangle proposal at point q (geometric normal
+// ng) from r with receiver normal n.
+RP_DEV float tri_pdf(const LightEntry &L, F3 p0, F3 p1, F3 p2, F3 r, F3 n, F3 q, F3 ng) {
+    const F3 dv = q - r;
+    const float d2 = dot(dv, dv);
+    if (!(d2 > 0)) return 0.0f;
+    const float area = tri_solid_angle(p0, p1, p2, r);
+    if (spherical_tri(area)) {
+        float s0, s1;
+        tri_inverse(p0, p1, p2, r, unit(dv), s0, s1);
+        return (nonzero(n) ? bilinear_density(s0, s1, tri_cosines(p0, p1, p2, r, n)) : 1.0f) / area;
+    }
+    const float cl = fabsf(dot(ng, dv / __builtin_sqrtf(d2)));
+    return cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+}
+
 // A light sample from p (normal n, zero in media): selection by the scene's
 // light sampler, then the light's proposal; pdf includes the selection pmf
 // (for a spot light it is that pmf alone).
@@ -1513,22 +1598,35 @@ RP_DEV LightSample sample_light(const Scene &S, F3 p, F3 n, const Wavelengths &w
             F3 a, b, c;
             uint32_t id;
             load_tri(S, L.index, a, b, c, id);
-            float b0, b1;
-            if (u0 < u1) {
-                b0 = u0 * 0.5f;
-                b1 = u1 - b0;
+            float su = omin(u0, 0x1.fffffep-1f), sv = omin(u1, 0x1.fffffep-1f), b1, b2;
+            const float area = tri_solid_angle(a, b, c, p);
+            bool spherical = false;
+            if (spherical_tri(area)) {
+                // the warp's bilinear density over the solid angle, at the warped sample
+                float dens = 1.0f;
+                if (nonzero(n)) {
+                    const Bilinear w = tri_cosines(a, b, c, p, n);
+                    bilinear_warp(su, sv, w);
+                    dens = bilinear_density(su, sv, w);
+                }
+                if (!tri_sample(a, b, c, p, su, sv, b1, b2)) return ls;
+                pdf_omega = dens / area;
+                spherical = true;
             } else {
-                b1 = u1 * 0.5f;
-                b0 = u0 - b1;
+                const float root = __builtin_sqrtf(su);
+                b1 = 1.0f - root;
+                b2 = sv * root;
             }
-            float b2 = 1.0f - b0 - b1;
+            const float b0 = 1.0f - b1 - b2;
             ls.p = b0 * a + b1 * b + b2 * c;
             ls.err = gamma_n(6) * (vabs(b0 * a) + vabs(b1 * b) + vabs(b2 * c));
             ls.n = normalize(cross(a - c, b - c));
-            const F3 dv = ls.p - p;
-            const float d2 = dot(dv, dv);
-            const float cl = fabsf(dot(ls.n, dv / __builtin_sqrtf(d2)));
-            pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+            if (!spherical) {
+                const F3 dv = ls.p - p;
+                const float d2 = dot(dv, dv);
+                const float cl = fabsf(dot(ls.n, dv / __builtin_sqrtf(d2)));
+                pdf_omega = d2 > 0 && cl > 0 ? L.inv_area * d2 / cl : 0.0f;
+            }
         } else {
             float su = omin(u0, 0x1.fffffep-1f), sv = omin(u1, 0x1.fffffep-1f), qu = su, qv = sv;
             bool spherical = false;
@@ -1611,10 +1709,10 @@ RP_DEV float area_light_pdf(const Scene &S, uint32_t id, F3 p, F3 n, F3 q, F3 ng
         quad_coords(L, q, u, v);
         return pmf * quad_pdf(L, p, n, u, v, q);
     }
-    const F3 dv = q - p;
-    const float d2 = dot(dv, dv);
-    const float cl = fabsf(dot(ng, dv / __builtin_sqrtf(d2)));
-    return cl > 0 ? pmf * (L.inv_area * d2 / cl) : 0.0f;
+    F3 a, b, c;
+    uint32_t tri_id;
+    load_tri(S, L.index, a, b, c, tri_id);
+    return pmf * tri_pdf(L, a, b, c, p, n, q, ng);
 }
 
 // Visibility with transmittance through null interfaces and media.

@@ -15,11 +15,12 @@ only calls them).  Where those sources are absent the tests skip.
   patch rectangle test, the normal cone and positional cone, and the alias
   table and light BVH that build_light_sampling makes from the oracle's
   per-light inputs, bit for bit, on scenes with many kinds of lights.
-* Quad emitters (src/light_geometry_sampling_shared.h): the spherical
-  rectangle proposal warped by the receiver's corner cosines, its density at
-  the sample and at an emission hit, against the reference's own functions
-  composed as its patch branch composes them (a planted unwarped density
-  fails it).
+* Quad and triangle emitters (src/light_geometry_sampling_shared.h): the
+  spherical rectangle and spherical triangle proposals warped by the
+  receiver's cosines, area sampling outside their solid-angle ranges, their
+  densities at the sample and at an emission hit, against the reference's
+  own functions composed as its patch and triangle branches compose them (a
+  planted unwarped density fails each).
 * Hero balance bookkeeping (src/hero_spectrum_shared.h:102-160): cumulative
   densities, throughputs and balance weights over event sequences, bit for
   bit.
@@ -499,3 +500,89 @@ def test_fresnel_equals_the_reference_double_models(orc):
         e = np.float32(rng.uniform(0.5, 2.5))
         cs = np.float32(c if rng.random() < 0.5 else -c)
         assert abs(orc.L.or_fresnel_dielectric(cs, e) - R.L.ref_dielectric_fresnel(cs, e)) < 5e-7
+
+
+def one_triangle_scene(rng):
+    """A single two-sided emissive triangle of random shape, size and
+    orientation; its vertices in scene order."""
+    from tests.scene_util import SceneBuilder
+
+    b = SceneBuilder(4, 4, 1)
+    c = rng.normal(size=3)
+    size = 10 ** rng.uniform(-1, 0.5)
+    pts = c + rng.normal(size=(3, 3)) * size
+    e = b.emission(b.constant(1.0), two_sided=True)
+    o = b.object(b.diffuse(b.constant(0.0)), e)
+    b.mesh(pts, [(0, 1, 2)], o)
+    b.look_at((0, 0, 10), (0, 0, 0), (0, 1, 0), 40)
+    d = b.build()
+    return d, np.ascontiguousarray(np.asarray(b.positions, np.float32).reshape(-1)), c, size
+
+
+def _tri_compare(orc, R, seed, count):
+    import ctypes as C
+
+    fp = C.POINTER(C.c_float)
+    orc.L.or_tri_sample.restype, orc.L.or_tri_sample.argtypes = C.c_int, [C.c_void_p, C.c_uint32, fp, fp, C.c_float,
+                                                                          C.c_float, fp, fp]
+    orc.L.or_tri_pdf.restype, orc.L.or_tri_pdf.argtypes = C.c_float, [C.c_void_p, C.c_uint32, fp, fp, fp]
+    R.L.ref_tri_sample.restype, R.L.ref_tri_sample.argtypes = C.c_int, [fp, fp, fp, C.c_float, C.c_float, fp, fp]
+    R.L.ref_tri_pdf.restype, R.L.ref_tri_pdf.argtypes = C.c_float, [fp, fp, fp, fp]
+    P = lambda a: a.ctypes.data_as(fp)  # noqa: E731
+    rng = np.random.default_rng(seed)
+    zero = np.zeros(3, np.float32)
+    dbary, dpdf, dhit, area, modes = [], [], [], [], [0, 0, 0]
+    while len(dbary) + modes[0] < count:
+        d, corners, c, size = one_triangle_scene(rng)
+        sc = orc.scene(d)
+        for _ in range(64):
+            dirn = rng.normal(size=3)
+            dirn /= np.linalg.norm(dirn)
+            p = (c + dirn * size * 10 ** rng.uniform(0, 1.5)).astype(np.float32)
+            n = rng.normal(size=3)
+            n = (n / np.linalg.norm(n)).astype(np.float32) if rng.random() < 0.75 else zero
+            u = rng.random(2).astype(np.float32)
+            bo, po = np.zeros(2, np.float32), np.zeros(1, np.float32)
+            br, pr = np.zeros(2, np.float32), np.zeros(1, np.float32)
+            mo = orc.L.or_tri_sample(sc.h, 0, P(p), P(n), float(u[0]), float(u[1]), P(bo), P(po))
+            mr = R.L.ref_tri_sample(P(p), P(corners), P(n), float(u[0]), float(u[1]), P(br), P(pr))
+            if mo != mr:
+                modes[2] += 1  # the 3e-4 / 6.22 sr thresholds met at a rounding boundary
+                continue
+            if mr == 0:  # area sampling: the same square-to-triangle map
+                modes[0] += 1
+                assert np.allclose(bo, br, atol=1e-6), (bo, br)
+                continue
+            if mr < 0:
+                continue
+            modes[1] += 1
+            dbary.append(float(np.max(np.abs(bo - br))))
+            dpdf.append(abs(float(po[0]) / float(pr[0]) - 1))
+            q = ((1 - bo[0] - bo[1]) * corners[:3] + bo[0] * corners[3:6] + bo[1] * corners[6:]).astype(np.float32)
+            dhit.append(abs(orc.L.or_tri_pdf(sc.h, 0, P(p), P(n), P(bo)) / R.L.ref_tri_pdf(P(p), P(corners), P(n),
+                                                                                            P(q)) - 1))
+            area.append(1.0 / R.L.ref_tri_pdf(P(p), P(corners), P(zero), P(q)))
+    return modes, np.array(dbary), np.array(dpdf), np.array(dhit), np.array(area)
+
+
+def test_triangle_proposal_follows_the_reference(orc):
+    """Emissive triangles: the spherical-triangle proposal warped by the
+    receiver's vertex cosines between 3e-4 and 6.22 sr, and area sampling
+    with the reference's square-to-triangle map outside, against the
+    reference's own triangle_sample / triangle_inverse / triangle_cosines /
+    bilinear_* composed as its triangle branch composes them.  Tolerances as
+    for the rectangle: tight where the triangle subtends 0.01 sr or more."""
+    R = ref()
+    modes, dbary, dpdf, dhit, area = _tri_compare(orc, R, 51, 2500)
+    assert modes[1] >= 1500 and modes[0] >= 50 and modes[2] <= 0.01 * sum(modes), modes
+    well = area >= 1e-2
+    for d in (dbary, dpdf, dhit):
+        assert d[well].max() < 1e-4, d[well].max()
+    assert np.quantile(dpdf, 0.99) < 1e-4 and np.quantile(dhit, 0.99) < 1e-4
+
+
+def test_triangle_proposal_pin_sees_an_unwarped_density(orc):
+    R = ref()
+    with orc.planted_slip(12):
+        _, _, dpdf, dhit, _ = _tri_compare(orc, R, 52, 400)
+    assert np.quantile(dhit, 0.5) > 1e-2
