@@ -248,7 +248,11 @@ static Hit closest(const Scene &s, V3 o, V3 d, float tmax) {
     stack[sp++] = 0;
     while (sp) {
         const auto &n = s.nodes[stack[--sp]];
-        if (!box_hit(n, o, inv, std::min(tmax, h.t))) continue;
+        // Cull beyond min(tmax, h.t) grown by 2^-16: a box's rounded entry
+        // distance can exceed the rounded distance of a triangle inside it
+        // that ties the closest hit (a shared edge), which the smallest-id
+        // rule must still reach.
+        if (!box_hit(n, o, inv, std::min(tmax, h.t) * (1.0f + 0x1p-16f))) continue;
         if (n.count) {
             for (uint32_t i = 0; i < n.count; ++i) {
                 uint32_t tri = s.order[n.left_or_first + i];

@@ -570,41 +570,77 @@ struct Stack {
 
 RP_DEV float byte_of(uint32_t w, int i) { return float((w >> (8 * i)) & 0xffu); }
 
+// Reciprocal direction component for the slab test.  An exact zero gives
+// +-2^100 instead of an infinity: q * A + B with A infinite is NaN, and an
+// axis without a finite scale would cull nothing (a vertical ray over a
+// heightfield would visit every node under it).  With 2^100 a ray parallel
+// to the slab gets an interval of at least +-(pad + 2^-20 w) * 2^100 around
+// 0 when its origin lies inside (pad >= 1e-30 is the builder's padding, w
+// the child's extent on that axis), beyond any hit distance unless the
+// child is thinner than ~1e-18 on that axis, whose geometry such a ray can
+// only meet in its own plane, where the triangle test reports no hit; when
+// the origin lies outside at a gap g, the interval starts near g * 2^100,
+// which culls the box unless g is below ~1e-24 (then the box costs a visit,
+// never a hit).  A and B stay finite for scenes under ~1e8 units.
+RP_DEV float slab_inv(float d) { return d == 0.0f ? copysignf(0x1p100f, d) : 1.0f / d; }
+
 // Tests the 4 children of a node; returns hit mask and entry distances.
+//
+// A child's plane on axis x sits at P = ox + q * sx (8-bit q, power-of-two
+// sx).  Along the ray its distance is (P - o.x) / d.x = q * A + B with
+// A = sx * inv.x (exact: sx is a power of two) and B = (ox - o.x) * inv.x,
+// so each plane costs one byte conversion and one fused multiply-add; the
+// direction's sign picks which of the two bounds is the near plane, so the
+// slab needs no min/max.  The computed distances differ from the exact ones
+// by at most 2^-24 * (510 |A| + 5.1 |B|) (the rounding of B, the reciprocal,
+// the fused sum, and the widened offsets below); E = 2^-20 * (255 |A| +
+// 2 |B|) exceeds that six-fold, and the near planes use B - E, the far
+// planes B + E.
+// The slab interval so computed contains the exact one, so no box that the
+// ray meets is culled.  The builder pads each child box by 2^-20 of its
+// magnitude before quantising, which keeps the exact P (not only its
+// rounded value) outside the child's geometry.  `inv` comes from slab_inv
+// (finite for a zero component); where A or B overflows on a tiny nonzero
+// component, the distances are NaN, which min/max ignore, or infinite on
+// the side the exact values lie.
 RP_DEV uint32_t test_children(const uint4 *nodes, uint32_t node, F3 o, F3 inv, float tmax, float tn[4], uint32_t child[4]) {
     const uint4 q0 = nodes[4 * node + 0];
     const uint4 q1 = nodes[4 * node + 1];
     const uint4 q2 = nodes[4 * node + 2];
     const uint4 q3 = nodes[4 * node + 3];
-    float ox = __uint_as_float(q0.x), oy = __uint_as_float(q0.y), oz = __uint_as_float(q0.z);
-    float sx = __uint_as_float((q0.w & 0xffu) << 23);
-    float sy = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
-    float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
+    const float sx = __uint_as_float((q0.w & 0xffu) << 23);
+    const float sy = __uint_as_float(((q0.w >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((q0.w >> 16) & 0xffu) << 23);
     child[0] = q2.z;
     child[1] = q2.w;
     child[2] = q3.x;
     child[3] = q3.y;
-    const float grow = 1.0f + 2.0f * gamma_n(3);
+    const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
+    const float bx = (__uint_as_float(q0.x) - o.x) * inv.x;
+    const float by = (__uint_as_float(q0.y) - o.y) * inv.y;
+    const float bz = (__uint_as_float(q0.z) - o.z) * inv.z;
+    const float ex = __builtin_fmaf(fabsf(bx), 0x1p-19f, fabsf(ax) * (255.0f * 0x1p-20f));
+    const float ey = __builtin_fmaf(fabsf(by), 0x1p-19f, fabsf(ay) * (255.0f * 0x1p-20f));
+    const float ez = __builtin_fmaf(fabsf(bz), 0x1p-19f, fabsf(az) * (255.0f * 0x1p-20f));
+    // near and far bounds of the four children (qlo words q1.x/y/z, qhi
+    // words q1.w, q2.x, q2.y), by the sign of the direction
+    const bool px = !(inv.x < 0.0f), py = !(inv.y < 0.0f), pz = !(inv.z < 0.0f);
+    const uint32_t nxw = px ? q1.x : q1.w, fxw = px ? q1.w : q1.x;
+    const uint32_t nyw = py ? q1.y : q2.x, fyw = py ? q2.x : q1.y;
+    const uint32_t nzw = pz ? q1.z : q2.y, fzw = pz ? q2.y : q1.z;
+    const float nbx = bx - ex, fbx = bx + ex;
+    const float nby = by - ey, fby = by + ey;
+    const float nbz = bz - ez, fbz = bz + ez;
     uint32_t mask = 0;
-    // A bound is origin + q * scale with an 8-bit q and a power-of-two scale:
-    // the product is exact, so one fused multiply-add rounds exactly like the
-    // separate multiply and add (the arithmetic the BVH builder verifies).
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        float lx = __builtin_fmaf(byte_of(q1.x, i), sx, ox), hx = __builtin_fmaf(byte_of(q1.w, i), sx, ox);
-        float ly = __builtin_fmaf(byte_of(q1.y, i), sy, oy), hy = __builtin_fmaf(byte_of(q2.x, i), sy, oy);
-        float lz = __builtin_fmaf(byte_of(q1.z, i), sz, oz), hz = __builtin_fmaf(byte_of(q2.y, i), sz, oz);
-        float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
-        float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
-        float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
-        float nx = fminf(t0x, t1x), fx = fmaxf(t0x, t1x);
-        float ny = fminf(t0y, t1y), fy = fmaxf(t0y, t1y);
-        float nz = fminf(t0z, t1z), fz = fmaxf(t0z, t1z);
-        float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
-        float tfar = fminf(fminf(fx, fy), fz) * grow;
-        tfar = fminf(tfar, tmax);
+        const float nx = __builtin_fmaf(byte_of(nxw, i), ax, nbx), fx = __builtin_fmaf(byte_of(fxw, i), ax, fbx);
+        const float ny = __builtin_fmaf(byte_of(nyw, i), ay, nby), fy = __builtin_fmaf(byte_of(fyw, i), ay, fby);
+        const float nz = __builtin_fmaf(byte_of(nzw, i), az, nbz), fz = __builtin_fmaf(byte_of(fzw, i), az, fbz);
+        const float tnear = fmaxf(fmaxf(nx, ny), fmaxf(nz, 0.0f));
+        const float tfar = fminf(fminf(fx, fy), fminf(fz, tmax));
         tn[i] = tnear;
-        bool hit = child[i] != kEmpty && tnear <= tfar;
+        const bool hit = child[i] != kEmpty && tnear <= tfar;
         mask |= hit ? (1u << i) : 0u;
     }
     return mask;
@@ -617,6 +653,8 @@ RP_DEV void load_tri(const Scene &S, uint32_t p, F3 &a, F3 &b, F3 &c, uint32_t &
     c = {t2.x, t2.y, t2.z};
     id = __float_as_uint(t0.w);
 }
+
+constexpr float kCutGrowth = 1.0f + 0x1p-16f;
 
 struct TraversalCounts {
     uint32_t nodes, tris;
@@ -643,7 +681,13 @@ RP_DEV bool traverse(const Scene &S, Stack &st, F3 o, F3 d, float tmax, Hit &h, 
     h.id = kEmpty;
     h.t = __builtin_inff();
     if (S.node_count == 0) return false;
-    F3 inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    F3 inv = {slab_inv(d.x), slab_inv(d.y), slab_inv(d.z)};
+    // Boxes and stack entries are culled beyond `cut` = min(tmax, h.t)
+    // grown by 2^-16: a box's computed entry distance and a triangle's
+    // computed hit distance round differently, and a triangle that ties the
+    // closest hit (a shared edge) must still be reached for the smallest-id
+    // rule, whatever the visiting order.  Growing the cut only visits more.
+    float cut = tmax * kCutGrowth;
     st.sp = 0;
     uint32_t cur = 0;
     for (;;) {
@@ -663,13 +707,14 @@ RP_DEV bool traverse(const Scene &S, Stack &st, F3 o, F3 d, float tmax, Hit &h, 
                     h.b1 = b1;
                     h.b2 = b2;
                     if (kAny) return true;
+                    cut = t * kCutGrowth; // t <= tmax
                 }
             }
         } else {
             float tn[4];
             uint32_t ch[4];
             if (kCount) tc->nodes += 1;
-            uint32_t mask = test_children(S.nodes, cur, o, inv, omin(tmax, h.t), tn, ch);
+            uint32_t mask = test_children(S.nodes, cur, o, inv, cut, tn, ch);
             if (mask) {
                 // Order hit children by entry distance with a 4-input sorting
                 // network on registers (no dynamically indexed arrays, which
@@ -702,7 +747,7 @@ RP_DEV bool traverse(const Scene &S, Stack &st, F3 o, F3 d, float tmax, Hit &h, 
             uint32_t n;
             float t;
             st.pop(n, t);
-            if (t <= h.t) {
+            if (t <= cut) {
                 cur = n;
                 found = true;
                 break;
