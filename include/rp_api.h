@@ -178,7 +178,7 @@ typedef enum rp_light_sampler {
 } rp_light_sampler;
 
 typedef struct rp_build_options {
-    uint32_t leaf_size;       /* max triangles per BVH leaf (1..8), 0 = default 4 */
+    uint32_t leaf_size;       /* max triangles per BVH leaf (1..8), 0 = default 2 */
     uint32_t sah_bins;        /* 0 = default 32 */
     uint32_t threads;         /* host build threads, 0 = hardware concurrency */
     uint32_t device;          /* HIP device ordinal */

@@ -272,7 +272,11 @@ Bvh4 build_bvh4(const float *positions, const uint32_t *indices, uint32_t triang
     (void)threads;
     if (triangle_count == 0) return out;
     Builder b;
-    b.leaf_size = std::min(std::max(leaf_size ? leaf_size : 4u, 1u), 15u);
+    // Default 2 triangles per leaf: measured wavefront frames (MI355X) with
+    // leaves of 4 / 2 / 1: 1080p terrain 27.80 / 27.22 / 29.50 ms, volume
+    // 512^2 32 spp 16.55 / 16.05 / 16.52 ms (64 bins), Cornell 512^2 64 spp
+    // 12.45 / 11.53 ms.
+    b.leaf_size = std::min(std::max(leaf_size ? leaf_size : 2u, 1u), 15u);
     b.bins = std::min(std::max(bins ? bins : 32u, 4u), 256u);
     b.prims.resize(triangle_count);
     for (uint32_t t = 0; t < triangle_count; ++t) {
