@@ -120,6 +120,21 @@ def test_frame_modes_identical(rp, cornell):
         assert c == counts[0]
 
 
+@pytest.mark.parametrize("width, height, spp", [(37, 21, 3), (40, 24, 2)])
+def test_frame_modes_identical_small_films(rp, gpu_available, width, height, spp):
+    """Wavefront films equal the megakernel's bit for bit on a film of whole
+    8x8 tiles and on one that is not (rows of 37 pixels), whose 2331 paths
+    leave the last wave of every queue launch part-empty."""
+    synth = rp.Synthetic(rp.SYNTH_CORNELL, seed=3, width=width, height=height)
+    scene = rp.Scene(synth.desc)
+    ref, _ = scene.render_frame(rp.frame_config(spp, mode=rp.MODE_MEGAKERNEL, seed=2))
+    for mode in (rp.MODE_WAVEFRONT, rp.MODE_WAVEFRONT_SORTED):
+        img, st = scene.render_frame(rp.frame_config(spp, mode=mode, seed=2, flags=rp.FRAME_NO_TAIL))
+        print(f"[small film] {width}x{height} mode {mode}: {st}", flush=True)
+        assert st["failed_paths"] == 0 and st["extend_rays"] >= width * height * spp
+        assert np.array_equal(ref.view(np.uint32), img.view(np.uint32))
+
+
 def test_frame_vs_oracle(rp, orc, cornell):
     synth, scene = cornell
     img, _ = scene.render_frame(rp.frame_config(16, mode=rp.MODE_WAVEFRONT_SORTED, seed=1))
