@@ -253,7 +253,7 @@ typedef struct rp_frame_config {
     uint32_t randomization;
     uint32_t capacity;         /* wavefront paths in flight, 0 = auto */
     uint32_t group_size;       /* threads per workgroup, 0 = auto */
-    uint32_t sort_bits;        /* material sort-key width, 0 = auto */
+    uint32_t sort_bits;        /* material sort-key width (1..8), 0 = default 4 */
     uint32_t flags;            /* RP_FRAME_* bits below */
     uint32_t tail_paths;       /* wavefront: once at most this many paths are live, one kernel
                                   finishes them, a thread per path; 0 = auto (the lanes the

@@ -203,6 +203,7 @@ struct LightNode {
 
 struct Scene {
     const float4 *tris;     // 3 per triangle in BVH leaf order: (v0, scene id) (v1, object) (v2, light)
+    const uint32_t *tri_material; // material index per triangle, BVH leaf order (sort keys)
     const uint4 *nodes;     // 4 per 64-byte node
     uint32_t node_count, tri_count;
     const rp_object *objects;

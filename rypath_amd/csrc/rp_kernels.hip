@@ -76,6 +76,7 @@ struct rp_scene {
     rp_build_info info{};
     int device = 0;
     DevBuf<float4> tris;
+    DevBuf<uint32_t> tri_material; // material index per triangle, BVH leaf order (sort keys)
     DevBuf<QNode> nodes;
     DevBuf<rp_object> objects;
     DevBuf<rp_material> materials;
@@ -357,8 +358,10 @@ void upload_scene(rp_scene *s) {
     std::vector<uint32_t> leaf_pos(d.triangle_count);
     for (uint32_t p = 0; p < d.triangle_count; ++p) leaf_pos[s->bvh.tri_order[p]] = p;
     std::vector<float4> tris(3 * size_t(d.triangle_count));
+    std::vector<uint32_t> tri_material(std::max(d.triangle_count, 1u), 0u);
     for (uint32_t p = 0; p < d.triangle_count; ++p) {
         uint32_t t = s->bvh.tri_order[p];
+        tri_material[p] = d.objects[d.tri_object[t]].material;
         const float *v[3];
         for (int k = 0; k < 3; ++k) v[k] = d.positions + 3 * size_t(d.indices[3 * t + k]);
         uint32_t w[3] = {t, d.tri_object[t], tri_light[t]};
@@ -416,6 +419,7 @@ void upload_scene(rp_scene *s) {
         e.flags = h.flags;
     }
     s->tris.upload(tris);
+    s->tri_material.upload(tri_material);
     s->nodes.upload(s->bvh.nodes);
     s->objects.upload(s->host.objects);
     s->materials.upload(s->host.materials);
@@ -440,6 +444,7 @@ void upload_scene(rp_scene *s) {
 
     rpd::Scene &S = s->dev;
     S.tris = s->tris.p;
+    S.tri_material = s->tri_material.p;
     S.nodes = reinterpret_cast<const uint4 *>(s->nodes.p);
     S.node_count = uint32_t(s->bvh.nodes.size());
     S.tri_count = d.triangle_count;
@@ -477,7 +482,8 @@ void upload_scene(rp_scene *s) {
     for (auto &l : dl)
         if (l.kind == 0 || l.kind == 3) ++s->info.emitter_count;
     s->info.light_count = uint32_t(dl.size());
-    s->info.device_bytes = s->tris.n * sizeof(float4) + s->nodes.n * sizeof(QNode) + s->spectra.n * 4 +
+    s->info.device_bytes = s->tris.n * sizeof(float4) + s->tri_material.n * 4 + s->nodes.n * sizeof(QNode) +
+                           s->spectra.n * 4 +
                            s->sobol_dir.n * 4 + s->lights.n * sizeof(rpd::LightEntry) +
                            s->light_nodes.n * sizeof(rpd::LightNode);
 }
