@@ -70,14 +70,18 @@ PMC_SUMMARY = ROOT / "profiles" / "terrain_1080p_hbm_traffic.md"
 TRAVERSAL_KERNELS = ("void rpd::k_wf_extend<false", "void rpd::k_wf_trace<false", "void rpd::k_wf_shadow<false")
 
 
-def pmc_traffic():
+def pmc_traffic(frame_mode):
     """HBM bytes per traversal launch (closest hit, fused closest hit + shadow,
-    shadow alone) of the 1080p terrain wavefront frame, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by its copy
-    calibration).  None when the summary is absent."""
+    shadow alone) of the 1080p terrain frame, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by its copy calibration).  None
+    when the summary is absent or was recorded for another frame mode than
+    the timed one."""
     launches, mib = 0, 0.0
     try:
-        for line in PMC_SUMMARY.read_text().splitlines():
+        text = PMC_SUMMARY.read_text()
+        if f"--modes {frame_mode} " not in text:
+            return None
+        for line in text.splitlines():
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             if cells and cells[0].startswith(TRAVERSAL_KERNELS):
                 launches += int(cells[1])
@@ -511,8 +515,8 @@ def main():
         launches_ = max(st["extend_launches"] + st["shadow_launches"], 1)
         achieved = alg / t_trav / 1e9
         avg_launch = t_trav / launches_
-        traffic = pmc_traffic() if (args.workload == "terrain" and (args.width, args.height, args.spp) ==
-                                    (1920, 1080, 16) and frame_mode == "wave") else None
+        traffic = pmc_traffic(frame_mode) if (args.workload == "terrain" and (args.width, args.height, args.spp) ==
+                                              (1920, 1080, 16)) else None
         # measured read peaks on this device (rp_probe_read_bandwidth): an
         # 8 GiB buffer streams from HBM; 24 MiB read 400 times stays in the
         # XCDs' L2 (3 MiB per XCD, above the 32 KiB L1 share of each CU)

@@ -257,23 +257,34 @@ __global__ void k_pool_sort_hist(PoolView V, uint32_t p, uint32_t bits) {
     for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
         if (h[k]) atomicAdd(&V.F.hist[k * kQueueStride], h[k]);
 }
-// Each tile of blockDim entries is ranked in LDS.
+// Each tile of kSortBlockItems entries (16 per thread, coalesced) is ranked
+// in LDS and reserves its key ranges with one global atomic per key: with
+// 256-entry tiles those serialised atomics dominated the sort.
 __global__ void k_pool_sort_scatter(PoolView V, uint32_t p, uint32_t bits) {
     __shared__ uint32_t h[256], base[256];
     const uint32_t K = 1u << bits, n = V.c->qn[p];
-    RP_GRID_STRIDE(n) {
+    for (uint32_t tile = blockIdx.x * kSortBlockItems; tile < n; tile += gridDim.x * kSortBlockItems) {
         for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) h[k] = 0;
         __syncthreads();
-        uint32_t i = base_ + threadIdx.x, key = 0, rank = 0;
-        if (i < n) {
-            key = pool_key(V, p, i, bits);
-            rank = atomicAdd(&h[key], 1u);
+        uint32_t packed[kSortItems]; // key << 12 | rank within the tile (< 4096)
+#pragma unroll
+        for (uint32_t j = 0; j < kSortItems; ++j) {
+            const uint32_t i = tile + j * 256 + threadIdx.x;
+            packed[j] = 0;
+            if (i < n) {
+                const uint32_t key = pool_key(V, p, i, bits);
+                packed[j] = key << 12 | atomicAdd(&h[key], 1u);
+            }
         }
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
             base[k] = h[k] ? atomicAdd(&V.F.hist[(256 + k) * kQueueStride], h[k]) : 0;
         __syncthreads();
-        if (i < n) V.F.perm[base[key] + rank] = i;
+#pragma unroll
+        for (uint32_t j = 0; j < kSortItems; ++j) {
+            const uint32_t i = tile + j * 256 + threadIdx.x;
+            if (i < n) V.F.perm[base[packed[j] >> 12] + (packed[j] & 4095u)] = i;
+        }
         __syncthreads();
     }
 }
