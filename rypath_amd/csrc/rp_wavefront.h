@@ -769,17 +769,21 @@ inline double wavefront_render(const Scene &S, const PathParams &P, FrameBuffers
         WF_CHECK(hipEventRecord(e1, stream));
         ++launches;
         hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(256), 0, stream, F.qcount, F.hist, bits);
-        if (sorted) {
+        // The first bounce stays in generation order: its waves are 8x8
+        // pixel tiles whose camera rays already hit alike surfaces, and
+        // sorting them cost 0.29 ms in passes for 0.09 ms of shading.
+        const bool sort_now = sorted && !gen;
+        if (sort_now) {
             const dim3 g(sort_blocks(qn));
             hipLaunchKernelGGL(k_sort_hist, g, dim3(256), 0, stream, S, F.hit_id, nullptr, qn, bits, F.hist);
             hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(64), 0, stream, bits, F.hist);
             hipLaunchKernelGGL(k_sort_scatter, g, dim3(256), 0, stream, S, F.hit_id, nullptr, qn, bits, F.hist, F.perm);
         }
         if (gen)
-            hipLaunchKernelGGL(k_wf_shade<true>, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx,
+            hipLaunchKernelGGL(k_wf_shade<true>, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sort_now, npx,
                                sample_begin);
         else
-            hipLaunchKernelGGL(k_wf_shade<false>, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sorted, npx,
+            hipLaunchKernelGGL(k_wf_shade<false>, grid, dim3(kWfBlock), 0, stream, S, P, C, N, F, qn, sort_now, npx,
                                sample_begin);
         gen = false;
         WF_CHECK(hipMemcpyAsync(FB.host_count, F.qcount, (kQueueStride + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost,
